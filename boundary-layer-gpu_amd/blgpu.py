@@ -1,0 +1,281 @@
+"""Python binding of the blgpu C ABI (include/blgpu.h) via ctypes.
+
+Fields are numpy arrays in the reference's Fortran layout (x fastest, full
+allocated extent).  The library must be built in-tree
+(`make -C boundary-layer-gpu_amd`); importing this module never falls back to
+another implementation.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libblgpu.so")
+
+FIELD_U, FIELD_V, FIELD_W, FIELD_P, FIELD_NUT = range(5)
+PREC_FP64, PREC_MIXED = 0, 1
+
+_D, _I, _P = ctypes.c_double, ctypes.c_int, ctypes.c_void_p
+_DP = ctypes.POINTER(ctypes.c_double)
+
+
+class BLConfig(ctypes.Structure):
+    _fields_ = [("nx", _I), ("ny", _I), ("nz", _I), ("x", _DP), ("y", _DP), ("z", _DP), ("nu", _D),
+                ("dPdx", _D), ("dPdz", _D), ("cfl", _D), ("scheme", _I), ("les", _I), ("precision", _I),
+                ("U_inlet", _DP), ("V_inlet", _DP), ("W_inlet", _DP), ("U_top", _DP), ("V_top", _DP),
+                ("W_top", _DP), ("n_modes", _I), ("m_modes", _I), ("zmode", _DP), ("tmode", _DP),
+                ("qu_re", _DP), ("qu_im", _DP), ("qv_re", _DP), ("qv_im", _DP), ("qw_re", _DP),
+                ("qw_im", _DP)]
+
+
+# every entry point declared in include/blgpu.h with its ctypes signature
+SIGNATURES = {
+    "bl_create": (_I, [ctypes.POINTER(BLConfig), ctypes.POINTER(_P)]),
+    "bl_destroy": (None, [_P]),
+    "bl_last_error": (ctypes.c_char_p, []),
+    "bl_upload": (_I, [_P, _I, _P]),
+    "bl_download": (_I, [_P, _I, _P]),
+    "bl_set_time": (_I, [_P, _D]),
+    "bl_get_time": (_I, [_P, _DP]),
+    "bl_compute_dt": (_I, [_P, _DP]),
+    "bl_step": (_I, [_P]),
+    "bl_run": (_I, [_P, _I]),
+    "bl_synchronize": (_I, [_P]),
+    "bl_eddy_viscosity": (_I, [_P]),
+    "bl_max_divergence": (_I, [_P, _DP]),
+    "bl_wall_stats": (_I, [_P, _P]),
+    "bl_monitor": (_I, [_P, _P]),
+    "bl_rhs": (_I, [_P, _P, _P, _P]),
+    "bl_set_old": (_I, [_P, _P, _P, _P]),
+    "bl_apply_bc": (_I, [_P, _I, _D]),
+    "bl_projection": (_I, [_P, _I, _D]),
+    "bl_poisson": (_I, [_P, _P]),
+    "bl_profile_enable": (_I, [_P, _I]),
+    "bl_profile_read": (_I, [_P, _P, _P, _I]),
+    "bl_profile_count": (_I, []),
+    "bl_profile_name": (ctypes.c_char_p, [_I]),
+    "bl_blasius_profile": (_I, [_I, _D, _D, _I, _P, _P, _P]),
+    "bl_blasius_tables": (_I, [_I, _I, _P, _P, _D, _I, _P, _P, _P] + [_P] * 6),
+    "bl_blasius_field": (_I, [_I, _I, _I, _P, _P, _D, _I, _P, _P, _P, _P, _P, _P]),
+    "bl_add_noise": (_I, [_I, _I, _I, _D, ctypes.c_ulonglong, _P, _P, _P]),
+    "bl_grid_tanh": (_I, [_I, _I, _I, _D, _D, _D, _D, _D, _P, _P, _P]),
+}
+
+_lib = None
+
+
+class BLError(RuntimeError):
+    pass
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise BLError(f"blgpu library not built: {LIB_PATH} (run make -C boundary-layer-gpu_amd)")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _check(rc, what):
+    if rc != 0:
+        msg = lib().bl_last_error().decode(errors="replace")
+        raise BLError(f"{what} failed ({rc}): {msg}")
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _ptr(a):
+    return a.ctypes.data_as(_DP)
+
+
+# ----------------------------------------------------------------- set-up ----
+def blasius_profile(n=1000, eta_max=20.0, fpp0=0.46960, sub=64):
+    eta, f, df = np.zeros(n), np.zeros(n), np.zeros(n)
+    _check(lib().bl_blasius_profile(n, eta_max, fpp0, sub, eta.ctypes.data, f.ctypes.data, df.ctypes.data),
+           "bl_blasius_profile")
+    return eta, f, df
+
+
+def blasius_tables(x, y, nu, eta, f, df):
+    x, y, eta, f, df = map(_f64, (x, y, eta, f, df))
+    nx, ny = x.size, y.size
+    t = dict(U_inlet=np.zeros(ny + 1), V_inlet=np.zeros(ny), W_inlet=np.zeros(ny + 1), U_top=np.zeros(nx),
+             V_top=np.zeros(nx + 1), W_top=np.zeros(nx + 1))
+    _check(lib().bl_blasius_tables(nx, ny, x.ctypes.data, y.ctypes.data, nu, eta.size, eta.ctypes.data,
+                                   f.ctypes.data, df.ctypes.data,
+                                   *[t[k].ctypes.data for k in ("U_inlet", "V_inlet", "W_inlet", "U_top",
+                                                                "V_top", "W_top")]), "bl_blasius_tables")
+    return t
+
+
+def blasius_field(x, y, z, nu, eta, f, df):
+    x, y, eta, f, df = map(_f64, (x, y, eta, f, df))
+    nx, ny, nz = x.size, y.size, len(z)
+    U = np.zeros((nx, ny + 1, nz + 1), order="F")
+    V = np.zeros((nx + 1, ny, nz + 1), order="F")
+    W = np.zeros((nx + 1, ny + 1, nz), order="F")
+    _check(lib().bl_blasius_field(nx, ny, nz, x.ctypes.data, y.ctypes.data, nu, eta.size, eta.ctypes.data,
+                                  f.ctypes.data, df.ctypes.data, U.ctypes.data, V.ctypes.data, W.ctypes.data),
+           "bl_blasius_field")
+    return U, V, W
+
+
+def add_noise(U, V, W, amp, seed):
+    for a in (U, V, W):
+        assert a.flags.f_contiguous and a.dtype == np.float64
+    nx, nyg, nzg = U.shape
+    _check(lib().bl_add_noise(nx, nyg - 1, nzg - 1, amp, seed, U.ctypes.data, V.ctypes.data, W.ctypes.data),
+           "bl_add_noise")
+
+
+def grid_tanh(nx, ny, nz, Lx, Ly, Lz, alpha=2.0, x0=1.0):
+    x, y, z = np.zeros(nx), np.zeros(ny), np.zeros(nz)
+    _check(lib().bl_grid_tanh(nx, ny, nz, Lx, Ly, Lz, alpha, x0, x.ctypes.data, y.ctypes.data, z.ctypes.data),
+           "bl_grid_tanh")
+    return x, y, z
+
+
+# ----------------------------------------------------------------- solver ----
+class Solver:
+    """One device-resident solver (bl_create ... bl_destroy)."""
+
+    def __init__(self, x, y, z, nu, tables, dPdx=0.0, dPdz=0.0, scheme=3, les=0, cfl=-1e-4, modes=None,
+                 precision=PREC_FP64):
+        self._keep = []
+        keep = lambda a: (self._keep.append(_f64(a)), self._keep[-1])[1]
+        x, y, z = keep(x), keep(y), keep(z)
+        self.nx, self.ny, self.nz = x.size, y.size, z.size
+        c = BLConfig()
+        c.nx, c.ny, c.nz = self.nx, self.ny, self.nz
+        c.x, c.y, c.z = _ptr(x), _ptr(y), _ptr(z)
+        c.nu, c.dPdx, c.dPdz, c.cfl = nu, dPdx, dPdz, cfl
+        c.scheme, c.les, c.precision = scheme, les, precision
+        for k in ("U_inlet", "V_inlet", "W_inlet", "U_top", "V_top", "W_top"):
+            setattr(c, k, _ptr(keep(tables[k])))
+        if modes is not None:
+            c.n_modes = len(modes["zmode"])
+            c.m_modes = len(modes["tmode"])
+            c.zmode = _ptr(keep(modes["zmode"]))
+            c.tmode = _ptr(keep(modes["tmode"]))
+            for name, key in (("qu", "qu"), ("qv", "qv"), ("qw", "qw")):
+                q = np.asfortranarray(modes[key])
+                re = self._keep.append(np.asfortranarray(q.real, dtype=np.float64)) or self._keep[-1]
+                im = self._keep.append(np.asfortranarray(q.imag, dtype=np.float64)) or self._keep[-1]
+                setattr(c, name + "_re", _ptr(re))
+                setattr(c, name + "_im", _ptr(im))
+        self.h = _P()
+        _check(lib().bl_create(ctypes.byref(c), ctypes.byref(self.h)), "bl_create")
+        self.shapes = [(self.nx, self.ny + 1, self.nz + 1), (self.nx + 1, self.ny, self.nz + 1),
+                       (self.nx + 1, self.ny + 1, self.nz), (self.nx + 1, self.ny + 1, self.nz + 1),
+                       (self.nx + 1, self.ny + 1, self.nz + 1)]
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().bl_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def upload(self, field, a):
+        a = np.asfortranarray(a, dtype=np.float64)
+        assert a.shape == self.shapes[field], (a.shape, self.shapes[field])
+        _check(lib().bl_upload(self.h, field, a.ctypes.data), "bl_upload")
+
+    def download(self, field):
+        a = np.zeros(self.shapes[field], order="F")
+        _check(lib().bl_download(self.h, field, a.ctypes.data), "bl_download")
+        return a
+
+    def set_fields(self, U, V, W):
+        self.upload(FIELD_U, U)
+        self.upload(FIELD_V, V)
+        self.upload(FIELD_W, W)
+
+    def fields(self):
+        return self.download(FIELD_U), self.download(FIELD_V), self.download(FIELD_W)
+
+    @property
+    def t(self):
+        v = _D()
+        _check(lib().bl_get_time(self.h, ctypes.byref(v)), "bl_get_time")
+        return v.value
+
+    @t.setter
+    def t(self, v):
+        _check(lib().bl_set_time(self.h, v), "bl_set_time")
+
+    def compute_dt(self):
+        v = _D()
+        _check(lib().bl_compute_dt(self.h, ctypes.byref(v)), "bl_compute_dt")
+        return v.value
+
+    def step(self):
+        _check(lib().bl_step(self.h), "bl_step")
+
+    def run(self, n):
+        _check(lib().bl_run(self.h, n), "bl_run")
+
+    def synchronize(self):
+        _check(lib().bl_synchronize(self.h), "bl_synchronize")
+
+    def eddy_viscosity(self):
+        _check(lib().bl_eddy_viscosity(self.h), "bl_eddy_viscosity")
+
+    def max_divergence(self):
+        v = _D()
+        _check(lib().bl_max_divergence(self.h, ctypes.byref(v)), "bl_max_divergence")
+        return v.value
+
+    def wall_stats(self):
+        cf = np.zeros(self.nx)
+        _check(lib().bl_wall_stats(self.h, cf.ctypes.data), "bl_wall_stats")
+        return cf
+
+    def monitor(self):
+        out = np.zeros(8)
+        _check(lib().bl_monitor(self.h, out.ctypes.data), "bl_monitor")
+        return out
+
+    def rhs(self):
+        r = [np.zeros(self.shapes[c], order="F") for c in range(3)]
+        _check(lib().bl_rhs(self.h, *[a.ctypes.data for a in r]), "bl_rhs")
+        return r
+
+    def set_old(self, Uo, Vo, Wo):
+        a = [np.asfortranarray(v, dtype=np.float64) for v in (Uo, Vo, Wo)]
+        _check(lib().bl_set_old(self.h, *[v.ctypes.data for v in a]), "bl_set_old")
+
+    def apply_bc(self, rk_step, dt):
+        _check(lib().bl_apply_bc(self.h, rk_step, dt), "bl_apply_bc")
+
+    def projection(self, rk_step, dt):
+        _check(lib().bl_projection(self.h, rk_step, dt), "bl_projection")
+
+    def profile(self, on=True):
+        _check(lib().bl_profile_enable(self.h, 1 if on else 0), "bl_profile_enable")
+
+    def profile_read(self):
+        n = lib().bl_profile_count()
+        ms = np.zeros(n)
+        cnt = np.zeros(n, dtype=np.int64)
+        _check(lib().bl_profile_read(self.h, ms.ctypes.data, cnt.ctypes.data, n), "bl_profile_read")
+        return {lib().bl_profile_name(i).decode(): (float(ms[i]), int(cnt[i])) for i in range(n)}
+
+    def poisson(self, rhs):
+        r = np.asfortranarray(rhs, dtype=np.float64).copy(order="F")
+        assert r.shape == self.shapes[FIELD_P]
+        _check(lib().bl_poisson(self.h, r.ctypes.data), "bl_poisson")
+        return r

@@ -1,0 +1,794 @@
+// C ABI of blgpu: handle life cycle, transfers and step orchestration.
+//
+// The per-step sequence mirrors compute_time_step_RK3 / RK2
+// (time_integration.f90:145-465): for each stage the fused RHS+update kernel
+// writes the next state buffer, then boundary conditions, projection and
+// boundary conditions again.  The T1 copy U->Uo of the reference is replaced
+// by rotating three state buffers: the step-start buffer is read as Uo for the
+// whole step and never written.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/blgpu.h"
+#include "bl_internal.hpp"
+
+namespace blgpu {
+
+std::string &last_error() {
+    static thread_local std::string msg;
+    return msg;
+}
+
+namespace {
+
+constexpr long double kPi = 3.141592653589793238462643383279502884L;
+constexpr double kPiD = 3.141592653589793238462643383279502884;  // = 4*atan(1) in double
+
+int round_up(int v, int m) { return (v + m - 1) / m * m; }
+
+// initialization.f90:181-207
+void centers(int n, const double *x, std::vector<double> &xg) {
+    std::vector<double> xm(n - 1);
+    for (int i = 0; i < n - 1; ++i) xm[i] = 0.5 * (x[i] + x[i + 1]);
+    xg.assign(n + 1, 0.0);
+    for (int i = 0; i < n - 1; ++i) xg[i + 1] = xm[i];
+    xg[0] = xm[0] - 2.0 * (xm[0] - x[0]);
+    xg[n] = xm[n - 2] + 2.0 * (x[n - 1] - xm[n - 2]);
+}
+
+struct HostSetup {
+    std::vector<double> x, y, z, xg, yg, zg, rdx, rdxg, rdy, rdyg, rdz, rdzg, wy0, wy1;
+    double dxmin, dymin, dzmin;
+};
+
+double *dmalloc(size_t n) {
+    void *p = nullptr;
+    hipError_t e = hipMalloc(&p, std::max<size_t>(n, 1) * sizeof(double));
+    if (e != hipSuccess) {
+        last_error() = "hipMalloc of " + std::to_string(n * sizeof(double)) + " bytes: " + hipGetErrorString(e);
+        throw std::bad_alloc();
+    }
+    return static_cast<double *>(p);
+}
+
+void upload_vec(double *dst, const std::vector<double> &v, hipStream_t s) {
+    BL_HIP(hipMemcpyAsync(dst, v.data(), v.size() * sizeof(double), hipMemcpyHostToDevice, s));
+}
+
+void field_extent(const Dims &d, int field, int &n1, int &n2, int &n3) {
+    switch (field) {
+    case BL_FIELD_U: n1 = d.nx; n2 = d.nyg; n3 = d.nzg; break;
+    case BL_FIELD_V: n1 = d.nxg; n2 = d.ny; n3 = d.nzg; break;
+    case BL_FIELD_W: n1 = d.nxg; n2 = d.nyg; n3 = d.nz; break;
+    default: n1 = d.nxg; n2 = d.nyg; n3 = d.nzg; break;
+    }
+}
+
+double *field_dev(Handle &h, int field) {
+    Field3 &f = h.buf[h.icur];
+    switch (field) {
+    case BL_FIELD_U: return f.u;
+    case BL_FIELD_V: return f.v;
+    case BL_FIELD_W: return f.w;
+    case BL_FIELD_P: return h.P;
+    case BL_FIELD_NUT: return h.nu_t;
+    default: last_error() = "unknown field id"; throw ArgError();
+    }
+}
+
+void copy_to_device(Handle &h, double *dev, const double *host, int n1, int n2, int n3) {
+    const Dims &d = h.d;
+    for (int k = 0; k < n3; ++k)
+        BL_HIP(hipMemcpy2DAsync(dev + (size_t)k * d.sxy, (size_t)d.sx * sizeof(double),
+                                host + (size_t)k * n1 * n2, (size_t)n1 * sizeof(double),
+                                (size_t)n1 * sizeof(double), n2, hipMemcpyHostToDevice, h.stream));
+    BL_HIP(hipStreamSynchronize(h.stream));
+}
+
+void copy_to_host(Handle &h, double *host, const double *dev, int n1, int n2, int n3) {
+    const Dims &d = h.d;
+    BL_HIP(hipStreamSynchronize(h.stream));
+    for (int k = 0; k < n3; ++k)
+        BL_HIP(hipMemcpy2DAsync(host + (size_t)k * n1 * n2, (size_t)n1 * sizeof(double),
+                                dev + (size_t)k * d.sxy, (size_t)d.sx * sizeof(double),
+                                (size_t)n1 * sizeof(double), n2, hipMemcpyDeviceToHost, h.stream));
+    BL_HIP(hipStreamSynchronize(h.stream));
+}
+
+void check_config(const bl_config *c) {
+    auto bad = [](const char *m) {
+        last_error() = m;
+        throw ArgError();
+    };
+    if (!c) bad("bl_create: null config");
+    if (c->nx < 5 || c->ny < 4 || c->nz < 5) bad("bl_create: need nx>=5, ny>=4, nz>=5");
+    if (!c->x || !c->y || !c->z) bad("bl_create: missing grid");
+    if (!c->U_inlet || !c->V_inlet || !c->W_inlet || !c->U_top || !c->V_top || !c->W_top)
+        bad("bl_create: missing boundary tables");
+    if (c->scheme != 2 && c->scheme != 3) bad("bl_create: scheme must be 2 (RK2) or 3 (RK3)");
+    if (c->les != 0 && c->les != 2) bad("bl_create: les must be 0 (DNS) or 2 (dynamic Smagorinsky)");
+    if (c->precision != BL_PREC_FP64) bad("bl_create: only the FP64 path is built in this library");
+    if (c->n_modes < 0 || c->m_modes < 0) bad("bl_create: negative mode count");
+    if (c->n_modes > 0 && (c->m_modes <= 0 || !c->zmode || !c->tmode || !c->qu_re || !c->qu_im ||
+                           !c->qv_re || !c->qv_im || !c->qw_re || !c->qw_im))
+        bad("bl_create: incomplete temporal-mode tables");
+    for (int i = 1; i < c->nx; ++i)
+        if (!(c->x[i] > c->x[i - 1])) bad("bl_create: x must increase");
+    for (int i = 1; i < c->ny; ++i)
+        if (!(c->y[i] > c->y[i - 1])) bad("bl_create: y must increase");
+    for (int i = 1; i < c->nz; ++i)
+        if (!(c->z[i] > c->z[i - 1])) bad("bl_create: z must increase");
+}
+
+// initialization.f90:394-425 (Dyy rows for j = 2..nyg-1, 0-based row r = j-2)
+void build_dyy(const HostSetup &g, int nyg, std::vector<double> &dd, std::vector<double> &dl,
+               std::vector<double> &du) {
+    const int nm = nyg - 2;
+    dd.assign(nm, 0.0);
+    dl.assign(nm, 0.0);
+    du.assign(nm, 0.0);
+    auto Y = [&](int j) { return g.y[j - 1]; };
+    auto YG = [&](int j) { return g.yg[j - 1]; };
+    for (int j = 2; j <= nyg - 1; ++j) {
+        const double a = 1.0 / (Y(j) - Y(j - 1)) / (YG(j + 1) - YG(j));
+        const double b = 1.0 / (Y(j) - Y(j - 1)) * (-1.0 / (YG(j + 1) - YG(j)) - 1.0 / (YG(j) - YG(j - 1)));
+        const double c = 1.0 / (Y(j) - Y(j - 1)) / (YG(j) - YG(j - 1));
+        const int r = j - 2;
+        if (j == 2) {
+            dd[r] = b + c;
+            du[r] = a;
+        } else if (j == nyg - 1) {
+            dd[r] = a + b;
+            dl[r - 1] = c;
+        } else {
+            dd[r] = b;
+            du[r] = a;
+            dl[r - 1] = c;
+        }
+    }
+}
+
+void build(Handle &h, const bl_config *c) {
+    Dims &d = h.d;
+    d.nx = c->nx;
+    d.ny = c->ny;
+    d.nz = c->nz;
+    d.nxg = d.nx + 1;
+    d.nyg = d.ny + 1;
+    d.nzg = d.nz + 1;
+    d.sx = round_up(d.nxg, 16);
+    d.sy = d.nyg;
+    d.sxy = (long long)d.sx * d.sy;
+    d.N = d.nx - 2;
+    d.M = d.nz - 2;
+    d.NM = d.ny - 1;
+    d.Kz = d.M / 2 + 1;
+    h.nu = c->nu;
+    h.dPdx = c->dPdx;
+    h.dPdz = c->dPdz;
+    h.cfl = c->cfl;
+    h.scheme = c->scheme;
+    h.les = c->les;
+    h.precision = c->precision;
+    h.t = 0.0;
+    h.dt = c->cfl < 0 ? -c->cfl : 0.0;
+    // RK tables (initialization.f90:503-530)
+    std::memset(h.rk_t, 0, sizeof(h.rk_t));
+    std::memset(h.rk_coef, 0, sizeof(h.rk_coef));
+    std::memset(h.rk2_t, 0, sizeof(h.rk2_t));
+    std::memset(h.rk2_coef, 0, sizeof(h.rk2_coef));
+    h.rk2_t[1] = 1.0 / 2.0;
+    h.rk2_t[2] = 1.0;
+    h.rk2_coef[1][1] = 1.0 / 2.0;
+    h.rk2_coef[2][1] = 0.0;
+    h.rk2_coef[2][2] = 1.0;
+    h.rk_t[1] = 8.0 / 15.0;
+    h.rk_t[2] = 2.0 / 3.0;
+    h.rk_t[3] = 1.0;
+    h.rk_coef[1][1] = 8.0 / 15.0;
+    h.rk_coef[2][1] = 1.0 / 4.0;
+    h.rk_coef[2][2] = 5.0 / 12.0;
+    h.rk_coef[3][1] = 1.0 / 4.0;
+    h.rk_coef[3][2] = 0.0;
+    h.rk_coef[3][3] = 3.0 / 4.0;
+
+    HostSetup g;
+    g.x.assign(c->x, c->x + d.nx);
+    g.y.assign(c->y, c->y + d.ny);
+    g.z.assign(c->z, c->z + d.nz);
+    centers(d.nx, c->x, g.xg);
+    centers(d.ny, c->y, g.yg);
+    centers(d.nz, c->z, g.zg);
+    auto recip_diff = [](const std::vector<double> &a, bool backward) {
+        std::vector<double> r(a.size(), 0.0);
+        for (size_t i = 0; i + 1 < a.size(); ++i) {
+            if (backward)
+                r[i + 1] = 1.0 / (a[i + 1] - a[i]);
+            else
+                r[i] = 1.0 / (a[i + 1] - a[i]);
+        }
+        return r;
+    };
+    g.rdx = recip_diff(g.x, true);
+    g.rdy = recip_diff(g.y, true);
+    g.rdz = recip_diff(g.z, true);
+    g.rdxg = recip_diff(g.xg, false);
+    g.rdyg = recip_diff(g.yg, false);
+    g.rdzg = recip_diff(g.zg, false);
+    g.wy0.resize(d.ny);
+    g.wy1.resize(d.ny);
+    for (int j = 0; j < d.ny; ++j) {
+        g.wy0[j] = (g.yg[j + 1] - g.y[j]) / (g.yg[j + 1] - g.yg[j]);
+        g.wy1[j] = 1.0 - g.wy0[j];
+    }
+    auto mindiff = [](const std::vector<double> &a) {
+        double m = 1e300;
+        for (size_t i = 0; i + 1 < a.size(); ++i) m = std::min(m, a[i + 1] - a[i]);
+        return m;
+    };
+    g.dxmin = mindiff(g.xg);
+    g.dymin = mindiff(g.yg);
+    g.dzmin = mindiff(g.zg);
+    h.length_y = 0.0;
+    for (int j = 1; j <= d.nyg - 2; ++j) h.length_y = h.length_y + (g.y[j] - g.y[j - 1]);
+    h.bc_uc = c->U_top[d.nx - 1];
+    h.dxmin = g.dxmin;
+    h.dymin = g.dymin;
+    h.dzmin = g.dzmin;
+
+    BL_HIP(hipStreamCreate(&h.stream));
+
+    // ---- grid, metric and table storage in one allocation
+    std::vector<std::pair<const double **, std::vector<double>>> tabs;
+    auto add = [&](const double **slot, std::vector<double> v) { tabs.emplace_back(slot, std::move(v)); };
+    add(&h.m.x, g.x);
+    add(&h.m.y, g.y);
+    add(&h.m.z, g.z);
+    add(&h.m.xg, g.xg);
+    add(&h.m.yg, g.yg);
+    add(&h.m.zg, g.zg);
+    add(&h.m.rdx, g.rdx);
+    add(&h.m.rdxg, g.rdxg);
+    add(&h.m.rdy, g.rdy);
+    add(&h.m.rdyg, g.rdyg);
+    add(&h.m.rdz, g.rdz);
+    add(&h.m.rdzg, g.rdzg);
+    add(&h.m.wy0, g.wy0);
+    add(&h.m.wy1, g.wy1);
+    add(&h.bc.U_inlet, std::vector<double>(c->U_inlet, c->U_inlet + d.nyg));
+    add(&h.bc.V_inlet, std::vector<double>(c->V_inlet, c->V_inlet + d.ny));
+    add(&h.bc.W_inlet, std::vector<double>(c->W_inlet, c->W_inlet + d.nyg));
+    add(&h.bc.U_top, std::vector<double>(c->U_top, c->U_top + d.nx));
+    add(&h.bc.V_top, std::vector<double>(c->V_top, c->V_top + d.nxg));
+    add(&h.bc.W_top, std::vector<double>(c->W_top, c->W_top + d.nxg));
+    h.bc.n_modes = c->n_modes;
+    h.bc.m_modes = c->m_modes;
+    if (c->n_modes > 0) {
+        const size_t nm = (size_t)c->n_modes * c->m_modes;
+        add(&h.bc.zmode, std::vector<double>(c->zmode, c->zmode + c->n_modes));
+        add(&h.bc.tmode, std::vector<double>(c->tmode, c->tmode + c->m_modes));
+        add(&h.bc.qu_re, std::vector<double>(c->qu_re, c->qu_re + nm * d.nyg));
+        add(&h.bc.qu_im, std::vector<double>(c->qu_im, c->qu_im + nm * d.nyg));
+        add(&h.bc.qv_re, std::vector<double>(c->qv_re, c->qv_re + nm * d.ny));
+        add(&h.bc.qv_im, std::vector<double>(c->qv_im, c->qv_im + nm * d.ny));
+        add(&h.bc.qw_re, std::vector<double>(c->qw_re, c->qw_re + nm * d.nyg));
+        add(&h.bc.qw_im, std::vector<double>(c->qw_im, c->qw_im + nm * d.nyg));
+    }
+    size_t total = 0;
+    for (auto &t : tabs) total += round_up((int)t.second.size(), 32);
+    double *blk = dmalloc(total);
+    h.mem_grid = blk;
+    size_t off = 0;
+    for (auto &t : tabs) {
+        upload_vec(blk + off, t.second, h.stream);
+        *t.first = blk + off;
+        off += round_up((int)t.second.size(), 32);
+    }
+
+    // ---- state buffers
+    const size_t nf = (size_t)d.sxy * d.nzg;
+    for (int b = 0; b < 3; ++b) {
+        h.buf[b].u = dmalloc(nf);
+        h.buf[b].v = dmalloc(nf);
+        h.buf[b].w = dmalloc(nf);
+        BL_HIP(hipMemsetAsync(h.buf[b].u, 0, nf * sizeof(double), h.stream));
+        BL_HIP(hipMemsetAsync(h.buf[b].v, 0, nf * sizeof(double), h.stream));
+        BL_HIP(hipMemsetAsync(h.buf[b].w, 0, nf * sizeof(double), h.stream));
+    }
+    h.icur = h.iold = 0;
+    h.F1.u = dmalloc(nf);
+    h.F1.v = dmalloc(nf);
+    h.F1.w = dmalloc(nf);
+    h.P = dmalloc(nf);
+    h.phi = dmalloc(nf);
+    h.nu_t = dmalloc(nf);
+    for (double *p : {h.F1.u, h.F1.v, h.F1.w, h.P, h.phi, h.nu_t})
+        BL_HIP(hipMemsetAsync(p, 0, nf * sizeof(double), h.stream));
+    const size_t nspec = (size_t)d.N * d.NM * d.M;
+    h.T1 = dmalloc(nspec);
+    h.S = dmalloc(nspec);
+
+    // ---- transform matrices, stored transposed AT[s][r] with padded rows
+    const int N = d.N, M = d.M;
+    h.lda_x = round_up(N, 64);
+    h.lda_z = round_up(M, 64);
+    std::vector<double> axf((size_t)N * h.lda_x, 0.0), axi((size_t)N * h.lda_x, 0.0);
+    for (int kx = 0; kx < N; ++kx)
+        for (int n = 0; n < N; ++n) {
+            // angle pi*kx*(2n+1)/(2N), reduced exactly modulo 4N
+            const long long a = ((long long)kx * (2 * n + 1)) % (4LL * N);
+            const long double cv = cosl(kPi * (long double)a / (2.0L * N));
+            axf[(size_t)n * h.lda_x + kx] = (double)(2.0L * cv);  // DCT-II x2
+            const long double w = kx == 0 ? 0.5L : 1.0L;
+            axi[(size_t)kx * h.lda_x + n] = (double)(w * cv / ((long double)N * M));
+        }
+    std::vector<double> azf((size_t)M * h.lda_z, 0.0), azi((size_t)M * h.lda_z, 0.0);
+    for (int q = 0; q < M; ++q) {
+        const int kz = (q + 1) / 2;
+        const bool im = q > 0 && (q % 2 == 0);
+        const bool nyq = (M % 2 == 0) && kz == M / 2;
+        for (int m = 0; m < M; ++m) {
+            const long long a = ((long long)kz * m) % M;
+            const long double ang = 2.0L * kPi * (long double)a / (long double)M;
+            const long double fv = q == 0 ? 1.0L : (im ? -sinl(ang) : cosl(ang));
+            const long double iv = q == 0 ? 1.0L : (nyq ? cosl(ang) : 2.0L * fv);
+            azf[(size_t)m * h.lda_z + q] = (double)fv;
+            azi[(size_t)q * h.lda_z + m] = (double)iv;
+        }
+    }
+    h.AxF = dmalloc(axf.size());
+    h.AxI = dmalloc(axi.size());
+    h.AzF = dmalloc(azf.size());
+    h.AzI = dmalloc(azi.size());
+    upload_vec(h.AxF, axf, h.stream);
+    upload_vec(h.AxI, axi, h.stream);
+    upload_vec(h.AzF, azf, h.stream);
+    upload_vec(h.AzI, azi, h.stream);
+
+    // ---- Thomas factors on the half spectrum (initialization.f90:328-344, 427-469)
+    const int NM = d.NM, Kz = d.Kz;
+    std::vector<double> kxx(N), kzz(Kz);
+    for (int i = 0; i < N; ++i) kxx[i] = 2.0 * (std::cos(kPiD * (double)i / (double)N) - 1.0) / std::pow(g.dxmin, 2.0);
+    for (int k = 0; k < Kz; ++k)
+        kzz[k] = 2.0 * (std::cos(2.0 * kPiD * (double)k / (double)M) - 1.0) / std::pow(g.dzmin, 2.0);
+    std::vector<double> dd, dlo, dup;
+    build_dyy(g, d.nyg, dd, dlo, dup);
+    const size_t plane = (size_t)N * NM;
+    std::vector<double> fdl(plane * Kz, 0.0), fdp(plane * Kz, 0.0);
+    std::vector<double> piv(NM), low(NM);
+    for (int kk = 0; kk < Kz; ++kk)
+        for (int kx = 0; kx < N; ++kx) {
+            for (int j = 0; j < NM; ++j) piv[j] = dd[j] + kxx[kx] + kzz[kk];
+            for (int j = 0; j < NM - 1; ++j) low[j] = dlo[j];
+            if (kx == 0 && kk == 0) piv[0] = 3.0 / 2.0 * piv[0];
+            for (int j = 1; j < NM; ++j) {
+                const double m = low[j - 1] / piv[j - 1];
+                low[j - 1] = m;
+                piv[j] = piv[j] - m * dup[j - 1];
+            }
+            for (int j = 0; j < NM; ++j) {
+                fdp[(size_t)kk * plane + (size_t)j * N + kx] = 1.0 / piv[j];
+                if (j < NM - 1) fdl[(size_t)kk * plane + (size_t)j * N + kx] = low[j];
+            }
+        }
+    h.lu_dl = dmalloc(fdl.size());
+    h.lu_dp = dmalloc(fdp.size());
+    h.lu_du = dmalloc(NM);
+    upload_vec(h.lu_dl, fdl, h.stream);
+    upload_vec(h.lu_dp, fdp, h.stream);
+    upload_vec(h.lu_du, dup, h.stream);
+
+    h.scal = dmalloc(64);
+    h.red = dmalloc(std::max<size_t>((size_t)std::max(d.nxg, 8 * d.nzg), 1024));
+    BL_HIP(hipStreamSynchronize(h.stream));
+}
+
+void prof_free(Handle &h);
+
+void free_all(Handle &h) {
+    auto f = [](void *p) {
+        if (p) (void)hipFree(p);
+    };
+    for (int b = 0; b < 3; ++b) {
+        f(h.buf[b].u);
+        f(h.buf[b].v);
+        f(h.buf[b].w);
+    }
+    for (void *p : {(void *)h.F1.u, (void *)h.F1.v, (void *)h.F1.w, (void *)h.P, (void *)h.phi,
+                    (void *)h.nu_t, (void *)h.T1, (void *)h.S, (void *)h.AxF, (void *)h.AxI,
+                    (void *)h.AzF, (void *)h.AzI, (void *)h.lu_dl, (void *)h.lu_dp, (void *)h.lu_du,
+                    (void *)h.scal, (void *)h.red, h.mem_grid})
+        f(p);
+    if (h.stream) (void)hipStreamSynchronize(h.stream);
+    prof_free(h);
+    if (h.stream) (void)hipStreamDestroy(h.stream);
+}
+
+// one projection sub-step on the current buffer (projection.f90:21-87)
+void projection_on(Handle &h, Field3 &f, int rk_step, double dt) {
+    launch_divergence(h, f, h.phi);
+    launch_poisson(h, h.phi);
+    launch_project(h, f, h.phi);
+    if (rk_step == 1) launch_pressure(h, h.phi, h.P, dt);
+}
+
+void do_step(Handle &h) {
+    const double dt = h.dt;
+    const double to = h.t;
+    const int X = h.icur;
+    const int Y = (X + 1) % 3, Z = (X + 2) % 3;
+    Field3 old = h.buf[X];
+    const int nst = h.scheme == 3 ? 3 : 2;
+    const int order3[3] = {Y, Z, Y};
+    int cur = X;
+    for (int s = 1; s <= nst; ++s) {
+        const int nxt = order3[s - 1];
+        launch_rk_stage(h, h.buf[cur], old, h.buf[nxt], s, dt);
+        const double ts = to + (nst == 3 ? h.rk_t[s] : h.rk2_t[s]) * dt;
+        launch_bc(h, h.buf[nxt], old, s, dt, ts);
+        projection_on(h, h.buf[nxt], s, dt);
+        launch_bc(h, h.buf[nxt], old, s, dt, ts);
+        h.t = ts;
+        cur = nxt;
+    }
+    h.icur = h.iold = cur;
+}
+
+}  // namespace
+
+struct ProfState {
+    struct Rec {
+        int id;
+        hipEvent_t a, b;
+    };
+    std::vector<Rec> pending;
+    std::vector<hipEvent_t> pool;
+    double ms[KID_COUNT] = {};
+    long long n[KID_COUNT] = {};
+    hipEvent_t open_ev[KID_COUNT] = {};
+    hipEvent_t take() {
+        if (!pool.empty()) {
+            hipEvent_t e = pool.back();
+            pool.pop_back();
+            return e;
+        }
+        hipEvent_t e;
+        BL_HIP(hipEventCreate(&e));
+        return e;
+    }
+    void drain() {
+        for (auto &r : pending) {
+            BL_HIP(hipEventSynchronize(r.b));
+            float t = 0.f;
+            BL_HIP(hipEventElapsedTime(&t, r.a, r.b));
+            ms[r.id] += t;
+            n[r.id] += 1;
+            pool.push_back(r.a);
+            pool.push_back(r.b);
+        }
+        pending.clear();
+    }
+    ~ProfState() {
+        for (auto &r : pending) {
+            (void)hipEventDestroy(r.a);
+            (void)hipEventDestroy(r.b);
+        }
+        for (auto e : pool) (void)hipEventDestroy(e);
+    }
+};
+
+namespace {
+void prof_free(Handle &h) {
+    delete h.prof;
+    h.prof = nullptr;
+}
+}  // namespace
+
+void prof_begin(Handle &h, int id) {
+    if (!h.prof) return;
+    hipEvent_t e = h.prof->take();
+    BL_HIP(hipEventRecord(e, h.stream));
+    h.prof->open_ev[id] = e;
+}
+
+void prof_end(Handle &h, int id) {
+    if (!h.prof || !h.prof->open_ev[id]) return;
+    hipEvent_t e = h.prof->take();
+    BL_HIP(hipEventRecord(e, h.stream));
+    h.prof->pending.push_back({id, h.prof->open_ev[id], e});
+    h.prof->open_ev[id] = nullptr;
+    if (h.prof->pending.size() > 4096) h.prof->drain();
+}
+
+namespace {
+
+template <class F>
+int guarded(F &&fn) {
+    try {
+        fn();
+        return BL_OK;
+    } catch (const ArgError &) {
+        return BL_EINVAL;
+    } catch (const DeviceError &) {
+        return BL_EDEVICE;
+    } catch (const std::bad_alloc &) {
+        if (last_error().empty()) last_error() = "out of memory";
+        return BL_ENOMEM;
+    } catch (...) {
+        last_error() = "unexpected exception";
+        return BL_EDEVICE;
+    }
+}
+
+Handle &H(bl_handle *h) {
+    if (!h) {
+        last_error() = "null handle";
+        throw ArgError();
+    }
+    return *h;
+}
+
+}  // namespace
+}  // namespace blgpu
+
+using namespace blgpu;
+
+extern "C" {
+
+const char *bl_last_error(void) { return last_error().c_str(); }
+
+int bl_create(const bl_config *cfg, bl_handle **out) {
+    last_error().clear();
+    if (!out) {
+        last_error() = "bl_create: null output pointer";
+        return BL_EINVAL;
+    }
+    *out = nullptr;
+    bl_handle *h = new (std::nothrow) bl_handle();
+    if (!h) return BL_ENOMEM;
+    std::memset(static_cast<Handle *>(h), 0, sizeof(Handle));
+    int rc = guarded([&] {
+        check_config(cfg);
+        int ndev = 0;
+        BL_HIP(hipGetDeviceCount(&ndev));
+        if (ndev < 1) {
+            last_error() = "bl_create: no HIP device";
+            throw DeviceError();
+        }
+        build(*h, cfg);
+    });
+    if (rc != BL_OK) {
+        free_all(*h);
+        delete h;
+        return rc;
+    }
+    *out = h;
+    return BL_OK;
+}
+
+void bl_destroy(bl_handle *h) {
+    if (!h) return;
+    (void)hipStreamSynchronize(h->stream);
+    free_all(*h);
+    delete h;
+}
+
+int bl_upload(bl_handle *hh, int field, const double *host) {
+    return guarded([&] {
+        Handle &h = H(hh);
+        if (!host) {
+            last_error() = "bl_upload: null host pointer";
+            throw ArgError();
+        }
+        int n1, n2, n3;
+        field_extent(h.d, field, n1, n2, n3);
+        copy_to_device(h, field_dev(h, field), host, n1, n2, n3);
+    });
+}
+
+int bl_download(bl_handle *hh, int field, double *host) {
+    return guarded([&] {
+        Handle &h = H(hh);
+        if (!host) {
+            last_error() = "bl_download: null host pointer";
+            throw ArgError();
+        }
+        int n1, n2, n3;
+        field_extent(h.d, field, n1, n2, n3);
+        copy_to_host(h, host, field_dev(h, field), n1, n2, n3);
+    });
+}
+
+int bl_set_time(bl_handle *hh, double t) {
+    return guarded([&] { H(hh).t = t; });
+}
+
+int bl_get_time(bl_handle *hh, double *t) {
+    return guarded([&] {
+        if (!t) throw ArgError();
+        *t = H(hh).t;
+    });
+}
+
+int bl_compute_dt(bl_handle *hh, double *dt) {
+    return guarded([&] {
+        Handle &h = H(hh);
+        // time_integration.f90:536-574
+        const double CFLa = std::fabs(h.cfl);
+        if (h.cfl < 0) {
+            h.dt = -h.cfl;
+        } else {
+            const double dt_conv = dt_convective(h, h.buf[h.icur]);
+            const double dvu = CFLa * std::pow(h.dxmin, 2.0) / h.nu;
+            const double dvv = CFLa * std::pow(h.dymin, 2.0) / h.nu;
+            const double dvw = CFLa * std::pow(h.dzmin, 2.0) / h.nu;
+            const double dt_vis = std::fmin(dvu, std::fmin(dvv, dvw));
+            h.dt = std::fmin(dt_conv, dt_vis);
+        }
+        if (dt) *dt = h.dt;
+    });
+}
+
+int bl_step(bl_handle *hh) {
+    return guarded([&] {
+        Handle &h = H(hh);
+        if (!(h.dt > 0.0)) {
+            last_error() = "bl_step: dt is not positive (call bl_compute_dt)";
+            throw ArgError();
+        }
+        do_step(h);
+    });
+}
+
+int bl_run(bl_handle *hh, int nsteps) {
+    return guarded([&] {
+        Handle &h = H(hh);
+        if (!(h.dt > 0.0)) {
+            last_error() = "bl_run: dt is not positive (call bl_compute_dt)";
+            throw ArgError();
+        }
+        for (int n = 0; n < nsteps; ++n) do_step(h);
+    });
+}
+
+int bl_synchronize(bl_handle *hh) {
+    return guarded([&] { BL_HIP(hipStreamSynchronize(H(hh).stream)); });
+}
+
+int bl_eddy_viscosity(bl_handle *hh) {
+    return guarded([&] {
+        Handle &h = H(hh);
+        if (h.les != 0) {
+            last_error() = "bl_eddy_viscosity: dynamic Smagorinsky is not available in this build";
+            throw ArgError();
+        }
+        BL_HIP(hipMemsetAsync(h.nu_t, 0, (size_t)h.d.sxy * h.d.nzg * sizeof(double), h.stream));
+    });
+}
+
+int bl_max_divergence(bl_handle *hh, double *div) {
+    return guarded([&] {
+        Handle &h = H(hh);
+        if (!div) throw ArgError();
+        *div = max_divergence(h, h.buf[h.icur]);
+    });
+}
+
+int bl_wall_stats(bl_handle *hh, double *cf) {
+    return guarded([&] {
+        Handle &h = H(hh);
+        if (!cf) throw ArgError();
+        wall_stats(h, h.buf[h.icur], cf);
+    });
+}
+
+int bl_monitor(bl_handle *hh, double *out) {
+    return guarded([&] {
+        Handle &h = H(hh);
+        if (!out) throw ArgError();
+        monitor(h, h.buf[h.icur], out);
+    });
+}
+
+int bl_rhs(bl_handle *hh, double *ru, double *rv, double *rw) {
+    return guarded([&] {
+        Handle &h = H(hh);
+        if (!ru || !rv || !rw) throw ArgError();
+        const size_t nf = (size_t)h.d.sxy * h.d.nzg;
+        for (double *p : {h.F1.u, h.F1.v, h.F1.w}) BL_HIP(hipMemsetAsync(p, 0, nf * sizeof(double), h.stream));
+        launch_rhs_only(h, h.buf[h.icur], h.F1);
+        int n1, n2, n3;
+        field_extent(h.d, BL_FIELD_U, n1, n2, n3);
+        copy_to_host(h, ru, h.F1.u, n1, n2, n3);
+        field_extent(h.d, BL_FIELD_V, n1, n2, n3);
+        copy_to_host(h, rv, h.F1.v, n1, n2, n3);
+        field_extent(h.d, BL_FIELD_W, n1, n2, n3);
+        copy_to_host(h, rw, h.F1.w, n1, n2, n3);
+    });
+}
+
+int bl_set_old(bl_handle *hh, const double *Uo, const double *Vo, const double *Wo) {
+    return guarded([&] {
+        Handle &h = H(hh);
+        if (!Uo || !Vo || !Wo) throw ArgError();
+        const int io = (h.icur + 1) % 3;
+        int n1, n2, n3;
+        field_extent(h.d, BL_FIELD_U, n1, n2, n3);
+        copy_to_device(h, h.buf[io].u, Uo, n1, n2, n3);
+        field_extent(h.d, BL_FIELD_V, n1, n2, n3);
+        copy_to_device(h, h.buf[io].v, Vo, n1, n2, n3);
+        field_extent(h.d, BL_FIELD_W, n1, n2, n3);
+        copy_to_device(h, h.buf[io].w, Wo, n1, n2, n3);
+        h.iold = io;
+    });
+}
+
+int bl_apply_bc(bl_handle *hh, int rk_step, double dt) {
+    return guarded([&] {
+        Handle &h = H(hh);
+        if (rk_step < 1 || rk_step > 3) throw ArgError();
+        launch_bc(h, h.buf[h.icur], h.buf[h.iold], rk_step, dt, h.t);
+        BL_HIP(hipStreamSynchronize(h.stream));
+    });
+}
+
+int bl_projection(bl_handle *hh, int rk_step, double dt) {
+    return guarded([&] {
+        Handle &h = H(hh);
+        projection_on(h, h.buf[h.icur], rk_step, dt);
+        BL_HIP(hipStreamSynchronize(h.stream));
+    });
+}
+
+int bl_profile_enable(bl_handle *hh, int on) {
+    return guarded([&] {
+        Handle &h = H(hh);
+        if (on && !h.prof) h.prof = new ProfState();
+        if (!on && h.prof) {
+            BL_HIP(hipStreamSynchronize(h.stream));
+            delete h.prof;
+            h.prof = nullptr;
+        }
+    });
+}
+
+int bl_profile_read(bl_handle *hh, double *ms, long long *launches, int n) {
+    return guarded([&] {
+        Handle &h = H(hh);
+        if (!ms || !launches || n < 1) throw ArgError();
+        for (int i = 0; i < n; ++i) {
+            ms[i] = 0.0;
+            launches[i] = 0;
+        }
+        if (!h.prof) return;
+        h.prof->drain();
+        for (int i = 0; i < n && i < KID_COUNT; ++i) {
+            ms[i] = h.prof->ms[i];
+            launches[i] = h.prof->n[i];
+        }
+    });
+}
+
+const char *bl_profile_name(int id) { return kernel_name(id); }
+
+int bl_profile_count(void) { return KID_COUNT; }
+
+int bl_poisson(bl_handle *hh, double *rhs) {
+    return guarded([&] {
+        Handle &h = H(hh);
+        if (!rhs) throw ArgError();
+        int n1, n2, n3;
+        field_extent(h.d, BL_FIELD_P, n1, n2, n3);
+        copy_to_device(h, h.phi, rhs, n1, n2, n3);
+        launch_poisson(h, h.phi);
+        copy_to_host(h, rhs, h.phi, n1, n2, n3);
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,750 @@
+// HIP kernels for the per-time-step path (gfx950, FP64 mode).
+//
+// Reference correspondence (0-based indices here, the reference is 1-based):
+//   k_rk_stage      equations.f90:24-415 fused with time_integration.f90:351-458
+//   k_bc_*          boundary_conditions.f90:127-262
+//   k_divergence    projection.f90:34-49
+//   k_xform         the DCT-II/DFT stages of solve_poisson_gpu projection.f90:110-202
+//   k_thomas        projection.f90:137-150
+//   k_project       projection.f90:355-386 with apply_periodic_z_pressure :293-303
+//   k_pressure      projection.f90:75-85
+//   reductions      projection.f90:391-413, statistics.f90:137-160, monitor.f90:29-52,
+//                   time_integration.f90:546-558
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <vector>
+
+#include "bl_internal.hpp"
+
+namespace blgpu {
+
+namespace {
+
+constexpr int kWave = 64;
+
+__device__ __forceinline__ double sq(double a) { return a * a; }
+
+// ------------------------------------------------------------------------
+// Fused RHS + Runge-Kutta stage update.
+// MODE 0: new = old + c0*F            (first stage, and RK2 second stage)
+// MODE 1: new = old + dt*(c1*F1 + c0*F) (RK3 stages 2 and 3)
+// MODE 2: write F only (operator test)
+// One thread per (i,j,k) of the union of the U, V, W interiors.
+// ------------------------------------------------------------------------
+template <int MODE, bool WRITE_F1>
+__global__ __launch_bounds__(256) void k_rk_stage(
+    Dims d, Metrics m, double nu, double dPdx, double dPdz, const double *__restrict__ U,
+    const double *__restrict__ V, const double *__restrict__ W, const double *__restrict__ Uo,
+    const double *__restrict__ Vo, const double *__restrict__ Wo, const double *__restrict__ F1u,
+    const double *__restrict__ F1v, const double *__restrict__ F1w, double *__restrict__ Un,
+    double *__restrict__ Vn, double *__restrict__ Wn, double *__restrict__ Fou,
+    double *__restrict__ Fov, double *__restrict__ Fow, double c0, double c1, double dt) {
+    const int i = blockIdx.x * kWave + threadIdx.x + 1;
+    const int j = blockIdx.y * 4 + threadIdx.y + 1;
+    const int k = blockIdx.z + 1;
+    if (i > d.nxg - 2 || j > d.nyg - 2) return;
+    const long long sx = d.sx, sxy = d.sxy;
+    const long long p = fidx(d, i, j, k);
+
+    // ---- u: i in [1,nx-2], j in [1,nyg-2], k in [1,nzg-2]  (equations.f90:47-94, 149-154)
+    if (i <= d.nx - 2) {
+        double r = 0.0;
+        if (i < d.nx - 2) {
+            const double up = U[p], ue = U[p + 1], uw = U[p - 1];
+            const double un = U[p + sx], us = U[p - sx], ut = U[p + sxy], ub = U[p - sxy];
+            r = -(sq(0.5 * (up + ue)) - sq(0.5 * (uw + up))) * m.rdxg[i];
+            r = r - ((m.wy0[j] * up + m.wy1[j] * un) * 0.5 * (V[p] + V[p + 1]) -
+                     (m.wy0[j - 1] * us + m.wy1[j - 1] * up) * 0.5 * (V[p - sx] + V[p + 1 - sx])) *
+                        m.rdy[j];
+            r = r - (0.5 * (up + ut) * 0.5 * (W[p] + W[p + 1]) -
+                     0.5 * (ub + up) * 0.5 * (W[p - sxy] + W[p + 1 - sxy])) *
+                        m.rdz[k];
+            r = r + dPdx +
+                nu * ((m.rdx[i + 1] * (ue - up) - m.rdx[i] * (up - uw)) * m.rdxg[i] +
+                      ((un - up) * m.rdyg[j] - (up - us) * m.rdyg[j - 1]) * m.rdy[j] +
+                      (m.rdzg[k] * (ut - up) - m.rdzg[k - 1] * (up - ub)) * m.rdz[k]);
+        }
+        if (MODE == 2) {
+            Fou[p] = r;
+        } else {
+            if (WRITE_F1) Fou[p] = r;
+            Un[p] = (MODE == 0) ? Uo[p] + c0 * r : Uo[p] + dt * (c1 * F1u[p] + c0 * r);
+        }
+    }
+    // ---- v: i in [1,nxg-2], j in [1,ny-2], k in [1,nzg-2]  (equations.f90:183-230, 287-292)
+    if (j <= d.ny - 2) {
+        double r = 0.0;
+        if (i < d.nxg - 2) {
+            const double vp = V[p], ve = V[p + 1], vw = V[p - 1];
+            const double vn = V[p + sx], vs = V[p - sx], vt = V[p + sxy], vb = V[p - sxy];
+            r = -(sq(0.5 * (vp + vn)) - sq(0.5 * (vs + vp))) * m.rdyg[j];
+            r = r - ((m.wy0[j] * U[p] + m.wy1[j] * U[p + sx]) * 0.5 * (vp + ve) -
+                     (m.wy0[j] * U[p - 1] + m.wy1[j] * U[p - 1 + sx]) * 0.5 * (vw + vp)) *
+                        m.rdx[i];
+            r = r - (0.5 * (vp + vt) * (m.wy0[j] * W[p] + m.wy1[j] * W[p + sx]) -
+                     0.5 * (vb + vp) * (m.wy0[j] * W[p - sxy] + m.wy1[j] * W[p + sx - sxy])) *
+                        m.rdz[k];
+            r = r + nu * ((m.rdxg[i] * (ve - vp) - m.rdxg[i - 1] * (vp - vw)) * m.rdx[i] +
+                          (m.rdy[j + 1] * (vn - vp) - m.rdy[j] * (vp - vs)) * m.rdyg[j] +
+                          (m.rdzg[k] * (vt - vp) - m.rdzg[k - 1] * (vp - vb)) * m.rdz[k]);
+        }
+        if (MODE == 2) {
+            Fov[p] = r;
+        } else {
+            if (WRITE_F1) Fov[p] = r;
+            Vn[p] = (MODE == 0) ? Vo[p] + c0 * r : Vo[p] + dt * (c1 * F1v[p] + c0 * r);
+        }
+    }
+    // ---- w: i in [1,nxg-2], j in [1,nyg-2], k in [1,nz-2]  (equations.f90:323-366, 408-413)
+    if (k <= d.nz - 2) {
+        double r = 0.0;
+        if (i < d.nxg - 2) {
+            const double wp = W[p], we = W[p + 1], ww = W[p - 1];
+            const double wn = W[p + sx], ws = W[p - sx], wt = W[p + sxy], wb = W[p - sxy];
+            r = -(sq(0.5 * (wp + wt)) - sq(0.5 * (wb + wp))) * m.rdzg[k];
+            r = r - (0.5 * (U[p] + U[p + sxy]) * 0.5 * (wp + we) -
+                     0.5 * (U[p - 1] + U[p - 1 + sxy]) * 0.5 * (ww + wp)) *
+                        m.rdx[i];
+            r = r - (0.5 * (V[p] + V[p + sxy]) * (m.wy0[j] * wp + m.wy1[j] * wn) -
+                     0.5 * (V[p - sx] + V[p - sx + sxy]) * (m.wy0[j - 1] * ws + m.wy1[j - 1] * wp)) *
+                        m.rdy[j];
+            r = r + dPdz +
+                nu * ((m.rdxg[i] * (we - wp) - m.rdxg[i - 1] * (wp - ww)) * m.rdx[i] +
+                      ((wn - wp) * m.rdyg[j] - (wp - ws) * m.rdyg[j - 1]) * m.rdy[j] +
+                      (m.rdz[k + 1] * (wt - wp) - m.rdz[k] * (wp - wb)) * m.rdzg[k]);
+        }
+        if (MODE == 2) {
+            Fow[p] = r;
+        } else {
+            if (WRITE_F1) Fow[p] = r;
+            Wn[p] = (MODE == 0) ? Wo[p] + c0 * r : Wo[p] + dt * (c1 * F1w[p] + c0 * r);
+        }
+    }
+}
+
+// ------------------------------------------------------------------------
+// Boundary conditions, in the reference's order (boundary_conditions.f90:
+// 135-260).  Each kernel covers one phase over the planes it writes.
+// ------------------------------------------------------------------------
+
+// inflow (:137-190) and outflow (:193-202); one thread per (j,k)
+__global__ __launch_bounds__(256) void k_bc_inout(Dims d, Metrics m, BCTables b, double *U,
+                                                  double *V, double *W, const double *Uo,
+                                                  const double *Vo, const double *Wo,
+                                                  double coef, double t) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    const int k = blockIdx.y;
+    if (j >= d.nyg) return;
+    const int nx = d.nx, nxg = d.nxg;
+    // temporal modes Re(q e^{i(beta z - omega t)})
+    double ut = 0.0, vt = 0.0, wt = 0.0;
+    if (b.n_modes > 0) {
+        const double zc = m.zg[k];
+        const double zf = (k < d.nz) ? m.z[k] : 0.0;
+        for (int n = 0; n < b.n_modes; ++n)
+            for (int mm = 0; mm < b.m_modes; ++mm) {
+                const double pc = b.zmode[n] * zc - b.tmode[mm] * t;
+                const double pf = b.zmode[n] * zf - b.tmode[mm] * t;
+                const long long qi = (long long)n + (long long)b.n_modes * mm;
+                const long long qu = j + (long long)d.nyg * qi;
+                const double cc = cos(pc), sc = sin(pc);
+                ut = ut + (b.qu_re[qu] * cc - b.qu_im[qu] * sc);
+                if (j < d.ny) {
+                    const long long qv = j + (long long)d.ny * qi;
+                    vt = vt + (b.qv_re[qv] * cc - b.qv_im[qv] * sc);
+                }
+                const double cf = cos(pf), sf = sin(pf);
+                wt = wt + (b.qw_re[qu] * cf - b.qw_im[qu] * sf);
+            }
+    }
+    const long long p0 = fidx(d, 0, j, k);
+    // U (all j < nyg, k < nzg)
+    U[p0] = b.U_inlet[j] + ut;
+    {
+        const long long pa = fidx(d, nx - 2, j, k);
+        const double o1 = Uo[pa], o0 = Uo[pa - 1];
+        const double v = o1 - coef * (o1 - o0) / (m.x[nx - 2] - m.x[nx - 3]);
+        U[pa] = v;
+        U[pa + 1] = v;
+    }
+    if (k < d.nz) {
+        W[p0] = b.W_inlet[j] + wt;
+        const long long pa = fidx(d, nxg - 2, j, k);
+        const double o1 = Wo[pa], o0 = Wo[pa - 1];
+        const double v = o1 - coef * (o1 - o0) / (m.xg[nxg - 2] - m.xg[nxg - 3]);
+        W[pa] = v;
+        W[pa + 1] = v;
+    }
+    if (j < d.ny) {
+        V[p0] = b.V_inlet[j] + vt;
+        const long long pa = fidx(d, nxg - 2, j, k);
+        const double o1 = Vo[pa], o0 = Vo[pa - 1];
+        const double v = o1 - coef * (o1 - o0) / (m.xg[nxg - 2] - m.xg[nxg - 3]);
+        V[pa] = v;
+        V[pa + 1] = v;
+    }
+}
+
+// periodic z (:205-217); one thread per (i,j)
+__global__ __launch_bounds__(256) void k_bc_periodic(Dims d, double *U, double *V, double *W) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y;
+    if (i >= d.nxg) return;
+    const long long base = fidx(d, i, j, 0), sxy = d.sxy;
+    const int nzg = d.nzg, nz = d.nz;
+    if (i < d.nx) {
+        U[base] = U[base + sxy * (nzg - 3)];
+        U[base + sxy * (nzg - 2)] = U[base + sxy * 1];
+        U[base + sxy * (nzg - 1)] = U[base + sxy * 2];
+    }
+    if (j < d.ny) {
+        V[base] = V[base + sxy * (nzg - 3)];
+        V[base + sxy * (nzg - 2)] = V[base + sxy * 1];
+        V[base + sxy * (nzg - 1)] = V[base + sxy * 2];
+    }
+    W[base] = W[base + sxy * (nz - 2)];
+    W[base + sxy * (nz - 1)] = W[base + sxy * 1];
+}
+
+// top Dirichlet (:220-228) then wall (:231-238); one thread per (i,k)
+__global__ __launch_bounds__(256) void k_bc_topwall(Dims d, BCTables b, double *U, double *V,
+                                                    double *W) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int k = blockIdx.y;
+    if (i >= d.nxg) return;
+    const long long sx = d.sx;
+    const long long p0 = fidx(d, i, 0, k);
+    if (i < d.nx) {
+        U[p0 + sx * (d.nyg - 1)] = b.U_top[i];
+        U[p0] = -U[p0 + sx];
+    }
+    V[p0 + sx * (d.ny - 1)] = b.V_top[i];
+    V[p0] = 0.0;
+    if (k < d.nz) {
+        W[p0 + sx * (d.nyg - 1)] = b.W_top[i];
+        W[p0] = -W[p0 + sx];
+    }
+}
+
+// global mass conservation (:241-260): single work-group reduction then
+// correction of the outflow planes.
+__global__ __launch_bounds__(1024) void k_bc_massflux(Dims d, Metrics m, double length_y, double *U,
+                                                      const double *V) {
+    __shared__ double red[1024];
+    __shared__ double dU;
+    const int tid = threadIdx.x;
+    double acc = 0.0;
+    const int nj = d.nyg - 2, nk = d.nzg - 3;
+    for (int t = tid; t < nj * nk; t += 1024) {
+        const int j = 1 + t % nj, k = 1 + t / nj;
+        acc += (U[fidx(d, 0, j, k)] - U[fidx(d, d.nx - 2, j, k)]) * (m.y[j] - m.y[j - 1]);
+    }
+    const int ni = d.nxg - 3;
+    for (int t = tid; t < ni * nk; t += 1024) {
+        const int i = 1 + t % ni, k = 1 + t / ni;
+        acc += (V[fidx(d, i, 0, k)] - V[fidx(d, i, d.ny - 1, k)]) * (m.x[i] - m.x[i - 1]);
+    }
+    red[tid] = acc;
+    __syncthreads();
+    for (int s = 512; s > 0; s >>= 1) {
+        if (tid < s) red[tid] += red[tid + s];
+        __syncthreads();
+    }
+    if (tid == 0) dU = red[0] / length_y / (double)(d.nzg - 3);
+    __syncthreads();
+    const double du = dU;
+    for (int t = tid; t < d.nyg * d.nzg; t += 1024) {
+        const int j = t % d.nyg, k = t / d.nyg;
+        const long long pa = fidx(d, d.nx - 2, j, k);
+        const double v = U[pa] + du;
+        U[pa] = v;
+        U[pa + 1] = v;
+    }
+}
+
+// ------------------------------------------------------------------------
+// Projection pieces
+// ------------------------------------------------------------------------
+// pseudo-pressure rhs on the cells the Poisson solve reads (projection.f90:38-47)
+__global__ __launch_bounds__(256) void k_divergence(Dims d, Metrics m, const double *U,
+                                                    const double *V, const double *W,
+                                                    double *out) {
+    const int i = blockIdx.x * kWave + threadIdx.x + 1;
+    const int j = blockIdx.y * 4 + threadIdx.y + 1;
+    const int k = blockIdx.z + 1;
+    if (i > d.nxg - 3 || j > d.nyg - 2) return;
+    const long long p = fidx(d, i, j, k);
+    out[p] = (U[p] - U[p - 1]) * m.rdx[i] + (V[p] - V[p - d.sx]) * m.rdy[j] +
+             (W[p] - W[p - d.sxy]) * m.rdz[k];
+}
+
+// phi with the x-Neumann and z-periodic copies of apply_periodic_z_pressure
+__device__ __forceinline__ long long phi_at(const Dims &d, int i, int j, int k) {
+    if (i > d.nxg - 3) i = d.nxg - 3;
+    if (k == d.nzg - 2) k = 1;
+    return fidx(d, i, j, k);
+}
+
+__global__ __launch_bounds__(256) void k_project(Dims d, Metrics m, double *U, double *V,
+                                                 double *W, const double *phi) {
+    const int i = blockIdx.x * kWave + threadIdx.x + 1;
+    const int j = blockIdx.y * 4 + threadIdx.y + 1;
+    const int k = blockIdx.z + 1;
+    if (i > d.nxg - 2 || j > d.nyg - 2) return;
+    const long long p = fidx(d, i, j, k);
+    const double pc = phi[phi_at(d, i, j, k)];
+    if (i <= d.nx - 2) U[p] = U[p] - (phi[phi_at(d, i + 1, j, k)] - pc) * m.rdxg[i];
+    if (j <= d.ny - 2) V[p] = V[p] - (phi[phi_at(d, i, j + 1, k)] - pc) * m.rdyg[j];
+    if (k <= d.nz - 2) W[p] = W[p] - (phi[phi_at(d, i, j, k + 1)] - pc) * m.rdzg[k];
+}
+
+__global__ __launch_bounds__(256) void k_pressure(Dims d, const double *phi, double *P, double den) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y;
+    const int k = blockIdx.z;
+    if (i >= d.nxg) return;
+    const int ci = i < 1 ? 1 : (i > d.nxg - 2 ? d.nxg - 2 : i);
+    const int cj = j < 1 ? 1 : (j > d.nyg - 2 ? d.nyg - 2 : j);
+    const int ck = k < 1 ? 1 : (k > d.nzg - 2 ? d.nzg - 2 : k);
+    P[fidx(d, i, j, k)] = phi[phi_at(d, ci, cj, ck)] / den;
+}
+
+// ------------------------------------------------------------------------
+// Dense transform along one index of many independent columns:
+//   out[r, c] = sum_s A[r, s] in[s, c]
+// The transpose A^T[s][r] (leading dimension lda, zero padded) is read with
+// wave-uniform addresses, i.e. through the scalar cache into SGPR operands of
+// v_fma_f64; each lane owns one column and RW rows of accumulators.
+// Columns are c = c1 + C1*c2 with independent strides.
+// ------------------------------------------------------------------------
+struct XformArgs {
+    int nrows, nsum, lda;
+    int C1, C;
+    long long in_base, in_s, in_c1, in_c2;
+    long long out_base, out_r, out_c1, out_c2;
+};
+
+template <int RW>
+__global__ __launch_bounds__(256) void k_xform(XformArgs a, const double *__restrict__ AT,
+                                               const double *__restrict__ in,
+                                               double *__restrict__ out) {
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wave = threadIdx.x >> 6;
+    const int r0 = __builtin_amdgcn_readfirstlane((blockIdx.y * 4 + wave) * RW);
+    if (r0 >= a.nrows) return;
+    const int c = blockIdx.x * kWave + lane;
+    const bool valid = c < a.C;
+    const int cc = valid ? c : a.C - 1;
+    const int c1 = cc % a.C1, c2 = cc / a.C1;
+    const double *src = in + a.in_base + c1 * a.in_c1 + c2 * a.in_c2;
+    double acc[RW];
+#pragma unroll
+    for (int r = 0; r < RW; ++r) acc[r] = 0.0;
+    for (int s = 0; s < a.nsum; ++s) {
+        const double xv = src[(long long)s * a.in_s];
+        const double *ar = AT + (long long)s * a.lda + r0;
+#pragma unroll
+        for (int r = 0; r < RW; ++r) acc[r] = fma(ar[r], xv, acc[r]);
+    }
+    if (!valid) return;
+    double *dst = out + a.out_base + c1 * a.out_c1 + c2 * a.out_c2;
+#pragma unroll
+    for (int r = 0; r < RW; ++r)
+        if (r0 + r < a.nrows) dst[(long long)(r0 + r) * a.out_r] = acc[r];
+}
+
+// Thomas sweep with precomputed factors, one (kx, kz) per lane; the Re and
+// Im rows of the half spectrum share the factors (projection.f90:137-150).
+__global__ __launch_bounds__(256) void k_thomas(Dims d, const double *__restrict__ dl,
+                                                const double *__restrict__ dp,
+                                                const double *__restrict__ du, double *S) {
+    const int kx = blockIdx.x * blockDim.x + threadIdx.x;
+    const int kk = blockIdx.y;
+    const int N = d.N, NM = d.NM, M = d.M;
+    if (kx >= N) return;
+    const long long plane = (long long)N * NM;
+    const int qre = kk == 0 ? 0 : 2 * kk - 1;
+    const int qim = (kk > 0 && 2 * kk <= M - 1) ? 2 * kk : -1;
+    double *a = S + qre * plane + kx;
+    double *b = qim >= 0 ? S + qim * plane + kx : nullptr;
+    const double *L = dl + kk * plane + kx;
+    const double *D = dp + kk * plane + kx;
+    double ra = a[0], rb = b ? b[0] : 0.0;
+    for (int j = 1; j < NM; ++j) {
+        const double l = L[(long long)(j - 1) * N];
+        ra = a[(long long)j * N] - l * ra;
+        a[(long long)j * N] = ra;
+        if (b) {
+            rb = b[(long long)j * N] - l * rb;
+            b[(long long)j * N] = rb;
+        }
+    }
+    const double dlast = D[(long long)(NM - 1) * N];
+    ra = ra * dlast;
+    a[(long long)(NM - 1) * N] = ra;
+    if (b) {
+        rb = rb * dlast;
+        b[(long long)(NM - 1) * N] = rb;
+    }
+    for (int j = NM - 2; j >= 0; --j) {
+        const double dj = D[(long long)j * N], uj = du[j];
+        ra = (a[(long long)j * N] - uj * ra) * dj;
+        a[(long long)j * N] = ra;
+        if (b) {
+            rb = (b[(long long)j * N] - uj * rb) * dj;
+            b[(long long)j * N] = rb;
+        }
+    }
+}
+
+// ------------------------------------------------------------------------
+// Reductions
+// ------------------------------------------------------------------------
+__device__ __forceinline__ void atomic_max_pos(double *addr, double v) {
+    // non-negative doubles order like their bit patterns
+    atomicMax(reinterpret_cast<unsigned long long *>(addr), (unsigned long long)__double_as_longlong(v));
+}
+
+__global__ __launch_bounds__(256) void k_maxdiv(Dims d, Metrics m, const double *U, const double *V,
+                                                const double *W, double *out) {
+    const int i = blockIdx.x * kWave + threadIdx.x + 1;
+    const int j = blockIdx.y * 4 + threadIdx.y + 1;
+    const int k = blockIdx.z + 1;
+    double v = 0.0;
+    if (i <= d.nxg - 3 && j <= d.nyg - 2) {
+        const long long p = fidx(d, i, j, k);
+        v = fabs((U[p] - U[p - 1]) * m.rdx[i] + (V[p] - V[p - d.sx]) * m.rdy[j] +
+                 (W[p] - W[p - d.sxy]) * m.rdz[k]);
+    }
+    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_down(v, o));
+    if ((threadIdx.x & 63) == 0) atomic_max_pos(out, v);
+}
+
+__global__ __launch_bounds__(256) void k_dtconv(Dims d, Metrics m, const double *U, const double *V,
+                                                const double *W, double *out) {
+    const int i = blockIdx.x * kWave + threadIdx.x + 1;
+    const int j = blockIdx.y * 4 + threadIdx.y + 1;
+    const int k = blockIdx.z + 1;
+    double a = 0.0, b = 0.0, c = 0.0;
+    if (i <= d.nxg - 2 && j <= d.nyg - 2) {
+        const long long p = fidx(d, i, j, k);
+        a = (m.xg[i + 1] - m.xg[i]) / fabs(U[p]);
+        b = (m.yg[j + 1] - m.yg[j]) / fabs(V[p]);
+        c = (m.zg[k + 1] - m.zg[k]) / fabs(W[p]);
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        a = fmax(a, __shfl_down(a, o));
+        b = fmax(b, __shfl_down(b, o));
+        c = fmax(c, __shfl_down(c, o));
+    }
+    if ((threadIdx.x & 63) == 0) {
+        atomic_max_pos(out + 0, a);
+        atomic_max_pos(out + 1, b);
+        atomic_max_pos(out + 2, c);
+    }
+}
+
+// Cf(x) (statistics.f90:137-160): one thread per x face
+__global__ __launch_bounds__(256) void k_wallstats(Dims d, Metrics m, double nu, const double *U,
+                                                   const double *V, const double *nu_t,
+                                                   double *cf) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= d.nx) return;
+    const double nz2 = (double)(d.nzg - 2);
+    double u1 = 0.0, u2 = 0.0, uv = 0.0, n1a = 0.0, n2a = 0.0, n1b = 0.0, n2b = 0.0;
+    for (int k = 1; k <= d.nzg - 2; ++k) {
+        const long long p = fidx(d, i, 0, k);
+        const double a = U[p], b = U[p + d.sx];
+        u1 += a;
+        u2 += b;
+        uv += (m.wy0[0] * a + m.wy1[0] * b) * (0.5 * (V[p] + V[p + 1]));
+        n1a += nu_t[p];
+        n2a += nu_t[p + d.sx];
+        n1b += nu_t[p + 1];
+        n2b += nu_t[p + 1 + d.sx];
+    }
+    const double dudy = (u2 / nz2 - u1 / nz2) / (m.yg[1] - m.yg[0]);
+    const double tau = 0.25 * (n1a / nz2 + n2a / nz2 + n1b / nz2 + n2b / nz2) * dudy;
+    cf[i] = 2.0 * (-(uv / nz2) + nu * dudy + tau) / 1.0;
+}
+
+// per-plane sums and maxima of the four monitored interiors (monitor.f90:29-52)
+__global__ __launch_bounds__(256) void k_monitor(Dims d, const double *U, const double *V,
+                                                 const double *W, const double *nu_t, double *part) {
+    const int k = blockIdx.x + 1;  // plane
+    __shared__ double sh[8][256];
+    double s[4] = {0, 0, 0, 0}, mx[4] = {-INFINITY, -INFINITY, -INFINITY, -INFINITY};
+    for (int j = 1; j <= d.nyg - 2; ++j)
+        for (int i = threadIdx.x + 1; i <= d.nxg - 2; i += 256) {
+            const long long p = fidx(d, i, j, k);
+            if (i <= d.nx - 2) {
+                s[0] += U[p];
+                mx[0] = fmax(mx[0], U[p]);
+            }
+            if (j <= d.ny - 2) {
+                s[1] += V[p];
+                mx[1] = fmax(mx[1], V[p]);
+            }
+            if (k <= d.nz - 2) {
+                s[2] += W[p];
+                mx[2] = fmax(mx[2], W[p]);
+            }
+            s[3] += nu_t[p];
+            mx[3] = fmax(mx[3], nu_t[p]);
+        }
+    for (int q = 0; q < 4; ++q) {
+        sh[q][threadIdx.x] = s[q];
+        sh[4 + q][threadIdx.x] = mx[q];
+    }
+    __syncthreads();
+    for (int st = 128; st > 0; st >>= 1) {
+        if ((int)threadIdx.x < st)
+            for (int q = 0; q < 4; ++q) {
+                sh[q][threadIdx.x] += sh[q][threadIdx.x + st];
+                sh[4 + q][threadIdx.x] = fmax(sh[4 + q][threadIdx.x], sh[4 + q][threadIdx.x + st]);
+            }
+        __syncthreads();
+    }
+    if (threadIdx.x < 8) part[(long long)(k - 1) * 8 + threadIdx.x] = sh[threadIdx.x][0];
+}
+
+dim3 interior_grid(const Dims &d) {
+    return dim3((d.nxg - 2 + kWave - 1) / kWave, (d.nyg - 2 + 3) / 4, d.nzg - 2);
+}
+
+void check_launch() {
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) {
+        last_error() = std::string("kernel launch: ") + hipGetErrorString(e);
+        throw DeviceError();
+    }
+}
+
+template <int RW>
+void xform_launch(Handle &h, int kid, const XformArgs &a, const double *AT, const double *in, double *out) {
+    ProfScope ps(h, kid);
+    dim3 grid((a.C + kWave - 1) / kWave, (a.nrows + 4 * RW - 1) / (4 * RW));
+    hipLaunchKernelGGL(k_xform<RW>, grid, dim3(256), 0, h.stream, a, AT, in, out);
+    check_launch();
+}
+
+}  // namespace
+
+const char *kernel_name(int id) {
+    static const char *names[KID_COUNT] = {
+        "rk_stage", "bc_inout", "bc_periodic", "bc_topwall", "bc_massflux", "divergence",
+        "xform_x_fwd", "xform_z_fwd", "thomas", "xform_z_inv", "xform_x_inv", "project",
+        "pressure", "reduce"};
+    return (id >= 0 && id < KID_COUNT) ? names[id] : "";
+}
+
+// ------------------------------------------------------------------------
+// launch wrappers
+// ------------------------------------------------------------------------
+void launch_rk_stage(Handle &h, const Field3 &cur, const Field3 &old, Field3 &nxt, int stage,
+                     double dt) {
+    const Dims &d = h.d;
+    dim3 grid = interior_grid(d), block(kWave, 4);
+    const bool rk3 = h.scheme == 3;
+    ProfScope ps(h, KID_RK_STAGE);
+    if (stage == 1) {
+        const double c0 = dt * (rk3 ? h.rk_coef[1][1] : h.rk2_coef[1][1]);
+        if (rk3)
+            hipLaunchKernelGGL((k_rk_stage<0, true>), grid, block, 0, h.stream, d, h.m, h.nu, h.dPdx,
+                               h.dPdz, cur.u, cur.v, cur.w, old.u, old.v, old.w, nullptr, nullptr,
+                               nullptr, nxt.u, nxt.v, nxt.w, h.F1.u, h.F1.v, h.F1.w, c0, 0.0, dt);
+        else
+            hipLaunchKernelGGL((k_rk_stage<0, false>), grid, block, 0, h.stream, d, h.m, h.nu, h.dPdx,
+                               h.dPdz, cur.u, cur.v, cur.w, old.u, old.v, old.w, nullptr, nullptr,
+                               nullptr, nxt.u, nxt.v, nxt.w, nullptr, nullptr, nullptr, c0, 0.0, dt);
+    } else if (!rk3) {
+        // RK2 second stage: Uo + dt*(0*F1 + 1*F2) == Uo + dt*F2 exactly
+        hipLaunchKernelGGL((k_rk_stage<0, false>), grid, block, 0, h.stream, d, h.m, h.nu, h.dPdx,
+                           h.dPdz, cur.u, cur.v, cur.w, old.u, old.v, old.w, nullptr, nullptr, nullptr,
+                           nxt.u, nxt.v, nxt.w, nullptr, nullptr, nullptr, dt * h.rk2_coef[2][2], 0.0,
+                           dt);
+    } else {
+        // RK3 stages 2, 3: a_32 = 0 so F2 is never needed
+        const double c1 = h.rk_coef[stage][1], c0 = h.rk_coef[stage][stage];
+        hipLaunchKernelGGL((k_rk_stage<1, false>), grid, block, 0, h.stream, d, h.m, h.nu, h.dPdx,
+                           h.dPdz, cur.u, cur.v, cur.w, old.u, old.v, old.w, h.F1.u, h.F1.v, h.F1.w,
+                           nxt.u, nxt.v, nxt.w, nullptr, nullptr, nullptr, c0, c1, dt);
+    }
+    check_launch();
+}
+
+void launch_rhs_only(Handle &h, const Field3 &cur, Field3 &rhs) {
+    const Dims &d = h.d;
+    hipLaunchKernelGGL((k_rk_stage<2, false>), interior_grid(d), dim3(kWave, 4), 0, h.stream, d, h.m,
+                       h.nu, h.dPdx, h.dPdz, cur.u, cur.v, cur.w, nullptr, nullptr, nullptr, nullptr,
+                       nullptr, nullptr, nullptr, nullptr, nullptr, rhs.u, rhs.v, rhs.w, 0.0, 0.0, 0.0);
+    check_launch();
+}
+
+void launch_bc(Handle &h, Field3 &f, const Field3 &old, int rk_step, double dt, double t) {
+    const Dims &d = h.d;
+    const double coef = h.rk_t[rk_step] * dt * h.bc_uc;
+    prof_begin(h, KID_BC_INOUT);
+    hipLaunchKernelGGL(k_bc_inout, dim3((d.nyg + 255) / 256, d.nzg), dim3(256), 0, h.stream, d, h.m,
+                       h.bc, f.u, f.v, f.w, old.u, old.v, old.w, coef, t);
+    check_launch();
+    prof_end(h, KID_BC_INOUT);
+    prof_begin(h, KID_BC_PERIODIC);
+    hipLaunchKernelGGL(k_bc_periodic, dim3((d.nxg + 255) / 256, d.nyg), dim3(256), 0, h.stream, d,
+                       f.u, f.v, f.w);
+    check_launch();
+    prof_end(h, KID_BC_PERIODIC);
+    prof_begin(h, KID_BC_TOPWALL);
+    hipLaunchKernelGGL(k_bc_topwall, dim3((d.nxg + 255) / 256, d.nzg), dim3(256), 0, h.stream, d,
+                       h.bc, f.u, f.v, f.w);
+    check_launch();
+    prof_end(h, KID_BC_TOPWALL);
+    prof_begin(h, KID_BC_MASSFLUX);
+    hipLaunchKernelGGL(k_bc_massflux, dim3(1), dim3(1024), 0, h.stream, d, h.m, h.length_y, f.u, f.v);
+    check_launch();
+    prof_end(h, KID_BC_MASSFLUX);
+}
+
+void launch_divergence(Handle &h, const Field3 &f, double *out) {
+    const Dims &d = h.d;
+    ProfScope ps(h, KID_DIVERGENCE);
+    hipLaunchKernelGGL(k_divergence, dim3((d.nxg - 3 + kWave - 1) / kWave, (d.nyg - 2 + 3) / 4, d.nzg - 3),
+                       dim3(kWave, 4), 0, h.stream, d, h.m, f.u, f.v, f.w, out);
+    check_launch();
+}
+
+void launch_poisson(Handle &h, double *phi) {
+    const Dims &d = h.d;
+    const int N = d.N, M = d.M, NM = d.NM;
+    const long long plane = (long long)N * NM;
+    XformArgs a{};
+    // forward x: columns (j, m) of the padded field -> T1[m][j][kx]
+    a.nrows = N;
+    a.nsum = N;
+    a.lda = h.lda_x;
+    a.C1 = NM;
+    a.C = NM * M;
+    a.in_base = fidx(d, 1, 1, 1);
+    a.in_s = 1;
+    a.in_c1 = d.sx;
+    a.in_c2 = d.sxy;
+    a.out_base = 0;
+    a.out_r = 1;
+    a.out_c1 = N;
+    a.out_c2 = plane;
+    xform_launch<16>(h, KID_XFORM_X_FWD, a, h.AxF, phi, h.T1);
+    // forward z: columns (kx, j) of T1 -> S[q][j][kx]
+    a.nrows = M;
+    a.nsum = M;
+    a.lda = h.lda_z;
+    a.C1 = N;
+    a.C = N * NM;
+    a.in_base = 0;
+    a.in_s = plane;
+    a.in_c1 = 1;
+    a.in_c2 = N;
+    a.out_base = 0;
+    a.out_r = plane;
+    a.out_c1 = 1;
+    a.out_c2 = N;
+    xform_launch<16>(h, KID_XFORM_Z_FWD, a, h.AzF, h.T1, h.S);
+    {
+    ProfScope ps(h, KID_THOMAS);
+    hipLaunchKernelGGL(k_thomas, dim3((N + 255) / 256, d.Kz), dim3(256), 0, h.stream, d, h.lu_dl,
+                       h.lu_dp, h.lu_du, h.S);
+    check_launch();
+    }
+    // inverse z: S -> T1
+    xform_launch<16>(h, KID_XFORM_Z_INV, a, h.AzI, h.S, h.T1);
+    // inverse x: T1[m][j][kx] -> phi cells (1..N, 1..NM, 1..M)
+    a.nrows = N;
+    a.nsum = N;
+    a.lda = h.lda_x;
+    a.C1 = NM;
+    a.C = NM * M;
+    a.in_base = 0;
+    a.in_s = 1;
+    a.in_c1 = N;
+    a.in_c2 = plane;
+    a.out_base = fidx(d, 1, 1, 1);
+    a.out_r = 1;
+    a.out_c1 = d.sx;
+    a.out_c2 = d.sxy;
+    xform_launch<16>(h, KID_XFORM_X_INV, a, h.AxI, h.T1, phi);
+}
+
+void launch_project(Handle &h, Field3 &f, const double *phi) {
+    const Dims &d = h.d;
+    ProfScope ps(h, KID_PROJECT);
+    hipLaunchKernelGGL(k_project, interior_grid(d), dim3(kWave, 4), 0, h.stream, d, h.m, f.u, f.v,
+                       f.w, phi);
+    check_launch();
+}
+
+void launch_pressure(Handle &h, const double *phi, double *P, double dt) {
+    const Dims &d = h.d;
+    ProfScope ps(h, KID_PRESSURE);
+    const double den = dt * h.rk2_coef[1][1];
+    hipLaunchKernelGGL(k_pressure, dim3((d.nxg + 255) / 256, d.nyg, d.nzg), dim3(256), 0, h.stream, d,
+                       phi, P, den);
+    check_launch();
+}
+
+double max_divergence(Handle &h, const Field3 &f) {
+    const Dims &d = h.d;
+    BL_HIP(hipMemsetAsync(h.scal, 0, sizeof(double), h.stream));
+    hipLaunchKernelGGL(k_maxdiv, interior_grid(d), dim3(kWave, 4), 0, h.stream, d, h.m, f.u, f.v, f.w,
+                       h.scal);
+    check_launch();
+    double v = 0.0;
+    BL_HIP(hipMemcpyAsync(&v, h.scal, sizeof(double), hipMemcpyDeviceToHost, h.stream));
+    BL_HIP(hipStreamSynchronize(h.stream));
+    return v;
+}
+
+double dt_convective(Handle &h, const Field3 &f) {
+    const Dims &d = h.d;
+    ProfScope ps(h, KID_REDUCE);
+    BL_HIP(hipMemsetAsync(h.scal, 0, 3 * sizeof(double), h.stream));
+    hipLaunchKernelGGL(k_dtconv, interior_grid(d), dim3(kWave, 4), 0, h.stream, d, h.m, f.u, f.v, f.w,
+                       h.scal);
+    check_launch();
+    double v[3];
+    BL_HIP(hipMemcpyAsync(v, h.scal, 3 * sizeof(double), hipMemcpyDeviceToHost, h.stream));
+    BL_HIP(hipStreamSynchronize(h.stream));
+    const double CFLa = std::fabs(h.cfl);
+    const double dcu = CFLa * v[0], dcv = CFLa * v[1], dcw = CFLa * v[2];
+    return std::fmin(dcu, std::fmin(dcv, dcw));
+}
+
+void wall_stats(Handle &h, const Field3 &f, double *cf_host) {
+    const Dims &d = h.d;
+    hipLaunchKernelGGL(k_wallstats, dim3((d.nx + 255) / 256), dim3(256), 0, h.stream, d, h.m, h.nu, f.u,
+                       f.v, h.nu_t, h.red);
+    check_launch();
+    BL_HIP(hipMemcpyAsync(cf_host, h.red, sizeof(double) * d.nx, hipMemcpyDeviceToHost, h.stream));
+    BL_HIP(hipStreamSynchronize(h.stream));
+}
+
+void monitor(Handle &h, const Field3 &f, double *out) {
+    const Dims &d = h.d;
+    const int np = d.nzg - 2;
+    hipLaunchKernelGGL(k_monitor, dim3(np), dim3(256), 0, h.stream, d, f.u, f.v, f.w, h.nu_t, h.red);
+    check_launch();
+    std::vector<double> part((size_t)np * 8);
+    BL_HIP(hipMemcpyAsync(part.data(), h.red, sizeof(double) * part.size(), hipMemcpyDeviceToHost,
+                          h.stream));
+    BL_HIP(hipStreamSynchronize(h.stream));
+    for (int q = 0; q < 4; ++q) {
+        out[q] = 0.0;
+        out[4 + q] = -INFINITY;
+    }
+    for (int k = 0; k < np; ++k)
+        for (int q = 0; q < 4; ++q) {
+            out[q] += part[(size_t)k * 8 + q];
+            out[4 + q] = std::fmax(out[4 + q], part[(size_t)k * 8 + 4 + q]);
+        }
+}
+
+}  // namespace blgpu

@@ -1,0 +1,174 @@
+/*
+ * blgpu — MI355X (gfx950) implementation of the boundary-layer solver's
+ * per-time-step path behind a plain C ABI.
+ *
+ * Conventions shared by every entry point:
+ *  - 3-D fields cross the boundary exactly as the reference Fortran stores
+ *    them: column-major, x fastest, full allocated extent
+ *      U(nx,nyg,nzg)  V(nxg,ny,nzg)  W(nxg,nyg,nz)  P(nxg,nyg,nzg)  nu_t(nxg,nyg,nzg)
+ *    with nxg = nx+1, nyg = ny+1, nzg = nz+1 (initialization.f90:84-104, 126-129).
+ *  - Every function returns 0 on success or a nonzero BL_E* code; the message
+ *    of the last failure on the calling thread is bl_last_error().  The
+ *    reference Stops with a message on errors (cufft_solver.f90:39); a Fortran
+ *    caller does `if (ierr /= 0) stop` with that text.
+ *  - One handle = one device-resident solver state on the current HIP device,
+ *    driven from one host thread on one stream.  Handles are not thread-safe.
+ *  - No host pointer passed in is retained after the call returns.
+ */
+#ifndef BLGPU_H
+#define BLGPU_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct bl_handle bl_handle;
+
+enum {
+    BL_OK = 0,
+    BL_EINVAL = 1,   /* bad argument or unsupported configuration   */
+    BL_EDEVICE = 2,  /* HIP runtime / device error                  */
+    BL_ENOMEM = 3,   /* device allocation failed                    */
+    BL_ESTATE = 4    /* call not valid in the handle's current state */
+};
+
+enum { BL_FIELD_U = 0, BL_FIELD_V = 1, BL_FIELD_W = 2, BL_FIELD_P = 3, BL_FIELD_NUT = 4 };
+
+enum { BL_PREC_FP64 = 0, BL_PREC_MIXED = 1 };
+
+/*
+ * Solver configuration.  Replaces the `!$acc data` region set-up
+ * (main.f90:61-81), cufft_init_plans (cufft_solver.f90:26-62) and the Thomas
+ * LU / twiddle precompute (initialization.f90:313-320, 427-469): the library
+ * derives metrics, transform matrices and LU factors itself from the grids.
+ *  x(nx), y(ny), z(nz): face coordinates (x_global, y_global, z_global).
+ *  scheme: RKscheme (2 or 3), les: LES flag (0 DNS, 2 dynamic Smagorinsky),
+ *  cfl: .turbb CFL (negative = fixed dt = -cfl).
+ *  Boundary tables as built by compute_blasius_solution_for_bc
+ *  (boundary_conditions.f90:1452-1797): U_inlet(nyg) V_inlet(ny) W_inlet(nyg)
+ *  U_top(nx) V_top(nxg) W_top(nxg).
+ *  Temporal inflow modes (optional, n_modes = 0 disables): zmode(n_modes),
+ *  tmode(m_modes) already multiplied by beta/omega, and q*_re/q*_im laid out
+ *  (j, n, m) with j fastest, already interpolated to yg (u, w) and y (v).
+ */
+typedef struct bl_config {
+    int nx, ny, nz;
+    const double *x, *y, *z;
+    double nu, dPdx, dPdz, cfl;
+    int scheme;
+    int les;
+    int precision;
+    const double *U_inlet, *V_inlet, *W_inlet;
+    const double *U_top, *V_top, *W_top;
+    int n_modes, m_modes;
+    const double *zmode, *tmode;
+    const double *qu_re, *qu_im, *qv_re, *qv_im, *qw_re, *qw_im;
+} bl_config;
+
+/* Create a solver on the current HIP device.  Replaces main.f90:61-81 +
+ * cufft_init_plans + initialization.f90:427-469. */
+int bl_create(const bl_config *cfg, bl_handle **out);
+/* Free all device memory (cufft_destroy_plans cufft_solver.f90:64-72,
+ * finalize finalization.f90:20-36). */
+void bl_destroy(bl_handle *h);
+/* Message of the last failure on this thread ("" if none). */
+const char *bl_last_error(void);
+
+/* `!$acc update device(field)` / `!$acc update self(field)` (main.f90:100,
+ * time_integration.f90:342-345).  Synchronous; full reference extent. */
+int bl_upload(bl_handle *h, int field, const double *host);
+int bl_download(bl_handle *h, int field, double *host);
+
+int bl_set_time(bl_handle *h, double t);
+int bl_get_time(bl_handle *h, double *t);
+
+/* compute_dt (time_integration.f90:528-591): fixed dt for cfl < 0, else the
+ * reference's max-based convective bound (a device reduction) and the
+ * viscous bound.  The result is also stored as the handle's dt. */
+int bl_compute_dt(bl_handle *h, double *dt);
+
+/* compute_time_step_RK2 / RK3 (time_integration.f90:145-300, 305-465):
+ * one full step with the handle's dt, asynchronous on the handle's stream
+ * (no host synchronisation inside).  t advances to t + dt. */
+int bl_step(bl_handle *h);
+/* Run nsteps steps back to back (the time loop of main.f90:84-113 without
+ * the I/O). */
+int bl_run(bl_handle *h, int nsteps);
+/* Block until queued work is finished. */
+int bl_synchronize(bl_handle *h);
+
+/* compute_eddy_viscosity (subgrid.f90:23-47) on the current fields:
+ * recomputes nu_t on the device (LES = 2: dynamic Smagorinsky). */
+int bl_eddy_viscosity(bl_handle *h);
+
+/* check_divergence (projection.f90:391-413), FP64 device max-reduction. */
+int bl_max_divergence(bl_handle *h, double *div);
+/* Cf part of compute_statistics (statistics.f90:137-160): cf(nx). */
+int bl_wall_stats(bl_handle *h, double *cf);
+/* output_monitor reductions (monitor.f90:29-52): out[8] = sum U, V, W,
+ * nu_t over the interiors, then max U, V, W, nu_t. */
+int bl_monitor(bl_handle *h, double *out);
+
+/* ---- single operators, for parity checks against the reference ---- */
+
+/* compute_rhs_u/v/w (equations.f90:24-415) of the current fields, written to
+ * host arrays of the full U/V/W extents (interior ranges only). */
+int bl_rhs(bl_handle *h, double *ru, double *rv, double *rw);
+/* Set the step-start fields Uo, Vo, Wo used by the outflow condition
+ * (time_integration.f90:314-337 copy). */
+int bl_set_old(bl_handle *h, const double *Uo, const double *Vo, const double *Wo);
+/* apply_boundary_conditions_gpu (boundary_conditions.f90:127-262) on the
+ * current fields with rk_step and dt at the handle's time. */
+int bl_apply_bc(bl_handle *h, int rk_step, double dt);
+/* compute_projection_step (projection.f90:21-87, 293-386) on the current
+ * fields; P is updated when rk_step == 1. */
+int bl_projection(bl_handle *h, int rk_step, double dt);
+/* Poisson solve alone (solve_poisson_gpu projection.f90:93-204): rhs and
+ * result in a host array of the P extent; cells i=2..nxg-2, j=2..nyg-1,
+ * k=2..nzg-2 are read and written. */
+int bl_poisson(bl_handle *h, double *rhs_inout);
+
+/* ---- per-kernel timing (no counterpart in the reference, which prints a
+ * system_clock sub-step profile, time_integration.f90:283-298) ---- */
+
+/* When enabled, every kernel launch of the handle is bracketed by HIP events
+ * on the handle's stream. */
+int bl_profile_enable(bl_handle *h, int on);
+/* Accumulated milliseconds and launch counts per kernel class id 0..n-1
+ * since profiling was enabled. */
+int bl_profile_read(bl_handle *h, double *ms, long long *launches, int n);
+/* Number of kernel classes and the name of class id. */
+int bl_profile_count(void);
+const char *bl_profile_name(int id);
+
+/* ---- host-side set-up helpers (not on the per-step path) ---- */
+
+/* Blasius similarity profile f''' + f f'' = 0, f(0)=f'(0)=0, f''(0)=fpp0,
+ * integrated by classical RK4 with `sub` sub-steps per output interval on
+ * eta = 0..eta_max (n points).  The reference ships the table it makes with
+ * fpp0 = 0.46960, eta_max = 20, n = 1000 (laminar_test/generate_blasius.py). */
+int bl_blasius_profile(int n, double eta_max, double fpp0, int sub, double *eta, double *f,
+                       double *df);
+/* Boundary tables from a Blasius table, inflow_flag 1/3/4
+ * (boundary_conditions.f90:1463-1627). */
+int bl_blasius_tables(int nx, int ny, const double *x, const double *y, double nu, int n,
+                      const double *eta, const double *f, const double *df, double *U_inlet,
+                      double *V_inlet, double *W_inlet, double *U_top, double *V_top,
+                      double *W_top);
+/* Blasius initial field on any grid (init_flow input_output.f90:206-259). */
+int bl_blasius_field(int nx, int ny, int nz, const double *x, const double *y, double nu, int n,
+                     const double *eta, const double *f, const double *df, double *U, double *V,
+                     double *W);
+/* Uniform noise amp*(2r-1) on the interior of U, V, W, r from a counter-based
+ * generator keyed by (seed, field, linear index): identical on any host. */
+int bl_add_noise(int nx, int ny, int nz, double amp, unsigned long long seed, double *U, double *V,
+                 double *W);
+/* tanh-stretched grid: x = x0 + Lx*i/(nx-1), y = Ly*(1 - tanh(a(1-j/(ny-1)))/tanh a),
+ * z = Lz*k/(nz-2). */
+int bl_grid_tanh(int nx, int ny, int nz, double Lx, double Ly, double Lz, double alpha, double x0,
+                 double *x, double *y, double *z);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

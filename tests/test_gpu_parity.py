@@ -1,0 +1,163 @@
+"""Parity of the HIP path (through the C ABI) with the FP64 oracle.
+
+Tolerances (FP64 mode): single operators agree to a relative L2 of 1e-12,
+full steps to 1e-12 after one step and 1e-10 after twenty (the bar for
+u, v, w, p after N steps); the maximum divergence per step stays below 1e-12
+relative to max|u|/min(dy).
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from . import cases, orc
+from .blgpu_import import blgpu, solver_for
+
+pytestmark = pytest.mark.gpu
+
+
+def rel_l2(a, b):
+    nb = np.linalg.norm(b)
+    return np.linalg.norm(a - b) / (nb if nb > 0 else 1.0)
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+CASES = {
+    "tanh_even": dict(nx=24, ny=16, nz=8, noise=0.01, seed=1),
+    "tanh_odd": dict(nx=25, ny=13, nz=9, noise=0.01, seed=2),
+    "sinh": dict(nx=26, ny=14, nz=10, grid="sinh", alpha=2.5, noise=0.01, seed=3),
+    "modes": dict(nx=24, ny=24, nz=10, Ly=0.02, Lz=0.1083, modes=True, noise=0.002, seed=4),
+}
+
+
+@pytest.fixture(params=sorted(CASES))
+def case(request):
+    say("case", request.param, CASES[request.param])
+    return cases.laminar_small(**CASES[request.param])
+
+
+def test_rhs_parity(case):
+    o = case.oracle()
+    s, _ = solver_for(case, o)
+    ro = o.rhs()
+    rg = s.rhs()
+    for c in range(3):
+        assert rel_l2(rg[c], ro[c]) < 1e-12, c
+
+
+def test_bc_parity(case):
+    o = case.oracle()
+    U0, V0, W0 = o.fields()
+    s, _ = solver_for(case, o)
+    Uo = U0 * (1 + 0.01 * np.sin(np.arange(U0.size).reshape(U0.shape, order="F")))
+    o.set_old(Uo, V0, W0)
+    s.set_old(Uo, V0, W0)
+    for t in (0.0, 0.0037):
+        o.t = t
+        s.t = t
+        o.apply_bc(2, 3e-4)
+        s.apply_bc(2, 3e-4)
+        for c, (a, b) in enumerate(zip(s.fields(), o.fields())):
+            assert rel_l2(a, b) < 1e-14, (t, c)
+
+
+def test_poisson_parity(case):
+    o = case.oracle()
+    s, _ = solver_for(case, o)
+    rng = np.random.default_rng(7)
+    rhs = np.zeros(o.shapes[3], order="F")
+    rhs[1:-2, 1:-1, 1:-2] = rng.standard_normal(rhs[1:-2, 1:-1, 1:-2].shape)
+    po = o.poisson(rhs)[1:-2, 1:-1, 1:-2]
+    pg = s.poisson(rhs)[1:-2, 1:-1, 1:-2]
+    assert rel_l2(pg, po) < 1e-12
+
+
+def test_projection_parity(case):
+    o = case.oracle()
+    s, _ = solver_for(case, o)
+    o.set_old(*o.fields())
+    s.set_old(*o.fields())
+    o.apply_bc(1, 2e-4)
+    s.apply_bc(1, 2e-4)
+    o.projection(1, 2e-4)
+    s.projection(1, 2e-4)
+    for c, (a, b) in enumerate(zip(s.fields(), o.fields())):
+        assert rel_l2(a, b) < 1e-12, c
+    assert rel_l2(s.download(blgpu.FIELD_P), o.get(3)) < 1e-11
+
+
+@pytest.mark.parametrize("scheme", [3, 2])
+def test_step_parity(case, scheme):
+    case.scheme = scheme
+    case.cfl = -2e-4
+    o = case.oracle()
+    s, _ = solver_for(case, o)
+    assert o.compute_dt() == s.compute_dt() == 2e-4
+    dmax = np.abs(o.get(0)).max() / np.diff(case.y).min()
+    for n in range(20):
+        o.step()
+        s.step()
+        if n in (0, 19):
+            tol = 1e-12 if n == 0 else 1e-10
+            for c, (a, b) in enumerate(zip(s.fields(), o.fields())):
+                assert rel_l2(a, b) < tol, (n, c)
+            assert rel_l2(s.download(blgpu.FIELD_P), o.get(3)) < 10 * tol
+        div = s.max_divergence()
+        assert div < 1e-12 * dmax, (n, div)
+    assert s.t == o.t
+    assert abs(s.max_divergence() - o.max_divergence()) <= 1e-12 * dmax
+    cf_o, cf_g = o.wall_stats(), s.wall_stats()
+    assert rel_l2(cf_g, cf_o) < 1e-10
+    mo, mg = o.monitor(), s.monitor()
+    assert np.allclose(mg, mo, rtol=1e-11, atol=1e-12)
+
+
+def test_compute_dt_cfl_parity():
+    c = cases.laminar_small(nx=20, ny=12, nz=9, noise=0.02, seed=8, cfl=0.4)
+    o = c.oracle()
+    s, _ = solver_for(c, o)
+    assert s.compute_dt() == o.compute_dt()
+    W = o.get(2) * 0.0
+    o.set(2, W)
+    s.upload(2, W)
+    assert s.compute_dt() == o.compute_dt()
+
+
+def test_larger_grid_parity():
+    c = cases.laminar_small(nx=96, ny=48, nz=35, Lx=0.3, Ly=0.02, Lz=0.02, noise=0.01, seed=11, cfl=-1e-4)
+    say("larger grid", c.x.size, c.y.size, c.z.size)
+    o = c.oracle()
+    s, _ = solver_for(c, o)
+    o.compute_dt()
+    s.compute_dt()
+    for _ in range(5):
+        o.step()
+        s.step()
+    for ci, (a, b) in enumerate(zip(s.fields(), o.fields())):
+        assert rel_l2(a, b) < 1e-11, ci
+    assert rel_l2(s.download(blgpu.FIELD_P), o.get(3)) < 1e-10
+
+
+def test_laminar_blasius_cf():
+    """Laminar case (scaled cfg-1 box): Cf stays within 0.5 % of 0.664/sqrt(Re_x)
+    away from the inlet and outlet, as the reference's laminar test checks."""
+    nu = 4.898844e-6
+    theta = 1.469653e-3
+    x, y, z = blgpu.grid_tanh(96, 32, 8, 100 * theta, 10 * theta, 4 * theta, 2.0, 1.0)
+    eta, f, df = blgpu.blasius_profile()
+    t = blgpu.blasius_tables(x, y, nu, eta, f, df)
+    U, V, W = blgpu.blasius_field(x, y, z, nu, eta, f, df)
+    s = blgpu.Solver(x, y, z, nu, t, scheme=3, cfl=0.5)
+    s.set_fields(U, V, W)
+    dt = s.compute_dt()
+    assert dt > 0
+    s.run(300)
+    cf = s.wall_stats()
+    rex = x / nu
+    ratio = cf[3:-5] / (0.664 / np.sqrt(rex[3:-5]))
+    assert np.abs(ratio - 1).max() < 5e-3, ratio
+    assert s.max_divergence() < 1e-12 / np.diff(y).min()
