@@ -13,6 +13,8 @@
 #include <cstring>
 #include <new>
 #include <string>
+#include <tuple>
+#include <utility>
 #include <vector>
 
 #include "../../include/blgpu.h"
@@ -342,6 +344,33 @@ void build(Handle &h, const bl_config *c) {
             azi[(size_t)q * h.lda_z + m] = (double)iv;
         }
     }
+    // x-DCT tables: N = N1*N2 with N1 the largest divisor <= sqrt(N)
+    {
+        int n1 = 1;
+        for (int a = 1; (long long)a * a <= N; ++a)
+            if (N % a == 0) n1 = a;
+        h.dct_n1 = n1;
+        const int n2 = N / n1;
+        auto cis = [](long long num, long long den, long double frac) {
+            // e^{-2 pi i num/den * frac}, argument reduced exactly
+            const long long r = num % den;
+            const long double ang = -2.0L * kPi * (long double)r / (long double)den * frac;
+            return std::pair<double, double>((double)cosl(ang), (double)sinl(ang));
+        };
+        std::vector<double> w1(2 * n1), w2(2 * n2), wN(2 * N), rot(2 * N);
+        for (int q = 0; q < n1; ++q) std::tie(w1[2 * q], w1[2 * q + 1]) = cis(q, n1, 1.0L);
+        for (int q = 0; q < n2; ++q) std::tie(w2[2 * q], w2[2 * q + 1]) = cis(q, n2, 1.0L);
+        for (int q = 0; q < N; ++q) std::tie(wN[2 * q], wN[2 * q + 1]) = cis(q, N, 1.0L);
+        for (int k = 0; k < N; ++k) std::tie(rot[2 * k], rot[2 * k + 1]) = cis(k, 4LL * N, 1.0L);
+        h.dct_w1 = dmalloc(w1.size());
+        h.dct_w2 = dmalloc(w2.size());
+        h.dct_wN = dmalloc(wN.size());
+        h.dct_rot = dmalloc(rot.size());
+        upload_vec(h.dct_w1, w1, h.stream);
+        upload_vec(h.dct_w2, w2, h.stream);
+        upload_vec(h.dct_wN, wN, h.stream);
+        upload_vec(h.dct_rot, rot, h.stream);
+    }
     h.AxF = dmalloc(axf.size());
     h.AxI = dmalloc(axi.size());
     h.AzF = dmalloc(azf.size());
@@ -385,7 +414,12 @@ void build(Handle &h, const bl_config *c) {
     upload_vec(h.lu_du, dup, h.stream);
 
     h.scal = dmalloc(64);
-    h.red = dmalloc(std::max<size_t>((size_t)std::max(d.nxg, 8 * d.nzg), 1024));
+    {
+        // reduction scratch: 3 partials per interior block, flux partials, Cf, monitor planes
+        const size_t nb = (size_t)((d.nxg - 2 + 63) / 64) * ((d.nyg - 2 + 3) / 4) * (d.nzg - 2);
+        const size_t nq = (size_t)(((d.nyg + 255) / 256) + ((d.nxg + 255) / 256)) * d.nzg;
+        h.red = dmalloc(std::max({3 * nb, nq, (size_t)d.nxg, (size_t)8 * d.nzg, (size_t)1024}));
+    }
     BL_HIP(hipStreamSynchronize(h.stream));
 }
 
@@ -402,7 +436,8 @@ void free_all(Handle &h) {
     }
     for (void *p : {(void *)h.F1.u, (void *)h.F1.v, (void *)h.F1.w, (void *)h.P, (void *)h.phi,
                     (void *)h.nu_t, (void *)h.T1, (void *)h.S, (void *)h.AxF, (void *)h.AxI,
-                    (void *)h.AzF, (void *)h.AzI, (void *)h.lu_dl, (void *)h.lu_dp, (void *)h.lu_du,
+                    (void *)h.AzF, (void *)h.AzI, (void *)h.dct_w1, (void *)h.dct_w2,
+                    (void *)h.dct_wN, (void *)h.dct_rot, (void *)h.lu_dl, (void *)h.lu_dp, (void *)h.lu_du,
                     (void *)h.scal, (void *)h.red, h.mem_grid})
         f(p);
     if (h.stream) (void)hipStreamSynchronize(h.stream);

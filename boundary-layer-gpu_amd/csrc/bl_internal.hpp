@@ -94,6 +94,8 @@ struct Handle {
     double *P, *phi, *nu_t;
     double *T1, *S;  // transform intermediates: [m][j][kx] and [q][j][kx]
     double *AxF, *AzF, *AzI, *AxI;  // transposed transform matrices
+    int dct_n1;                      // x-DCT factorisation N = dct_n1 * (N / dct_n1)
+    double *dct_w1, *dct_w2, *dct_wN, *dct_rot;  // x-DCT twiddle tables (complex)
     double *lu_dl, *lu_dp, *lu_du;  // Thomas factors [k][j][kx], [k][j][kx], [j]
     double *scal;                   // device scalars (mass-flux correction, reductions)
     double *red;                    // reduction scratch

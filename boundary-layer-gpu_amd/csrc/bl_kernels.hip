@@ -129,13 +129,28 @@ __global__ __launch_bounds__(256) void k_rk_stage(
 // ------------------------------------------------------------------------
 
 // inflow (:137-190) and outflow (:193-202); one thread per (j,k)
+// deterministic block sum of 256 values; returns the sum in thread 0
+__device__ __forceinline__ double block_sum_256(double v, double *sh) {
+    sh[threadIdx.x] = v;
+    __syncthreads();
+    for (int st = 128; st > 0; st >>= 1) {
+        if ((int)threadIdx.x < st) sh[threadIdx.x] += sh[threadIdx.x + st];
+        __syncthreads();
+    }
+    return sh[0];
+}
+
 __global__ __launch_bounds__(256) void k_bc_inout(Dims d, Metrics m, BCTables b, double *U,
                                                   double *V, double *W, const double *Uo,
                                                   const double *Vo, const double *Wo,
-                                                  double coef, double t) {
+                                                  double coef, double t, double *qpart) {
+    __shared__ double sh[256];
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
     const int k = blockIdx.y;
-    if (j >= d.nyg) return;
+    // mass flux through the x boundaries (boundary_conditions.f90:246-249):
+    // (U(1,j,k) - U(nx-1,j,k))*(y(j)-y(j-1)) for j in [1,nyg-2], k in [1,nzg-3]
+    double q = 0.0;
+    if (j < d.nyg) {
     const int nx = d.nx, nxg = d.nxg;
     // temporal modes Re(q e^{i(beta z - omega t)})
     double ut = 0.0, vt = 0.0, wt = 0.0;
@@ -160,13 +175,15 @@ __global__ __launch_bounds__(256) void k_bc_inout(Dims d, Metrics m, BCTables b,
     }
     const long long p0 = fidx(d, 0, j, k);
     // U (all j < nyg, k < nzg)
-    U[p0] = b.U_inlet[j] + ut;
+    const double uin = b.U_inlet[j] + ut;
+    U[p0] = uin;
     {
         const long long pa = fidx(d, nx - 2, j, k);
         const double o1 = Uo[pa], o0 = Uo[pa - 1];
         const double v = o1 - coef * (o1 - o0) / (m.x[nx - 2] - m.x[nx - 3]);
         U[pa] = v;
         U[pa + 1] = v;
+        if (j >= 1 && j <= d.nyg - 2 && k >= 1 && k <= d.nzg - 3) q = (uin - v) * (m.y[j] - m.y[j - 1]);
     }
     if (k < d.nz) {
         W[p0] = b.W_inlet[j] + wt;
@@ -184,6 +201,9 @@ __global__ __launch_bounds__(256) void k_bc_inout(Dims d, Metrics m, BCTables b,
         V[pa] = v;
         V[pa + 1] = v;
     }
+    }
+    q = block_sum_256(q, sh);
+    if (threadIdx.x == 0) qpart[blockIdx.x + gridDim.x * blockIdx.y] = q;
 }
 
 // periodic z (:205-217); one thread per (i,j)
@@ -208,10 +228,17 @@ __global__ __launch_bounds__(256) void k_bc_periodic(Dims d, double *U, double *
 }
 
 // top Dirichlet (:220-228) then wall (:231-238); one thread per (i,k)
-__global__ __launch_bounds__(256) void k_bc_topwall(Dims d, BCTables b, double *U, double *V,
-                                                    double *W) {
+__global__ __launch_bounds__(256) void k_bc_topwall(Dims d, Metrics m, BCTables b, double *U,
+                                                    double *V, double *W, double *qpart) {
+    __shared__ double sh[256];
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int k = blockIdx.y;
+    // mass flux through the y boundaries (boundary_conditions.f90:251-253) from
+    // the values written here: (V(i,1,k) - V(i,ny,k))*(x(i)-x(i-1))
+    double q = 0.0;
+    if (i >= 1 && i <= d.nxg - 3 && k >= 1 && k <= d.nzg - 3) q = (0.0 - b.V_top[i]) * (m.x[i] - m.x[i - 1]);
+    q = block_sum_256(q, sh);
+    if (threadIdx.x == 0) qpart[blockIdx.x + gridDim.x * blockIdx.y] = q;
     if (i >= d.nxg) return;
     const long long sx = d.sx;
     const long long p0 = fidx(d, i, 0, k);
@@ -227,40 +254,23 @@ __global__ __launch_bounds__(256) void k_bc_topwall(Dims d, BCTables b, double *
     }
 }
 
-// global mass conservation (:241-260): single work-group reduction then
-// correction of the outflow planes.
-__global__ __launch_bounds__(1024) void k_bc_massflux(Dims d, Metrics m, double length_y, double *U,
-                                                      const double *V) {
-    __shared__ double red[1024];
-    __shared__ double dU;
-    const int tid = threadIdx.x;
+// global mass conservation (:241-260): every block re-sums the flux partials
+// in the same order (so all agree bit for bit) and corrects its part of the
+// outflow planes.  One thread per (j,k).
+__global__ __launch_bounds__(256) void k_bc_massflux(Dims d, double length_y, const double *qpart,
+                                                     int nq, double *U) {
+    __shared__ double sh[256];
     double acc = 0.0;
-    const int nj = d.nyg - 2, nk = d.nzg - 3;
-    for (int t = tid; t < nj * nk; t += 1024) {
-        const int j = 1 + t % nj, k = 1 + t / nj;
-        acc += (U[fidx(d, 0, j, k)] - U[fidx(d, d.nx - 2, j, k)]) * (m.y[j] - m.y[j - 1]);
-    }
-    const int ni = d.nxg - 3;
-    for (int t = tid; t < ni * nk; t += 1024) {
-        const int i = 1 + t % ni, k = 1 + t / ni;
-        acc += (V[fidx(d, i, 0, k)] - V[fidx(d, i, d.ny - 1, k)]) * (m.x[i] - m.x[i - 1]);
-    }
-    red[tid] = acc;
-    __syncthreads();
-    for (int s = 512; s > 0; s >>= 1) {
-        if (tid < s) red[tid] += red[tid + s];
-        __syncthreads();
-    }
-    if (tid == 0) dU = red[0] / length_y / (double)(d.nzg - 3);
-    __syncthreads();
-    const double du = dU;
-    for (int t = tid; t < d.nyg * d.nzg; t += 1024) {
-        const int j = t % d.nyg, k = t / d.nyg;
-        const long long pa = fidx(d, d.nx - 2, j, k);
-        const double v = U[pa] + du;
-        U[pa] = v;
-        U[pa + 1] = v;
-    }
+    for (int t = threadIdx.x; t < nq; t += 256) acc += qpart[t];
+    const double Q = block_sum_256(acc, sh);
+    const double du = Q / length_y / (double)(d.nzg - 3);
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    const int k = blockIdx.y;
+    if (j >= d.nyg) return;
+    const long long pa = fidx(d, d.nx - 2, j, k);
+    const double v = U[pa] + du;
+    U[pa] = v;
+    U[pa + 1] = v;
 }
 
 // ------------------------------------------------------------------------
@@ -354,6 +364,172 @@ __global__ __launch_bounds__(256) void k_xform(XformArgs a, const double *__rest
         if (r0 + r < a.nrows) dst[(long long)(r0 + r) * a.out_r] = acc[r];
 }
 
+// ------------------------------------------------------------------------
+// DCT-II x2 along x (forward) and its normalised inverse, per line, by
+// Makhoul's reordering and a two-factor (N = N1*N2) DFT of the real
+// reordered sequence, all in LDS.  Only k1 <= N1/2 is formed; the other half
+// follows from Hermitian symmetry.  Forward: X[k] = sum_n 2 f[n]
+// cos(pi k (2n+1)/(2N)) (the reference's 2N-point symmetric extension plus
+// twiddle, projection.f90:110-134).  Inverse: f[n] = (1/(N M)) (g[0]/2 +
+// sum_k g[k] cos(pi k (2n+1)/(2N))) (projection.f90:160-202 with the
+// 1/(2 nxp nzp) factor).
+// Tables (cos, sin interleaved): w1[q] = e^{-2 pi i q/N1}, w2[q] = e^{-2 pi i q/N2},
+// wN[q] = e^{-2 pi i q/N}, rot[k] = e^{-i pi k/(2N)}.
+// ------------------------------------------------------------------------
+struct DctArgs {
+    int N, N1, N2, K1h;
+    int C1, C;  // lines c = c1 + C1*c2
+    long long in_base, in_c1, in_c2;
+    long long out_base, out_c1, out_c2;
+    double scale;  // inverse only: 1/(N*M)
+    const double2 *w1, *w2, *wN, *rot;
+};
+
+__device__ __forceinline__ double2 cmul(double2 a, double2 b) {
+    return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ double2 cmulc(double2 a, double2 b) {  // a * conj(b)
+    return make_double2(a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y);
+}
+
+template <int LB>
+__global__ __launch_bounds__(256) void k_dct_fwd(DctArgs a, const double *__restrict__ in,
+                                                 double *__restrict__ out) {
+    extern __shared__ double smem[];
+    const int N = a.N, N1 = a.N1, N2 = a.N2, K1 = a.K1h + 1;
+    double *v = smem;                                    // LB*N
+    double2 *B = reinterpret_cast<double2 *>(v + LB * N);  // LB*K1*N2
+    double2 *w1 = B + LB * K1 * N2;                      // N1
+    double2 *w2 = w1 + N1;                               // N2
+    const int tid = threadIdx.x;
+    for (int q = tid; q < N1; q += 256) w1[q] = a.w1[q];
+    for (int q = tid; q < N2; q += 256) w2[q] = a.w2[q];
+    const int line0 = blockIdx.x * LB;
+    // load and reorder: v[n/2] = f[n] (n even), v[N-1-(n-1)/2] = f[n] (n odd)
+    for (int idx = tid; idx < LB * N; idx += 256) {
+        const int l = idx / N, n = idx - l * N;
+        const int c = min(line0 + l, a.C - 1);
+        const int c1 = c % a.C1, c2 = c / a.C1;
+        const double f = in[a.in_base + c1 * a.in_c1 + c2 * a.in_c2 + n];
+        v[l * N + ((n & 1) ? N - 1 - (n >> 1) : (n >> 1))] = f;
+    }
+    __syncthreads();
+    // step A: B[k1][n2] = W_N^{n2 k1} sum_n1 v[N2 n1 + n2] W_N1^{n1 k1}
+    for (int idx = tid; idx < LB * K1 * N2; idx += 256) {
+        const int l = idx / (K1 * N2), r = idx - l * K1 * N2;
+        const int k1 = r / N2, n2 = r - k1 * N2;
+        const double *vl = v + l * N + n2;
+        double re = 0.0, im = 0.0;
+        int q = 0;
+        for (int n1 = 0; n1 < N1; ++n1) {
+            const double x = vl[n1 * N2];
+            const double2 w = w1[q];
+            re = fma(x, w.x, re);
+            im = fma(x, w.y, im);
+            q += k1;
+            if (q >= N1) q -= N1;
+        }
+        B[idx] = cmul(make_double2(re, im), a.wN[(n2 * k1) % N]);
+    }
+    __syncthreads();
+    // step C: V[k1 + N1 k2] = sum_n2 B[k1][n2] W_N2^{n2 k2}; X[k] = 2 Re(rot_k V),
+    // X[N-k] = -2 Im(rot_k V)
+    for (int idx = tid; idx < LB * K1 * N2; idx += 256) {
+        const int l = idx / (K1 * N2), r = idx - l * K1 * N2;
+        const int k1 = r / N2, k2 = r - k1 * N2;
+        const double2 *Bl = B + l * K1 * N2 + k1 * N2;
+        double re = 0.0, im = 0.0;
+        int q = 0;
+        for (int n2 = 0; n2 < N2; ++n2) {
+            const double2 b = Bl[n2], w = w2[q];
+            re = fma(b.x, w.x, fma(-b.y, w.y, re));
+            im = fma(b.x, w.y, fma(b.y, w.x, im));
+            q += k2;
+            if (q >= N2) q -= N2;
+        }
+        const int k = k1 + N1 * k2;
+        const double2 wv = cmul(a.rot[k], make_double2(re, im));
+        v[l * N + k] = 2.0 * wv.x;
+        if (k > 0) v[l * N + N - k] = -2.0 * wv.y;
+    }
+    __syncthreads();
+    for (int idx = tid; idx < LB * N; idx += 256) {
+        const int l = idx / N, kx = idx - l * N;
+        const int c = line0 + l;
+        if (c >= a.C) continue;
+        const int c1 = c % a.C1, c2 = c / a.C1;
+        out[a.out_base + c1 * a.out_c1 + c2 * a.out_c2 + kx] = v[l * N + kx];
+    }
+}
+
+template <int LB>
+__global__ __launch_bounds__(256) void k_dct_inv(DctArgs a, const double *__restrict__ in,
+                                                 double *__restrict__ out) {
+    extern __shared__ double smem[];
+    const int N = a.N, N1 = a.N1, N2 = a.N2, K1h = a.K1h, K1 = K1h + 1;
+    double *g = smem;                                    // LB*N (later the output line)
+    double2 *C = reinterpret_cast<double2 *>(g + LB * N);  // LB*K1*N2
+    double2 *w1 = C + LB * K1 * N2;
+    double2 *w2 = w1 + N1;
+    const int tid = threadIdx.x;
+    for (int q = tid; q < N1; q += 256) w1[q] = a.w1[q];
+    for (int q = tid; q < N2; q += 256) w2[q] = a.w2[q];
+    const int line0 = blockIdx.x * LB;
+    for (int idx = tid; idx < LB * N; idx += 256) {
+        const int l = idx / N, k = idx - l * N;
+        const int c = min(line0 + l, a.C - 1);
+        const int c1 = c % a.C1, c2 = c / a.C1;
+        g[idx] = in[a.in_base + c1 * a.in_c1 + c2 * a.in_c2 + k];
+    }
+    __syncthreads();
+    // step C': C~[k1][n2] = W_N^{-n2 k1} sum_k2 V[k1 + N1 k2] W_N2^{-n2 k2},
+    // V[k] = conj(rot_k) (g[k] - i g[N-k]) / 2
+    for (int idx = tid; idx < LB * K1 * N2; idx += 256) {
+        const int l = idx / (K1 * N2), r = idx - l * K1 * N2;
+        const int k1 = r / N2, n2 = r - k1 * N2;
+        const double *gl = g + l * N;
+        double re = 0.0, im = 0.0;
+        int q = 0;
+        for (int k2 = 0; k2 < N2; ++k2) {
+            const int k = k1 + N1 * k2;
+            const double gk = gl[k], gm = (k == 0) ? 0.0 : gl[N - k];
+            const double2 V = cmulc(make_double2(0.5 * gk, -0.5 * gm), a.rot[k]);
+            const double2 w = w2[q];  // use conj(w)
+            re = fma(V.x, w.x, fma(V.y, w.y, re));
+            im = fma(V.y, w.x, fma(-V.x, w.y, im));
+            q += n2;
+            if (q >= N2) q -= N2;
+        }
+        C[idx] = cmulc(make_double2(re, im), a.wN[(n2 * k1) % N]);
+    }
+    __syncthreads();
+    // step A': v[N2 n1 + n2] = (1/N) (Re C~[0] + sum_{k1>=1} c_k1 Re(C~[k1] W_N1^{-n1 k1}))
+    for (int idx = tid; idx < LB * N; idx += 256) {
+        const int l = idx / N, n = idx - l * N;
+        const int n1 = n / N2, n2 = n - n1 * N2;
+        const double2 *Cl = C + l * K1 * N2 + n2;
+        double acc = Cl[0].x;
+        int q = 0;
+        for (int k1 = 1; k1 <= K1h; ++k1) {
+            q += n1;
+            if (q >= N1) q -= N1;
+            const double2 c = Cl[k1 * N2], w = w1[q];
+            const double t = c.x * w.x + c.y * w.y;  // Re(c * conj(w))
+            acc += ((N1 % 2 == 0) && k1 == K1h) ? t : 2.0 * t;
+        }
+        const int pos = (n < (N + 1) / 2) ? 2 * n : 2 * (N - 1 - n) + 1;
+        g[l * N + pos] = acc * a.scale;
+    }
+    __syncthreads();
+    for (int idx = tid; idx < LB * N; idx += 256) {
+        const int l = idx / N, n = idx - l * N;
+        const int c = line0 + l;
+        if (c >= a.C) continue;
+        const int c1 = c % a.C1, c2 = c / a.C1;
+        out[a.out_base + c1 * a.out_c1 + c2 * a.out_c2 + n] = g[idx];
+    }
+}
+
 // Thomas sweep with precomputed factors, one (kx, kz) per lane; the Re and
 // Im rows of the half spectrum share the factors (projection.f90:137-150).
 __global__ __launch_bounds__(256) void k_thomas(Dims d, const double *__restrict__ dl,
@@ -401,13 +577,18 @@ __global__ __launch_bounds__(256) void k_thomas(Dims d, const double *__restrict
 // ------------------------------------------------------------------------
 // Reductions
 // ------------------------------------------------------------------------
-__device__ __forceinline__ void atomic_max_pos(double *addr, double v) {
-    // non-negative doubles order like their bit patterns
-    atomicMax(reinterpret_cast<unsigned long long *>(addr), (unsigned long long)__double_as_longlong(v));
+// block max of non-negative values; one atomic per block
+__device__ __forceinline__ double block_max_256(double v, double *sh) {
+    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_down(v, o));
+    const int t = threadIdx.x + blockDim.x * threadIdx.y;
+    if ((t & 63) == 0) sh[t >> 6] = v;
+    __syncthreads();
+    return fmax(fmax(sh[0], sh[1]), fmax(sh[2], sh[3]));
 }
 
 __global__ __launch_bounds__(256) void k_maxdiv(Dims d, Metrics m, const double *U, const double *V,
                                                 const double *W, double *out) {
+    __shared__ double sh[4];
     const int i = blockIdx.x * kWave + threadIdx.x + 1;
     const int j = blockIdx.y * 4 + threadIdx.y + 1;
     const int k = blockIdx.z + 1;
@@ -417,12 +598,14 @@ __global__ __launch_bounds__(256) void k_maxdiv(Dims d, Metrics m, const double 
         v = fabs((U[p] - U[p - 1]) * m.rdx[i] + (V[p] - V[p - d.sx]) * m.rdy[j] +
                  (W[p] - W[p - d.sxy]) * m.rdz[k]);
     }
-    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_down(v, o));
-    if ((threadIdx.x & 63) == 0) atomic_max_pos(out, v);
+    v = block_max_256(v, sh);
+    const long long bid = blockIdx.x + (long long)gridDim.x * (blockIdx.y + (long long)gridDim.y * blockIdx.z);
+    if (threadIdx.x == 0 && threadIdx.y == 0) out[bid] = v;
 }
 
 __global__ __launch_bounds__(256) void k_dtconv(Dims d, Metrics m, const double *U, const double *V,
                                                 const double *W, double *out) {
+    __shared__ double sh[3][4];
     const int i = blockIdx.x * kWave + threadIdx.x + 1;
     const int j = blockIdx.y * 4 + threadIdx.y + 1;
     const int k = blockIdx.z + 1;
@@ -433,15 +616,32 @@ __global__ __launch_bounds__(256) void k_dtconv(Dims d, Metrics m, const double 
         b = (m.yg[j + 1] - m.yg[j]) / fabs(V[p]);
         c = (m.zg[k + 1] - m.zg[k]) / fabs(W[p]);
     }
-    for (int o = 32; o > 0; o >>= 1) {
-        a = fmax(a, __shfl_down(a, o));
-        b = fmax(b, __shfl_down(b, o));
-        c = fmax(c, __shfl_down(c, o));
+    a = block_max_256(a, sh[0]);
+    b = block_max_256(b, sh[1]);
+    c = block_max_256(c, sh[2]);
+    const long long bid = blockIdx.x + (long long)gridDim.x * (blockIdx.y + (long long)gridDim.y * blockIdx.z);
+    if (threadIdx.x == 0 && threadIdx.y == 0) {
+        out[3 * bid + 0] = a;
+        out[3 * bid + 1] = b;
+        out[3 * bid + 2] = c;
     }
-    if ((threadIdx.x & 63) == 0) {
-        atomic_max_pos(out + 0, a);
-        atomic_max_pos(out + 1, b);
-        atomic_max_pos(out + 2, c);
+}
+
+// max over n partial values part[c + stride*q] for c < ncomp, one block
+__global__ __launch_bounds__(1024) void k_reduce_max(const double *part, long long n, int ncomp,
+                                                     double *out) {
+    __shared__ double sh[1024];
+    for (int c = 0; c < ncomp; ++c) {
+        double v = 0.0;
+        for (long long q = threadIdx.x; q < n; q += 1024) v = fmax(v, part[c + ncomp * q]);
+        sh[threadIdx.x] = v;
+        __syncthreads();
+        for (int st = 512; st > 0; st >>= 1) {
+            if ((int)threadIdx.x < st) sh[threadIdx.x] = fmax(sh[threadIdx.x], sh[threadIdx.x + st]);
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) out[c] = sh[0];
+        __syncthreads();
     }
 }
 
@@ -529,6 +729,25 @@ void xform_launch(Handle &h, int kid, const XformArgs &a, const double *AT, cons
     check_launch();
 }
 
+constexpr int kDctLines = 2;
+
+void dct_launch(Handle &h, int kid, const DctArgs &a, bool forward, const double *in, double *out) {
+    ProfScope ps(h, kid);
+    const int K1 = a.K1h + 1;
+    const size_t lds = sizeof(double) * (size_t)kDctLines * a.N +
+                       sizeof(double2) * ((size_t)kDctLines * K1 * a.N2 + a.N1 + a.N2);
+    if (lds > 65536) {
+        last_error() = "x transform length too large for the LDS-resident DCT (N = " + std::to_string(a.N) + ")";
+        throw ArgError();
+    }
+    const dim3 grid((a.C + kDctLines - 1) / kDctLines);
+    if (forward)
+        hipLaunchKernelGGL(k_dct_fwd<kDctLines>, grid, dim3(256), lds, h.stream, a, in, out);
+    else
+        hipLaunchKernelGGL(k_dct_inv<kDctLines>, grid, dim3(256), lds, h.stream, a, in, out);
+    check_launch();
+}
+
 }  // namespace
 
 const char *kernel_name(int id) {
@@ -586,8 +805,11 @@ void launch_bc(Handle &h, Field3 &f, const Field3 &old, int rk_step, double dt, 
     const Dims &d = h.d;
     const double coef = h.rk_t[rk_step] * dt * h.bc_uc;
     prof_begin(h, KID_BC_INOUT);
-    hipLaunchKernelGGL(k_bc_inout, dim3((d.nyg + 255) / 256, d.nzg), dim3(256), 0, h.stream, d, h.m,
-                       h.bc, f.u, f.v, f.w, old.u, old.v, old.w, coef, t);
+    const dim3 g_in((d.nyg + 255) / 256, d.nzg), g_tw((d.nxg + 255) / 256, d.nzg);
+    const int nq_in = g_in.x * g_in.y, nq_tw = g_tw.x * g_tw.y;
+    double *qpart = h.red;
+    hipLaunchKernelGGL(k_bc_inout, g_in, dim3(256), 0, h.stream, d, h.m, h.bc, f.u, f.v, f.w, old.u,
+                       old.v, old.w, coef, t, qpart);
     check_launch();
     prof_end(h, KID_BC_INOUT);
     prof_begin(h, KID_BC_PERIODIC);
@@ -596,12 +818,13 @@ void launch_bc(Handle &h, Field3 &f, const Field3 &old, int rk_step, double dt, 
     check_launch();
     prof_end(h, KID_BC_PERIODIC);
     prof_begin(h, KID_BC_TOPWALL);
-    hipLaunchKernelGGL(k_bc_topwall, dim3((d.nxg + 255) / 256, d.nzg), dim3(256), 0, h.stream, d,
-                       h.bc, f.u, f.v, f.w);
+    hipLaunchKernelGGL(k_bc_topwall, g_tw, dim3(256), 0, h.stream, d, h.m, h.bc, f.u, f.v, f.w,
+                       qpart + nq_in);
     check_launch();
     prof_end(h, KID_BC_TOPWALL);
     prof_begin(h, KID_BC_MASSFLUX);
-    hipLaunchKernelGGL(k_bc_massflux, dim3(1), dim3(1024), 0, h.stream, d, h.m, h.length_y, f.u, f.v);
+    hipLaunchKernelGGL(k_bc_massflux, g_in, dim3(256), 0, h.stream, d, h.length_y, qpart,
+                       nq_in + nq_tw, f.u);
     check_launch();
     prof_end(h, KID_BC_MASSFLUX);
 }
@@ -619,21 +842,26 @@ void launch_poisson(Handle &h, double *phi) {
     const int N = d.N, M = d.M, NM = d.NM;
     const long long plane = (long long)N * NM;
     XformArgs a{};
-    // forward x: columns (j, m) of the padded field -> T1[m][j][kx]
-    a.nrows = N;
-    a.nsum = N;
-    a.lda = h.lda_x;
-    a.C1 = NM;
-    a.C = NM * M;
-    a.in_base = fidx(d, 1, 1, 1);
-    a.in_s = 1;
-    a.in_c1 = d.sx;
-    a.in_c2 = d.sxy;
-    a.out_base = 0;
-    a.out_r = 1;
-    a.out_c1 = N;
-    a.out_c2 = plane;
-    xform_launch<16>(h, KID_XFORM_X_FWD, a, h.AxF, phi, h.T1);
+    // forward x: lines (j, m) of the padded field -> T1[m][j][kx]
+    DctArgs da{};
+    da.N = N;
+    da.N1 = h.dct_n1;
+    da.N2 = N / h.dct_n1;
+    da.K1h = h.dct_n1 / 2;
+    da.C1 = NM;
+    da.C = NM * M;
+    da.w1 = reinterpret_cast<const double2 *>(h.dct_w1);
+    da.w2 = reinterpret_cast<const double2 *>(h.dct_w2);
+    da.wN = reinterpret_cast<const double2 *>(h.dct_wN);
+    da.rot = reinterpret_cast<const double2 *>(h.dct_rot);
+    da.in_base = fidx(d, 1, 1, 1);
+    da.in_c1 = d.sx;
+    da.in_c2 = d.sxy;
+    da.out_base = 0;
+    da.out_c1 = N;
+    da.out_c2 = plane;
+    da.scale = 1.0;
+    dct_launch(h, KID_XFORM_X_FWD, da, true, phi, h.T1);
     // forward z: columns (kx, j) of T1 -> S[q][j][kx]
     a.nrows = M;
     a.nsum = M;
@@ -658,20 +886,14 @@ void launch_poisson(Handle &h, double *phi) {
     // inverse z: S -> T1
     xform_launch<16>(h, KID_XFORM_Z_INV, a, h.AzI, h.S, h.T1);
     // inverse x: T1[m][j][kx] -> phi cells (1..N, 1..NM, 1..M)
-    a.nrows = N;
-    a.nsum = N;
-    a.lda = h.lda_x;
-    a.C1 = NM;
-    a.C = NM * M;
-    a.in_base = 0;
-    a.in_s = 1;
-    a.in_c1 = N;
-    a.in_c2 = plane;
-    a.out_base = fidx(d, 1, 1, 1);
-    a.out_r = 1;
-    a.out_c1 = d.sx;
-    a.out_c2 = d.sxy;
-    xform_launch<16>(h, KID_XFORM_X_INV, a, h.AxI, h.T1, phi);
+    da.in_base = 0;
+    da.in_c1 = N;
+    da.in_c2 = plane;
+    da.out_base = fidx(d, 1, 1, 1);
+    da.out_c1 = d.sx;
+    da.out_c2 = d.sxy;
+    da.scale = 1.0 / ((double)N * (double)M);
+    dct_launch(h, KID_XFORM_X_INV, da, false, h.T1, phi);
 }
 
 void launch_project(Handle &h, Field3 &f, const double *phi) {
@@ -693,9 +915,11 @@ void launch_pressure(Handle &h, const double *phi, double *P, double dt) {
 
 double max_divergence(Handle &h, const Field3 &f) {
     const Dims &d = h.d;
-    BL_HIP(hipMemsetAsync(h.scal, 0, sizeof(double), h.stream));
-    hipLaunchKernelGGL(k_maxdiv, interior_grid(d), dim3(kWave, 4), 0, h.stream, d, h.m, f.u, f.v, f.w,
-                       h.scal);
+    const dim3 g = interior_grid(d);
+    const long long nb = (long long)g.x * g.y * g.z;
+    hipLaunchKernelGGL(k_maxdiv, g, dim3(kWave, 4), 0, h.stream, d, h.m, f.u, f.v, f.w, h.red);
+    check_launch();
+    hipLaunchKernelGGL(k_reduce_max, dim3(1), dim3(1024), 0, h.stream, h.red, nb, 1, h.scal);
     check_launch();
     double v = 0.0;
     BL_HIP(hipMemcpyAsync(&v, h.scal, sizeof(double), hipMemcpyDeviceToHost, h.stream));
@@ -706,9 +930,11 @@ double max_divergence(Handle &h, const Field3 &f) {
 double dt_convective(Handle &h, const Field3 &f) {
     const Dims &d = h.d;
     ProfScope ps(h, KID_REDUCE);
-    BL_HIP(hipMemsetAsync(h.scal, 0, 3 * sizeof(double), h.stream));
-    hipLaunchKernelGGL(k_dtconv, interior_grid(d), dim3(kWave, 4), 0, h.stream, d, h.m, f.u, f.v, f.w,
-                       h.scal);
+    const dim3 g = interior_grid(d);
+    const long long nb = (long long)g.x * g.y * g.z;
+    hipLaunchKernelGGL(k_dtconv, g, dim3(kWave, 4), 0, h.stream, d, h.m, f.u, f.v, f.w, h.red);
+    check_launch();
+    hipLaunchKernelGGL(k_reduce_max, dim3(1), dim3(1024), 0, h.stream, h.red, nb, 3, h.scal);
     check_launch();
     double v[3];
     BL_HIP(hipMemcpyAsync(v, h.scal, 3 * sizeof(double), hipMemcpyDeviceToHost, h.stream));
