@@ -316,34 +316,7 @@ void build(Handle &h, const bl_config *c) {
     h.T1 = dmalloc(nspec);
     h.S = dmalloc(nspec);
 
-    // ---- transform matrices, stored transposed AT[s][r] with padded rows
     const int N = d.N, M = d.M;
-    h.lda_x = round_up(N, 64);
-    h.lda_z = round_up(M, 64);
-    std::vector<double> axf((size_t)N * h.lda_x, 0.0), axi((size_t)N * h.lda_x, 0.0);
-    for (int kx = 0; kx < N; ++kx)
-        for (int n = 0; n < N; ++n) {
-            // angle pi*kx*(2n+1)/(2N), reduced exactly modulo 4N
-            const long long a = ((long long)kx * (2 * n + 1)) % (4LL * N);
-            const long double cv = cosl(kPi * (long double)a / (2.0L * N));
-            axf[(size_t)n * h.lda_x + kx] = (double)(2.0L * cv);  // DCT-II x2
-            const long double w = kx == 0 ? 0.5L : 1.0L;
-            axi[(size_t)kx * h.lda_x + n] = (double)(w * cv / ((long double)N * M));
-        }
-    std::vector<double> azf((size_t)M * h.lda_z, 0.0), azi((size_t)M * h.lda_z, 0.0);
-    for (int q = 0; q < M; ++q) {
-        const int kz = (q + 1) / 2;
-        const bool im = q > 0 && (q % 2 == 0);
-        const bool nyq = (M % 2 == 0) && kz == M / 2;
-        for (int m = 0; m < M; ++m) {
-            const long long a = ((long long)kz * m) % M;
-            const long double ang = 2.0L * kPi * (long double)a / (long double)M;
-            const long double fv = q == 0 ? 1.0L : (im ? -sinl(ang) : cosl(ang));
-            const long double iv = q == 0 ? 1.0L : (nyq ? cosl(ang) : 2.0L * fv);
-            azf[(size_t)m * h.lda_z + q] = (double)fv;
-            azi[(size_t)q * h.lda_z + m] = (double)iv;
-        }
-    }
     // x-DCT tables: N = N1*N2 with N1 the largest divisor <= sqrt(N)
     {
         int n1 = 1;
@@ -371,15 +344,22 @@ void build(Handle &h, const bl_config *c) {
         upload_vec(h.dct_wN, wN, h.stream);
         upload_vec(h.dct_rot, rot, h.stream);
     }
-    h.AxF = dmalloc(axf.size());
-    h.AxI = dmalloc(axi.size());
-    h.AzF = dmalloc(azf.size());
-    h.AzI = dmalloc(azi.size());
-    upload_vec(h.AxF, axf, h.stream);
-    upload_vec(h.AxI, axi, h.stream);
-    upload_vec(h.AzF, azf, h.stream);
-    upload_vec(h.AzI, azi, h.stream);
-
+    // z cos/sin tables, symmetric in (k, m): lda covers the padded row blocks
+    {
+        h.lda_zt = round_up(M + 1, 64);
+        std::vector<double> zc((size_t)h.lda_zt * h.lda_zt, 0.0), zs((size_t)h.lda_zt * h.lda_zt, 0.0);
+        for (int m = 0; m < M; ++m)
+            for (int k = 0; k < M; ++k) {
+                const long long r = ((long long)k * m) % M;
+                const long double ang = 2.0L * kPi * (long double)r / (long double)M;
+                zc[(size_t)m * h.lda_zt + k] = (double)cosl(ang);
+                zs[(size_t)m * h.lda_zt + k] = (double)sinl(ang);
+            }
+        h.zc = dmalloc(zc.size());
+        h.zs = dmalloc(zs.size());
+        upload_vec(h.zc, zc, h.stream);
+        upload_vec(h.zs, zs, h.stream);
+    }
     // ---- Thomas factors on the half spectrum (initialization.f90:328-344, 427-469)
     const int NM = d.NM, Kz = d.Kz;
     std::vector<double> kxx(N), kzz(Kz);
@@ -435,9 +415,8 @@ void free_all(Handle &h) {
         f(h.buf[b].w);
     }
     for (void *p : {(void *)h.F1.u, (void *)h.F1.v, (void *)h.F1.w, (void *)h.P, (void *)h.phi,
-                    (void *)h.nu_t, (void *)h.T1, (void *)h.S, (void *)h.AxF, (void *)h.AxI,
-                    (void *)h.AzF, (void *)h.AzI, (void *)h.dct_w1, (void *)h.dct_w2,
-                    (void *)h.dct_wN, (void *)h.dct_rot, (void *)h.lu_dl, (void *)h.lu_dp, (void *)h.lu_du,
+                    (void *)h.nu_t, (void *)h.T1, (void *)h.S, (void *)h.dct_w1, (void *)h.dct_w2,
+                    (void *)h.dct_wN, (void *)h.dct_rot, (void *)h.zc, (void *)h.zs, (void *)h.lu_dl, (void *)h.lu_dp, (void *)h.lu_du,
                     (void *)h.scal, (void *)h.red, h.mem_grid})
         f(p);
     if (h.stream) (void)hipStreamSynchronize(h.stream);
@@ -447,8 +426,7 @@ void free_all(Handle &h) {
 
 // one projection sub-step on the current buffer (projection.f90:21-87)
 void projection_on(Handle &h, Field3 &f, int rk_step, double dt) {
-    launch_divergence(h, f, h.phi);
-    launch_poisson(h, h.phi);
+    launch_poisson(h, h.phi, &f);
     launch_project(h, f, h.phi);
     if (rk_step == 1) launch_pressure(h, h.phi, h.P, dt);
 }

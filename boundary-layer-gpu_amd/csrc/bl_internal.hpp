@@ -79,7 +79,6 @@ struct Handle {
     double length_y;
     double bc_uc;      // U_top(nx): outflow convection speed
     double dxmin, dymin, dzmin;  // smallest centre spacings (initialization.f90:220-222)
-    int lda_x, lda_z;  // leading dimensions of the transposed transform matrices
     double rk_t[4];
     double rk_coef[4][4];
     double rk2_t[3];
@@ -93,8 +92,9 @@ struct Handle {
     Field3 F1;      // first-stage RHS (RK3)
     double *P, *phi, *nu_t;
     double *T1, *S;  // transform intermediates: [m][j][kx] and [q][j][kx]
-    double *AxF, *AzF, *AzI, *AxI;  // transposed transform matrices
     int dct_n1;                      // x-DCT factorisation N = dct_n1 * (N / dct_n1)
+    int lda_zt;                      // leading dimension of the z cos/sin tables
+    double *zc, *zs;                 // z tables cos/sin(2 pi k m / M), [m][k]
     double *dct_w1, *dct_w2, *dct_wN, *dct_rot;  // x-DCT twiddle tables (complex)
     double *lu_dl, *lu_dp, *lu_du;  // Thomas factors [k][j][kx], [k][j][kx], [j]
     double *scal;                   // device scalars (mass-flux correction, reductions)
@@ -121,7 +121,8 @@ void launch_rk_stage(Handle &h, const Field3 &cur, const Field3 &old, Field3 &nx
                      double dt);
 void launch_bc(Handle &h, Field3 &f, const Field3 &old, int rk_step, double dt, double t);
 void launch_divergence(Handle &h, const Field3 &f, double *out);
-void launch_poisson(Handle &h, double *phi_inout);
+// div != null: the rhs is the divergence of *div (computed inside the x transform)
+void launch_poisson(Handle &h, double *phi_inout, const Field3 *div = nullptr);
 void launch_project(Handle &h, Field3 &f, const double *phi);
 void launch_pressure(Handle &h, const double *phi, double *P, double dt);
 void launch_rhs_only(Handle &h, const Field3 &cur, Field3 &rhs);

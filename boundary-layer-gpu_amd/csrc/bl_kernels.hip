@@ -1,10 +1,10 @@
 // HIP kernels for the per-time-step path (gfx950, FP64 mode).
 //
 // Reference correspondence (0-based indices here, the reference is 1-based):
-//   k_rk_stage      equations.f90:24-415 fused with time_integration.f90:351-458
+//   k_rk_march      equations.f90:24-415 fused with time_integration.f90:351-458
 //   k_bc_*          boundary_conditions.f90:127-262
 //   k_divergence    projection.f90:34-49
-//   k_xform         the DCT-II/DFT stages of solve_poisson_gpu projection.f90:110-202
+//   k_dct_*, k_zdft_*  the DCT-II/DFT stages of solve_poisson_gpu projection.f90:110-202
 //   k_thomas        projection.f90:137-150
 //   k_project       projection.f90:355-386 with apply_periodic_z_pressure :293-303
 //   k_pressure      projection.f90:75-85
@@ -26,100 +26,166 @@ constexpr int kWave = 64;
 __device__ __forceinline__ double sq(double a) { return a * a; }
 
 // ------------------------------------------------------------------------
-// Fused RHS + Runge-Kutta stage update.
-// MODE 0: new = old + c0*F            (first stage, and RK2 second stage)
+// Fused RHS + Runge-Kutta stage update, marching in z.
+// MODE 0: new = old + c0*F              (first stage, and RK2 second stage)
 // MODE 1: new = old + dt*(c1*F1 + c0*F) (RK3 stages 2 and 3)
-// MODE 2: write F only (operator test)
-// One thread per (i,j,k) of the union of the U, V, W interiors.
+// MODE 2: write F only (operator test)  A block owns a
+// 64 (x) x 8 (y) column of cells and a chunk of z planes; U, V, W of planes
+// k-1, k, k+1 live in an LDS ring with a one-cell halo in x and y, so every
+// velocity value is read from HBM once per chunk (plus halo) instead of once
+// per neighbour.  Loads are x-fastest and 512-byte aligned rows.
+// Thread (tx, ty) computes cells (i0+tx, j0+2ty) and (i0+tx, j0+2ty+1).
 // ------------------------------------------------------------------------
-template <int MODE, bool WRITE_F1>
-__global__ __launch_bounds__(256) void k_rk_stage(
-    Dims d, Metrics m, double nu, double dPdx, double dPdz, const double *__restrict__ U,
-    const double *__restrict__ V, const double *__restrict__ W, const double *__restrict__ Uo,
-    const double *__restrict__ Vo, const double *__restrict__ Wo, const double *__restrict__ F1u,
-    const double *__restrict__ F1v, const double *__restrict__ F1w, double *__restrict__ Un,
-    double *__restrict__ Vn, double *__restrict__ Wn, double *__restrict__ Fou,
-    double *__restrict__ Fov, double *__restrict__ Fow, double c0, double c1, double dt) {
-    const int i = blockIdx.x * kWave + threadIdx.x + 1;
-    const int j = blockIdx.y * 4 + threadIdx.y + 1;
-    const int k = blockIdx.z + 1;
-    if (i > d.nxg - 2 || j > d.nyg - 2) return;
-    const long long sx = d.sx, sxy = d.sxy;
-    const long long p = fidx(d, i, j, k);
+constexpr int kTX = 64, kTY = 8, kHX = kTX + 2, kHY = kTY + 2, kPlane = kHX * kHY;
 
-    // ---- u: i in [1,nx-2], j in [1,nyg-2], k in [1,nzg-2]  (equations.f90:47-94, 149-154)
-    if (i <= d.nx - 2) {
-        double r = 0.0;
-        if (i < d.nx - 2) {
-            const double up = U[p], ue = U[p + 1], uw = U[p - 1];
-            const double un = U[p + sx], us = U[p - sx], ut = U[p + sxy], ub = U[p - sxy];
-            r = -(sq(0.5 * (up + ue)) - sq(0.5 * (uw + up))) * m.rdxg[i];
-            r = r - ((m.wy0[j] * up + m.wy1[j] * un) * 0.5 * (V[p] + V[p + 1]) -
-                     (m.wy0[j - 1] * us + m.wy1[j - 1] * up) * 0.5 * (V[p - sx] + V[p + 1 - sx])) *
-                        m.rdy[j];
-            r = r - (0.5 * (up + ut) * 0.5 * (W[p] + W[p + 1]) -
-                     0.5 * (ub + up) * 0.5 * (W[p - sxy] + W[p + 1 - sxy])) *
-                        m.rdz[k];
-            r = r + dPdx +
-                nu * ((m.rdx[i + 1] * (ue - up) - m.rdx[i] * (up - uw)) * m.rdxg[i] +
-                      ((un - up) * m.rdyg[j] - (up - us) * m.rdyg[j - 1]) * m.rdy[j] +
-                      (m.rdzg[k] * (ut - up) - m.rdzg[k - 1] * (up - ub)) * m.rdz[k]);
-        }
-        if (MODE == 2) {
-            Fou[p] = r;
-        } else {
-            if (WRITE_F1) Fou[p] = r;
-            Un[p] = (MODE == 0) ? Uo[p] + c0 * r : Uo[p] + dt * (c1 * F1u[p] + c0 * r);
-        }
+template <int MODE, bool WRITE_F1>
+__global__ __launch_bounds__(256) void k_rk_march(
+    Dims d, Metrics m, double nu, double dPdx, double dPdz, int kchunk,
+    const double *__restrict__ U, const double *__restrict__ V, const double *__restrict__ W,
+    const double *__restrict__ Uo, const double *__restrict__ Vo, const double *__restrict__ Wo,
+    const double *__restrict__ F1u, const double *__restrict__ F1v, const double *__restrict__ F1w,
+    double *__restrict__ Un, double *__restrict__ Vn, double *__restrict__ Wn, double *__restrict__ Fou,
+    double *__restrict__ Fov, double *__restrict__ Fow, double c0, double c1, double dt) {
+    __shared__ double sU[3][kPlane], sV[3][kPlane], sW[3][kPlane];
+    const int tid = threadIdx.x;
+    const int tx = tid & 63, ty = tid >> 6;
+    const int i0 = blockIdx.x * kTX + 1, j0 = blockIdx.y * kTY + 1;
+    const int kb = 1 + blockIdx.z * kchunk;
+    const int ke = min(kb + kchunk, d.nzg - 1);  // exclusive
+    const long long sx = d.sx, sxy = d.sxy;
+    // halo tile origin (i0-1, j0-1); loads clamped into the allocation
+    // each thread moves up to kPer halo elements of a plane; they are fetched
+    // into registers one iteration ahead and committed to LDS after the sync
+    constexpr int kPer = (kPlane + 255) / 256;
+    long long off[kPer];
+#pragma unroll
+    for (int q = 0; q < kPer; ++q) {
+        const int e = tid + 256 * q;
+        const int ee = e < kPlane ? e : kPlane - 1;
+        const int hy = ee / kHX, hx = ee - hy * kHX;
+        const int ii = min(i0 - 1 + hx, d.sx - 1), jj = min(j0 - 1 + hy, d.nyg - 1);
+        off[q] = (long long)jj * sx + ii;
     }
-    // ---- v: i in [1,nxg-2], j in [1,ny-2], k in [1,nzg-2]  (equations.f90:183-230, 287-292)
-    if (j <= d.ny - 2) {
-        double r = 0.0;
-        if (i < d.nxg - 2) {
-            const double vp = V[p], ve = V[p + 1], vw = V[p - 1];
-            const double vn = V[p + sx], vs = V[p - sx], vt = V[p + sxy], vb = V[p - sxy];
-            r = -(sq(0.5 * (vp + vn)) - sq(0.5 * (vs + vp))) * m.rdyg[j];
-            r = r - ((m.wy0[j] * U[p] + m.wy1[j] * U[p + sx]) * 0.5 * (vp + ve) -
-                     (m.wy0[j] * U[p - 1] + m.wy1[j] * U[p - 1 + sx]) * 0.5 * (vw + vp)) *
-                        m.rdx[i];
-            r = r - (0.5 * (vp + vt) * (m.wy0[j] * W[p] + m.wy1[j] * W[p + sx]) -
-                     0.5 * (vb + vp) * (m.wy0[j] * W[p - sxy] + m.wy1[j] * W[p + sx - sxy])) *
-                        m.rdz[k];
-            r = r + nu * ((m.rdxg[i] * (ve - vp) - m.rdxg[i - 1] * (vp - vw)) * m.rdx[i] +
-                          (m.rdy[j + 1] * (vn - vp) - m.rdy[j] * (vp - vs)) * m.rdyg[j] +
-                          (m.rdzg[k] * (vt - vp) - m.rdzg[k - 1] * (vp - vb)) * m.rdz[k]);
+    double ru[kPer], rv[kPer], rw[kPer];
+    auto fetch = [&](int k) {
+        const long long base = (long long)k * sxy;
+#pragma unroll
+        for (int q = 0; q < kPer; ++q) {
+            ru[q] = U[base + off[q]];
+            rv[q] = V[base + off[q]];
+            rw[q] = W[base + off[q]];
         }
-        if (MODE == 2) {
-            Fov[p] = r;
-        } else {
-            if (WRITE_F1) Fov[p] = r;
-            Vn[p] = (MODE == 0) ? Vo[p] + c0 * r : Vo[p] + dt * (c1 * F1v[p] + c0 * r);
+    };
+    auto commit = [&](int slot) {
+#pragma unroll
+        for (int q = 0; q < kPer; ++q) {
+            const int e = tid + 256 * q;
+            if (e < kPlane) {
+                sU[slot][e] = ru[q];
+                sV[slot][e] = rv[q];
+                sW[slot][e] = rw[q];
+            }
         }
-    }
-    // ---- w: i in [1,nxg-2], j in [1,nyg-2], k in [1,nz-2]  (equations.f90:323-366, 408-413)
-    if (k <= d.nz - 2) {
-        double r = 0.0;
-        if (i < d.nxg - 2) {
-            const double wp = W[p], we = W[p + 1], ww = W[p - 1];
-            const double wn = W[p + sx], ws = W[p - sx], wt = W[p + sxy], wb = W[p - sxy];
-            r = -(sq(0.5 * (wp + wt)) - sq(0.5 * (wb + wp))) * m.rdzg[k];
-            r = r - (0.5 * (U[p] + U[p + sxy]) * 0.5 * (wp + we) -
-                     0.5 * (U[p - 1] + U[p - 1 + sxy]) * 0.5 * (ww + wp)) *
-                        m.rdx[i];
-            r = r - (0.5 * (V[p] + V[p + sxy]) * (m.wy0[j] * wp + m.wy1[j] * wn) -
-                     0.5 * (V[p - sx] + V[p - sx + sxy]) * (m.wy0[j - 1] * ws + m.wy1[j - 1] * wp)) *
-                        m.rdy[j];
-            r = r + dPdz +
-                nu * ((m.rdxg[i] * (we - wp) - m.rdxg[i - 1] * (wp - ww)) * m.rdx[i] +
-                      ((wn - wp) * m.rdyg[j] - (wp - ws) * m.rdyg[j - 1]) * m.rdy[j] +
-                      (m.rdz[k + 1] * (wt - wp) - m.rdz[k] * (wp - wb)) * m.rdzg[k]);
+    };
+    fetch(kb - 1);
+    commit((kb - 1) % 3);
+    fetch(kb);
+    commit(kb % 3);
+    fetch(kb + 1);
+    const int i = i0 + tx;
+    for (int k = kb; k < ke; ++k) {
+        commit((k + 1) % 3);
+        __syncthreads();
+        if (k + 2 <= ke) fetch(k + 2);
+        const int sm = (k - 1) % 3, sc = k % 3, sp = (k + 1) % 3;
+        const double rdzk = m.rdz[k], rdzgk = m.rdzg[k], rdzgm = m.rdzg[k - 1];
+#pragma unroll
+        for (int r = 0; r < 2; ++r) {
+            const int jt = 2 * ty + r;       // row in the tile
+            const int j = j0 + jt;
+            if (i > d.nxg - 2 || j > d.nyg - 2) continue;
+            const int e = (jt + 1) * kHX + (tx + 1);  // LDS index of (i, j)
+            const long long p = fidx(d, i, j, k);
+            // ---- u (equations.f90:47-94, 149-154)
+            if (i <= d.nx - 2) {
+                double rr = 0.0;
+                if (i < d.nx - 2) {
+                    const double up = sU[sc][e], ue = sU[sc][e + 1], uw = sU[sc][e - 1];
+                    const double un = sU[sc][e + kHX], us = sU[sc][e - kHX];
+                    const double ut = sU[sp][e], ub = sU[sm][e];
+                    rr = -(sq(0.5 * (up + ue)) - sq(0.5 * (uw + up))) * m.rdxg[i];
+                    rr = rr - ((m.wy0[j] * up + m.wy1[j] * un) * 0.5 * (sV[sc][e] + sV[sc][e + 1]) -
+                               (m.wy0[j - 1] * us + m.wy1[j - 1] * up) * 0.5 *
+                                   (sV[sc][e - kHX] + sV[sc][e + 1 - kHX])) *
+                                  m.rdy[j];
+                    rr = rr - (0.5 * (up + ut) * 0.5 * (sW[sc][e] + sW[sc][e + 1]) -
+                               0.5 * (ub + up) * 0.5 * (sW[sm][e] + sW[sm][e + 1])) *
+                                  rdzk;
+                    rr = rr + dPdx +
+                         nu * ((m.rdx[i + 1] * (ue - up) - m.rdx[i] * (up - uw)) * m.rdxg[i] +
+                               ((un - up) * m.rdyg[j] - (up - us) * m.rdyg[j - 1]) * m.rdy[j] +
+                               (rdzgk * (ut - up) - rdzgm * (up - ub)) * rdzk);
+                }
+                if (MODE == 2) {
+                    Fou[p] = rr;
+                } else {
+                    if (WRITE_F1) Fou[p] = rr;
+                    Un[p] = (MODE == 0) ? Uo[p] + c0 * rr : Uo[p] + dt * (c1 * F1u[p] + c0 * rr);
+                }
+            }
+            // ---- v (equations.f90:183-230, 287-292)
+            if (j <= d.ny - 2) {
+                double rr = 0.0;
+                if (i < d.nxg - 2) {
+                    const double vp = sV[sc][e], ve = sV[sc][e + 1], vw = sV[sc][e - 1];
+                    const double vn = sV[sc][e + kHX], vs = sV[sc][e - kHX];
+                    const double vt = sV[sp][e], vb = sV[sm][e];
+                    rr = -(sq(0.5 * (vp + vn)) - sq(0.5 * (vs + vp))) * m.rdyg[j];
+                    rr = rr - ((m.wy0[j] * sU[sc][e] + m.wy1[j] * sU[sc][e + kHX]) * 0.5 * (vp + ve) -
+                               (m.wy0[j] * sU[sc][e - 1] + m.wy1[j] * sU[sc][e - 1 + kHX]) * 0.5 * (vw + vp)) *
+                                  m.rdx[i];
+                    rr = rr - (0.5 * (vp + vt) * (m.wy0[j] * sW[sc][e] + m.wy1[j] * sW[sc][e + kHX]) -
+                               0.5 * (vb + vp) * (m.wy0[j] * sW[sm][e] + m.wy1[j] * sW[sm][e + kHX])) *
+                                  rdzk;
+                    rr = rr + nu * ((m.rdxg[i] * (ve - vp) - m.rdxg[i - 1] * (vp - vw)) * m.rdx[i] +
+                                    (m.rdy[j + 1] * (vn - vp) - m.rdy[j] * (vp - vs)) * m.rdyg[j] +
+                                    (rdzgk * (vt - vp) - rdzgm * (vp - vb)) * rdzk);
+                }
+                if (MODE == 2) {
+                    Fov[p] = rr;
+                } else {
+                    if (WRITE_F1) Fov[p] = rr;
+                    Vn[p] = (MODE == 0) ? Vo[p] + c0 * rr : Vo[p] + dt * (c1 * F1v[p] + c0 * rr);
+                }
+            }
+            // ---- w (equations.f90:323-366, 408-413)
+            if (k <= d.nz - 2) {
+                double rr = 0.0;
+                if (i < d.nxg - 2) {
+                    const double wp = sW[sc][e], we = sW[sc][e + 1], ww = sW[sc][e - 1];
+                    const double wn = sW[sc][e + kHX], ws = sW[sc][e - kHX];
+                    const double wt = sW[sp][e], wb = sW[sm][e];
+                    rr = -(sq(0.5 * (wp + wt)) - sq(0.5 * (wb + wp))) * rdzgk;
+                    rr = rr - (0.5 * (sU[sc][e] + sU[sp][e]) * 0.5 * (wp + we) -
+                               0.5 * (sU[sc][e - 1] + sU[sp][e - 1]) * 0.5 * (ww + wp)) *
+                                  m.rdx[i];
+                    rr = rr - (0.5 * (sV[sc][e] + sV[sp][e]) * (m.wy0[j] * wp + m.wy1[j] * wn) -
+                               0.5 * (sV[sc][e - kHX] + sV[sp][e - kHX]) * (m.wy0[j - 1] * ws + m.wy1[j - 1] * wp)) *
+                                  m.rdy[j];
+                    rr = rr + dPdz +
+                         nu * ((m.rdxg[i] * (we - wp) - m.rdxg[i - 1] * (wp - ww)) * m.rdx[i] +
+                               ((wn - wp) * m.rdyg[j] - (wp - ws) * m.rdyg[j - 1]) * m.rdy[j] +
+                               (m.rdz[k + 1] * (wt - wp) - rdzk * (wp - wb)) * rdzgk);
+                }
+                if (MODE == 2) {
+                    Fow[p] = rr;
+                } else {
+                    if (WRITE_F1) Fow[p] = rr;
+                    Wn[p] = (MODE == 0) ? Wo[p] + c0 * rr : Wo[p] + dt * (c1 * F1w[p] + c0 * rr);
+                }
+            }
         }
-        if (MODE == 2) {
-            Fow[p] = r;
-        } else {
-            if (WRITE_F1) Fow[p] = r;
-            Wn[p] = (MODE == 0) ? Wo[p] + c0 * r : Wo[p] + dt * (c1 * F1w[p] + c0 * r);
-        }
+        __syncthreads();
     }
 }
 
@@ -321,47 +387,106 @@ __global__ __launch_bounds__(256) void k_pressure(Dims d, const double *phi, dou
 }
 
 // ------------------------------------------------------------------------
-// Dense transform along one index of many independent columns:
-//   out[r, c] = sum_s A[r, s] in[s, c]
-// The transpose A^T[s][r] (leading dimension lda, zero padded) is read with
-// wave-uniform addresses, i.e. through the scalar cache into SGPR operands of
-// v_fma_f64; each lane owns one column and RW rows of accumulators.
-// Columns are c = c1 + C1*c2 with independent strides.
+// Real DFT along z (M = nzp points, periodic) of many columns, folded by the
+// symmetry m <-> M-m so that each (cos, sin) pair of outputs costs M FMAs.
+// Spectrum rows: q = 0 (k = 0), q = 2k-1 Re X_k, q = 2k Im X_k with
+// X_k = sum_m T[m] e^{-2 pi i k m/M} (the z part of the reference's 2-D FFT,
+// projection.f90:123); the inverse is the unnormalised sum over all k.
+// Tables (transposed for wave-uniform scalar loads): ctab[m][k] = cos(2 pi k m/M),
+// stab[m][k] = sin(2 pi k m/M), m, k < lda rows/cols (zero padded).
+// Columns are c = kx + N*j, contiguous; stride between z planes is `plane`.
 // ------------------------------------------------------------------------
-struct XformArgs {
-    int nrows, nsum, lda;
-    int C1, C;
-    long long in_base, in_s, in_c1, in_c2;
-    long long out_base, out_r, out_c1, out_c2;
-};
-
 template <int RW>
-__global__ __launch_bounds__(256) void k_xform(XformArgs a, const double *__restrict__ AT,
-                                               const double *__restrict__ in,
-                                               double *__restrict__ out) {
-    const int lane = threadIdx.x & (kWave - 1);
-    const int wave = threadIdx.x >> 6;
-    const int r0 = __builtin_amdgcn_readfirstlane((blockIdx.y * 4 + wave) * RW);
-    if (r0 >= a.nrows) return;
-    const int c = blockIdx.x * kWave + lane;
-    const bool valid = c < a.C;
-    const int cc = valid ? c : a.C - 1;
-    const int c1 = cc % a.C1, c2 = cc / a.C1;
-    const double *src = in + a.in_base + c1 * a.in_c1 + c2 * a.in_c2;
-    double acc[RW];
+__global__ __launch_bounds__(256) void k_zdft_fwd(int M, int C, long long plane, int lda,
+                                                  const double *__restrict__ ctab,
+                                                  const double *__restrict__ stab,
+                                                  const double *__restrict__ in,
+                                                  double *__restrict__ out) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int k0 = __builtin_amdgcn_readfirstlane((blockIdx.y * 4 + wave) * RW);
+    const int Kc = M / 2 + 1;  // cos rows k = 0..M/2
+    if (k0 >= Kc) return;
+    const int c = blockIdx.x * 64 + lane;
+    if (c >= C) return;
+    const double *src = in + c;
+    const double t0 = src[0];
+    double ac[RW], as[RW];
 #pragma unroll
-    for (int r = 0; r < RW; ++r) acc[r] = 0.0;
-    for (int s = 0; s < a.nsum; ++s) {
-        const double xv = src[(long long)s * a.in_s];
-        const double *ar = AT + (long long)s * a.lda + r0;
-#pragma unroll
-        for (int r = 0; r < RW; ++r) acc[r] = fma(ar[r], xv, acc[r]);
+    for (int r = 0; r < RW; ++r) {
+        ac[r] = t0;
+        as[r] = 0.0;
     }
-    if (!valid) return;
-    double *dst = out + a.out_base + c1 * a.out_c1 + c2 * a.out_c2;
+    const int hm = M / 2;
+    for (int m = 1; m <= hm; ++m) {
+        const double tp = src[(long long)m * plane];
+        const double tm = src[(long long)(M - m) * plane];
+        const double e = (2 * m == M) ? tp : tp + tm;
+        const double o = tp - tm;  // zero for the middle element of even M
+        const double *cr = ctab + (long long)m * lda + k0;
+        const double *sr = stab + (long long)m * lda + k0;
 #pragma unroll
-    for (int r = 0; r < RW; ++r)
-        if (r0 + r < a.nrows) dst[(long long)(r0 + r) * a.out_r] = acc[r];
+        for (int r = 0; r < RW; ++r) {
+            ac[r] = fma(e, cr[r], ac[r]);
+            as[r] = fma(o, sr[r], as[r]);
+        }
+    }
+    double *dst = out + c;
+#pragma unroll
+    for (int r = 0; r < RW; ++r) {
+        const int k = k0 + r;
+        if (k >= Kc) break;
+        dst[(long long)(k == 0 ? 0 : 2 * k - 1) * plane] = ac[r];
+        if (k >= 1 && 2 * k <= M - 1) dst[(long long)(2 * k) * plane] = -as[r];
+    }
+}
+
+// inverse: g[m] = S0 + sum_k c_k (Re_k cos - Im_k sin)(2 pi k m/M), c_k = 2
+// (1 for the Nyquist row of even M); outputs m and M-m come from the same
+// cos and sin sums.
+template <int RW>
+__global__ __launch_bounds__(256) void k_zdft_inv(int M, int C, long long plane, int lda,
+                                                  const double *__restrict__ ctab,
+                                                  const double *__restrict__ stab,
+                                                  const double *__restrict__ in,
+                                                  double *__restrict__ out) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int m0 = __builtin_amdgcn_readfirstlane((blockIdx.y * 4 + wave) * RW);
+    const int hm = M / 2;  // outputs m = 0..M/2 (paired with M-m)
+    if (m0 > hm) return;
+    const int c = blockIdx.x * 64 + lane;
+    if (c >= C) return;
+    const double *src = in + c;
+    const double s0 = src[0];
+    double cc[RW], ss[RW];
+#pragma unroll
+    for (int r = 0; r < RW; ++r) {
+        cc[r] = s0;
+        ss[r] = 0.0;
+    }
+    const int Kc = M / 2 + 1;
+    for (int k = 1; k < Kc; ++k) {
+        const double re = src[(long long)(2 * k - 1) * plane];
+        const bool nyq = (2 * k == M);
+        const double im = nyq ? 0.0 : src[(long long)(2 * k) * plane];
+        const double cw = nyq ? re : 2.0 * re;
+        const double sw = 2.0 * im;
+        // tables indexed [k][m] here: ctab is symmetric in (k, m)
+        const double *cr = ctab + (long long)k * lda + m0;
+        const double *sr = stab + (long long)k * lda + m0;
+#pragma unroll
+        for (int r = 0; r < RW; ++r) {
+            cc[r] = fma(cw, cr[r], cc[r]);
+            ss[r] = fma(sw, sr[r], ss[r]);
+        }
+    }
+    double *dst = out + c;
+#pragma unroll
+    for (int r = 0; r < RW; ++r) {
+        const int m = m0 + r;
+        if (m > hm) break;
+        dst[(long long)m * plane] = cc[r] - ss[r];
+        if (m >= 1 && M - m != m) dst[(long long)(M - m) * plane] = cc[r] + ss[r];
+    }
 }
 
 // ------------------------------------------------------------------------
@@ -392,8 +517,14 @@ __device__ __forceinline__ double2 cmulc(double2 a, double2 b) {  // a * conj(b)
     return make_double2(a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y);
 }
 
-template <int LB>
-__global__ __launch_bounds__(256) void k_dct_fwd(DctArgs a, const double *__restrict__ in,
+// DIV: the line is the pseudo-pressure rhs (projection.f90:38-47) evaluated
+// on the fly from U, V, W (in = U, in2 = V, in3 = W, lines over cells
+// i = 1..N, j = 1..NM, k = 1..M) instead of being read from memory.
+template <int LB, bool DIV>
+__global__ __launch_bounds__(256) void k_dct_fwd(DctArgs a, Dims d, Metrics mt,
+                                                 const double *__restrict__ in,
+                                                 const double *__restrict__ in2,
+                                                 const double *__restrict__ in3,
                                                  double *__restrict__ out) {
     extern __shared__ double smem[];
     const int N = a.N, N1 = a.N1, N2 = a.N2, K1 = a.K1h + 1;
@@ -410,7 +541,14 @@ __global__ __launch_bounds__(256) void k_dct_fwd(DctArgs a, const double *__rest
         const int l = idx / N, n = idx - l * N;
         const int c = min(line0 + l, a.C - 1);
         const int c1 = c % a.C1, c2 = c / a.C1;
-        const double f = in[a.in_base + c1 * a.in_c1 + c2 * a.in_c2 + n];
+        const long long p = a.in_base + c1 * a.in_c1 + c2 * a.in_c2 + n;
+        double f;
+        if (DIV) {
+            f = (in[p] - in[p - 1]) * mt.rdx[n + 1] + (in2[p] - in2[p - d.sx]) * mt.rdy[c1 + 1] +
+                (in3[p] - in3[p - d.sxy]) * mt.rdz[c2 + 1];
+        } else {
+            f = in[p];
+        }
         v[l * N + ((n & 1) ? N - 1 - (n >> 1) : (n >> 1))] = f;
     }
     __syncthreads();
@@ -467,33 +605,33 @@ __global__ __launch_bounds__(256) void k_dct_inv(DctArgs a, const double *__rest
                                                  double *__restrict__ out) {
     extern __shared__ double smem[];
     const int N = a.N, N1 = a.N1, N2 = a.N2, K1h = a.K1h, K1 = K1h + 1;
-    double *g = smem;                                    // LB*N (later the output line)
-    double2 *C = reinterpret_cast<double2 *>(g + LB * N);  // LB*K1*N2
+    double2 *Vs = reinterpret_cast<double2 *>(smem);  // LB*N, later the output lines (as doubles)
+    double2 *C = Vs + LB * N;                         // LB*K1*N2
     double2 *w1 = C + LB * K1 * N2;
     double2 *w2 = w1 + N1;
     const int tid = threadIdx.x;
     for (int q = tid; q < N1; q += 256) w1[q] = a.w1[q];
     for (int q = tid; q < N2; q += 256) w2[q] = a.w2[q];
     const int line0 = blockIdx.x * LB;
+    // V[k] = conj(rot_k) (g[k] - i g[N-k]) / 2, g read straight from the spectrum line
     for (int idx = tid; idx < LB * N; idx += 256) {
         const int l = idx / N, k = idx - l * N;
         const int c = min(line0 + l, a.C - 1);
         const int c1 = c % a.C1, c2 = c / a.C1;
-        g[idx] = in[a.in_base + c1 * a.in_c1 + c2 * a.in_c2 + k];
+        const double *gl = in + a.in_base + c1 * a.in_c1 + c2 * a.in_c2;
+        const double gk = gl[k], gm = (k == 0) ? 0.0 : gl[N - k];
+        Vs[idx] = cmulc(make_double2(0.5 * gk, -0.5 * gm), a.rot[k]);
     }
     __syncthreads();
-    // step C': C~[k1][n2] = W_N^{-n2 k1} sum_k2 V[k1 + N1 k2] W_N2^{-n2 k2},
-    // V[k] = conj(rot_k) (g[k] - i g[N-k]) / 2
+    // step C': C~[k1][n2] = W_N^{-n2 k1} sum_k2 V[k1 + N1 k2] W_N2^{-n2 k2}
     for (int idx = tid; idx < LB * K1 * N2; idx += 256) {
         const int l = idx / (K1 * N2), r = idx - l * K1 * N2;
         const int k1 = r / N2, n2 = r - k1 * N2;
-        const double *gl = g + l * N;
+        const double2 *Vl = Vs + l * N + k1;
         double re = 0.0, im = 0.0;
         int q = 0;
         for (int k2 = 0; k2 < N2; ++k2) {
-            const int k = k1 + N1 * k2;
-            const double gk = gl[k], gm = (k == 0) ? 0.0 : gl[N - k];
-            const double2 V = cmulc(make_double2(0.5 * gk, -0.5 * gm), a.rot[k]);
+            const double2 V = Vl[N1 * k2];
             const double2 w = w2[q];  // use conj(w)
             re = fma(V.x, w.x, fma(V.y, w.y, re));
             im = fma(V.y, w.x, fma(-V.x, w.y, im));
@@ -504,6 +642,7 @@ __global__ __launch_bounds__(256) void k_dct_inv(DctArgs a, const double *__rest
     }
     __syncthreads();
     // step A': v[N2 n1 + n2] = (1/N) (Re C~[0] + sum_{k1>=1} c_k1 Re(C~[k1] W_N1^{-n1 k1}))
+    double *o = smem;  // LB*N doubles inside the V area, which is no longer read
     for (int idx = tid; idx < LB * N; idx += 256) {
         const int l = idx / N, n = idx - l * N;
         const int n1 = n / N2, n2 = n - n1 * N2;
@@ -518,7 +657,7 @@ __global__ __launch_bounds__(256) void k_dct_inv(DctArgs a, const double *__rest
             acc += ((N1 % 2 == 0) && k1 == K1h) ? t : 2.0 * t;
         }
         const int pos = (n < (N + 1) / 2) ? 2 * n : 2 * (N - 1 - n) + 1;
-        g[l * N + pos] = acc * a.scale;
+        o[l * N + pos] = acc * a.scale;
     }
     __syncthreads();
     for (int idx = tid; idx < LB * N; idx += 256) {
@@ -526,51 +665,72 @@ __global__ __launch_bounds__(256) void k_dct_inv(DctArgs a, const double *__rest
         const int c = line0 + l;
         if (c >= a.C) continue;
         const int c1 = c % a.C1, c2 = c / a.C1;
-        out[a.out_base + c1 * a.out_c1 + c2 * a.out_c2 + n] = g[idx];
+        out[a.out_base + c1 * a.out_c1 + c2 * a.out_c2 + n] = o[idx];
     }
 }
 
-// Thomas sweep with precomputed factors, one (kx, kz) per lane; the Re and
-// Im rows of the half spectrum share the factors (projection.f90:137-150).
+// Thomas sweep with precomputed factors (projection.f90:137-150).  One real
+// system (kx, q) per lane: q = 0 is the kz = 0 row, q = 2k-1 / 2k the real /
+// imaginary rows of wavenumber k, which share the factors of k.  The sweep is
+// unrolled by TJ rows so that the loads of a chunk are issued together.
+template <int TJ>
 __global__ __launch_bounds__(256) void k_thomas(Dims d, const double *__restrict__ dl,
                                                 const double *__restrict__ dp,
                                                 const double *__restrict__ du, double *S) {
     const int kx = blockIdx.x * blockDim.x + threadIdx.x;
-    const int kk = blockIdx.y;
-    const int N = d.N, NM = d.NM, M = d.M;
+    const int q = blockIdx.y;
+    const int N = d.N, NM = d.NM;
     if (kx >= N) return;
     const long long plane = (long long)N * NM;
-    const int qre = kk == 0 ? 0 : 2 * kk - 1;
-    const int qim = (kk > 0 && 2 * kk <= M - 1) ? 2 * kk : -1;
-    double *a = S + qre * plane + kx;
-    double *b = qim >= 0 ? S + qim * plane + kx : nullptr;
+    const int kk = (q + 1) / 2;
+    double *a = S + q * plane + kx;
     const double *L = dl + kk * plane + kx;
     const double *D = dp + kk * plane + kx;
-    double ra = a[0], rb = b ? b[0] : 0.0;
-    for (int j = 1; j < NM; ++j) {
-        const double l = L[(long long)(j - 1) * N];
-        ra = a[(long long)j * N] - l * ra;
-        a[(long long)j * N] = ra;
-        if (b) {
-            rb = b[(long long)j * N] - l * rb;
-            b[(long long)j * N] = rb;
+    // forward elimination with the stored multipliers
+    double r = a[0];
+    int j = 1;
+    for (; j + TJ <= NM; j += TJ) {
+        double x[TJ], l[TJ];
+#pragma unroll
+        for (int t = 0; t < TJ; ++t) {
+            x[t] = a[(long long)(j + t) * N];
+            l[t] = L[(long long)(j + t - 1) * N];
         }
-    }
-    const double dlast = D[(long long)(NM - 1) * N];
-    ra = ra * dlast;
-    a[(long long)(NM - 1) * N] = ra;
-    if (b) {
-        rb = rb * dlast;
-        b[(long long)(NM - 1) * N] = rb;
-    }
-    for (int j = NM - 2; j >= 0; --j) {
-        const double dj = D[(long long)j * N], uj = du[j];
-        ra = (a[(long long)j * N] - uj * ra) * dj;
-        a[(long long)j * N] = ra;
-        if (b) {
-            rb = (b[(long long)j * N] - uj * rb) * dj;
-            b[(long long)j * N] = rb;
+#pragma unroll
+        for (int t = 0; t < TJ; ++t) {
+            r = x[t] - l[t] * r;
+            x[t] = r;
         }
+#pragma unroll
+        for (int t = 0; t < TJ; ++t) a[(long long)(j + t) * N] = x[t];
+    }
+    for (; j < NM; ++j) {
+        r = a[(long long)j * N] - L[(long long)(j - 1) * N] * r;
+        a[(long long)j * N] = r;
+    }
+    // back substitution with the stored reciprocal pivots
+    r = r * D[(long long)(NM - 1) * N];
+    a[(long long)(NM - 1) * N] = r;
+    j = NM - 2;
+    for (; j - TJ + 1 >= 0; j -= TJ) {
+        double x[TJ], p[TJ], u[TJ];
+#pragma unroll
+        for (int t = 0; t < TJ; ++t) {
+            x[t] = a[(long long)(j - t) * N];
+            p[t] = D[(long long)(j - t) * N];
+            u[t] = du[j - t];
+        }
+#pragma unroll
+        for (int t = 0; t < TJ; ++t) {
+            r = (x[t] - u[t] * r) * p[t];
+            x[t] = r;
+        }
+#pragma unroll
+        for (int t = 0; t < TJ; ++t) a[(long long)(j - t) * N] = x[t];
+    }
+    for (; j >= 0; --j) {
+        r = (a[(long long)j * N] - du[j] * r) * D[(long long)j * N];
+        a[(long long)j * N] = r;
     }
 }
 
@@ -721,28 +881,42 @@ void check_launch() {
     }
 }
 
-template <int RW>
-void xform_launch(Handle &h, int kid, const XformArgs &a, const double *AT, const double *in, double *out) {
+constexpr int kDctLines = 2;
+constexpr int kZRows = 16;
+
+void zdft_launch(Handle &h, int kid, bool forward, const double *in, double *out) {
     ProfScope ps(h, kid);
-    dim3 grid((a.C + kWave - 1) / kWave, (a.nrows + 4 * RW - 1) / (4 * RW));
-    hipLaunchKernelGGL(k_xform<RW>, grid, dim3(256), 0, h.stream, a, AT, in, out);
+    const Dims &d = h.d;
+    const int C = d.N * d.NM;
+    const long long plane = (long long)C;
+    const int rows = d.M / 2 + 1;
+    const dim3 grid((C + 63) / 64, (rows + 4 * kZRows - 1) / (4 * kZRows));
+    if (forward)
+        hipLaunchKernelGGL(k_zdft_fwd<kZRows>, grid, dim3(256), 0, h.stream, d.M, C, plane, h.lda_zt,
+                           h.zc, h.zs, in, out);
+    else
+        hipLaunchKernelGGL(k_zdft_inv<kZRows>, grid, dim3(256), 0, h.stream, d.M, C, plane, h.lda_zt,
+                           h.zc, h.zs, in, out);
     check_launch();
 }
 
-constexpr int kDctLines = 2;
-
-void dct_launch(Handle &h, int kid, const DctArgs &a, bool forward, const double *in, double *out) {
+void dct_launch(Handle &h, int kid, const DctArgs &a, bool forward, const double *in, double *out,
+                const Field3 *div = nullptr) {
     ProfScope ps(h, kid);
     const int K1 = a.K1h + 1;
-    const size_t lds = sizeof(double) * (size_t)kDctLines * a.N +
+    const size_t lds = (forward ? sizeof(double) : sizeof(double2)) * (size_t)kDctLines * a.N +
                        sizeof(double2) * ((size_t)kDctLines * K1 * a.N2 + a.N1 + a.N2);
     if (lds > 65536) {
         last_error() = "x transform length too large for the LDS-resident DCT (N = " + std::to_string(a.N) + ")";
         throw ArgError();
     }
     const dim3 grid((a.C + kDctLines - 1) / kDctLines);
-    if (forward)
-        hipLaunchKernelGGL(k_dct_fwd<kDctLines>, grid, dim3(256), lds, h.stream, a, in, out);
+    if (forward && div)
+        hipLaunchKernelGGL((k_dct_fwd<kDctLines, true>), grid, dim3(256), lds, h.stream, a, h.d, h.m,
+                           div->u, div->v, div->w, out);
+    else if (forward)
+        hipLaunchKernelGGL((k_dct_fwd<kDctLines, false>), grid, dim3(256), lds, h.stream, a, h.d, h.m, in,
+                           nullptr, nullptr, out);
     else
         hipLaunchKernelGGL(k_dct_inv<kDctLines>, grid, dim3(256), lds, h.stream, a, in, out);
     check_launch();
@@ -761,43 +935,48 @@ const char *kernel_name(int id) {
 // ------------------------------------------------------------------------
 // launch wrappers
 // ------------------------------------------------------------------------
+constexpr int kChunk = 16;
+
+dim3 march_grid(const Dims &d) {
+    return dim3((d.nxg - 2 + kTX - 1) / kTX, (d.nyg - 2 + kTY - 1) / kTY, (d.nzg - 2 + kChunk - 1) / kChunk);
+}
+
 void launch_rk_stage(Handle &h, const Field3 &cur, const Field3 &old, Field3 &nxt, int stage,
                      double dt) {
     const Dims &d = h.d;
-    dim3 grid = interior_grid(d), block(kWave, 4);
+    const dim3 grid = march_grid(d), block(256);
     const bool rk3 = h.scheme == 3;
     ProfScope ps(h, KID_RK_STAGE);
     if (stage == 1) {
         const double c0 = dt * (rk3 ? h.rk_coef[1][1] : h.rk2_coef[1][1]);
         if (rk3)
-            hipLaunchKernelGGL((k_rk_stage<0, true>), grid, block, 0, h.stream, d, h.m, h.nu, h.dPdx,
-                               h.dPdz, cur.u, cur.v, cur.w, old.u, old.v, old.w, nullptr, nullptr,
-                               nullptr, nxt.u, nxt.v, nxt.w, h.F1.u, h.F1.v, h.F1.w, c0, 0.0, dt);
+            hipLaunchKernelGGL((k_rk_march<0, true>), grid, block, 0, h.stream, d, h.m, h.nu, h.dPdx, h.dPdz,
+                               kChunk, cur.u, cur.v, cur.w, old.u, old.v, old.w, nullptr, nullptr, nullptr,
+                               nxt.u, nxt.v, nxt.w, h.F1.u, h.F1.v, h.F1.w, c0, 0.0, dt);
         else
-            hipLaunchKernelGGL((k_rk_stage<0, false>), grid, block, 0, h.stream, d, h.m, h.nu, h.dPdx,
-                               h.dPdz, cur.u, cur.v, cur.w, old.u, old.v, old.w, nullptr, nullptr,
-                               nullptr, nxt.u, nxt.v, nxt.w, nullptr, nullptr, nullptr, c0, 0.0, dt);
+            hipLaunchKernelGGL((k_rk_march<0, false>), grid, block, 0, h.stream, d, h.m, h.nu, h.dPdx, h.dPdz,
+                               kChunk, cur.u, cur.v, cur.w, old.u, old.v, old.w, nullptr, nullptr, nullptr,
+                               nxt.u, nxt.v, nxt.w, nullptr, nullptr, nullptr, c0, 0.0, dt);
     } else if (!rk3) {
         // RK2 second stage: Uo + dt*(0*F1 + 1*F2) == Uo + dt*F2 exactly
-        hipLaunchKernelGGL((k_rk_stage<0, false>), grid, block, 0, h.stream, d, h.m, h.nu, h.dPdx,
-                           h.dPdz, cur.u, cur.v, cur.w, old.u, old.v, old.w, nullptr, nullptr, nullptr,
-                           nxt.u, nxt.v, nxt.w, nullptr, nullptr, nullptr, dt * h.rk2_coef[2][2], 0.0,
-                           dt);
+        hipLaunchKernelGGL((k_rk_march<0, false>), grid, block, 0, h.stream, d, h.m, h.nu, h.dPdx, h.dPdz,
+                           kChunk, cur.u, cur.v, cur.w, old.u, old.v, old.w, nullptr, nullptr, nullptr, nxt.u,
+                           nxt.v, nxt.w, nullptr, nullptr, nullptr, dt * h.rk2_coef[2][2], 0.0, dt);
     } else {
         // RK3 stages 2, 3: a_32 = 0 so F2 is never needed
         const double c1 = h.rk_coef[stage][1], c0 = h.rk_coef[stage][stage];
-        hipLaunchKernelGGL((k_rk_stage<1, false>), grid, block, 0, h.stream, d, h.m, h.nu, h.dPdx,
-                           h.dPdz, cur.u, cur.v, cur.w, old.u, old.v, old.w, h.F1.u, h.F1.v, h.F1.w,
-                           nxt.u, nxt.v, nxt.w, nullptr, nullptr, nullptr, c0, c1, dt);
+        hipLaunchKernelGGL((k_rk_march<1, false>), grid, block, 0, h.stream, d, h.m, h.nu, h.dPdx, h.dPdz,
+                           kChunk, cur.u, cur.v, cur.w, old.u, old.v, old.w, h.F1.u, h.F1.v, h.F1.w, nxt.u,
+                           nxt.v, nxt.w, nullptr, nullptr, nullptr, c0, c1, dt);
     }
     check_launch();
 }
 
 void launch_rhs_only(Handle &h, const Field3 &cur, Field3 &rhs) {
     const Dims &d = h.d;
-    hipLaunchKernelGGL((k_rk_stage<2, false>), interior_grid(d), dim3(kWave, 4), 0, h.stream, d, h.m,
-                       h.nu, h.dPdx, h.dPdz, cur.u, cur.v, cur.w, nullptr, nullptr, nullptr, nullptr,
-                       nullptr, nullptr, nullptr, nullptr, nullptr, rhs.u, rhs.v, rhs.w, 0.0, 0.0, 0.0);
+    hipLaunchKernelGGL((k_rk_march<2, false>), march_grid(d), dim3(256), 0, h.stream, d, h.m, h.nu, h.dPdx,
+                       h.dPdz, kChunk, cur.u, cur.v, cur.w, nullptr, nullptr, nullptr, nullptr, nullptr,
+                       nullptr, nullptr, nullptr, nullptr, rhs.u, rhs.v, rhs.w, 0.0, 0.0, 0.0);
     check_launch();
 }
 
@@ -837,11 +1016,10 @@ void launch_divergence(Handle &h, const Field3 &f, double *out) {
     check_launch();
 }
 
-void launch_poisson(Handle &h, double *phi) {
+void launch_poisson(Handle &h, double *phi, const Field3 *div) {
     const Dims &d = h.d;
     const int N = d.N, M = d.M, NM = d.NM;
     const long long plane = (long long)N * NM;
-    XformArgs a{};
     // forward x: lines (j, m) of the padded field -> T1[m][j][kx]
     DctArgs da{};
     da.N = N;
@@ -861,30 +1039,17 @@ void launch_poisson(Handle &h, double *phi) {
     da.out_c1 = N;
     da.out_c2 = plane;
     da.scale = 1.0;
-    dct_launch(h, KID_XFORM_X_FWD, da, true, phi, h.T1);
+    dct_launch(h, KID_XFORM_X_FWD, da, true, phi, h.T1, div);
     // forward z: columns (kx, j) of T1 -> S[q][j][kx]
-    a.nrows = M;
-    a.nsum = M;
-    a.lda = h.lda_z;
-    a.C1 = N;
-    a.C = N * NM;
-    a.in_base = 0;
-    a.in_s = plane;
-    a.in_c1 = 1;
-    a.in_c2 = N;
-    a.out_base = 0;
-    a.out_r = plane;
-    a.out_c1 = 1;
-    a.out_c2 = N;
-    xform_launch<16>(h, KID_XFORM_Z_FWD, a, h.AzF, h.T1, h.S);
+    zdft_launch(h, KID_XFORM_Z_FWD, true, h.T1, h.S);
     {
     ProfScope ps(h, KID_THOMAS);
-    hipLaunchKernelGGL(k_thomas, dim3((N + 255) / 256, d.Kz), dim3(256), 0, h.stream, d, h.lu_dl,
+    hipLaunchKernelGGL(k_thomas<8>, dim3((N + 255) / 256, M), dim3(256), 0, h.stream, d, h.lu_dl,
                        h.lu_dp, h.lu_du, h.S);
     check_launch();
     }
     // inverse z: S -> T1
-    xform_launch<16>(h, KID_XFORM_Z_INV, a, h.AzI, h.S, h.T1);
+    zdft_launch(h, KID_XFORM_Z_INV, false, h.S, h.T1);
     // inverse x: T1[m][j][kx] -> phi cells (1..N, 1..NM, 1..M)
     da.in_base = 0;
     da.in_c1 = N;
