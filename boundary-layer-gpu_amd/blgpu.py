@@ -26,7 +26,7 @@ class BLConfig(ctypes.Structure):
                 ("U_inlet", _DP), ("V_inlet", _DP), ("W_inlet", _DP), ("U_top", _DP), ("V_top", _DP),
                 ("W_top", _DP), ("n_modes", _I), ("m_modes", _I), ("zmode", _DP), ("tmode", _DP),
                 ("qu_re", _DP), ("qu_im", _DP), ("qv_re", _DP), ("qv_im", _DP), ("qw_re", _DP),
-                ("qw_im", _DP)]
+                ("qw_im", _DP), ("nut_bc", _I)]
 
 
 # every entry point declared in include/blgpu.h with its ctypes signature
@@ -149,7 +149,7 @@ class Solver:
     """One device-resident solver (bl_create ... bl_destroy)."""
 
     def __init__(self, x, y, z, nu, tables, dPdx=0.0, dPdz=0.0, scheme=3, les=0, cfl=-1e-4, modes=None,
-                 precision=PREC_FP64):
+                 precision=PREC_FP64, nut_bc=0):
         self._keep = []
         keep = lambda a: (self._keep.append(_f64(a)), self._keep[-1])[1]
         x, y, z = keep(x), keep(y), keep(z)
@@ -159,6 +159,7 @@ class Solver:
         c.x, c.y, c.z = _ptr(x), _ptr(y), _ptr(z)
         c.nu, c.dPdx, c.dPdz, c.cfl = nu, dPdx, dPdz, cfl
         c.scheme, c.les, c.precision = scheme, les, precision
+        c.nut_bc = nut_bc
         for k in ("U_inlet", "V_inlet", "W_inlet", "U_top", "V_top", "W_top"):
             setattr(c, k, _ptr(keep(tables[k])))
         if modes is not None:

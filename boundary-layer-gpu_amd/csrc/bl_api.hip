@@ -177,6 +177,7 @@ void build(Handle &h, const bl_config *c) {
     h.cfl = c->cfl;
     h.scheme = c->scheme;
     h.les = c->les;
+    h.nut_bc = c->nut_bc;
     h.precision = c->precision;
     h.t = 0.0;
     h.dt = c->cfl < 0 ? -c->cfl : 0.0;
@@ -312,6 +313,10 @@ void build(Handle &h, const bl_config *c) {
     h.nu_t = dmalloc(nf);
     for (double *p : {h.F1.u, h.F1.v, h.F1.w, h.P, h.phi, h.nu_t})
         BL_HIP(hipMemsetAsync(p, 0, nf * sizeof(double), h.stream));
+    if (h.les == 2) {
+        h.dsm = dmalloc(nf * kDsmArrays);
+        BL_HIP(hipMemsetAsync(h.dsm, 0, nf * kDsmArrays * sizeof(double), h.stream));
+    }
     const size_t nspec = (size_t)d.N * d.NM * d.M;
     h.T1 = dmalloc(nspec);
     h.S = dmalloc(nspec);
@@ -417,7 +422,7 @@ void free_all(Handle &h) {
     for (void *p : {(void *)h.F1.u, (void *)h.F1.v, (void *)h.F1.w, (void *)h.P, (void *)h.phi,
                     (void *)h.nu_t, (void *)h.T1, (void *)h.S, (void *)h.dct_w1, (void *)h.dct_w2,
                     (void *)h.dct_wN, (void *)h.dct_rot, (void *)h.zc, (void *)h.zs, (void *)h.lu_dl, (void *)h.lu_dp, (void *)h.lu_du,
-                    (void *)h.scal, (void *)h.red, h.mem_grid})
+                    (void *)h.scal, (void *)h.red, (void *)h.dsm, h.mem_grid})
         f(p);
     if (h.stream) (void)hipStreamSynchronize(h.stream);
     prof_free(h);
@@ -442,6 +447,8 @@ void do_step(Handle &h) {
     int cur = X;
     for (int s = 1; s <= nst; ++s) {
         const int nxt = order3[s - 1];
+        // eddy viscosity of the stage's input fields (time_integration.f90:341-346)
+        if (h.les == 2) launch_dsm(h, h.buf[cur]);
         launch_rk_stage(h, h.buf[cur], old, h.buf[nxt], s, dt);
         const double ts = to + (nst == 3 ? h.rk_t[s] : h.rk2_t[s]) * dt;
         launch_bc(h, h.buf[nxt], old, s, dt, ts);
@@ -677,11 +684,10 @@ int bl_synchronize(bl_handle *hh) {
 int bl_eddy_viscosity(bl_handle *hh) {
     return guarded([&] {
         Handle &h = H(hh);
-        if (h.les != 0) {
-            last_error() = "bl_eddy_viscosity: dynamic Smagorinsky is not available in this build";
-            throw ArgError();
-        }
-        BL_HIP(hipMemsetAsync(h.nu_t, 0, (size_t)h.d.sxy * h.d.nzg * sizeof(double), h.stream));
+        if (h.les == 2)
+            launch_dsm(h, h.buf[h.icur]);
+        else
+            BL_HIP(hipMemsetAsync(h.nu_t, 0, (size_t)h.d.sxy * h.d.nzg * sizeof(double), h.stream));
     });
 }
 

@@ -65,6 +65,10 @@ enum KernelId {
     KID_PROJECT,
     KID_PRESSURE,
     KID_REDUCE,
+    KID_DSM_CELLS,
+    KID_DSM_VFILTER,
+    KID_DSM_FILTER,
+    KID_DSM_COLUMN,
     KID_COUNT
 };
 const char *kernel_name(int id);
@@ -75,6 +79,7 @@ struct Handle {
     Dims d;
     double nu, dPdx, dPdz, cfl;
     int scheme, les, precision;
+    int nut_bc;  // nutBC: 1 = Dirichlet (antisymmetric) nu_t ghost rows in y, else Neumann
     double t, dt;
     double length_y;
     double bc_uc;      // U_top(nx): outflow convection speed
@@ -91,6 +96,7 @@ struct Handle {
     int icur;       // index of the current (most recent) buffer
     Field3 F1;      // first-stage RHS (RK3)
     double *P, *phi, *nu_t;
+    double *dsm;  // dynamic Smagorinsky work arrays (LES = 2 only), kDsmArrays field-sized slots
     double *T1, *S;  // transform intermediates: [m][j][kx] and [q][j][kx]
     int dct_n1;                      // x-DCT factorisation N = dct_n1 * (N / dct_n1)
     int lda_zt;                      // leading dimension of the z cos/sin tables
@@ -130,6 +136,9 @@ double max_divergence(Handle &h, const Field3 &f);
 void wall_stats(Handle &h, const Field3 &f, double *cf_host);
 void monitor(Handle &h, const Field3 &f, double *out_host);
 double dt_convective(Handle &h, const Field3 &f);
+// dynamic Smagorinsky eddy viscosity of f into h.nu_t (bl_dsm.hip)
+constexpr int kDsmArrays = 15;
+void launch_dsm(Handle &h, const Field3 &f);
 
 std::string &last_error();
 

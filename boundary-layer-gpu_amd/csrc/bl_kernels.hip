@@ -38,15 +38,19 @@ __device__ __forceinline__ double sq(double a) { return a * a; }
 // ------------------------------------------------------------------------
 constexpr int kTX = 64, kTY = 8, kHX = kTX + 2, kHY = kTY + 2, kPlane = kHX * kHY;
 
-template <int MODE, bool WRITE_F1>
+// LES: the viscous terms use nu + nu_t (equations.f90:97-132, 231-270,
+// 367-404); nu_t is staged in a fourth LDS ring.
+template <int MODE, bool WRITE_F1, bool LES>
 __global__ __launch_bounds__(256) void k_rk_march(
     Dims d, Metrics m, double nu, double dPdx, double dPdz, int kchunk,
     const double *__restrict__ U, const double *__restrict__ V, const double *__restrict__ W,
+    const double *__restrict__ NT,
     const double *__restrict__ Uo, const double *__restrict__ Vo, const double *__restrict__ Wo,
     const double *__restrict__ F1u, const double *__restrict__ F1v, const double *__restrict__ F1w,
     double *__restrict__ Un, double *__restrict__ Vn, double *__restrict__ Wn, double *__restrict__ Fou,
     double *__restrict__ Fov, double *__restrict__ Fow, double c0, double c1, double dt) {
     __shared__ double sU[3][kPlane], sV[3][kPlane], sW[3][kPlane];
+    __shared__ double sN[LES ? 3 : 1][LES ? kPlane : 1];
     const int tid = threadIdx.x;
     const int tx = tid & 63, ty = tid >> 6;
     const int i0 = blockIdx.x * kTX + 1, j0 = blockIdx.y * kTY + 1;
@@ -66,7 +70,7 @@ __global__ __launch_bounds__(256) void k_rk_march(
         const int ii = min(i0 - 1 + hx, d.sx - 1), jj = min(j0 - 1 + hy, d.nyg - 1);
         off[q] = (long long)jj * sx + ii;
     }
-    double ru[kPer], rv[kPer], rw[kPer];
+    double ru[kPer], rv[kPer], rw[kPer], rn[LES ? kPer : 1];
     auto fetch = [&](int k) {
         const long long base = (long long)k * sxy;
 #pragma unroll
@@ -74,6 +78,7 @@ __global__ __launch_bounds__(256) void k_rk_march(
             ru[q] = U[base + off[q]];
             rv[q] = V[base + off[q]];
             rw[q] = W[base + off[q]];
+            if constexpr (LES) rn[q] = NT[base + off[q]];
         }
     };
     auto commit = [&](int slot) {
@@ -84,6 +89,7 @@ __global__ __launch_bounds__(256) void k_rk_march(
                 sU[slot][e] = ru[q];
                 sV[slot][e] = rv[q];
                 sW[slot][e] = rw[q];
+                if constexpr (LES) sN[slot][e] = rn[q];
             }
         }
     };
@@ -121,10 +127,25 @@ __global__ __launch_bounds__(256) void k_rk_march(
                     rr = rr - (0.5 * (up + ut) * 0.5 * (sW[sc][e] + sW[sc][e + 1]) -
                                0.5 * (ub + up) * 0.5 * (sW[sm][e] + sW[sm][e + 1])) *
                                   rdzk;
-                    rr = rr + dPdx +
-                         nu * ((m.rdx[i + 1] * (ue - up) - m.rdx[i] * (up - uw)) * m.rdxg[i] +
-                               ((un - up) * m.rdyg[j] - (up - us) * m.rdyg[j - 1]) * m.rdy[j] +
-                               (rdzgk * (ut - up) - rdzgm * (up - ub)) * rdzk);
+                    if constexpr (!LES) {
+                        rr = rr + dPdx +
+                             nu * ((m.rdx[i + 1] * (ue - up) - m.rdx[i] * (up - uw)) * m.rdxg[i] +
+                                   ((un - up) * m.rdyg[j] - (up - us) * m.rdyg[j - 1]) * m.rdy[j] +
+                                   (rdzgk * (ut - up) - rdzgm * (up - ub)) * rdzk);
+                    } else {
+                        const double *N = sN[sc], *Nm = sN[sm], *Np = sN[sp];
+                        const double a0 = m.wy0[j - 1], a1 = m.wy1[j - 1], b0 = m.wy0[j], b1 = m.wy1[j];
+                        const double nx1 = nu + N[e];
+                        const double nx2 = nu + N[e + 1];
+                        const double ny1 = nu + 0.5 * (a0 * N[e - kHX] + a1 * N[e] + a0 * N[e + 1 - kHX] + a1 * N[e + 1]);
+                        const double ny2 = nu + 0.5 * (b0 * N[e] + b1 * N[e + kHX] + b0 * N[e + 1] + b1 * N[e + 1 + kHX]);
+                        const double nz1 = nu + 0.25 * (N[e] + Nm[e] + N[e + 1] + Nm[e + 1]);
+                        const double nz2 = nu + 0.25 * (N[e] + Np[e] + N[e + 1] + Np[e + 1]);
+                        rr = rr + dPdx +
+                             (nx2 * m.rdx[i + 1] * (ue - up) - nx1 * m.rdx[i] * (up - uw)) * m.rdxg[i] +
+                             (ny2 * ((un - up) * m.rdyg[j]) - ny1 * ((up - us) * m.rdyg[j - 1])) * m.rdy[j] +
+                             (nz2 * rdzgk * (ut - up) - nz1 * rdzgm * (up - ub)) * rdzk;
+                    }
                 }
                 if (MODE == 2) {
                     Fou[p] = rr;
@@ -147,9 +168,23 @@ __global__ __launch_bounds__(256) void k_rk_march(
                     rr = rr - (0.5 * (vp + vt) * (m.wy0[j] * sW[sc][e] + m.wy1[j] * sW[sc][e + kHX]) -
                                0.5 * (vb + vp) * (m.wy0[j] * sW[sm][e] + m.wy1[j] * sW[sm][e + kHX])) *
                                   rdzk;
-                    rr = rr + nu * ((m.rdxg[i] * (ve - vp) - m.rdxg[i - 1] * (vp - vw)) * m.rdx[i] +
-                                    (m.rdy[j + 1] * (vn - vp) - m.rdy[j] * (vp - vs)) * m.rdyg[j] +
-                                    (rdzgk * (vt - vp) - rdzgm * (vp - vb)) * rdzk);
+                    if constexpr (!LES) {
+                        rr = rr + nu * ((m.rdxg[i] * (ve - vp) - m.rdxg[i - 1] * (vp - vw)) * m.rdx[i] +
+                                        (m.rdy[j + 1] * (vn - vp) - m.rdy[j] * (vp - vs)) * m.rdyg[j] +
+                                        (rdzgk * (vt - vp) - rdzgm * (vp - vb)) * rdzk);
+                    } else {
+                        const double *N = sN[sc], *Nm = sN[sm], *Np = sN[sp];
+                        const double b0 = m.wy0[j], b1 = m.wy1[j];
+                        const double nx1 = nu + 0.5 * (b0 * N[e - 1] + b1 * N[e - 1 + kHX] + b0 * N[e] + b1 * N[e + kHX]);
+                        const double nx2 = nu + 0.5 * (b0 * N[e] + b1 * N[e + kHX] + b0 * N[e + 1] + b1 * N[e + 1 + kHX]);
+                        const double ny1 = nu + N[e];
+                        const double ny2 = nu + N[e + kHX];
+                        const double nz1 = nu + 0.5 * (b0 * Nm[e] + b1 * Nm[e + kHX] + b0 * N[e] + b1 * N[e + kHX]);
+                        const double nz2 = nu + 0.5 * (b0 * N[e] + b1 * N[e + kHX] + b0 * Np[e] + b1 * Np[e + kHX]);
+                        rr = rr + m.rdx[i] * (nx2 * m.rdxg[i] * (ve - vp) - nx1 * m.rdxg[i - 1] * (vp - vw)) +
+                             m.rdyg[j] * (ny2 * m.rdy[j + 1] * (vn - vp) - ny1 * m.rdy[j] * (vp - vs)) +
+                             rdzk * (nz2 * rdzgk * (vt - vp) - nz1 * rdzgm * (vp - vb));
+                    }
                 }
                 if (MODE == 2) {
                     Fov[p] = rr;
@@ -172,10 +207,25 @@ __global__ __launch_bounds__(256) void k_rk_march(
                     rr = rr - (0.5 * (sV[sc][e] + sV[sp][e]) * (m.wy0[j] * wp + m.wy1[j] * wn) -
                                0.5 * (sV[sc][e - kHX] + sV[sp][e - kHX]) * (m.wy0[j - 1] * ws + m.wy1[j - 1] * wp)) *
                                   m.rdy[j];
-                    rr = rr + dPdz +
-                         nu * ((m.rdxg[i] * (we - wp) - m.rdxg[i - 1] * (wp - ww)) * m.rdx[i] +
-                               ((wn - wp) * m.rdyg[j] - (wp - ws) * m.rdyg[j - 1]) * m.rdy[j] +
-                               (m.rdz[k + 1] * (wt - wp) - rdzk * (wp - wb)) * rdzgk);
+                    if constexpr (!LES) {
+                        rr = rr + dPdz +
+                             nu * ((m.rdxg[i] * (we - wp) - m.rdxg[i - 1] * (wp - ww)) * m.rdx[i] +
+                                   ((wn - wp) * m.rdyg[j] - (wp - ws) * m.rdyg[j - 1]) * m.rdy[j] +
+                                   (m.rdz[k + 1] * (wt - wp) - rdzk * (wp - wb)) * rdzgk);
+                    } else {
+                        const double *N = sN[sc], *Np = sN[sp];
+                        const double a0 = m.wy0[j - 1], a1 = m.wy1[j - 1], b0 = m.wy0[j], b1 = m.wy1[j];
+                        const double nx1 = nu + 0.25 * (N[e] + Np[e] + N[e - 1] + Np[e - 1]);
+                        const double nx2 = nu + 0.25 * (N[e] + Np[e] + N[e + 1] + Np[e + 1]);
+                        const double ny1 = nu + 0.5 * (a0 * N[e - kHX] + a1 * N[e] + a0 * Np[e - kHX] + a1 * Np[e]);
+                        const double ny2 = nu + 0.5 * (b0 * N[e] + b1 * N[e + kHX] + b0 * Np[e] + b1 * Np[e + kHX]);
+                        const double nz1 = nu + N[e];
+                        const double nz2 = nu + Np[e];
+                        rr = rr + dPdz +
+                             (nx2 * m.rdxg[i] * (we - wp) - nx1 * m.rdxg[i - 1] * (wp - ww)) * m.rdx[i] +
+                             (ny2 * ((wn - wp) * m.rdyg[j]) - ny1 * ((wp - ws) * m.rdyg[j - 1])) * m.rdy[j] +
+                             (nz2 * m.rdz[k + 1] * (wt - wp) - nz1 * rdzk * (wp - wb)) * rdzgk;
+                    }
                 }
                 if (MODE == 2) {
                     Fow[p] = rr;
@@ -928,7 +978,7 @@ const char *kernel_name(int id) {
     static const char *names[KID_COUNT] = {
         "rk_stage", "bc_inout", "bc_periodic", "bc_topwall", "bc_massflux", "divergence",
         "xform_x_fwd", "xform_z_fwd", "thomas", "xform_z_inv", "xform_x_inv", "project",
-        "pressure", "reduce"};
+        "pressure", "reduce", "dsm_cells", "dsm_vfilter", "dsm_filter", "dsm_column"};
     return (id >= 0 && id < KID_COUNT) ? names[id] : "";
 }
 
@@ -941,43 +991,48 @@ dim3 march_grid(const Dims &d) {
     return dim3((d.nxg - 2 + kTX - 1) / kTX, (d.nyg - 2 + kTY - 1) / kTY, (d.nzg - 2 + kChunk - 1) / kChunk);
 }
 
-void launch_rk_stage(Handle &h, const Field3 &cur, const Field3 &old, Field3 &nxt, int stage,
-                     double dt) {
+template <int MODE, bool WRITE_F1>
+void march(Handle &h, const Field3 &cur, const Field3 *old, const Field3 *f1, Field3 *nxt, Field3 *fo,
+           double c0, double c1, double dt) {
     const Dims &d = h.d;
     const dim3 grid = march_grid(d), block(256);
+    const double *o[3] = {old ? old->u : nullptr, old ? old->v : nullptr, old ? old->w : nullptr};
+    const double *f[3] = {f1 ? f1->u : nullptr, f1 ? f1->v : nullptr, f1 ? f1->w : nullptr};
+    double *n[3] = {nxt ? nxt->u : nullptr, nxt ? nxt->v : nullptr, nxt ? nxt->w : nullptr};
+    double *r[3] = {fo ? fo->u : nullptr, fo ? fo->v : nullptr, fo ? fo->w : nullptr};
+    if (h.les != 0)
+        hipLaunchKernelGGL((k_rk_march<MODE, WRITE_F1, true>), grid, block, 0, h.stream, d, h.m, h.nu, h.dPdx,
+                           h.dPdz, kChunk, cur.u, cur.v, cur.w, h.nu_t, o[0], o[1], o[2], f[0], f[1], f[2],
+                           n[0], n[1], n[2], r[0], r[1], r[2], c0, c1, dt);
+    else
+        hipLaunchKernelGGL((k_rk_march<MODE, WRITE_F1, false>), grid, block, 0, h.stream, d, h.m, h.nu, h.dPdx,
+                           h.dPdz, kChunk, cur.u, cur.v, cur.w, nullptr, o[0], o[1], o[2], f[0], f[1], f[2],
+                           n[0], n[1], n[2], r[0], r[1], r[2], c0, c1, dt);
+    check_launch();
+}
+
+void launch_rk_stage(Handle &h, const Field3 &cur, const Field3 &old, Field3 &nxt, int stage,
+                     double dt) {
     const bool rk3 = h.scheme == 3;
     ProfScope ps(h, KID_RK_STAGE);
     if (stage == 1) {
         const double c0 = dt * (rk3 ? h.rk_coef[1][1] : h.rk2_coef[1][1]);
         if (rk3)
-            hipLaunchKernelGGL((k_rk_march<0, true>), grid, block, 0, h.stream, d, h.m, h.nu, h.dPdx, h.dPdz,
-                               kChunk, cur.u, cur.v, cur.w, old.u, old.v, old.w, nullptr, nullptr, nullptr,
-                               nxt.u, nxt.v, nxt.w, h.F1.u, h.F1.v, h.F1.w, c0, 0.0, dt);
+            march<0, true>(h, cur, &old, nullptr, &nxt, &h.F1, c0, 0.0, dt);
         else
-            hipLaunchKernelGGL((k_rk_march<0, false>), grid, block, 0, h.stream, d, h.m, h.nu, h.dPdx, h.dPdz,
-                               kChunk, cur.u, cur.v, cur.w, old.u, old.v, old.w, nullptr, nullptr, nullptr,
-                               nxt.u, nxt.v, nxt.w, nullptr, nullptr, nullptr, c0, 0.0, dt);
+            march<0, false>(h, cur, &old, nullptr, &nxt, nullptr, c0, 0.0, dt);
     } else if (!rk3) {
         // RK2 second stage: Uo + dt*(0*F1 + 1*F2) == Uo + dt*F2 exactly
-        hipLaunchKernelGGL((k_rk_march<0, false>), grid, block, 0, h.stream, d, h.m, h.nu, h.dPdx, h.dPdz,
-                           kChunk, cur.u, cur.v, cur.w, old.u, old.v, old.w, nullptr, nullptr, nullptr, nxt.u,
-                           nxt.v, nxt.w, nullptr, nullptr, nullptr, dt * h.rk2_coef[2][2], 0.0, dt);
+        march<0, false>(h, cur, &old, nullptr, &nxt, nullptr, dt * h.rk2_coef[2][2], 0.0, dt);
     } else {
         // RK3 stages 2, 3: a_32 = 0 so F2 is never needed
         const double c1 = h.rk_coef[stage][1], c0 = h.rk_coef[stage][stage];
-        hipLaunchKernelGGL((k_rk_march<1, false>), grid, block, 0, h.stream, d, h.m, h.nu, h.dPdx, h.dPdz,
-                           kChunk, cur.u, cur.v, cur.w, old.u, old.v, old.w, h.F1.u, h.F1.v, h.F1.w, nxt.u,
-                           nxt.v, nxt.w, nullptr, nullptr, nullptr, c0, c1, dt);
+        march<1, false>(h, cur, &old, &h.F1, &nxt, nullptr, c0, c1, dt);
     }
-    check_launch();
 }
 
 void launch_rhs_only(Handle &h, const Field3 &cur, Field3 &rhs) {
-    const Dims &d = h.d;
-    hipLaunchKernelGGL((k_rk_march<2, false>), march_grid(d), dim3(256), 0, h.stream, d, h.m, h.nu, h.dPdx,
-                       h.dPdz, kChunk, cur.u, cur.v, cur.w, nullptr, nullptr, nullptr, nullptr, nullptr,
-                       nullptr, nullptr, nullptr, nullptr, rhs.u, rhs.v, rhs.w, 0.0, 0.0, 0.0);
-    check_launch();
+    march<2, false>(h, cur, nullptr, nullptr, nullptr, &rhs, 0.0, 0.0, 0.0);
 }
 
 void launch_bc(Handle &h, Field3 &f, const Field3 &old, int rk_step, double dt, double t) {

@@ -50,6 +50,9 @@ enum { BL_PREC_FP64 = 0, BL_PREC_MIXED = 1 };
  *  Temporal inflow modes (optional, n_modes = 0 disables): zmode(n_modes),
  *  tmode(m_modes) already multiplied by beta/omega, and q*_re/q*_im laid out
  *  (j, n, m) with j fastest, already interpolated to yg (u, w) and y (v).
+ *  nut_bc: .turbb nutBC (params.f90:156), the y ghost rows of nu_t after the
+ *  dynamic Smagorinsky model: 1 = antisymmetric (apply_Dirichlet_bc_y),
+ *  otherwise symmetric (apply_Neumann_bc_y), subgrid.f90:406-411.
  */
 typedef struct bl_config {
     int nx, ny, nz;
@@ -63,6 +66,7 @@ typedef struct bl_config {
     int n_modes, m_modes;
     const double *zmode, *tmode;
     const double *qu_re, *qu_im, *qv_re, *qv_im, *qw_re, *qw_im;
+    int nut_bc;
 } bl_config;
 
 /* Create a solver on the current HIP device.  Replaces main.f90:61-81 +
@@ -98,7 +102,10 @@ int bl_run(bl_handle *h, int nsteps);
 int bl_synchronize(bl_handle *h);
 
 /* compute_eddy_viscosity (subgrid.f90:23-47) on the current fields:
- * recomputes nu_t on the device (LES = 2: dynamic Smagorinsky). */
+ * recomputes nu_t on the device (LES = 2: dynamic Smagorinsky,
+ * sgs_dynamic_Smagorinsky subgrid.f90:261-414; LES = 0: nu_t = 0).  bl_step
+ * does this itself at the start of every RK stage, as the reference does
+ * (time_integration.f90:341-346, 383-388, 425-430). */
 int bl_eddy_viscosity(bl_handle *h);
 
 /* check_divergence (projection.f90:391-413), FP64 device max-reduction. */

@@ -81,6 +81,8 @@ void orc_wall_stats(const orc_state *s, double *cf);
 void orc_monitor(const orc_state *s, double *out);
 /* eddy viscosity (subgrid.f90:23-47) on the current U,V,W into nu_t */
 void orc_eddy_viscosity(orc_state *s);
+/* nutBC (params.f90:156): 1 = Dirichlet (antisymmetric) nu_t ghost in y, else Neumann */
+void orc_set_nut_bc(orc_state *s, int dirichlet);
 
 /* accessors for precomputed data used by tests */
 void orc_get_kxx_kzz(const orc_state *s, double *kxx, double *kzz);

@@ -47,6 +47,7 @@ def lib():
             "orc_wall_stats": (None, [_P, _P]),
             "orc_monitor": (None, [_P, _P]),
             "orc_eddy_viscosity": (None, [_P]),
+            "orc_set_nut_bc": (None, [_P, _I]),
             "orc_get_kxx_kzz": (None, [_P, _P, _P]),
             "orc_get_dyy": (None, [_P, _P, _P, _P]),
             "orc_get_dt": (_D, [_P]),
@@ -229,7 +230,12 @@ class Oracle:
         return out
 
     def eddy_viscosity(self):
+        """compute_eddy_viscosity on the current fields; returns nu_t."""
         lib().orc_eddy_viscosity(self.h)
+        return self.get(4)
+
+    def set_nut_bc(self, dirichlet):
+        lib().orc_set_nut_bc(self.h, int(dirichlet))
 
     def kxx_kzz(self):
         kxx = np.zeros(self.nx - 2)
