@@ -12,10 +12,14 @@ algorithm diverges within ~15 steps.  The bench therefore runs CFL = 0.5; the
 work per step does not depend on dt.  Inputs are resident
 in HBM before the timed region.  One "step" = compute_dt + one RK3 step.
 
-Prints one JSON line.  `roofline` describes the kernel with the largest
-share of GPU time, measured with HIP events on the solver's stream during the
-timed steps; `cpu_baseline` is the FP64 C oracle timed on this host for one
-RK3 step of a z-slab of the same workload.
+Prints one JSON line.  `value` is the FP64 path.  `roofline` describes the
+kernel with the largest share of GPU time, measured with HIP events on the
+solver's stream during the timed steps; `cpu_baseline` is the FP64 C oracle
+timed on this host for one RK3 step of a z-slab of the same workload.
+`mixed` is the same workload in mixed precision (FP32 fields, split-BF16x3
+MFMA transforms, FP64 Thomas / mass flux / statistics): its ns/cell/step, the
+MFMA rate of its x-transform GEMM, and the relative L2 difference of u, v, w,
+p and Cf from the FP64 run after the same warm-up + timed steps.
 """
 import argparse
 import json
@@ -33,6 +37,7 @@ NU = 4.898844e-6  # 1 / Re_x0, Re_x0 = (300/0.664)^2
 CFL = 0.5  # see the module docstring: 0.8 is unstable for the reference algorithm
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
+BF16_PEAK_TFLOPS = 2500.0  # dense
 
 
 def workload(nx, ny, nz, seed=0, noise=0.01):
@@ -95,37 +100,17 @@ def cpu_baseline(nz_sample):
                        f"z-slab of the same workload, OpenMP on {cores} threads")
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=200)
-    ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--nx", type=int, default=814)
-    ap.add_argument("--ny", type=int, default=257)
-    ap.add_argument("--nz", type=int, default=193)
-    ap.add_argument("--cpu-nz", type=int, default=17, help="z extent of the CPU-baseline sample (0 = skip)")
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print("bench.py: this build measures one GPU; multi-GPU runs are not part of it "
-              f"(asked for --gpus {args.gpus})", file=sys.stderr)
-        sys.exit(2)
-
-    from tests.blgpu_import import blgpu
-
-    nx, ny, nz = args.nx, args.ny, args.nz
-    print(f"bench: building workload {nx}x{ny}x{nz}", file=sys.stderr, flush=True)
-    x, y, z, tables, (U, V, W) = workload(nx, ny, nz)
-    s = blgpu.Solver(x, y, z, NU, tables, scheme=3, les=0, cfl=CFL)
+def run_leg(blgpu, precision, grid, args, keep_fields=False):
+    """Warm-up, then the timed steps with the HIP-event profile on."""
+    x, y, z, tables, (U, V, W) = grid
+    s = blgpu.Solver(x, y, z, NU, tables, scheme=3, les=0, cfl=CFL, precision=precision)
     s.set_fields(U, V, W)
-    del U, V, W
     dt = s.compute_dt()
-    print(f"bench: dt = {dt:.6e}; warm-up {args.warmup} steps", file=sys.stderr, flush=True)
     for _ in range(args.warmup):
         s.compute_dt()
         s.step()
     s.synchronize()
     s.profile(True)
-    print(f"bench: timing {args.steps} steps", file=sys.stderr, flush=True)
     t0 = time.perf_counter()
     for _ in range(args.steps):
         s.compute_dt()
@@ -134,14 +119,14 @@ def main():
     el = time.perf_counter() - t0
     prof = s.profile_read()
     s.profile(False)
-    div = s.max_divergence()
-    cf = s.wall_stats()
-    if not np.all(np.isfinite(cf)):
-        raise RuntimeError("non-finite Cf after the timed steps")
-    cells = nx * ny * nz
-    ms_per_step = el / args.steps * 1e3
-    value = el / args.steps / cells * 1e9
+    out = dict(el=el, dt=dt, prof=prof, div=s.max_divergence(), cf=s.wall_stats())
+    if keep_fields:
+        out["fields"] = s.fields() + (s.download(3),)
+    s.close()
+    return out
 
+
+def roofline_of(prof, nx, ny, nz):
     tot = {k: v[0] for k, v in prof.items() if v[1] > 0}
     gpu_ms = sum(tot.values())
     dom = max(tot, key=tot.get)
@@ -157,8 +142,75 @@ def main():
         roof = dict(bound="hbm", achieved=achieved, peak=HBM_PEAK_GBS, unit="GB/s",
                     frac=(achieved / HBM_PEAK_GBS) if achieved else None, traffic=None)
     roof.update(kernel=dom, avg_launch_ms=dom_ms / dom_n, share_of_gpu_time=dom_ms / gpu_ms)
+    return roof
+
+
+def gemm_mfma(prof, nx, ny, nz):
+    """MFMA rate of the mixed-precision x-transform GEMM: BF16 flops issued
+    (three products per term, padded tile extents) per launch / launch time."""
+    name = "xform_x_fwd"
+    if name not in prof or prof[name][1] == 0:
+        return None
+    N, M, NM = nx - 2, nz - 2, ny - 1
+    rp, kp = -(-N // 128) * 128, -(-N // 32) * 32
+    cols = -(-(NM * M) // 128) * 128
+    issued = 3 * 2.0 * rp * kp * cols
+    avg_s = prof[name][0] / prof[name][1] * 1e-3
+    ach = issued / avg_s / 1e12
+    return dict(kernel=name, bf16_tflops_issued=ach, peak=BF16_PEAK_TFLOPS, frac=ach / BF16_PEAK_TFLOPS,
+                avg_launch_ms=prof[name][0] / prof[name][1])
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--nx", type=int, default=814)
+    ap.add_argument("--ny", type=int, default=257)
+    ap.add_argument("--nz", type=int, default=193)
+    ap.add_argument("--cpu-nz", type=int, default=17, help="z extent of the CPU-baseline sample (0 = skip)")
+    ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision leg")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build measures one GPU; multi-GPU runs are not part of it "
+              f"(asked for --gpus {args.gpus})", file=sys.stderr)
+        sys.exit(2)
+
+    from tests.blgpu_import import blgpu
+
+    nx, ny, nz = args.nx, args.ny, args.nz
+    cells = nx * ny * nz
+    print(f"bench: building workload {nx}x{ny}x{nz}", file=sys.stderr, flush=True)
+    grid = workload(nx, ny, nz)
+    mixed = not args.no_mixed
+    print(f"bench: FP64 leg, warm-up {args.warmup}, timing {args.steps} steps", file=sys.stderr, flush=True)
+    r64 = run_leg(blgpu, blgpu.PREC_FP64, grid, args, keep_fields=mixed)
+    if not np.all(np.isfinite(r64["cf"])):
+        raise RuntimeError("non-finite Cf after the timed steps")
+    ms_per_step = r64["el"] / args.steps * 1e3
+    value = r64["el"] / args.steps / cells * 1e9
+    roof = roofline_of(r64["prof"], nx, ny, nz)
     kernels = {k: dict(ms_per_step=v[0] / args.steps, launches_per_step=v[1] / args.steps)
-               for k, v in prof.items() if v[1] > 0}
+               for k, v in r64["prof"].items() if v[1] > 0}
+
+    mixed_out = None
+    if mixed:
+        print("bench: mixed-precision leg", file=sys.stderr, flush=True)
+        rmx = run_leg(blgpu, blgpu.PREC_MIXED, grid, args, keep_fields=True)
+        err = {}
+        for name, a, b in zip(("u", "v", "w", "p"), rmx["fields"], r64["fields"]):
+            err[name] = float(np.linalg.norm(a - b) / np.linalg.norm(b))
+        err["cf"] = float(np.linalg.norm(rmx["cf"] - r64["cf"]) / np.linalg.norm(r64["cf"]))
+        v_mx = rmx["el"] / args.steps / cells * 1e9
+        mixed_out = dict(value=v_mx, unit="ns/cell/step", ms_per_step=rmx["el"] / args.steps * 1e3,
+                         speedup_vs_fp64=value / v_mx,
+                         rel_l2_vs_fp64_after_steps=err, steps_compared=args.warmup + args.steps,
+                         max_divergence=rmx["div"], mfma=gemm_mfma(rmx["prof"], nx, ny, nz),
+                         kernels={k: dict(ms_per_step=v[0] / args.steps, launches_per_step=v[1] / args.steps)
+                                  for k, v in rmx["prof"].items() if v[1] > 0})
+        del rmx
+    del r64["fields"]
 
     cpu = None
     if args.cpu_nz > 0:
@@ -179,10 +231,11 @@ def main():
         "dtype": "f64",
         "data": "synthetic",
         "config": {"workload": f"{nx}x{ny}x{nz} tanh(alpha=2) Blasius + 1% noise seed 0, RK3, CFL={CFL}, FP64",
-                   "nx": nx, "ny": ny, "nz": nz, "dt": dt, "precision": "fp64",
-                   "max_divergence_after": div},
+                   "nx": nx, "ny": ny, "nz": nz, "dt": r64["dt"], "precision": "fp64",
+                   "max_divergence_after": r64["div"]},
         "roofline": roof,
         "cpu_baseline": cpu,
+        "mixed": mixed_out,
         "kernels": kernels,
     }
     print(json.dumps(out), flush=True)

@@ -60,7 +60,20 @@ SIGNATURES = {
     "bl_blasius_field": (_I, [_I, _I, _I, _P, _P, _D, _I, _P, _P, _P, _P, _P, _P]),
     "bl_add_noise": (_I, [_I, _I, _I, _D, ctypes.c_ulonglong, _P, _P, _P]),
     "bl_grid_tanh": (_I, [_I, _I, _I, _D, _D, _D, _D, _D, _P, _P, _P]),
+    "bl_mlp_create": (_I, [ctypes.c_longlong, _I, ctypes.POINTER(_P)]),
+    "bl_mlp_destroy": (None, [_P]),
+    "bl_mlp_set_params": (_I, [_P, _P]),
+    "bl_mlp_get_params": (_I, [_P, _P]),
+    "bl_mlp_forward": (_I, [_P, _P, _I, _P]),
+    "bl_mlp_gradient": (_I, [_P, _P, _P, _I, _P, _P]),
+    "bl_mlp_add_rows": (_I, [_P, _P, ctypes.c_longlong]),
+    "bl_mlp_get_rows": (_I, [_P, ctypes.c_longlong, ctypes.c_longlong, _P]),
+    "bl_mlp_count": (_I, [_P, ctypes.POINTER(ctypes.c_longlong)]),
+    "bl_mlp_clear": (_I, [_P]),
+    "bl_mlp_train": (_I, [_P, _I, _I, ctypes.c_float, ctypes.c_ulonglong, _P]),
 }
+
+MLP_IN, MLP_HIDDEN, MLP_PARAMS, MLP_ROW = 10, 64, 9089, 28
 
 _lib = None
 
@@ -280,3 +293,80 @@ class Solver:
         assert r.shape == self.shapes[FIELD_P]
         _check(lib().bl_poisson(self.h, r.ctypes.data), "bl_poisson")
         return r
+
+
+def _f32(a):
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+class Closure:
+    """Learned subgrid closure: device dataset, FP32 master parameters, Adam
+    (bl_mlp_*)."""
+
+    def __init__(self, capacity, batch_max=65536):
+        self.h = _P()
+        _check(lib().bl_mlp_create(int(capacity), int(batch_max), ctypes.byref(self.h)), "bl_mlp_create")
+        self.batch_max = batch_max
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().bl_mlp_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def params(self):
+        t = np.zeros(MLP_PARAMS, dtype=np.float32)
+        _check(lib().bl_mlp_get_params(self.h, t.ctypes.data), "bl_mlp_get_params")
+        return t
+
+    @params.setter
+    def params(self, theta):
+        t = _f32(theta)
+        assert t.size == MLP_PARAMS
+        _check(lib().bl_mlp_set_params(self.h, t.ctypes.data), "bl_mlp_set_params")
+
+    def forward(self, X):
+        X = _f32(X)
+        y = np.zeros(X.shape[0], dtype=np.float32)
+        _check(lib().bl_mlp_forward(self.h, X.ctypes.data, X.shape[0], y.ctypes.data), "bl_mlp_forward")
+        return y
+
+    def gradient(self, X, t):
+        X, t = _f32(X), _f32(t)
+        g = np.zeros(MLP_PARAMS, dtype=np.float32)
+        loss = ctypes.c_float()
+        _check(lib().bl_mlp_gradient(self.h, X.ctypes.data, t.ctypes.data, X.shape[0], g.ctypes.data,
+                                     ctypes.byref(loss)), "bl_mlp_gradient")
+        return g, loss.value
+
+    def add_rows(self, rows):
+        r = _f32(rows)
+        assert r.ndim == 2 and r.shape[1] == MLP_ROW
+        _check(lib().bl_mlp_add_rows(self.h, r.ctypes.data, r.shape[0]), "bl_mlp_add_rows")
+
+    def rows(self, first=0, n=None):
+        n = self.count - first if n is None else n
+        r = np.zeros((n, MLP_ROW), dtype=np.float32)
+        _check(lib().bl_mlp_get_rows(self.h, int(first), int(n), r.ctypes.data), "bl_mlp_get_rows")
+        return r
+
+    @property
+    def count(self):
+        n = ctypes.c_longlong()
+        _check(lib().bl_mlp_count(self.h, ctypes.byref(n)), "bl_mlp_count")
+        return n.value
+
+    def clear(self):
+        _check(lib().bl_mlp_clear(self.h), "bl_mlp_clear")
+
+    def train(self, steps, batch, lr=1e-3, seed=0, history=True):
+        hist = np.zeros(steps, dtype=np.float32) if history else None
+        _check(lib().bl_mlp_train(self.h, int(steps), int(batch), float(lr), int(seed),
+                                  hist.ctypes.data if history else None), "bl_mlp_train")
+        return hist

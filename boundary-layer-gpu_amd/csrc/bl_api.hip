@@ -84,23 +84,50 @@ double *field_dev(Handle &h, int field) {
     }
 }
 
-void copy_to_device(Handle &h, double *dev, const double *host, int n1, int n2, int n3) {
+// strided copies between a host array of extent (n1, n2, n3) and the padded
+// device layout; element type E on the device
+template <class E>
+void copy_in(Handle &h, E *dev, const E *host, int n1, int n2, int n3) {
     const Dims &d = h.d;
     for (int k = 0; k < n3; ++k)
-        BL_HIP(hipMemcpy2DAsync(dev + (size_t)k * d.sxy, (size_t)d.sx * sizeof(double),
-                                host + (size_t)k * n1 * n2, (size_t)n1 * sizeof(double),
-                                (size_t)n1 * sizeof(double), n2, hipMemcpyHostToDevice, h.stream));
+        BL_HIP(hipMemcpy2DAsync(dev + (size_t)k * d.sxy, (size_t)d.sx * sizeof(E), host + (size_t)k * n1 * n2,
+                                (size_t)n1 * sizeof(E), (size_t)n1 * sizeof(E), n2, hipMemcpyHostToDevice,
+                                h.stream));
     BL_HIP(hipStreamSynchronize(h.stream));
 }
 
-void copy_to_host(Handle &h, double *host, const double *dev, int n1, int n2, int n3) {
+template <class E>
+void copy_out(Handle &h, E *host, const E *dev, int n1, int n2, int n3) {
     const Dims &d = h.d;
     BL_HIP(hipStreamSynchronize(h.stream));
     for (int k = 0; k < n3; ++k)
-        BL_HIP(hipMemcpy2DAsync(host + (size_t)k * n1 * n2, (size_t)n1 * sizeof(double),
-                                dev + (size_t)k * d.sxy, (size_t)d.sx * sizeof(double),
-                                (size_t)n1 * sizeof(double), n2, hipMemcpyDeviceToHost, h.stream));
+        BL_HIP(hipMemcpy2DAsync(host + (size_t)k * n1 * n2, (size_t)n1 * sizeof(E), dev + (size_t)k * d.sxy,
+                                (size_t)d.sx * sizeof(E), (size_t)n1 * sizeof(E), n2, hipMemcpyDeviceToHost,
+                                h.stream));
     BL_HIP(hipStreamSynchronize(h.stream));
+}
+
+// host FP64 arrays <-> device fields (rounded to float in mixed precision)
+size_t elem_size(const Handle &h) { return h.precision == BL_PREC_MIXED ? sizeof(float) : sizeof(double); }
+
+void copy_to_device(Handle &h, double *dev, const double *host, int n1, int n2, int n3) {
+    if (h.precision != BL_PREC_MIXED) {
+        copy_in(h, dev, host, n1, n2, n3);
+        return;
+    }
+    std::vector<float> tmp((size_t)n1 * n2 * n3);
+    for (size_t q = 0; q < tmp.size(); ++q) tmp[q] = (float)host[q];
+    copy_in(h, reinterpret_cast<float *>(dev), tmp.data(), n1, n2, n3);
+}
+
+void copy_to_host(Handle &h, double *host, const double *dev, int n1, int n2, int n3) {
+    if (h.precision != BL_PREC_MIXED) {
+        copy_out(h, host, dev, n1, n2, n3);
+        return;
+    }
+    std::vector<float> tmp((size_t)n1 * n2 * n3);
+    copy_out(h, tmp.data(), reinterpret_cast<const float *>(dev), n1, n2, n3);
+    for (size_t q = 0; q < tmp.size(); ++q) host[q] = (double)tmp[q];
 }
 
 void check_config(const bl_config *c) {
@@ -115,7 +142,10 @@ void check_config(const bl_config *c) {
         bad("bl_create: missing boundary tables");
     if (c->scheme != 2 && c->scheme != 3) bad("bl_create: scheme must be 2 (RK2) or 3 (RK3)");
     if (c->les != 0 && c->les != 2) bad("bl_create: les must be 0 (DNS) or 2 (dynamic Smagorinsky)");
-    if (c->precision != BL_PREC_FP64) bad("bl_create: only the FP64 path is built in this library");
+    if (c->precision != BL_PREC_FP64 && c->precision != BL_PREC_MIXED)
+        bad("bl_create: precision must be BL_PREC_FP64 or BL_PREC_MIXED");
+    if (c->precision == BL_PREC_MIXED && c->les != 0)
+        bad("bl_create: the mixed-precision path runs DNS (les = 0) only");
     if (c->n_modes < 0 || c->m_modes < 0) bad("bl_create: negative mode count");
     if (c->n_modes > 0 && (c->m_modes <= 0 || !c->zmode || !c->tmode || !c->qu_re || !c->qu_im ||
                            !c->qv_re || !c->qv_im || !c->qw_re || !c->qw_im))
@@ -293,33 +323,68 @@ void build(Handle &h, const bl_config *c) {
         *t.first = blk + off;
         off += round_up((int)t.second.size(), 32);
     }
+    // float copies of the metrics for the FP32 stencils
+    {
+        const std::vector<double> *src[14] = {&g.x,   &g.y,    &g.z,   &g.xg,   &g.yg,   &g.zg,  &g.rdx,
+                                              &g.rdxg, &g.rdy, &g.rdyg, &g.rdz, &g.rdzg, &g.wy0, &g.wy1};
+        const float **dst[14] = {&h.mf.x,   &h.mf.y,   &h.mf.z,    &h.mf.xg,  &h.mf.yg,  &h.mf.zg,  &h.mf.rdx,
+                                 &h.mf.rdxg, &h.mf.rdy, &h.mf.rdyg, &h.mf.rdz, &h.mf.rdzg, &h.mf.wy0, &h.mf.wy1};
+        size_t tot = 0;
+        for (auto *v : src) tot += round_up((int)v->size(), 64);
+        std::vector<float> host(tot, 0.f);
+        size_t o = 0;
+        std::vector<size_t> offs;
+        for (auto *v : src) {
+            offs.push_back(o);
+            for (size_t q = 0; q < v->size(); ++q) host[o + q] = (float)(*v)[q];
+            o += round_up((int)v->size(), 64);
+        }
+        float *fb = nullptr;
+        BL_HIP(hipMalloc(&fb, tot * sizeof(float)));
+        h.mem_gridf = fb;
+        BL_HIP(hipMemcpyAsync(fb, host.data(), tot * sizeof(float), hipMemcpyHostToDevice, h.stream));
+        BL_HIP(hipStreamSynchronize(h.stream));
+        for (int q = 0; q < 14; ++q) *dst[q] = fb + offs[q];
+    }
 
-    // ---- state buffers
+    // ---- state buffers (float in mixed precision)
     const size_t nf = (size_t)d.sxy * d.nzg;
+    const size_t es = h.precision == BL_PREC_MIXED ? sizeof(float) : sizeof(double);
+    auto falloc = [&]() {
+        double *p = dmalloc((nf * es + sizeof(double) - 1) / sizeof(double));
+        BL_HIP(hipMemsetAsync(p, 0, nf * es, h.stream));
+        return p;
+    };
     for (int b = 0; b < 3; ++b) {
-        h.buf[b].u = dmalloc(nf);
-        h.buf[b].v = dmalloc(nf);
-        h.buf[b].w = dmalloc(nf);
-        BL_HIP(hipMemsetAsync(h.buf[b].u, 0, nf * sizeof(double), h.stream));
-        BL_HIP(hipMemsetAsync(h.buf[b].v, 0, nf * sizeof(double), h.stream));
-        BL_HIP(hipMemsetAsync(h.buf[b].w, 0, nf * sizeof(double), h.stream));
+        h.buf[b].u = falloc();
+        h.buf[b].v = falloc();
+        h.buf[b].w = falloc();
     }
     h.icur = h.iold = 0;
-    h.F1.u = dmalloc(nf);
-    h.F1.v = dmalloc(nf);
-    h.F1.w = dmalloc(nf);
-    h.P = dmalloc(nf);
-    h.phi = dmalloc(nf);
-    h.nu_t = dmalloc(nf);
-    for (double *p : {h.F1.u, h.F1.v, h.F1.w, h.P, h.phi, h.nu_t})
-        BL_HIP(hipMemsetAsync(p, 0, nf * sizeof(double), h.stream));
+    h.F1.u = falloc();
+    h.F1.v = falloc();
+    h.F1.w = falloc();
+    h.P = falloc();
+    h.phi = falloc();
+    h.nu_t = falloc();
     if (h.les == 2) {
         h.dsm = dmalloc(nf * kDsmArrays);
         BL_HIP(hipMemsetAsync(h.dsm, 0, nf * kDsmArrays * sizeof(double), h.stream));
     }
     const size_t nspec = (size_t)d.N * d.NM * d.M;
-    h.T1 = dmalloc(nspec);
-    h.S = dmalloc(nspec);
+    if (h.precision == BL_PREC_MIXED) {
+        // FP32 spectra with kx stride NP; T1 doubles as the FP64 Thomas scratch
+        build_mixed(h);
+        const size_t nsf = (size_t)h.NP * d.NM * d.M;
+        const size_t t1 = std::max(nspec, (nsf + 1) / 2), s1 = (nsf + 1) / 2;
+        h.T1 = dmalloc(t1);
+        h.S = dmalloc(s1);
+        BL_HIP(hipMemsetAsync(h.T1, 0, t1 * sizeof(double), h.stream));
+        BL_HIP(hipMemsetAsync(h.S, 0, s1 * sizeof(double), h.stream));
+    } else {
+        h.T1 = dmalloc(nspec);
+        h.S = dmalloc(nspec);
+    }
 
     const int N = d.N, M = d.M;
     // x-DCT tables: N = N1*N2 with N1 the largest divisor <= sqrt(N)
@@ -422,7 +487,8 @@ void free_all(Handle &h) {
     for (void *p : {(void *)h.F1.u, (void *)h.F1.v, (void *)h.F1.w, (void *)h.P, (void *)h.phi,
                     (void *)h.nu_t, (void *)h.T1, (void *)h.S, (void *)h.dct_w1, (void *)h.dct_w2,
                     (void *)h.dct_wN, (void *)h.dct_rot, (void *)h.zc, (void *)h.zs, (void *)h.lu_dl, (void *)h.lu_dp, (void *)h.lu_du,
-                    (void *)h.scal, (void *)h.red, (void *)h.dsm, h.mem_grid})
+                    (void *)h.scal, (void *)h.red, (void *)h.dsm, h.mem_grid, h.mem_gridf, (void *)h.mx_tab,
+                    (void *)h.p00})
         f(p);
     if (h.stream) (void)hipStreamSynchronize(h.stream);
     prof_free(h);
@@ -431,17 +497,32 @@ void free_all(Handle &h) {
 
 // one projection sub-step on the current buffer (projection.f90:21-87)
 void projection_on(Handle &h, Field3 &f, int rk_step, double dt) {
+    if (h.precision == BL_PREC_MIXED) {
+        Fld3<float> ff = as<float>(f);
+        float *phi = reinterpret_cast<float *>(h.phi);
+        launch_poisson_mixed(h, phi, ff);
+        launch_project(h, ff, (const float *)phi);
+        if (rk_step == 1) launch_pressure(h, (const float *)phi, reinterpret_cast<float *>(h.P), dt);
+        return;
+    }
     launch_poisson(h, h.phi, &f);
-    launch_project(h, f, h.phi);
-    if (rk_step == 1) launch_pressure(h, h.phi, h.P, dt);
+    launch_project(h, f, (const double *)h.phi);
+    if (rk_step == 1) launch_pressure(h, (const double *)h.phi, h.P, dt);
 }
 
-void do_step(Handle &h) {
+template <class T>
+void bc_on(Handle &h, Field3 &f, const Field3 &old, int rk_step, double dt, double t) {
+    Fld3<T> ff = as<T>(f);
+    launch_bc(h, ff, as<T>(old), rk_step, dt, t);
+}
+
+template <class T>
+void do_step_t(Handle &h) {
     const double dt = h.dt;
     const double to = h.t;
     const int X = h.icur;
     const int Y = (X + 1) % 3, Z = (X + 2) % 3;
-    Field3 old = h.buf[X];
+    const Fld3<T> old = as<T>(h.buf[X]);
     const int nst = h.scheme == 3 ? 3 : 2;
     const int order3[3] = {Y, Z, Y};
     int cur = X;
@@ -449,15 +530,23 @@ void do_step(Handle &h) {
         const int nxt = order3[s - 1];
         // eddy viscosity of the stage's input fields (time_integration.f90:341-346)
         if (h.les == 2) launch_dsm(h, h.buf[cur]);
-        launch_rk_stage(h, h.buf[cur], old, h.buf[nxt], s, dt);
+        Fld3<T> next = as<T>(h.buf[nxt]);
+        launch_rk_stage(h, as<T>(h.buf[cur]), old, next, s, dt);
         const double ts = to + (nst == 3 ? h.rk_t[s] : h.rk2_t[s]) * dt;
-        launch_bc(h, h.buf[nxt], old, s, dt, ts);
+        launch_bc(h, next, old, s, dt, ts);
         projection_on(h, h.buf[nxt], s, dt);
-        launch_bc(h, h.buf[nxt], old, s, dt, ts);
+        launch_bc(h, next, old, s, dt, ts);
         h.t = ts;
         cur = nxt;
     }
     h.icur = h.iold = cur;
+}
+
+void do_step(Handle &h) {
+    if (h.precision == BL_PREC_MIXED)
+        do_step_t<float>(h);
+    else
+        do_step_t<double>(h);
 }
 
 }  // namespace
@@ -644,7 +733,8 @@ int bl_compute_dt(bl_handle *hh, double *dt) {
         if (h.cfl < 0) {
             h.dt = -h.cfl;
         } else {
-            const double dt_conv = dt_convective(h, h.buf[h.icur]);
+            const double dt_conv = h.precision == BL_PREC_MIXED ? dt_convective(h, as<float>(h.buf[h.icur]))
+                                                                : dt_convective(h, as<double>(h.buf[h.icur]));
             const double dvu = CFLa * std::pow(h.dxmin, 2.0) / h.nu;
             const double dvv = CFLa * std::pow(h.dymin, 2.0) / h.nu;
             const double dvw = CFLa * std::pow(h.dzmin, 2.0) / h.nu;
@@ -687,7 +777,7 @@ int bl_eddy_viscosity(bl_handle *hh) {
         if (h.les == 2)
             launch_dsm(h, h.buf[h.icur]);
         else
-            BL_HIP(hipMemsetAsync(h.nu_t, 0, (size_t)h.d.sxy * h.d.nzg * sizeof(double), h.stream));
+            BL_HIP(hipMemsetAsync(h.nu_t, 0, (size_t)h.d.sxy * h.d.nzg * elem_size(h), h.stream));
     });
 }
 
@@ -695,7 +785,8 @@ int bl_max_divergence(bl_handle *hh, double *div) {
     return guarded([&] {
         Handle &h = H(hh);
         if (!div) throw ArgError();
-        *div = max_divergence(h, h.buf[h.icur]);
+        *div = h.precision == BL_PREC_MIXED ? max_divergence(h, as<float>(h.buf[h.icur]))
+                                            : max_divergence(h, as<double>(h.buf[h.icur]));
     });
 }
 
@@ -703,7 +794,10 @@ int bl_wall_stats(bl_handle *hh, double *cf) {
     return guarded([&] {
         Handle &h = H(hh);
         if (!cf) throw ArgError();
-        wall_stats(h, h.buf[h.icur], cf);
+        if (h.precision == BL_PREC_MIXED)
+            wall_stats(h, as<float>(h.buf[h.icur]), cf);
+        else
+            wall_stats(h, as<double>(h.buf[h.icur]), cf);
     });
 }
 
@@ -711,7 +805,10 @@ int bl_monitor(bl_handle *hh, double *out) {
     return guarded([&] {
         Handle &h = H(hh);
         if (!out) throw ArgError();
-        monitor(h, h.buf[h.icur], out);
+        if (h.precision == BL_PREC_MIXED)
+            monitor(h, as<float>(h.buf[h.icur]), out);
+        else
+            monitor(h, as<double>(h.buf[h.icur]), out);
     });
 }
 
@@ -720,8 +817,13 @@ int bl_rhs(bl_handle *hh, double *ru, double *rv, double *rw) {
         Handle &h = H(hh);
         if (!ru || !rv || !rw) throw ArgError();
         const size_t nf = (size_t)h.d.sxy * h.d.nzg;
-        for (double *p : {h.F1.u, h.F1.v, h.F1.w}) BL_HIP(hipMemsetAsync(p, 0, nf * sizeof(double), h.stream));
-        launch_rhs_only(h, h.buf[h.icur], h.F1);
+        for (double *p : {h.F1.u, h.F1.v, h.F1.w}) BL_HIP(hipMemsetAsync(p, 0, nf * elem_size(h), h.stream));
+        if (h.precision == BL_PREC_MIXED) {
+            Fld3<float> r = as<float>(h.F1);
+            launch_rhs_only(h, as<float>(h.buf[h.icur]), r);
+        } else {
+            launch_rhs_only(h, as<double>(h.buf[h.icur]), h.F1);
+        }
         int n1, n2, n3;
         field_extent(h.d, BL_FIELD_U, n1, n2, n3);
         copy_to_host(h, ru, h.F1.u, n1, n2, n3);
@@ -752,7 +854,10 @@ int bl_apply_bc(bl_handle *hh, int rk_step, double dt) {
     return guarded([&] {
         Handle &h = H(hh);
         if (rk_step < 1 || rk_step > 3) throw ArgError();
-        launch_bc(h, h.buf[h.icur], h.buf[h.iold], rk_step, dt, h.t);
+        if (h.precision == BL_PREC_MIXED)
+            bc_on<float>(h, h.buf[h.icur], h.buf[h.iold], rk_step, dt, h.t);
+        else
+            bc_on<double>(h, h.buf[h.icur], h.buf[h.iold], rk_step, dt, h.t);
         BL_HIP(hipStreamSynchronize(h.stream));
     });
 }
@@ -802,6 +907,11 @@ int bl_poisson(bl_handle *hh, double *rhs) {
     return guarded([&] {
         Handle &h = H(hh);
         if (!rhs) throw ArgError();
+        if (h.precision == BL_PREC_MIXED) {
+            last_error() = "bl_poisson: the stand-alone Poisson solve is the FP64 path (the mixed path forms its "
+                           "rhs from the fields, see bl_projection)";
+            throw ArgError();
+        }
         int n1, n2, n3;
         field_extent(h.d, BL_FIELD_P, n1, n2, n3);
         copy_to_device(h, h.phi, rhs, n1, n2, n3);

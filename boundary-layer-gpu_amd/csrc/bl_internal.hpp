@@ -27,17 +27,20 @@ __host__ __device__ inline long long fidx(const Dims &d, int i, int j, int k) {
     return (long long)i + (long long)d.sx * ((long long)j + (long long)d.sy * k);
 }
 
-// Metric arrays (0-based, reciprocal spacings) living on the device.
-struct Metrics {
-    const double *x, *y, *z, *xg, *yg, *zg;
-    const double *rdx;   // 1/(x[i]-x[i-1]), i>=1
-    const double *rdxg;  // 1/(xg[i+1]-xg[i])
-    const double *rdy;   // 1/(y[j]-y[j-1]), j>=1
-    const double *rdyg;  // 1/(yg[j+1]-yg[j])
-    const double *rdz;   // 1/(z[k]-z[k-1]), k>=1
-    const double *rdzg;  // 1/(zg[k+1]-zg[k])
-    const double *wy0, *wy1;
+// Metric arrays (0-based, reciprocal spacings) living on the device, in the
+// arithmetic type of the kernels that read them.
+template <class T>
+struct MetricsT {
+    const T *x, *y, *z, *xg, *yg, *zg;
+    const T *rdx;   // 1/(x[i]-x[i-1]), i>=1
+    const T *rdxg;  // 1/(xg[i+1]-xg[i])
+    const T *rdy;   // 1/(y[j]-y[j-1]), j>=1
+    const T *rdyg;  // 1/(yg[j+1]-yg[j])
+    const T *rdz;   // 1/(z[k]-z[k-1]), k>=1
+    const T *rdzg;  // 1/(zg[k+1]-zg[k])
+    const T *wy0, *wy1;
 };
+using Metrics = MetricsT<double>;
 
 struct BCTables {
     const double *U_inlet, *V_inlet, *W_inlet, *U_top, *V_top, *W_top;
@@ -45,9 +48,17 @@ struct BCTables {
     const double *zmode, *tmode, *qu_re, *qu_im, *qv_re, *qv_im, *qw_re, *qw_im;
 };
 
-struct Field3 {
-    double *u, *v, *w;
+template <class T>
+struct Fld3 {
+    T *u, *v, *w;
 };
+// Storage of a velocity triple.  In mixed precision the three arrays hold
+// float; as<float>() gives the typed view.
+using Field3 = Fld3<double>;
+template <class T>
+inline Fld3<T> as(const Field3 &f) {
+    return {reinterpret_cast<T *>(f.u), reinterpret_cast<T *>(f.v), reinterpret_cast<T *>(f.w)};
+}
 
 // Kernel classes timed by the optional HIP-event profiler.
 enum KernelId {
@@ -95,7 +106,7 @@ struct Handle {
     int iold;       // index of the step-start buffer (Uo)
     int icur;       // index of the current (most recent) buffer
     Field3 F1;      // first-stage RHS (RK3)
-    double *P, *phi, *nu_t;
+    double *P, *phi, *nu_t;  // float in mixed precision
     double *dsm;  // dynamic Smagorinsky work arrays (LES = 2 only), kDsmArrays field-sized slots
     double *T1, *S;  // transform intermediates: [m][j][kx] and [q][j][kx]
     int dct_n1;                      // x-DCT factorisation N = dct_n1 * (N / dct_n1)
@@ -106,8 +117,18 @@ struct Handle {
     double *scal;                   // device scalars (mass-flux correction, reductions)
     double *red;                    // reduction scratch
     void *mem_grid;                 // one allocation for metrics and tables
+    void *mem_gridf;                // the float metric copies
     Metrics m;
+    MetricsT<float> mf;             // float copies for the FP32 stencils
     BCTables bc;
+    // mixed precision: BF16 hi/lo transform matrices, FP64 (0,0) mode
+    int NP;                         // kx stride of the FP32 spectral arrays (N rounded up to 32)
+    int kpx, kpz;                   // GEMM depths N and M rounded up to 32
+    unsigned short *mx_tab;         // one allocation for the eight matrices below
+    const unsigned short *dxf_hi, *dxf_lo, *dxi_hi, *dxi_lo, *zf_hi, *zf_lo, *zi_hi, *zi_lo;
+    double *p00;                    // FP64 plane partials of the divergence
+    double *c00;                    // phi00(j) / (2 N M)
+    int np00;                       // partials per y row
     ProfState *prof;                // null unless profiling is enabled
 };
 
@@ -122,20 +143,32 @@ struct ProfScope {
     ~ProfScope() { prof_end(h, id); }
 };
 
-// ---- kernels (bl_kernels.hip) ----
-void launch_rk_stage(Handle &h, const Field3 &cur, const Field3 &old, Field3 &nxt, int stage,
-                     double dt);
-void launch_bc(Handle &h, Field3 &f, const Field3 &old, int rk_step, double dt, double t);
-void launch_divergence(Handle &h, const Field3 &f, double *out);
+// ---- kernels (bl_kernels.hip); T = double (FP64 mode) or float (mixed) ----
+template <class T>
+const MetricsT<T> &metrics(const Handle &h);
+template <class T>
+void launch_rk_stage(Handle &h, const Fld3<T> &cur, const Fld3<T> &old, Fld3<T> &nxt, int stage, double dt);
+template <class T>
+void launch_bc(Handle &h, Fld3<T> &f, const Fld3<T> &old, int rk_step, double dt, double t);
 // div != null: the rhs is the divergence of *div (computed inside the x transform)
 void launch_poisson(Handle &h, double *phi_inout, const Field3 *div = nullptr);
-void launch_project(Handle &h, Field3 &f, const double *phi);
-void launch_pressure(Handle &h, const double *phi, double *P, double dt);
-void launch_rhs_only(Handle &h, const Field3 &cur, Field3 &rhs);
-double max_divergence(Handle &h, const Field3 &f);
-void wall_stats(Handle &h, const Field3 &f, double *cf_host);
-void monitor(Handle &h, const Field3 &f, double *out_host);
-double dt_convective(Handle &h, const Field3 &f);
+template <class T>
+void launch_project(Handle &h, Fld3<T> &f, const T *phi);
+template <class T>
+void launch_pressure(Handle &h, const T *phi, T *P, double dt);
+template <class T>
+void launch_rhs_only(Handle &h, const Fld3<T> &cur, Fld3<T> &rhs);
+template <class T>
+double max_divergence(Handle &h, const Fld3<T> &f);
+template <class T>
+void wall_stats(Handle &h, const Fld3<T> &f, double *cf_host);
+template <class T>
+void monitor(Handle &h, const Fld3<T> &f, double *out_host);
+template <class T>
+double dt_convective(Handle &h, const Fld3<T> &f);
+// ---- mixed precision (bl_mixed.hip) ----
+void build_mixed(Handle &h);
+void launch_poisson_mixed(Handle &h, float *phi, const Fld3<float> &f);
 // dynamic Smagorinsky eddy viscosity of f into h.nu_t (bl_dsm.hip)
 constexpr int kDsmArrays = 15;
 void launch_dsm(Handle &h, const Field3 &f);

@@ -23,7 +23,10 @@ namespace {
 
 constexpr int kWave = 64;
 
-__device__ __forceinline__ double sq(double a) { return a * a; }
+template <class T>
+__device__ __forceinline__ T sq(T a) {
+    return a * a;
+}
 
 // ------------------------------------------------------------------------
 // Fused RHS + Runge-Kutta stage update, marching in z.
@@ -40,17 +43,17 @@ constexpr int kTX = 64, kTY = 8, kHX = kTX + 2, kHY = kTY + 2, kPlane = kHX * kH
 
 // LES: the viscous terms use nu + nu_t (equations.f90:97-132, 231-270,
 // 367-404); nu_t is staged in a fourth LDS ring.
-template <int MODE, bool WRITE_F1, bool LES>
+template <int MODE, bool WRITE_F1, bool LES, class T>
 __global__ __launch_bounds__(256) void k_rk_march(
-    Dims d, Metrics m, double nu, double dPdx, double dPdz, int kchunk,
-    const double *__restrict__ U, const double *__restrict__ V, const double *__restrict__ W,
-    const double *__restrict__ NT,
-    const double *__restrict__ Uo, const double *__restrict__ Vo, const double *__restrict__ Wo,
-    const double *__restrict__ F1u, const double *__restrict__ F1v, const double *__restrict__ F1w,
-    double *__restrict__ Un, double *__restrict__ Vn, double *__restrict__ Wn, double *__restrict__ Fou,
-    double *__restrict__ Fov, double *__restrict__ Fow, double c0, double c1, double dt) {
-    __shared__ double sU[3][kPlane], sV[3][kPlane], sW[3][kPlane];
-    __shared__ double sN[LES ? 3 : 1][LES ? kPlane : 1];
+    Dims d, MetricsT<T> m, T nu, T dPdx, T dPdz, int kchunk,
+    const T *__restrict__ U, const T *__restrict__ V, const T *__restrict__ W,
+    const T *__restrict__ NT,
+    const T *__restrict__ Uo, const T *__restrict__ Vo, const T *__restrict__ Wo,
+    const T *__restrict__ F1u, const T *__restrict__ F1v, const T *__restrict__ F1w,
+    T *__restrict__ Un, T *__restrict__ Vn, T *__restrict__ Wn, T *__restrict__ Fou,
+    T *__restrict__ Fov, T *__restrict__ Fow, T c0, T c1, T dt) {
+    __shared__ T sU[3][kPlane], sV[3][kPlane], sW[3][kPlane];
+    __shared__ T sN[LES ? 3 : 1][LES ? kPlane : 1];
     const int tid = threadIdx.x;
     const int tx = tid & 63, ty = tid >> 6;
     const int i0 = blockIdx.x * kTX + 1, j0 = blockIdx.y * kTY + 1;
@@ -70,7 +73,7 @@ __global__ __launch_bounds__(256) void k_rk_march(
         const int ii = min(i0 - 1 + hx, d.sx - 1), jj = min(j0 - 1 + hy, d.nyg - 1);
         off[q] = (long long)jj * sx + ii;
     }
-    double ru[kPer], rv[kPer], rw[kPer], rn[LES ? kPer : 1];
+    T ru[kPer], rv[kPer], rw[kPer], rn[LES ? kPer : 1];
     auto fetch = [&](int k) {
         const long long base = (long long)k * sxy;
 #pragma unroll
@@ -104,7 +107,7 @@ __global__ __launch_bounds__(256) void k_rk_march(
         __syncthreads();
         if (k + 2 <= ke) fetch(k + 2);
         const int sm = (k - 1) % 3, sc = k % 3, sp = (k + 1) % 3;
-        const double rdzk = m.rdz[k], rdzgk = m.rdzg[k], rdzgm = m.rdzg[k - 1];
+        const T rdzk = m.rdz[k], rdzgk = m.rdzg[k], rdzgm = m.rdzg[k - 1];
 #pragma unroll
         for (int r = 0; r < 2; ++r) {
             const int jt = 2 * ty + r;       // row in the tile
@@ -114,11 +117,11 @@ __global__ __launch_bounds__(256) void k_rk_march(
             const long long p = fidx(d, i, j, k);
             // ---- u (equations.f90:47-94, 149-154)
             if (i <= d.nx - 2) {
-                double rr = 0.0;
+                T rr = 0.0;
                 if (i < d.nx - 2) {
-                    const double up = sU[sc][e], ue = sU[sc][e + 1], uw = sU[sc][e - 1];
-                    const double un = sU[sc][e + kHX], us = sU[sc][e - kHX];
-                    const double ut = sU[sp][e], ub = sU[sm][e];
+                    const T up = sU[sc][e], ue = sU[sc][e + 1], uw = sU[sc][e - 1];
+                    const T un = sU[sc][e + kHX], us = sU[sc][e - kHX];
+                    const T ut = sU[sp][e], ub = sU[sm][e];
                     rr = -(sq(0.5 * (up + ue)) - sq(0.5 * (uw + up))) * m.rdxg[i];
                     rr = rr - ((m.wy0[j] * up + m.wy1[j] * un) * 0.5 * (sV[sc][e] + sV[sc][e + 1]) -
                                (m.wy0[j - 1] * us + m.wy1[j - 1] * up) * 0.5 *
@@ -133,14 +136,14 @@ __global__ __launch_bounds__(256) void k_rk_march(
                                    ((un - up) * m.rdyg[j] - (up - us) * m.rdyg[j - 1]) * m.rdy[j] +
                                    (rdzgk * (ut - up) - rdzgm * (up - ub)) * rdzk);
                     } else {
-                        const double *N = sN[sc], *Nm = sN[sm], *Np = sN[sp];
-                        const double a0 = m.wy0[j - 1], a1 = m.wy1[j - 1], b0 = m.wy0[j], b1 = m.wy1[j];
-                        const double nx1 = nu + N[e];
-                        const double nx2 = nu + N[e + 1];
-                        const double ny1 = nu + 0.5 * (a0 * N[e - kHX] + a1 * N[e] + a0 * N[e + 1 - kHX] + a1 * N[e + 1]);
-                        const double ny2 = nu + 0.5 * (b0 * N[e] + b1 * N[e + kHX] + b0 * N[e + 1] + b1 * N[e + 1 + kHX]);
-                        const double nz1 = nu + 0.25 * (N[e] + Nm[e] + N[e + 1] + Nm[e + 1]);
-                        const double nz2 = nu + 0.25 * (N[e] + Np[e] + N[e + 1] + Np[e + 1]);
+                        const T *N = sN[sc], *Nm = sN[sm], *Np = sN[sp];
+                        const T a0 = m.wy0[j - 1], a1 = m.wy1[j - 1], b0 = m.wy0[j], b1 = m.wy1[j];
+                        const T nx1 = nu + N[e];
+                        const T nx2 = nu + N[e + 1];
+                        const T ny1 = nu + 0.5 * (a0 * N[e - kHX] + a1 * N[e] + a0 * N[e + 1 - kHX] + a1 * N[e + 1]);
+                        const T ny2 = nu + 0.5 * (b0 * N[e] + b1 * N[e + kHX] + b0 * N[e + 1] + b1 * N[e + 1 + kHX]);
+                        const T nz1 = nu + 0.25 * (N[e] + Nm[e] + N[e + 1] + Nm[e + 1]);
+                        const T nz2 = nu + 0.25 * (N[e] + Np[e] + N[e + 1] + Np[e + 1]);
                         rr = rr + dPdx +
                              (nx2 * m.rdx[i + 1] * (ue - up) - nx1 * m.rdx[i] * (up - uw)) * m.rdxg[i] +
                              (ny2 * ((un - up) * m.rdyg[j]) - ny1 * ((up - us) * m.rdyg[j - 1])) * m.rdy[j] +
@@ -156,11 +159,11 @@ __global__ __launch_bounds__(256) void k_rk_march(
             }
             // ---- v (equations.f90:183-230, 287-292)
             if (j <= d.ny - 2) {
-                double rr = 0.0;
+                T rr = 0.0;
                 if (i < d.nxg - 2) {
-                    const double vp = sV[sc][e], ve = sV[sc][e + 1], vw = sV[sc][e - 1];
-                    const double vn = sV[sc][e + kHX], vs = sV[sc][e - kHX];
-                    const double vt = sV[sp][e], vb = sV[sm][e];
+                    const T vp = sV[sc][e], ve = sV[sc][e + 1], vw = sV[sc][e - 1];
+                    const T vn = sV[sc][e + kHX], vs = sV[sc][e - kHX];
+                    const T vt = sV[sp][e], vb = sV[sm][e];
                     rr = -(sq(0.5 * (vp + vn)) - sq(0.5 * (vs + vp))) * m.rdyg[j];
                     rr = rr - ((m.wy0[j] * sU[sc][e] + m.wy1[j] * sU[sc][e + kHX]) * 0.5 * (vp + ve) -
                                (m.wy0[j] * sU[sc][e - 1] + m.wy1[j] * sU[sc][e - 1 + kHX]) * 0.5 * (vw + vp)) *
@@ -173,14 +176,14 @@ __global__ __launch_bounds__(256) void k_rk_march(
                                         (m.rdy[j + 1] * (vn - vp) - m.rdy[j] * (vp - vs)) * m.rdyg[j] +
                                         (rdzgk * (vt - vp) - rdzgm * (vp - vb)) * rdzk);
                     } else {
-                        const double *N = sN[sc], *Nm = sN[sm], *Np = sN[sp];
-                        const double b0 = m.wy0[j], b1 = m.wy1[j];
-                        const double nx1 = nu + 0.5 * (b0 * N[e - 1] + b1 * N[e - 1 + kHX] + b0 * N[e] + b1 * N[e + kHX]);
-                        const double nx2 = nu + 0.5 * (b0 * N[e] + b1 * N[e + kHX] + b0 * N[e + 1] + b1 * N[e + 1 + kHX]);
-                        const double ny1 = nu + N[e];
-                        const double ny2 = nu + N[e + kHX];
-                        const double nz1 = nu + 0.5 * (b0 * Nm[e] + b1 * Nm[e + kHX] + b0 * N[e] + b1 * N[e + kHX]);
-                        const double nz2 = nu + 0.5 * (b0 * N[e] + b1 * N[e + kHX] + b0 * Np[e] + b1 * Np[e + kHX]);
+                        const T *N = sN[sc], *Nm = sN[sm], *Np = sN[sp];
+                        const T b0 = m.wy0[j], b1 = m.wy1[j];
+                        const T nx1 = nu + 0.5 * (b0 * N[e - 1] + b1 * N[e - 1 + kHX] + b0 * N[e] + b1 * N[e + kHX]);
+                        const T nx2 = nu + 0.5 * (b0 * N[e] + b1 * N[e + kHX] + b0 * N[e + 1] + b1 * N[e + 1 + kHX]);
+                        const T ny1 = nu + N[e];
+                        const T ny2 = nu + N[e + kHX];
+                        const T nz1 = nu + 0.5 * (b0 * Nm[e] + b1 * Nm[e + kHX] + b0 * N[e] + b1 * N[e + kHX]);
+                        const T nz2 = nu + 0.5 * (b0 * N[e] + b1 * N[e + kHX] + b0 * Np[e] + b1 * Np[e + kHX]);
                         rr = rr + m.rdx[i] * (nx2 * m.rdxg[i] * (ve - vp) - nx1 * m.rdxg[i - 1] * (vp - vw)) +
                              m.rdyg[j] * (ny2 * m.rdy[j + 1] * (vn - vp) - ny1 * m.rdy[j] * (vp - vs)) +
                              rdzk * (nz2 * rdzgk * (vt - vp) - nz1 * rdzgm * (vp - vb));
@@ -195,11 +198,11 @@ __global__ __launch_bounds__(256) void k_rk_march(
             }
             // ---- w (equations.f90:323-366, 408-413)
             if (k <= d.nz - 2) {
-                double rr = 0.0;
+                T rr = 0.0;
                 if (i < d.nxg - 2) {
-                    const double wp = sW[sc][e], we = sW[sc][e + 1], ww = sW[sc][e - 1];
-                    const double wn = sW[sc][e + kHX], ws = sW[sc][e - kHX];
-                    const double wt = sW[sp][e], wb = sW[sm][e];
+                    const T wp = sW[sc][e], we = sW[sc][e + 1], ww = sW[sc][e - 1];
+                    const T wn = sW[sc][e + kHX], ws = sW[sc][e - kHX];
+                    const T wt = sW[sp][e], wb = sW[sm][e];
                     rr = -(sq(0.5 * (wp + wt)) - sq(0.5 * (wb + wp))) * rdzgk;
                     rr = rr - (0.5 * (sU[sc][e] + sU[sp][e]) * 0.5 * (wp + we) -
                                0.5 * (sU[sc][e - 1] + sU[sp][e - 1]) * 0.5 * (ww + wp)) *
@@ -213,14 +216,14 @@ __global__ __launch_bounds__(256) void k_rk_march(
                                    ((wn - wp) * m.rdyg[j] - (wp - ws) * m.rdyg[j - 1]) * m.rdy[j] +
                                    (m.rdz[k + 1] * (wt - wp) - rdzk * (wp - wb)) * rdzgk);
                     } else {
-                        const double *N = sN[sc], *Np = sN[sp];
-                        const double a0 = m.wy0[j - 1], a1 = m.wy1[j - 1], b0 = m.wy0[j], b1 = m.wy1[j];
-                        const double nx1 = nu + 0.25 * (N[e] + Np[e] + N[e - 1] + Np[e - 1]);
-                        const double nx2 = nu + 0.25 * (N[e] + Np[e] + N[e + 1] + Np[e + 1]);
-                        const double ny1 = nu + 0.5 * (a0 * N[e - kHX] + a1 * N[e] + a0 * Np[e - kHX] + a1 * Np[e]);
-                        const double ny2 = nu + 0.5 * (b0 * N[e] + b1 * N[e + kHX] + b0 * Np[e] + b1 * Np[e + kHX]);
-                        const double nz1 = nu + N[e];
-                        const double nz2 = nu + Np[e];
+                        const T *N = sN[sc], *Np = sN[sp];
+                        const T a0 = m.wy0[j - 1], a1 = m.wy1[j - 1], b0 = m.wy0[j], b1 = m.wy1[j];
+                        const T nx1 = nu + 0.25 * (N[e] + Np[e] + N[e - 1] + Np[e - 1]);
+                        const T nx2 = nu + 0.25 * (N[e] + Np[e] + N[e + 1] + Np[e + 1]);
+                        const T ny1 = nu + 0.5 * (a0 * N[e - kHX] + a1 * N[e] + a0 * Np[e - kHX] + a1 * Np[e]);
+                        const T ny2 = nu + 0.5 * (b0 * N[e] + b1 * N[e + kHX] + b0 * Np[e] + b1 * Np[e + kHX]);
+                        const T nz1 = nu + N[e];
+                        const T nz2 = nu + Np[e];
                         rr = rr + dPdz +
                              (nx2 * m.rdxg[i] * (we - wp) - nx1 * m.rdxg[i - 1] * (wp - ww)) * m.rdx[i] +
                              (ny2 * ((wn - wp) * m.rdyg[j]) - ny1 * ((wp - ws) * m.rdyg[j - 1])) * m.rdy[j] +
@@ -256,10 +259,10 @@ __device__ __forceinline__ double block_sum_256(double v, double *sh) {
     return sh[0];
 }
 
-__global__ __launch_bounds__(256) void k_bc_inout(Dims d, Metrics m, BCTables b, double *U,
-                                                  double *V, double *W, const double *Uo,
-                                                  const double *Vo, const double *Wo,
-                                                  double coef, double t, double *qpart) {
+template <class T>
+__global__ __launch_bounds__(256) void k_bc_inout(Dims d, Metrics m, BCTables b, T *U, T *V, T *W,
+                                                  const T *Uo, const T *Vo, const T *Wo, double coef,
+                                                  double t, double *qpart) {
     __shared__ double sh[256];
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
     const int k = blockIdx.y;
@@ -291,31 +294,31 @@ __global__ __launch_bounds__(256) void k_bc_inout(Dims d, Metrics m, BCTables b,
     }
     const long long p0 = fidx(d, 0, j, k);
     // U (all j < nyg, k < nzg)
-    const double uin = b.U_inlet[j] + ut;
-    U[p0] = uin;
+    const double uin = (double)(T)(b.U_inlet[j] + ut);  // the value stored
+    U[p0] = (T)uin;
     {
         const long long pa = fidx(d, nx - 2, j, k);
         const double o1 = Uo[pa], o0 = Uo[pa - 1];
-        const double v = o1 - coef * (o1 - o0) / (m.x[nx - 2] - m.x[nx - 3]);
-        U[pa] = v;
-        U[pa + 1] = v;
+        const double v = (double)(T)(o1 - coef * (o1 - o0) / (m.x[nx - 2] - m.x[nx - 3]));
+        U[pa] = (T)v;
+        U[pa + 1] = (T)v;
         if (j >= 1 && j <= d.nyg - 2 && k >= 1 && k <= d.nzg - 3) q = (uin - v) * (m.y[j] - m.y[j - 1]);
     }
     if (k < d.nz) {
-        W[p0] = b.W_inlet[j] + wt;
+        W[p0] = (T)(b.W_inlet[j] + wt);
         const long long pa = fidx(d, nxg - 2, j, k);
         const double o1 = Wo[pa], o0 = Wo[pa - 1];
         const double v = o1 - coef * (o1 - o0) / (m.xg[nxg - 2] - m.xg[nxg - 3]);
-        W[pa] = v;
-        W[pa + 1] = v;
+        W[pa] = (T)v;
+        W[pa + 1] = (T)v;
     }
     if (j < d.ny) {
-        V[p0] = b.V_inlet[j] + vt;
+        V[p0] = (T)(b.V_inlet[j] + vt);
         const long long pa = fidx(d, nxg - 2, j, k);
         const double o1 = Vo[pa], o0 = Vo[pa - 1];
         const double v = o1 - coef * (o1 - o0) / (m.xg[nxg - 2] - m.xg[nxg - 3]);
-        V[pa] = v;
-        V[pa + 1] = v;
+        V[pa] = (T)v;
+        V[pa + 1] = (T)v;
     }
     }
     q = block_sum_256(q, sh);
@@ -323,7 +326,8 @@ __global__ __launch_bounds__(256) void k_bc_inout(Dims d, Metrics m, BCTables b,
 }
 
 // periodic z (:205-217); one thread per (i,j)
-__global__ __launch_bounds__(256) void k_bc_periodic(Dims d, double *U, double *V, double *W) {
+template <class T>
+__global__ __launch_bounds__(256) void k_bc_periodic(Dims d, T *U, T *V, T *W) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int j = blockIdx.y;
     if (i >= d.nxg) return;
@@ -344,28 +348,29 @@ __global__ __launch_bounds__(256) void k_bc_periodic(Dims d, double *U, double *
 }
 
 // top Dirichlet (:220-228) then wall (:231-238); one thread per (i,k)
-__global__ __launch_bounds__(256) void k_bc_topwall(Dims d, Metrics m, BCTables b, double *U,
-                                                    double *V, double *W, double *qpart) {
+template <class T>
+__global__ __launch_bounds__(256) void k_bc_topwall(Dims d, Metrics m, BCTables b, T *U, T *V, T *W,
+                                                    double *qpart) {
     __shared__ double sh[256];
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int k = blockIdx.y;
     // mass flux through the y boundaries (boundary_conditions.f90:251-253) from
     // the values written here: (V(i,1,k) - V(i,ny,k))*(x(i)-x(i-1))
     double q = 0.0;
-    if (i >= 1 && i <= d.nxg - 3 && k >= 1 && k <= d.nzg - 3) q = (0.0 - b.V_top[i]) * (m.x[i] - m.x[i - 1]);
+    if (i >= 1 && i <= d.nxg - 3 && k >= 1 && k <= d.nzg - 3) q = (0.0 - (double)(T)b.V_top[i]) * (m.x[i] - m.x[i - 1]);
     q = block_sum_256(q, sh);
     if (threadIdx.x == 0) qpart[blockIdx.x + gridDim.x * blockIdx.y] = q;
     if (i >= d.nxg) return;
     const long long sx = d.sx;
     const long long p0 = fidx(d, i, 0, k);
     if (i < d.nx) {
-        U[p0 + sx * (d.nyg - 1)] = b.U_top[i];
+        U[p0 + sx * (d.nyg - 1)] = (T)b.U_top[i];
         U[p0] = -U[p0 + sx];
     }
-    V[p0 + sx * (d.ny - 1)] = b.V_top[i];
-    V[p0] = 0.0;
+    V[p0 + sx * (d.ny - 1)] = (T)b.V_top[i];
+    V[p0] = (T)0;
     if (k < d.nz) {
-        W[p0 + sx * (d.nyg - 1)] = b.W_top[i];
+        W[p0 + sx * (d.nyg - 1)] = (T)b.W_top[i];
         W[p0] = -W[p0 + sx];
     }
 }
@@ -373,8 +378,9 @@ __global__ __launch_bounds__(256) void k_bc_topwall(Dims d, Metrics m, BCTables 
 // global mass conservation (:241-260): every block re-sums the flux partials
 // in the same order (so all agree bit for bit) and corrects its part of the
 // outflow planes.  One thread per (j,k).
+template <class T>
 __global__ __launch_bounds__(256) void k_bc_massflux(Dims d, double length_y, const double *qpart,
-                                                     int nq, double *U) {
+                                                     int nq, T *U) {
     __shared__ double sh[256];
     double acc = 0.0;
     for (int t = threadIdx.x; t < nq; t += 256) acc += qpart[t];
@@ -384,27 +390,14 @@ __global__ __launch_bounds__(256) void k_bc_massflux(Dims d, double length_y, co
     const int k = blockIdx.y;
     if (j >= d.nyg) return;
     const long long pa = fidx(d, d.nx - 2, j, k);
-    const double v = U[pa] + du;
-    U[pa] = v;
-    U[pa + 1] = v;
+    const double v = (double)U[pa] + du;
+    U[pa] = (T)v;
+    U[pa + 1] = (T)v;
 }
 
 // ------------------------------------------------------------------------
 // Projection pieces
 // ------------------------------------------------------------------------
-// pseudo-pressure rhs on the cells the Poisson solve reads (projection.f90:38-47)
-__global__ __launch_bounds__(256) void k_divergence(Dims d, Metrics m, const double *U,
-                                                    const double *V, const double *W,
-                                                    double *out) {
-    const int i = blockIdx.x * kWave + threadIdx.x + 1;
-    const int j = blockIdx.y * 4 + threadIdx.y + 1;
-    const int k = blockIdx.z + 1;
-    if (i > d.nxg - 3 || j > d.nyg - 2) return;
-    const long long p = fidx(d, i, j, k);
-    out[p] = (U[p] - U[p - 1]) * m.rdx[i] + (V[p] - V[p - d.sx]) * m.rdy[j] +
-             (W[p] - W[p - d.sxy]) * m.rdz[k];
-}
-
 // phi with the x-Neumann and z-periodic copies of apply_periodic_z_pressure
 __device__ __forceinline__ long long phi_at(const Dims &d, int i, int j, int k) {
     if (i > d.nxg - 3) i = d.nxg - 3;
@@ -412,20 +405,21 @@ __device__ __forceinline__ long long phi_at(const Dims &d, int i, int j, int k) 
     return fidx(d, i, j, k);
 }
 
-__global__ __launch_bounds__(256) void k_project(Dims d, Metrics m, double *U, double *V,
-                                                 double *W, const double *phi) {
+template <class T>
+__global__ __launch_bounds__(256) void k_project(Dims d, MetricsT<T> m, T *U, T *V, T *W, const T *phi) {
     const int i = blockIdx.x * kWave + threadIdx.x + 1;
     const int j = blockIdx.y * 4 + threadIdx.y + 1;
     const int k = blockIdx.z + 1;
     if (i > d.nxg - 2 || j > d.nyg - 2) return;
     const long long p = fidx(d, i, j, k);
-    const double pc = phi[phi_at(d, i, j, k)];
+    const T pc = phi[phi_at(d, i, j, k)];
     if (i <= d.nx - 2) U[p] = U[p] - (phi[phi_at(d, i + 1, j, k)] - pc) * m.rdxg[i];
     if (j <= d.ny - 2) V[p] = V[p] - (phi[phi_at(d, i, j + 1, k)] - pc) * m.rdyg[j];
     if (k <= d.nz - 2) W[p] = W[p] - (phi[phi_at(d, i, j, k + 1)] - pc) * m.rdzg[k];
 }
 
-__global__ __launch_bounds__(256) void k_pressure(Dims d, const double *phi, double *P, double den) {
+template <class T>
+__global__ __launch_bounds__(256) void k_pressure(Dims d, const T *phi, T *P, double den) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int j = blockIdx.y;
     const int k = blockIdx.z;
@@ -433,7 +427,7 @@ __global__ __launch_bounds__(256) void k_pressure(Dims d, const double *phi, dou
     const int ci = i < 1 ? 1 : (i > d.nxg - 2 ? d.nxg - 2 : i);
     const int cj = j < 1 ? 1 : (j > d.nyg - 2 ? d.nyg - 2 : j);
     const int ck = k < 1 ? 1 : (k > d.nzg - 2 ? d.nzg - 2 : k);
-    P[fidx(d, i, j, k)] = phi[phi_at(d, ci, cj, ck)] / den;
+    P[fidx(d, i, j, k)] = (T)((double)phi[phi_at(d, ci, cj, ck)] / den);
 }
 
 // ------------------------------------------------------------------------
@@ -796,8 +790,9 @@ __device__ __forceinline__ double block_max_256(double v, double *sh) {
     return fmax(fmax(sh[0], sh[1]), fmax(sh[2], sh[3]));
 }
 
-__global__ __launch_bounds__(256) void k_maxdiv(Dims d, Metrics m, const double *U, const double *V,
-                                                const double *W, double *out) {
+template <class T>
+__global__ __launch_bounds__(256) void k_maxdiv(Dims d, Metrics m, const T *U, const T *V, const T *W,
+                                                double *out) {
     __shared__ double sh[4];
     const int i = blockIdx.x * kWave + threadIdx.x + 1;
     const int j = blockIdx.y * 4 + threadIdx.y + 1;
@@ -805,16 +800,17 @@ __global__ __launch_bounds__(256) void k_maxdiv(Dims d, Metrics m, const double 
     double v = 0.0;
     if (i <= d.nxg - 3 && j <= d.nyg - 2) {
         const long long p = fidx(d, i, j, k);
-        v = fabs((U[p] - U[p - 1]) * m.rdx[i] + (V[p] - V[p - d.sx]) * m.rdy[j] +
-                 (W[p] - W[p - d.sxy]) * m.rdz[k]);
+        v = fabs(((double)U[p] - (double)U[p - 1]) * m.rdx[i] + ((double)V[p] - (double)V[p - d.sx]) * m.rdy[j] +
+                 ((double)W[p] - (double)W[p - d.sxy]) * m.rdz[k]);
     }
     v = block_max_256(v, sh);
     const long long bid = blockIdx.x + (long long)gridDim.x * (blockIdx.y + (long long)gridDim.y * blockIdx.z);
     if (threadIdx.x == 0 && threadIdx.y == 0) out[bid] = v;
 }
 
-__global__ __launch_bounds__(256) void k_dtconv(Dims d, Metrics m, const double *U, const double *V,
-                                                const double *W, double *out) {
+template <class T>
+__global__ __launch_bounds__(256) void k_dtconv(Dims d, Metrics m, const T *U, const T *V, const T *W,
+                                                double *out) {
     __shared__ double sh[3][4];
     const int i = blockIdx.x * kWave + threadIdx.x + 1;
     const int j = blockIdx.y * 4 + threadIdx.y + 1;
@@ -822,9 +818,9 @@ __global__ __launch_bounds__(256) void k_dtconv(Dims d, Metrics m, const double 
     double a = 0.0, b = 0.0, c = 0.0;
     if (i <= d.nxg - 2 && j <= d.nyg - 2) {
         const long long p = fidx(d, i, j, k);
-        a = (m.xg[i + 1] - m.xg[i]) / fabs(U[p]);
-        b = (m.yg[j + 1] - m.yg[j]) / fabs(V[p]);
-        c = (m.zg[k + 1] - m.zg[k]) / fabs(W[p]);
+        a = (m.xg[i + 1] - m.xg[i]) / fabs((double)U[p]);
+        b = (m.yg[j + 1] - m.yg[j]) / fabs((double)V[p]);
+        c = (m.zg[k + 1] - m.zg[k]) / fabs((double)W[p]);
     }
     a = block_max_256(a, sh[0]);
     b = block_max_256(b, sh[1]);
@@ -856,23 +852,23 @@ __global__ __launch_bounds__(1024) void k_reduce_max(const double *part, long lo
 }
 
 // Cf(x) (statistics.f90:137-160): one thread per x face
-__global__ __launch_bounds__(256) void k_wallstats(Dims d, Metrics m, double nu, const double *U,
-                                                   const double *V, const double *nu_t,
-                                                   double *cf) {
+template <class T>
+__global__ __launch_bounds__(256) void k_wallstats(Dims d, Metrics m, double nu, const T *U, const T *V,
+                                                   const T *nu_t, double *cf) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= d.nx) return;
     const double nz2 = (double)(d.nzg - 2);
     double u1 = 0.0, u2 = 0.0, uv = 0.0, n1a = 0.0, n2a = 0.0, n1b = 0.0, n2b = 0.0;
     for (int k = 1; k <= d.nzg - 2; ++k) {
         const long long p = fidx(d, i, 0, k);
-        const double a = U[p], b = U[p + d.sx];
+        const double a = (double)U[p], b = (double)U[p + d.sx];
         u1 += a;
         u2 += b;
-        uv += (m.wy0[0] * a + m.wy1[0] * b) * (0.5 * (V[p] + V[p + 1]));
-        n1a += nu_t[p];
-        n2a += nu_t[p + d.sx];
-        n1b += nu_t[p + 1];
-        n2b += nu_t[p + 1 + d.sx];
+        uv += (m.wy0[0] * a + m.wy1[0] * b) * (0.5 * ((double)V[p] + (double)V[p + 1]));
+        n1a += (double)nu_t[p];
+        n2a += (double)nu_t[p + d.sx];
+        n1b += (double)nu_t[p + 1];
+        n2b += (double)nu_t[p + 1 + d.sx];
     }
     const double dudy = (u2 / nz2 - u1 / nz2) / (m.yg[1] - m.yg[0]);
     const double tau = 0.25 * (n1a / nz2 + n2a / nz2 + n1b / nz2 + n2b / nz2) * dudy;
@@ -880,8 +876,9 @@ __global__ __launch_bounds__(256) void k_wallstats(Dims d, Metrics m, double nu,
 }
 
 // per-plane sums and maxima of the four monitored interiors (monitor.f90:29-52)
-__global__ __launch_bounds__(256) void k_monitor(Dims d, const double *U, const double *V,
-                                                 const double *W, const double *nu_t, double *part) {
+template <class T>
+__global__ __launch_bounds__(256) void k_monitor(Dims d, const T *U, const T *V, const T *W, const T *nu_t,
+                                                 double *part) {
     const int k = blockIdx.x + 1;  // plane
     __shared__ double sh[8][256];
     double s[4] = {0, 0, 0, 0}, mx[4] = {-INFINITY, -INFINITY, -INFINITY, -INFINITY};
@@ -889,19 +886,19 @@ __global__ __launch_bounds__(256) void k_monitor(Dims d, const double *U, const 
         for (int i = threadIdx.x + 1; i <= d.nxg - 2; i += 256) {
             const long long p = fidx(d, i, j, k);
             if (i <= d.nx - 2) {
-                s[0] += U[p];
-                mx[0] = fmax(mx[0], U[p]);
+                s[0] += (double)U[p];
+                mx[0] = fmax(mx[0], (double)U[p]);
             }
             if (j <= d.ny - 2) {
-                s[1] += V[p];
-                mx[1] = fmax(mx[1], V[p]);
+                s[1] += (double)V[p];
+                mx[1] = fmax(mx[1], (double)V[p]);
             }
             if (k <= d.nz - 2) {
-                s[2] += W[p];
-                mx[2] = fmax(mx[2], W[p]);
+                s[2] += (double)W[p];
+                mx[2] = fmax(mx[2], (double)W[p]);
             }
-            s[3] += nu_t[p];
-            mx[3] = fmax(mx[3], nu_t[p]);
+            s[3] += (double)nu_t[p];
+            mx[3] = fmax(mx[3], (double)nu_t[p]);
         }
     for (int q = 0; q < 4; ++q) {
         sh[q][threadIdx.x] = s[q];
@@ -987,88 +984,95 @@ const char *kernel_name(int id) {
 // ------------------------------------------------------------------------
 constexpr int kChunk = 16;
 
+template <>
+const MetricsT<double> &metrics<double>(const Handle &h) {
+    return h.m;
+}
+template <>
+const MetricsT<float> &metrics<float>(const Handle &h) {
+    return h.mf;
+}
+
+
 dim3 march_grid(const Dims &d) {
     return dim3((d.nxg - 2 + kTX - 1) / kTX, (d.nyg - 2 + kTY - 1) / kTY, (d.nzg - 2 + kChunk - 1) / kChunk);
 }
 
-template <int MODE, bool WRITE_F1>
-void march(Handle &h, const Field3 &cur, const Field3 *old, const Field3 *f1, Field3 *nxt, Field3 *fo,
+template <int MODE, bool WRITE_F1, class T>
+void march(Handle &h, const Fld3<T> &cur, const Fld3<T> *old, const Fld3<T> *f1, Fld3<T> *nxt, Fld3<T> *fo,
            double c0, double c1, double dt) {
     const Dims &d = h.d;
     const dim3 grid = march_grid(d), block(256);
-    const double *o[3] = {old ? old->u : nullptr, old ? old->v : nullptr, old ? old->w : nullptr};
-    const double *f[3] = {f1 ? f1->u : nullptr, f1 ? f1->v : nullptr, f1 ? f1->w : nullptr};
-    double *n[3] = {nxt ? nxt->u : nullptr, nxt ? nxt->v : nullptr, nxt ? nxt->w : nullptr};
-    double *r[3] = {fo ? fo->u : nullptr, fo ? fo->v : nullptr, fo ? fo->w : nullptr};
+    const T *o[3] = {old ? old->u : nullptr, old ? old->v : nullptr, old ? old->w : nullptr};
+    const T *f[3] = {f1 ? f1->u : nullptr, f1 ? f1->v : nullptr, f1 ? f1->w : nullptr};
+    T *n[3] = {nxt ? nxt->u : nullptr, nxt ? nxt->v : nullptr, nxt ? nxt->w : nullptr};
+    T *r[3] = {fo ? fo->u : nullptr, fo ? fo->v : nullptr, fo ? fo->w : nullptr};
+    const MetricsT<T> &m = metrics<T>(h);
     if (h.les != 0)
-        hipLaunchKernelGGL((k_rk_march<MODE, WRITE_F1, true>), grid, block, 0, h.stream, d, h.m, h.nu, h.dPdx,
-                           h.dPdz, kChunk, cur.u, cur.v, cur.w, h.nu_t, o[0], o[1], o[2], f[0], f[1], f[2],
-                           n[0], n[1], n[2], r[0], r[1], r[2], c0, c1, dt);
+        hipLaunchKernelGGL((k_rk_march<MODE, WRITE_F1, true, T>), grid, block, 0, h.stream, d, m, (T)h.nu,
+                           (T)h.dPdx, (T)h.dPdz, kChunk, cur.u, cur.v, cur.w, reinterpret_cast<const T *>(h.nu_t),
+                           o[0], o[1], o[2], f[0], f[1], f[2], n[0], n[1], n[2], r[0], r[1], r[2], (T)c0, (T)c1,
+                           (T)dt);
     else
-        hipLaunchKernelGGL((k_rk_march<MODE, WRITE_F1, false>), grid, block, 0, h.stream, d, h.m, h.nu, h.dPdx,
-                           h.dPdz, kChunk, cur.u, cur.v, cur.w, nullptr, o[0], o[1], o[2], f[0], f[1], f[2],
-                           n[0], n[1], n[2], r[0], r[1], r[2], c0, c1, dt);
+        hipLaunchKernelGGL((k_rk_march<MODE, WRITE_F1, false, T>), grid, block, 0, h.stream, d, m, (T)h.nu,
+                           (T)h.dPdx, (T)h.dPdz, kChunk, cur.u, cur.v, cur.w, nullptr, o[0], o[1], o[2], f[0],
+                           f[1], f[2], n[0], n[1], n[2], r[0], r[1], r[2], (T)c0, (T)c1, (T)dt);
     check_launch();
 }
 
-void launch_rk_stage(Handle &h, const Field3 &cur, const Field3 &old, Field3 &nxt, int stage,
-                     double dt) {
+template <class T>
+void launch_rk_stage(Handle &h, const Fld3<T> &cur, const Fld3<T> &old, Fld3<T> &nxt, int stage, double dt) {
     const bool rk3 = h.scheme == 3;
+    Fld3<T> F1 = as<T>(h.F1);
     ProfScope ps(h, KID_RK_STAGE);
     if (stage == 1) {
         const double c0 = dt * (rk3 ? h.rk_coef[1][1] : h.rk2_coef[1][1]);
         if (rk3)
-            march<0, true>(h, cur, &old, nullptr, &nxt, &h.F1, c0, 0.0, dt);
+            march<0, true>(h, cur, &old, (const Fld3<T> *)nullptr, &nxt, &F1, c0, 0.0, dt);
         else
-            march<0, false>(h, cur, &old, nullptr, &nxt, nullptr, c0, 0.0, dt);
+            march<0, false>(h, cur, &old, (const Fld3<T> *)nullptr, &nxt, (Fld3<T> *)nullptr, c0, 0.0, dt);
     } else if (!rk3) {
         // RK2 second stage: Uo + dt*(0*F1 + 1*F2) == Uo + dt*F2 exactly
-        march<0, false>(h, cur, &old, nullptr, &nxt, nullptr, dt * h.rk2_coef[2][2], 0.0, dt);
+        march<0, false>(h, cur, &old, (const Fld3<T> *)nullptr, &nxt, (Fld3<T> *)nullptr, dt * h.rk2_coef[2][2],
+                        0.0, dt);
     } else {
         // RK3 stages 2, 3: a_32 = 0 so F2 is never needed
         const double c1 = h.rk_coef[stage][1], c0 = h.rk_coef[stage][stage];
-        march<1, false>(h, cur, &old, &h.F1, &nxt, nullptr, c0, c1, dt);
+        march<1, false>(h, cur, &old, &F1, &nxt, (Fld3<T> *)nullptr, c0, c1, dt);
     }
 }
 
-void launch_rhs_only(Handle &h, const Field3 &cur, Field3 &rhs) {
-    march<2, false>(h, cur, nullptr, nullptr, nullptr, &rhs, 0.0, 0.0, 0.0);
+template <class T>
+void launch_rhs_only(Handle &h, const Fld3<T> &cur, Fld3<T> &rhs) {
+    march<2, false>(h, cur, (const Fld3<T> *)nullptr, (const Fld3<T> *)nullptr, (Fld3<T> *)nullptr, &rhs, 0.0,
+                    0.0, 0.0);
 }
 
-void launch_bc(Handle &h, Field3 &f, const Field3 &old, int rk_step, double dt, double t) {
+template <class T>
+void launch_bc(Handle &h, Fld3<T> &f, const Fld3<T> &old, int rk_step, double dt, double t) {
     const Dims &d = h.d;
     const double coef = h.rk_t[rk_step] * dt * h.bc_uc;
     prof_begin(h, KID_BC_INOUT);
     const dim3 g_in((d.nyg + 255) / 256, d.nzg), g_tw((d.nxg + 255) / 256, d.nzg);
     const int nq_in = g_in.x * g_in.y, nq_tw = g_tw.x * g_tw.y;
     double *qpart = h.red;
-    hipLaunchKernelGGL(k_bc_inout, g_in, dim3(256), 0, h.stream, d, h.m, h.bc, f.u, f.v, f.w, old.u,
-                       old.v, old.w, coef, t, qpart);
+    hipLaunchKernelGGL(k_bc_inout<T>, g_in, dim3(256), 0, h.stream, d, h.m, h.bc, f.u, f.v, f.w, old.u, old.v,
+                       old.w, coef, t, qpart);
     check_launch();
     prof_end(h, KID_BC_INOUT);
     prof_begin(h, KID_BC_PERIODIC);
-    hipLaunchKernelGGL(k_bc_periodic, dim3((d.nxg + 255) / 256, d.nyg), dim3(256), 0, h.stream, d,
-                       f.u, f.v, f.w);
+    hipLaunchKernelGGL(k_bc_periodic<T>, dim3((d.nxg + 255) / 256, d.nyg), dim3(256), 0, h.stream, d, f.u, f.v,
+                       f.w);
     check_launch();
     prof_end(h, KID_BC_PERIODIC);
     prof_begin(h, KID_BC_TOPWALL);
-    hipLaunchKernelGGL(k_bc_topwall, g_tw, dim3(256), 0, h.stream, d, h.m, h.bc, f.u, f.v, f.w,
-                       qpart + nq_in);
+    hipLaunchKernelGGL(k_bc_topwall<T>, g_tw, dim3(256), 0, h.stream, d, h.m, h.bc, f.u, f.v, f.w, qpart + nq_in);
     check_launch();
     prof_end(h, KID_BC_TOPWALL);
     prof_begin(h, KID_BC_MASSFLUX);
-    hipLaunchKernelGGL(k_bc_massflux, g_in, dim3(256), 0, h.stream, d, h.length_y, qpart,
-                       nq_in + nq_tw, f.u);
+    hipLaunchKernelGGL(k_bc_massflux<T>, g_in, dim3(256), 0, h.stream, d, h.length_y, qpart, nq_in + nq_tw, f.u);
     check_launch();
     prof_end(h, KID_BC_MASSFLUX);
-}
-
-void launch_divergence(Handle &h, const Field3 &f, double *out) {
-    const Dims &d = h.d;
-    ProfScope ps(h, KID_DIVERGENCE);
-    hipLaunchKernelGGL(k_divergence, dim3((d.nxg - 3 + kWave - 1) / kWave, (d.nyg - 2 + 3) / 4, d.nzg - 3),
-                       dim3(kWave, 4), 0, h.stream, d, h.m, f.u, f.v, f.w, out);
-    check_launch();
 }
 
 void launch_poisson(Handle &h, double *phi, const Field3 *div) {
@@ -1116,28 +1120,31 @@ void launch_poisson(Handle &h, double *phi, const Field3 *div) {
     dct_launch(h, KID_XFORM_X_INV, da, false, h.T1, phi);
 }
 
-void launch_project(Handle &h, Field3 &f, const double *phi) {
+template <class T>
+void launch_project(Handle &h, Fld3<T> &f, const T *phi) {
     const Dims &d = h.d;
     ProfScope ps(h, KID_PROJECT);
-    hipLaunchKernelGGL(k_project, interior_grid(d), dim3(kWave, 4), 0, h.stream, d, h.m, f.u, f.v,
+    hipLaunchKernelGGL(k_project<T>, interior_grid(d), dim3(kWave, 4), 0, h.stream, d, metrics<T>(h), f.u, f.v,
                        f.w, phi);
     check_launch();
 }
 
-void launch_pressure(Handle &h, const double *phi, double *P, double dt) {
+template <class T>
+void launch_pressure(Handle &h, const T *phi, T *P, double dt) {
     const Dims &d = h.d;
     ProfScope ps(h, KID_PRESSURE);
     const double den = dt * h.rk2_coef[1][1];
-    hipLaunchKernelGGL(k_pressure, dim3((d.nxg + 255) / 256, d.nyg, d.nzg), dim3(256), 0, h.stream, d,
-                       phi, P, den);
+    hipLaunchKernelGGL(k_pressure<T>, dim3((d.nxg + 255) / 256, d.nyg, d.nzg), dim3(256), 0, h.stream, d, phi, P,
+                       den);
     check_launch();
 }
 
-double max_divergence(Handle &h, const Field3 &f) {
+template <class T>
+double max_divergence(Handle &h, const Fld3<T> &f) {
     const Dims &d = h.d;
     const dim3 g = interior_grid(d);
     const long long nb = (long long)g.x * g.y * g.z;
-    hipLaunchKernelGGL(k_maxdiv, g, dim3(kWave, 4), 0, h.stream, d, h.m, f.u, f.v, f.w, h.red);
+    hipLaunchKernelGGL(k_maxdiv<T>, g, dim3(kWave, 4), 0, h.stream, d, h.m, f.u, f.v, f.w, h.red);
     check_launch();
     hipLaunchKernelGGL(k_reduce_max, dim3(1), dim3(1024), 0, h.stream, h.red, nb, 1, h.scal);
     check_launch();
@@ -1147,12 +1154,13 @@ double max_divergence(Handle &h, const Field3 &f) {
     return v;
 }
 
-double dt_convective(Handle &h, const Field3 &f) {
+template <class T>
+double dt_convective(Handle &h, const Fld3<T> &f) {
     const Dims &d = h.d;
     ProfScope ps(h, KID_REDUCE);
     const dim3 g = interior_grid(d);
     const long long nb = (long long)g.x * g.y * g.z;
-    hipLaunchKernelGGL(k_dtconv, g, dim3(kWave, 4), 0, h.stream, d, h.m, f.u, f.v, f.w, h.red);
+    hipLaunchKernelGGL(k_dtconv<T>, g, dim3(kWave, 4), 0, h.stream, d, h.m, f.u, f.v, f.w, h.red);
     check_launch();
     hipLaunchKernelGGL(k_reduce_max, dim3(1), dim3(1024), 0, h.stream, h.red, nb, 3, h.scal);
     check_launch();
@@ -1164,23 +1172,25 @@ double dt_convective(Handle &h, const Field3 &f) {
     return std::fmin(dcu, std::fmin(dcv, dcw));
 }
 
-void wall_stats(Handle &h, const Field3 &f, double *cf_host) {
+template <class T>
+void wall_stats(Handle &h, const Fld3<T> &f, double *cf_host) {
     const Dims &d = h.d;
-    hipLaunchKernelGGL(k_wallstats, dim3((d.nx + 255) / 256), dim3(256), 0, h.stream, d, h.m, h.nu, f.u,
-                       f.v, h.nu_t, h.red);
+    hipLaunchKernelGGL(k_wallstats<T>, dim3((d.nx + 255) / 256), dim3(256), 0, h.stream, d, h.m, h.nu, f.u, f.v,
+                       reinterpret_cast<const T *>(h.nu_t), h.red);
     check_launch();
     BL_HIP(hipMemcpyAsync(cf_host, h.red, sizeof(double) * d.nx, hipMemcpyDeviceToHost, h.stream));
     BL_HIP(hipStreamSynchronize(h.stream));
 }
 
-void monitor(Handle &h, const Field3 &f, double *out) {
+template <class T>
+void monitor(Handle &h, const Fld3<T> &f, double *out) {
     const Dims &d = h.d;
     const int np = d.nzg - 2;
-    hipLaunchKernelGGL(k_monitor, dim3(np), dim3(256), 0, h.stream, d, f.u, f.v, f.w, h.nu_t, h.red);
+    hipLaunchKernelGGL(k_monitor<T>, dim3(np), dim3(256), 0, h.stream, d, f.u, f.v, f.w,
+                       reinterpret_cast<const T *>(h.nu_t), h.red);
     check_launch();
     std::vector<double> part((size_t)np * 8);
-    BL_HIP(hipMemcpyAsync(part.data(), h.red, sizeof(double) * part.size(), hipMemcpyDeviceToHost,
-                          h.stream));
+    BL_HIP(hipMemcpyAsync(part.data(), h.red, sizeof(double) * part.size(), hipMemcpyDeviceToHost, h.stream));
     BL_HIP(hipStreamSynchronize(h.stream));
     for (int q = 0; q < 4; ++q) {
         out[q] = 0.0;
@@ -1192,5 +1202,19 @@ void monitor(Handle &h, const Field3 &f, double *out) {
             out[4 + q] = std::fmax(out[4 + q], part[(size_t)k * 8 + 4 + q]);
         }
 }
+
+#define BL_INSTANTIATE(T)                                                                                   \
+    template void launch_rk_stage<T>(Handle &, const Fld3<T> &, const Fld3<T> &, Fld3<T> &, int, double); \
+    template void launch_rhs_only<T>(Handle &, const Fld3<T> &, Fld3<T> &);                                \
+    template void launch_bc<T>(Handle &, Fld3<T> &, const Fld3<T> &, int, double, double);                 \
+    template void launch_project<T>(Handle &, Fld3<T> &, const T *);                                      \
+    template void launch_pressure<T>(Handle &, const T *, T *, double);                                   \
+    template double max_divergence<T>(Handle &, const Fld3<T> &);                                         \
+    template double dt_convective<T>(Handle &, const Fld3<T> &);                                          \
+    template void wall_stats<T>(Handle &, const Fld3<T> &, double *);                                     \
+    template void monitor<T>(Handle &, const Fld3<T> &, double *);
+BL_INSTANTIATE(double)
+BL_INSTANTIATE(float)
+#undef BL_INSTANTIATE
 
 }  // namespace blgpu

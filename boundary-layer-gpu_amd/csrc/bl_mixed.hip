@@ -1,0 +1,485 @@
+// Mixed-precision Poisson solve (precision = BL_PREC_MIXED).
+//
+// The pseudo-pressure rhs (projection.f90:34-49) is formed in FP32; the x
+// cosine transform and the z Fourier transform run as batched GEMMs on the
+// matrix cores, each product split into BF16 pieces (a = a_hi + a_lo):
+//   a*b ~ a_hi*b_hi + a_hi*b_lo + a_lo*b_hi, accumulated in FP32;
+// the tridiagonal solves in y use the FP64 LU factors of
+// initialization.f90:427-469 with FP64 arithmetic.  The near-singular
+// (kx, kz) = (0, 0) mode bypasses the GEMMs: its rhs is an FP64 sum of the
+// divergence over each x-z plane, it is solved in FP64, and its constant
+// contribution phi00(j) / (2 N M) is added back after the inverse x GEMM.
+//
+// Transform matrices (N = nx-2, M = nz-2), identical in meaning to the FP64
+// path's DCT/DFT (projection.f90:110-202 with the 1/(2 nxp nzp) factor):
+//   Dxf[k][n] = 2 cos(pi k (2n+1) / (2N))
+//   Dxi[n][k] = c_k cos(pi k (2n+1) / (2N)) / (N M),  c_0 = 1/2, c_k = 1
+//   Zf[q][m]: q = 0 -> 1, q = 2k-1 -> cos(2 pi k m / M), q = 2k -> -sin(2 pi k m / M)
+//   Zi[m][q]: q = 0 -> 1, q = 2k-1 -> 2 cos (cos at k = M/2), q = 2k -> -2 sin
+// Spectral work arrays: T1[m][j][kx] and S[q][j][kx], kx stride NP = N
+// rounded up to 32 (the padding stays zero).
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <vector>
+
+#include "bl_internal.hpp"
+
+namespace blgpu {
+
+namespace {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef unsigned short u16x8 __attribute__((ext_vector_type(8)));
+
+__device__ __forceinline__ unsigned short bf16_bits(float x) {
+    const __bf16 b = (__bf16)x;  // round to nearest even
+    return __builtin_bit_cast(unsigned short, b);
+}
+__device__ __forceinline__ float bf16_float(unsigned short u) {
+    return __builtin_bit_cast(float, (unsigned int)u << 16);
+}
+
+// ------------------------------------------------------------------------
+// C[r][c] = sum_k A[r][k] B[k][c] (+ add[c % o_C1]) with A pre-split into
+// BF16 hi/lo rows of length Kp (zero padded past K) and B split on the fly.
+// Workgroup tile BM x BN, K step 32, four waves of 64x64 each; MFMA
+// v_mfma_f32_16x16x32_bf16: lane l holds A[l%16][8(l/16)+e],
+// B[8(l/16)+e][l%16] (e < 8) and C[4(l/16)+r][l%16] (r < 4).
+// B layout: KCONTIG -> B[c*ldb + k], else B[k*ldb + c].
+// Output layout: RCONTIG -> out[o_base + (c%o_C1)*o_c1 + (c/o_C1)*o_c2 + r],
+// else out[r*ldc + c].
+// ------------------------------------------------------------------------
+struct GemmArgs {
+    int R, C, K, Kp;
+    const unsigned short *Ahi, *Alo;
+    const float *B;
+    long long ldb;
+    float *out;
+    long long o_base, o_c1, o_c2, ldc;
+    int o_C1;
+    const double *add;  // optional per-(c % o_C1) addend
+};
+
+constexpr int kBK = 32, kLDK = kBK + 8;  // LDS row of one B column, padded
+
+template <int BM, int BN, bool KCONTIG, bool RCONTIG>
+__global__ __launch_bounds__(256) void k_gemm_bf16x3(GemmArgs g, int tiles_r, int nblk) {
+    static_assert((BM / 64) * (BN / 64) == 4, "four 64x64 wave tiles");
+    __shared__ unsigned short sBh[BN * kLDK], sBl[BN * kLDK];
+    // XCD-aware order: blocks that share a column tile run on one XCD (the
+    // hardware deals consecutive block ids round-robin over 8 XCDs)
+    const int b = blockIdx.x;
+    const int per = nblk / 8;
+    const int lb = (b % 8) < (nblk % 8) ? (b % 8) * (per + 1) + b / 8
+                                        : (nblk % 8) * (per + 1) + ((b % 8) - (nblk % 8)) * per + b / 8;
+    const int tr = lb % tiles_r, tc = lb / tiles_r;
+    const int r0 = tr * BM, c0 = tc * BN;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wr = (wave % (BM / 64)) * 64, wc = (wave / (BM / 64)) * 64;
+    const int l16 = lane & 15, lq = lane >> 4;
+
+    f32x4 acc[4][4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int c = 0; c < 4; ++c) acc[a][c] = f32x4{0.f, 0.f, 0.f, 0.f};
+
+    for (int kk = 0; kk < g.Kp; kk += kBK) {
+        // ---- stage the B tile (32 x BN) as BF16 hi/lo, [column][k] in LDS
+        if constexpr (KCONTIG) {
+            constexpr int TPC = 256 / BN;         // threads per column
+            constexpr int KPT = kBK / TPC;        // k values per thread
+            const int col = tid / TPC, k0 = (tid % TPC) * KPT;
+            const int c = c0 + col;
+            float v[KPT];
+#pragma unroll
+            for (int e = 0; e < KPT; ++e) {
+                const int k = kk + k0 + e;
+                v[e] = (c < g.C && k < g.K) ? g.B[(long long)c * g.ldb + k] : 0.f;
+            }
+#pragma unroll
+            for (int e = 0; e < KPT; e += 8) {
+                u16x8 hi, lo;
+#pragma unroll
+                for (int q = 0; q < 8; ++q) {
+                    const unsigned short h = bf16_bits(v[e + q]);
+                    hi[q] = h;
+                    lo[q] = bf16_bits(v[e + q] - bf16_float(h));
+                }
+                *reinterpret_cast<u16x8 *>(&sBh[col * kLDK + k0 + e]) = hi;
+                *reinterpret_cast<u16x8 *>(&sBl[col * kLDK + k0 + e]) = lo;
+            }
+        } else {
+            // thread: two consecutive k rows, CPT consecutive columns
+            constexpr int CPT = BN * kBK / 2 / 256;
+            const int kp = tid / (BN / CPT), cc0 = (tid % (BN / CPT)) * CPT;
+            const int ka = kk + 2 * kp;
+#pragma unroll
+            for (int e = 0; e < CPT; ++e) {
+                const int c = c0 + cc0 + e;
+                const float x0 = (c < g.C && ka < g.K) ? g.B[(long long)ka * g.ldb + c] : 0.f;
+                const float x1 = (c < g.C && ka + 1 < g.K) ? g.B[(long long)(ka + 1) * g.ldb + c] : 0.f;
+                const unsigned short h0 = bf16_bits(x0), h1 = bf16_bits(x1);
+                const unsigned short l0 = bf16_bits(x0 - bf16_float(h0)), l1 = bf16_bits(x1 - bf16_float(h1));
+                const int o = (cc0 + e) * kLDK + 2 * kp;
+                *reinterpret_cast<unsigned int *>(&sBh[o]) = (unsigned int)h0 | ((unsigned int)h1 << 16);
+                *reinterpret_cast<unsigned int *>(&sBl[o]) = (unsigned int)l0 | ((unsigned int)l1 << 16);
+            }
+        }
+        __syncthreads();
+        // ---- 4 x 4 blocks of 16 x 16, three MFMAs each
+        bf16x8 bh[4], bl[4];
+#pragma unroll
+        for (int cb = 0; cb < 4; ++cb) {
+            const int o = (wc + cb * 16 + l16) * kLDK + 8 * lq;
+            bh[cb] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const u16x8 *>(&sBh[o]));
+            bl[cb] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const u16x8 *>(&sBl[o]));
+        }
+#pragma unroll
+        for (int rb = 0; rb < 4; ++rb) {
+            const long long ao = (long long)(r0 + wr + rb * 16 + l16) * g.Kp + kk + 8 * lq;
+            const bf16x8 ah = __builtin_bit_cast(bf16x8, *reinterpret_cast<const u16x8 *>(g.Ahi + ao));
+            const bf16x8 al = __builtin_bit_cast(bf16x8, *reinterpret_cast<const u16x8 *>(g.Alo + ao));
+#pragma unroll
+            for (int cb = 0; cb < 4; ++cb) {
+                acc[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh[cb], acc[rb][cb], 0, 0, 0);
+                acc[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl[cb], acc[rb][cb], 0, 0, 0);
+                acc[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh[cb], acc[rb][cb], 0, 0, 0);
+            }
+        }
+        __syncthreads();
+    }
+    // ---- epilogue
+#pragma unroll
+    for (int cb = 0; cb < 4; ++cb) {
+        const int c = c0 + wc + cb * 16 + l16;
+        if (c >= g.C) continue;
+        const float addv = g.add ? (float)g.add[c % g.o_C1] : 0.f;
+        const long long ob = RCONTIG ? g.o_base + (long long)(c % g.o_C1) * g.o_c1 + (long long)(c / g.o_C1) * g.o_c2
+                                     : (long long)c;
+#pragma unroll
+        for (int rb = 0; rb < 4; ++rb)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int r = r0 + wr + rb * 16 + 4 * lq + q;
+                if (r >= g.R) continue;
+                const float v = acc[rb][cb][q] + addv;
+                if (RCONTIG)
+                    g.out[ob + r] = v;
+                else
+                    g.out[(long long)r * g.ldc + ob] = v;
+            }
+    }
+}
+
+template <int BM, int BN, bool KCONTIG, bool RCONTIG>
+void gemm(Handle &h, int kid, const GemmArgs &g) {
+    ProfScope ps(h, kid);
+    const int tr = (g.R + BM - 1) / BM, tc = (g.C + BN - 1) / BN;
+    const int nblk = tr * tc;
+    hipLaunchKernelGGL((k_gemm_bf16x3<BM, BN, KCONTIG, RCONTIG>), dim3(nblk), dim3(256), 0, h.stream, g, tr, nblk);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) {
+        last_error() = std::string("kernel launch: ") + hipGetErrorString(e);
+        throw DeviceError();
+    }
+}
+
+// ------------------------------------------------------------------------
+// pseudo-pressure rhs (projection.f90:38-47) in compact lines
+// DIV[(m*NM + j)*NP + i] (cells i+1, j+1, m+1), and per x-z plane FP64
+// partial sums of the same divergence for the (0,0) mode: one wave = one y
+// row of 64 cells, part[j][bx + gx*m].
+// ------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_div_mixed(Dims d, Metrics m, const float *__restrict__ U,
+                                                   const float *__restrict__ V, const float *__restrict__ W,
+                                                   int NP, float *__restrict__ div, double *__restrict__ part) {
+    const int ii = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int jj = blockIdx.y * 4 + (threadIdx.x >> 6);
+    const int mm = blockIdx.z;
+    double v = 0.0;
+    if (ii < d.N && jj < d.NM) {
+        const int i = ii + 1, j = jj + 1, k = mm + 1;
+        const long long p = fidx(d, i, j, k);
+        v = ((double)U[p] - (double)U[p - 1]) * m.rdx[i] + ((double)V[p] - (double)V[p - d.sx]) * m.rdy[j] +
+            ((double)W[p] - (double)W[p - d.sxy]) * m.rdz[k];
+        div[((long long)mm * d.NM + jj) * NP + ii] = (float)v;
+    }
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o);
+    if ((threadIdx.x & 63) == 0 && jj < d.NM)
+        part[(long long)jj * gridDim.x * gridDim.z + blockIdx.x + (long long)gridDim.x * mm] = v;
+}
+
+// FP64 (0,0) mode: rhs(j) = 2 sum over the plane (DCT-II x2 at kx = 0, DFT
+// at kz = 0), Thomas with the kx = kz = 0 factors (pinned row), and the
+// constant phi00(j) / (2 N M) the inverse transforms would add.
+__global__ __launch_bounds__(256) void k_solve00(Dims d, const double *__restrict__ part, int np,
+                                                 const double *__restrict__ dl, const double *__restrict__ dp,
+                                                 const double *__restrict__ du, double *__restrict__ c00) {
+    __shared__ double r[1024];
+    const int NM = d.NM, N = d.N;
+    for (int j = threadIdx.x; j < NM; j += 256) {
+        double s = 0.0;
+        for (int q = 0; q < np; ++q) s += part[(long long)j * np + q];
+        r[j] = 2.0 * s;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double x = r[0];
+        for (int j = 1; j < NM; ++j) {
+            x = r[j] - dl[(long long)(j - 1) * N] * x;
+            r[j] = x;
+        }
+        x = x * dp[(long long)(NM - 1) * N];
+        r[NM - 1] = x;
+        for (int j = NM - 2; j >= 0; --j) {
+            x = (r[j] - du[j] * x) * dp[(long long)j * N];
+            r[j] = x;
+        }
+    }
+    __syncthreads();
+    const double sc = 1.0 / (2.0 * (double)N * (double)d.M);
+    for (int j = threadIdx.x; j < NM; j += 256) c00[j] = r[j] * sc;
+}
+
+// Thomas sweep in FP64 on the FP32 spectrum (kx stride NP): the forward
+// values are kept in FP64 in X ([q][j][kx], kx stride N) for the back
+// substitution.  The (0,0) system is left to k_solve00 and zeroed here.
+template <int TJ>
+__global__ __launch_bounds__(256) void k_thomas_mixed(Dims d, int NP, const double *__restrict__ dl,
+                                                      const double *__restrict__ dp,
+                                                      const double *__restrict__ du, float *S,
+                                                      double *__restrict__ X) {
+    const int kx = blockIdx.x * blockDim.x + threadIdx.x;
+    const int q = blockIdx.y;
+    const int N = d.N, NM = d.NM;
+    if (kx >= N) return;
+    const long long sp = (long long)NP * NM, fp = (long long)N * NM;
+    float *a = S + q * sp + kx;
+    if (kx == 0 && q == 0) {
+        for (int j = 0; j < NM; ++j) a[(long long)j * NP] = 0.f;
+        return;
+    }
+    const int kk = (q + 1) / 2;
+    const double *L = dl + kk * fp + kx;
+    const double *D = dp + kk * fp + kx;
+    double *x = X + q * fp + kx;
+    double r = a[0];
+    x[0] = r;
+    int j = 1;
+    for (; j + TJ <= NM; j += TJ) {
+        double v[TJ], l[TJ];
+#pragma unroll
+        for (int t = 0; t < TJ; ++t) {
+            v[t] = a[(long long)(j + t) * NP];
+            l[t] = L[(long long)(j + t - 1) * N];
+        }
+#pragma unroll
+        for (int t = 0; t < TJ; ++t) {
+            r = v[t] - l[t] * r;
+            x[(long long)(j + t) * N] = r;
+        }
+    }
+    for (; j < NM; ++j) {
+        r = (double)a[(long long)j * NP] - L[(long long)(j - 1) * N] * r;
+        x[(long long)j * N] = r;
+    }
+    r = r * D[(long long)(NM - 1) * N];
+    a[(long long)(NM - 1) * NP] = (float)r;
+    j = NM - 2;
+    for (; j - TJ + 1 >= 0; j -= TJ) {
+        double v[TJ], p[TJ], u[TJ];
+#pragma unroll
+        for (int t = 0; t < TJ; ++t) {
+            v[t] = x[(long long)(j - t) * N];
+            p[t] = D[(long long)(j - t) * N];
+            u[t] = du[j - t];
+        }
+#pragma unroll
+        for (int t = 0; t < TJ; ++t) {
+            r = (v[t] - u[t] * r) * p[t];
+            a[(long long)(j - t) * NP] = (float)r;
+        }
+    }
+    for (; j >= 0; --j) {
+        r = (x[(long long)j * N] - du[j] * r) * D[(long long)j * N];
+        a[(long long)j * NP] = (float)r;
+    }
+}
+
+void check() {
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) {
+        last_error() = std::string("kernel launch: ") + hipGetErrorString(e);
+        throw DeviceError();
+    }
+}
+
+// host: BF16 hi/lo bits of a float (round to nearest even; finite inputs)
+uint16_t host_bf16(float x) {
+    uint32_t u;
+    std::memcpy(&u, &x, 4);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+float host_bf16_float(uint16_t b) {
+    const uint32_t u = (uint32_t)b << 16;
+    float f;
+    std::memcpy(&f, &u, 4);
+    return f;
+}
+
+// rows x K matrix (row-major, entries from f(r, k)) into hi/lo BF16 rows of
+// length Kp, rows padded to Rp; padding is zero
+template <class F>
+void split_matrix(int rows, int K, int Rp, int Kp, F f, std::vector<uint16_t> &hi, std::vector<uint16_t> &lo) {
+    hi.assign((size_t)Rp * Kp, 0);
+    lo.assign((size_t)Rp * Kp, 0);
+    for (int r = 0; r < rows; ++r)
+        for (int k = 0; k < K; ++k) {
+            const float v = (float)f(r, k);
+            const uint16_t h = host_bf16(v);
+            hi[(size_t)r * Kp + k] = h;
+            lo[(size_t)r * Kp + k] = host_bf16(v - host_bf16_float(h));
+        }
+}
+
+constexpr int kBMx = 128, kBNx = 128, kBMz = 64, kBNz = 256;
+
+}  // namespace
+
+void build_mixed(Handle &h) {
+    const Dims &d = h.d;
+    const int N = d.N, M = d.M;
+    if (d.NM > 1024) {
+        last_error() = "bl_create: the mixed-precision path supports ny <= 1025";
+        throw ArgError();
+    }
+    const long double pi = 3.141592653589793238462643383279502884L;
+    h.NP = (N + 31) / 32 * 32;
+    h.kpx = (N + kBK - 1) / kBK * kBK;
+    h.kpz = (M + kBK - 1) / kBK * kBK;
+    const int rpx = (N + kBMx - 1) / kBMx * kBMx, rpz = (M + kBMz - 1) / kBMz * kBMz;
+    // exact integer argument reduction before the long double cosine
+    auto cosx = [&](int k, int n) {  // cos(pi k (2n+1) / (2N))
+        const long long a = ((long long)k * (2 * n + 1)) % (4LL * N);
+        return cosl(pi * (long double)a / (2.0L * N));
+    };
+    auto cz = [&](int k, int m) { return cosl(2.0L * pi * (long double)(((long long)k * m) % M) / M); };
+    auto sz = [&](int k, int m) { return sinl(2.0L * pi * (long double)(((long long)k * m) % M) / M); };
+    const long double nm = (long double)N * (long double)M;
+    std::vector<uint16_t> t[8];
+    split_matrix(N, N, rpx, h.kpx, [&](int k, int n) { return 2.0L * cosx(k, n); }, t[0], t[1]);
+    split_matrix(N, N, rpx, h.kpx,
+                 [&](int n, int k) { return (k == 0 ? 0.5L : 1.0L) * cosx(k, n) / nm; }, t[2], t[3]);
+    split_matrix(M, M, rpz, h.kpz,
+                 [&](int q, int m) -> long double {
+                     if (q == 0) return 1.0L;
+                     const int k = (q + 1) / 2;
+                     return (q % 2 == 1) ? cz(k, m) : -sz(k, m);
+                 },
+                 t[4], t[5]);
+    split_matrix(M, M, rpz, h.kpz,
+                 [&](int m, int q) -> long double {
+                     if (q == 0) return 1.0L;
+                     const int k = (q + 1) / 2;
+                     if (q % 2 == 1) return (2 * k == M ? 1.0L : 2.0L) * cz(k, m);
+                     return -2.0L * sz(k, m);
+                 },
+                 t[6], t[7]);
+    size_t total = 0;
+    for (auto &v : t) total += v.size();
+    BL_HIP(hipMalloc(&h.mx_tab, total * sizeof(uint16_t)));
+    const unsigned short **dst[8] = {&h.dxf_hi, &h.dxf_lo, &h.dxi_hi, &h.dxi_lo,
+                                     &h.zf_hi,  &h.zf_lo,  &h.zi_hi,  &h.zi_lo};
+    size_t off = 0;
+    for (int q = 0; q < 8; ++q) {
+        BL_HIP(hipMemcpyAsync(h.mx_tab + off, t[q].data(), t[q].size() * sizeof(uint16_t), hipMemcpyHostToDevice,
+                              h.stream));
+        *dst[q] = h.mx_tab + off;
+        off += t[q].size();
+    }
+    // FP64 (0,0) mode scratch: plane partials of the divergence, then phi00/(2NM)
+    h.np00 = ((N + 63) / 64) * M;
+    BL_HIP(hipMalloc(&h.p00, sizeof(double) * ((size_t)h.np00 * d.NM + d.NM)));
+    h.c00 = h.p00 + (size_t)h.np00 * d.NM;
+    BL_HIP(hipStreamSynchronize(h.stream));
+}
+
+void launch_poisson_mixed(Handle &h, float *phi, const Fld3<float> &f) {
+    const Dims &d = h.d;
+    const int N = d.N, M = d.M, NM = d.NM, NP = h.NP;
+    float *S = reinterpret_cast<float *>(h.S);
+    float *T1 = reinterpret_cast<float *>(h.T1);
+    // rhs: divergence lines into S (as scratch), FP64 plane partials
+    {
+        ProfScope ps(h, KID_DIVERGENCE);
+        hipLaunchKernelGGL(k_div_mixed, dim3((N + 63) / 64, (NM + 3) / 4, M), dim3(256), 0, h.stream, d, h.m,
+                           f.u, f.v, f.w, NP, S, h.p00);
+        check();
+    }
+    // forward x: T1[(m*NM + j)*NP + kx] = sum_n Dxf[kx][n] div[(m*NM + j)*NP + n]
+    GemmArgs g{};
+    g.R = N;
+    g.C = NM * M;
+    g.K = N;
+    g.Kp = h.kpx;
+    g.Ahi = h.dxf_hi;
+    g.Alo = h.dxf_lo;
+    g.B = S;
+    g.ldb = NP;
+    g.out = T1;
+    g.o_base = 0;
+    g.o_c1 = NP;
+    g.o_c2 = 0;
+    g.o_C1 = g.C;
+    g.add = nullptr;
+    gemm<kBMx, kBNx, true, true>(h, KID_XFORM_X_FWD, g);
+    // forward z: S[q][col] = sum_m Zf[q][m] T1[m][col], col = j*NP + kx
+    GemmArgs z{};
+    z.R = M;
+    z.C = NM * NP;
+    z.K = M;
+    z.Kp = h.kpz;
+    z.Ahi = h.zf_hi;
+    z.Alo = h.zf_lo;
+    z.B = T1;
+    z.ldb = (long long)NM * NP;
+    z.out = S;
+    z.ldc = (long long)NM * NP;
+    z.o_C1 = 1;
+    gemm<kBMz, kBNz, false, false>(h, KID_XFORM_Z_FWD, z);
+    // FP64 Thomas on the spectrum, FP64 (0,0) mode
+    {
+        ProfScope ps(h, KID_THOMAS);
+        hipLaunchKernelGGL(k_thomas_mixed<8>, dim3((N + 255) / 256, M), dim3(256), 0, h.stream, d, NP, h.lu_dl,
+                           h.lu_dp, h.lu_du, S, h.T1);
+        check();
+        hipLaunchKernelGGL(k_solve00, dim3(1), dim3(256), 0, h.stream, d, h.p00, h.np00, h.lu_dl, h.lu_dp, h.lu_du,
+                           h.c00);
+        check();
+    }
+    // inverse z: T1[m][col] = sum_q Zi[m][q] S[q][col]
+    z.Ahi = h.zi_hi;
+    z.Alo = h.zi_lo;
+    z.B = S;
+    z.out = T1;
+    gemm<kBMz, kBNz, false, false>(h, KID_XFORM_Z_INV, z);
+    // inverse x into the field layout, plus the (0,0) constant of each y row
+    g.Ahi = h.dxi_hi;
+    g.Alo = h.dxi_lo;
+    g.B = T1;
+    g.out = phi;
+    g.o_base = fidx(d, 1, 1, 1);
+    g.o_C1 = NM;
+    g.o_c1 = d.sx;
+    g.o_c2 = d.sxy;
+    g.add = h.c00;
+    gemm<kBMx, kBNx, true, true>(h, KID_XFORM_X_INV, g);
+}
+
+}  // namespace blgpu

@@ -135,7 +135,7 @@ module bl_setup
 contains
 
   subroutine read_parameters()
-    integer :: i, ist, nxyz(3)
+    integer :: i, ist, ics, nxyz(3)
     logical :: f, f1
     character(len=512) :: arg
     ! -i <file> (params.f90:96-110)
@@ -179,7 +179,7 @@ contains
     call get_str('fileout', fileout, f)
     ! create the folder of fileout if needed (params.f90:158-176)
     i = index(fileout, '/', back=.true.)
-    if (i > 1) call execute_command_line('mkdir -p ' // trim(adjustl(fileout(1:i-1))), exitstat=ist)
+    if (i > 1) call execute_command_line('mkdir -p ' // trim(adjustl(fileout(1:i-1))), exitstat=ist, cmdstat=ics)
     call get_str('timeinflow_file', file_temporal, f)
     call get_int('RKscheme', itime_step, f)
     nx = nxyz(1); ny = nxyz(2); nz = nxyz(3)
@@ -503,7 +503,7 @@ contains
   subroutine output_data()
     character(len=512) :: fname, lnk
     character(len=8) :: ext
-    integer :: iu, ist
+    integer :: iu, ist, ics
     logical :: ex
     write(ext, '(I0.8)') istep + nstep_init
     fname = trim(adjustl(fileout)) // '.' // trim(adjustl(ext))
@@ -532,8 +532,8 @@ contains
     ! `call system`, a failing command is not an error
     lnk = trim(adjustl(fileout)) // '.restart'
     inquire(file=trim(lnk), exist=ex)
-    if (ex) call execute_command_line('rm ' // trim(lnk), exitstat=ist)
-    call execute_command_line('ln -s ' // trim(adjustl(fname)) // ' ' // trim(lnk), exitstat=ist)
+    if (ex) call execute_command_line('rm ' // trim(lnk), exitstat=ist, cmdstat=ics)
+    call execute_command_line('ln -s ' // trim(adjustl(fname)) // ' ' // trim(lnk), exitstat=ist, cmdstat=ics)
   end subroutine output_data
 
 end module bl_output

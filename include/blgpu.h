@@ -148,6 +148,34 @@ int bl_profile_read(bl_handle *h, double *ms, long long *launches, int n);
 int bl_profile_count(void);
 const char *bl_profile_name(int id);
 
+/* ---- learned subgrid closure: training (no counterpart in the reference,
+ * whose only dynamic model is subgrid.f90:261-414) ----
+ * A 10 -> 64 -> 64 -> 64 -> 1 SiLU network for c = nu_t / (Delta^2 |S|),
+ * FP32 master parameters in the layout W1[64][10], b1[64], W2[64][64], b2,
+ * W3[64][64], b3, W4[64], b4 (BL_MLP_PARAMS floats), trained with Adam on a
+ * device-resident dataset of BL_MLP_ROW-float rows: 10 features, target c,
+ * weight, Delta^2 |S|, S_ij (6: 11 22 33 12 13 23), tau_ij (6). */
+typedef struct bl_mlp bl_mlp;
+enum { BL_MLP_IN = 10, BL_MLP_HIDDEN = 64, BL_MLP_PARAMS = 9089, BL_MLP_ROW = 28 };
+int bl_mlp_create(long long capacity, int batch_max, bl_mlp **out);
+void bl_mlp_destroy(bl_mlp *m);
+/* set the FP32 master parameters (and reset the Adam moments) / read them */
+int bl_mlp_set_params(bl_mlp *m, const float *theta);
+int bl_mlp_get_params(bl_mlp *m, float *theta);
+/* FP32 forward of n <= batch_max samples x[n][10] -> y[n] (training path) */
+int bl_mlp_forward(bl_mlp *m, const float *x, int n, float *y);
+/* gradient of mean((y - t)^2) over n samples, and that loss */
+int bl_mlp_gradient(bl_mlp *m, const float *x, const float *t, int n, float *g, float *loss);
+/* dataset: append host rows, read rows back, count, clear */
+int bl_mlp_add_rows(bl_mlp *m, const float *rows, long long n);
+int bl_mlp_get_rows(bl_mlp *m, long long first, long long n, float *rows);
+int bl_mlp_count(bl_mlp *m, long long *n);
+int bl_mlp_clear(bl_mlp *m);
+/* steps of Adam (lr; betas 0.9, 0.999; eps 1e-8) on batches of `batch` rows
+ * drawn by a counter hash of (seed, step, slot); loss_hist[steps] optional:
+ * the weighted batch loss before each update */
+int bl_mlp_train(bl_mlp *m, int steps, int batch, float lr, unsigned long long seed, float *loss_hist);
+
 /* ---- host-side set-up helpers (not on the per-step path) ---- */
 
 /* Blasius similarity profile f''' + f f'' = 0, f(0)=f'(0)=0, f''(0)=fpp0,

@@ -1,0 +1,111 @@
+"""Mixed-precision path (BL_PREC_MIXED) against the FP64 oracle.
+
+FP32 fields, FP32 stencils and RK updates, split-BF16x3 MFMA transforms with
+FP32 accumulation, FP64 Thomas solves, FP64 (0,0) pressure mode, FP64
+mass-flux correction and FP64 statistics.  Error budget (relative L2 against
+the FP64 oracle): single operators 1e-5 (RHS 2e-5: FP32 differences of
+nearly equal neighbours), one projection 1e-5 on the velocities, and 1e-4 on
+u, v, w after twenty RK3 steps (the stated target for the mixed mode).
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from . import cases
+from .blgpu_import import blgpu, solver_for
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def rel_l2(a, b):
+    nb = np.linalg.norm(b)
+    return np.linalg.norm(a - b) / (nb if nb > 0 else 1.0)
+
+
+CASES = {
+    "tanh_even": dict(nx=24, ny=16, nz=8, noise=0.01, seed=1),
+    "tanh_odd": dict(nx=37, ny=21, nz=11, noise=0.01, seed=2),
+    "modes": dict(nx=40, ny=24, nz=12, Ly=0.02, Lz=0.1083, modes=True, noise=0.002, seed=4),
+}
+
+
+@pytest.fixture(params=sorted(CASES))
+def case(request):
+    say("mixed case", request.param, CASES[request.param])
+    return cases.laminar_small(**CASES[request.param])
+
+
+def mixed_solver(case, o):
+    return solver_for(case, o, precision=blgpu.PREC_MIXED)[0]
+
+
+def test_mixed_rhs(case):
+    o = case.oracle()
+    s = mixed_solver(case, o)
+    ro = o.rhs()
+    rg = s.rhs()
+    for c in range(3):
+        e = rel_l2(rg[c], ro[c])
+        say("rhs", c, e)
+        assert e < 2e-5, c
+
+
+def test_mixed_bc(case):
+    o = case.oracle()
+    s = mixed_solver(case, o)
+    U0, V0, W0 = o.fields()
+    Uo = U0 * (1 + 0.01 * np.sin(np.arange(U0.size).reshape(U0.shape, order="F")))
+    o.set_old(Uo, V0, W0)
+    s.set_old(Uo, V0, W0)
+    o.t = s.t = 0.0037
+    o.apply_bc(2, 2e-4)
+    s.apply_bc(2, 2e-4)
+    for name, a, b in zip("UVW", s.fields(), o.fields()):
+        e = np.abs(a - b).max() / np.abs(o.fields()[0]).max()
+        say("bc", name, e)
+        assert e < 1e-6, name
+
+
+def test_mixed_projection(case):
+    o = case.oracle()
+    s = mixed_solver(case, o)
+    o.projection(2, 2e-4)
+    s.projection(2, 2e-4)
+    U = o.fields()[0]
+    for name, a, b in zip("UVW", s.fields(), o.fields()):
+        e = np.linalg.norm(a - b) / np.linalg.norm(U)
+        say("projection", name, e)
+        assert e < 1e-5, name
+    # the divergence left by the mixed projection, relative to max|u|/min(dy)
+    scale = np.abs(U).max() / np.diff(case.y).min()
+    dg, do = s.max_divergence(), o.max_divergence()
+    say("max div mixed", dg / scale, "fp64", do / scale)
+    assert dg / scale < 1e-5
+
+
+def test_mixed_steps(case):
+    o = case.oracle()
+    s = mixed_solver(case, o)
+    for n in range(1, 21):
+        o.compute_dt()
+        s.compute_dt()
+        o.step()
+        s.step()
+    U = o.fields()[0]
+    for name, a, b in zip("UVW", s.fields(), o.fields()):
+        e = np.linalg.norm(a - b) / np.linalg.norm(U)
+        say("20 steps", name, e)
+        assert e < 1e-4, name
+    P_o, P_g = o.get(3), s.download(3)
+    say("20 steps P", rel_l2(P_g, P_o))
+    cf_o, cf_g = o.wall_stats(), s.wall_stats()
+    e = rel_l2(cf_g, cf_o)
+    say("20 steps Cf", e)
+    assert e < 1e-4
+    assert np.isclose(s.t, o.t, rtol=1e-14)
