@@ -161,6 +161,99 @@ def gemm_mfma(prof, nx, ny, nz):
                 avg_launch_ms=prof[name][0] / prof[name][1])
 
 
+def les_leg(blgpu, args, log):
+    """Learned-closure LES against dynamic Smagorinsky on an LES grid matched
+    to a DNS that refines it by (4, 2, 4) cells.  The DNS (FP64, Blasius + 1 %
+    noise, seed 1) provides filtered snapshots for training and, continued,
+    the reference Cf; both LES start from the same filtered DNS field and run
+    the same physical time."""
+    import importlib.util
+
+    spec = importlib.util.spec_from_file_location("closure_train",
+                                                  os.path.join(ROOT, "boundary-layer-gpu_amd", "closure_train.py"))
+    T = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(T)
+    nc, fac = (args.les_nx, args.les_ny, args.les_nz), (4, 2, 4)
+    box = (400 * THETA, 40 * THETA, 40 * THETA)
+    (xf, yf, zf), (xc, yc, zc) = T.lesgrid_pair(blgpu, nc, fac, box)
+    eta, f, df = blgpu.blasius_profile()
+    Uf, Vf, Wf = blgpu.blasius_field(xf, yf, zf, NU, eta, f, df)
+    blgpu.add_noise(Uf, Vf, Wf, 0.01, 1)
+    tab_f = blgpu.blasius_tables(xf, yf, NU, eta, f, df)
+    tab_c = blgpu.blasius_tables(xc, yc, NU, eta, f, df)
+    fine = blgpu.Solver(xf, yf, zf, NU, tab_f, cfl=CFL)
+    fine.set_fields(Uf, Vf, Wf)
+    del Uf, Vf, Wf
+    judge = blgpu.Solver(xc, yc, zc, NU, tab_c, les=2, cfl=CFL)
+    d99 = 7.5 * THETA  # Blasius delta99 / theta ~ 4.91 / 0.664 = 7.4 at the inlet
+    ncell_c = (nc[0] - 1) * (nc[1] - 1) * (nc[2] - 2)
+    snaps = args.les_snapshots
+    c = blgpu.Closure(capacity=(snaps + 1) * ncell_c, batch_max=65536)
+    log(f"bench: LES leg: DNS {xf.size}x{yf.size}x{zf.size}, LES {nc}, {snaps} training snapshots")
+    T.collect(fine, judge, c, fac, d99, snapshots=snaps + 1, every=args.les_every, smin=1e-3 / THETA)
+    rows = c.rows()
+    train, held = rows[: snaps * ncell_c], rows[snaps * ncell_c:]
+    nu_dsm = T.dsm_at_rows(judge)
+    c.clear()
+    c.add_rows(train)
+    c.params = T.init_params(0)
+    before = T.apriori(c, held, nu_dsm)
+    t0 = time.perf_counter()
+    hist = c.train(args.les_train_steps, 65536, lr=1e-3, seed=0)
+    t_train = time.perf_counter() - t0
+    after = T.apriori(c, held, nu_dsm)
+    # both LES start from the held-out filtered field at the DNS time
+    Uc, Vc, Wc = judge.fields()
+    t_start = judge.t
+    runs = {}
+    for name, les in (("dynamic_smagorinsky", 2), ("learned", 4)):
+        s = blgpu.Solver(xc, yc, zc, NU, tab_c, les=les, cfl=CFL)
+        s.set_fields(Uc, Vc, Wc)
+        s.t = t_start
+        if les == 4:
+            c.install(s, d99, 3.0)
+        dt = s.compute_dt()
+        s.step()  # first step untimed (code objects load on first launch)
+        s.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.les_steps - 1):
+            s.compute_dt()
+            s.step()
+        s.synchronize()
+        el = time.perf_counter() - t0
+        runs[name] = dict(solver=s, el=el, dt=dt)
+    # DNS to the same time
+    t_end = runs["learned"]["solver"].t
+    while fine.t < t_end - 1e-12:
+        fine.compute_dt()
+        fine.step()
+    cf_dns = fine.wall_stats()[:: fac[0]]
+    cells_c = nc[0] * nc[1] * nc[2]
+    out = dict(grid_les=list(nc), grid_dns=[xf.size, yf.size, zf.size], les_steps=args.les_steps,
+               time_span=t_end - t_start, dns_time_matched=fine.t,
+               training=dict(rows=int(train.shape[0]), weighted=int((train[:, 11] > 0).sum()),
+                             steps=args.les_train_steps, batch=65536, seconds=t_train,
+                             loss_first=float(hist[:20].mean()), loss_last=float(hist[-20:].mean())),
+               apriori_heldout=after, apriori_untrained_network=before["learned"])
+    for name, r in runs.items():
+        s = r["solver"]
+        cf = s.wall_stats()
+        inner = slice(2, -2)
+        err = float(np.linalg.norm(cf[inner] - cf_dns[inner]) / np.linalg.norm(cf_dns[inner]))
+        nt = args.les_steps - 1
+        d = dict(ns_per_cell_step=r["el"] / nt / cells_c * 1e9, ms_per_step=r["el"] / nt * 1e3,
+                 cf_rel_l2_vs_dns=err, dt=r["dt"])
+        if name == "learned":
+            d["fallback_cells"], d["cells"] = s.closure_fallback()[1], s.closure_fallback()[0]
+        out[name] = d
+        s.close()
+    out["cost_ratio_learned_over_dsm"] = out["learned"]["ns_per_cell_step"] / out["dynamic_smagorinsky"]["ns_per_cell_step"]
+    c.close()
+    judge.close()
+    fine.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -171,6 +264,17 @@ def main():
     ap.add_argument("--nz", type=int, default=193)
     ap.add_argument("--cpu-nz", type=int, default=17, help="z extent of the CPU-baseline sample (0 = skip)")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision leg")
+    ap.add_argument("--no-les", action="store_true", help="skip the learned-closure LES leg")
+    ap.add_argument("--calib-copy", type=int, default=0, metavar="MIB",
+                    help="first copy MIB MiB device to device once (a plain copy of known size that "
+                         "calibrates hardware byte counters in a profiled run)")
+    ap.add_argument("--les-nx", type=int, default=129)
+    ap.add_argument("--les-ny", type=int, default=49)
+    ap.add_argument("--les-nz", type=int, default=34)
+    ap.add_argument("--les-snapshots", type=int, default=8)
+    ap.add_argument("--les-every", type=int, default=25)
+    ap.add_argument("--les-train-steps", type=int, default=2000)
+    ap.add_argument("--les-steps", type=int, default=100)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU; multi-GPU runs are not part of it "
@@ -178,6 +282,17 @@ def main():
         sys.exit(2)
 
     from tests.blgpu_import import blgpu
+
+    if args.calib_copy > 0:
+        import torch
+
+        src = torch.ones(args.calib_copy << 20, dtype=torch.uint8, device="cuda")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        print(f"bench: calibration copy of {args.calib_copy} MiB done", file=sys.stderr, flush=True)
+        del src, dst
 
     nx, ny, nz = args.nx, args.ny, args.nz
     cells = nx * ny * nz
@@ -200,6 +315,11 @@ def main():
         rmx = run_leg(blgpu, blgpu.PREC_MIXED, grid, args, keep_fields=True)
         err = {}
         for name, a, b in zip(("u", "v", "w", "p"), rmx["fields"], r64["fields"]):
+            if name == "p":
+                # P is defined up to the constant the pinned (0,0) mode takes from
+                # the rhs round-off: compare the interior with its mean removed
+                a, b = a[1:-1, 1:-1, 1:-2], b[1:-1, 1:-1, 1:-2]
+                a, b = a - a.mean(), b - b.mean()
             err[name] = float(np.linalg.norm(a - b) / np.linalg.norm(b))
         err["cf"] = float(np.linalg.norm(rmx["cf"] - r64["cf"]) / np.linalg.norm(r64["cf"]))
         v_mx = rmx["el"] / args.steps / cells * 1e9
@@ -211,6 +331,10 @@ def main():
                                   for k, v in rmx["prof"].items() if v[1] > 0})
         del rmx
     del r64["fields"]
+
+    les_out = None
+    if not args.no_les:
+        les_out = les_leg(blgpu, args, lambda m: print(m, file=sys.stderr, flush=True))
 
     cpu = None
     if args.cpu_nz > 0:
@@ -236,6 +360,7 @@ def main():
         "roofline": roof,
         "cpu_baseline": cpu,
         "mixed": mixed_out,
+        "les": les_out,
         "kernels": kernels,
     }
     print(json.dumps(out), flush=True)

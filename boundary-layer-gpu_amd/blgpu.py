@@ -71,6 +71,10 @@ SIGNATURES = {
     "bl_mlp_count": (_I, [_P, ctypes.POINTER(ctypes.c_longlong)]),
     "bl_mlp_clear": (_I, [_P]),
     "bl_mlp_train": (_I, [_P, _I, _I, ctypes.c_float, ctypes.c_ulonglong, _P]),
+    "bl_closure_samples": (_I, [_P, _P, _P, _I, _I, _I, _D, _D]),
+    "bl_set_closure": (_I, [_P, _P, _D, _D]),
+    "bl_closure_fallback": (_I, [_P, ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(ctypes.c_longlong)]),
+    "bl_closure_infer": (_I, [_P, _P, ctypes.c_longlong, _P]),
 }
 
 MLP_IN, MLP_HIDDEN, MLP_PARAMS, MLP_ROW = 10, 64, 9089, 28
@@ -278,6 +282,12 @@ class Solver:
     def projection(self, rk_step, dt):
         _check(lib().bl_projection(self.h, rk_step, dt), "bl_projection")
 
+    def closure_fallback(self):
+        """(cells evaluated, cells that kept the DSM value) in the last evaluation."""
+        c, f = ctypes.c_longlong(), ctypes.c_longlong()
+        _check(lib().bl_closure_fallback(self.h, ctypes.byref(c), ctypes.byref(f)), "bl_closure_fallback")
+        return c.value, f.value
+
     def profile(self, on=True):
         _check(lib().bl_profile_enable(self.h, 1 if on else 0), "bl_profile_enable")
 
@@ -364,6 +374,20 @@ class Closure:
 
     def clear(self):
         _check(lib().bl_mlp_clear(self.h), "bl_mlp_clear")
+
+    def infer_bf16(self, X):
+        """The BF16-MFMA inference kernel on feature rows."""
+        X = _f32(X)
+        y = np.zeros(X.shape[0], dtype=np.float32)
+        _check(lib().bl_closure_infer(self.h, X.ctypes.data, X.shape[0], y.ctypes.data), "bl_closure_infer")
+        return y
+
+    def samples_from(self, fine, coarse, fx, fy, fz, delta99, smin=0.0):
+        """Rows from the DNS solver `fine` filtered onto the LES solver `coarse`."""
+        _check(lib().bl_closure_samples(fine.h, coarse.h, self.h, fx, fy, fz, delta99, smin), "bl_closure_samples")
+
+    def install(self, solver, delta99, nsigma=3.0):
+        _check(lib().bl_set_closure(solver.h, self.h, delta99, nsigma), "bl_set_closure")
 
     def train(self, steps, batch, lr=1e-3, seed=0, history=True):
         hist = np.zeros(steps, dtype=np.float32) if history else None

@@ -19,6 +19,7 @@
 
 #include "../../include/blgpu.h"
 #include "bl_internal.hpp"
+#include "bl_mlp.hpp"
 
 namespace blgpu {
 
@@ -141,7 +142,8 @@ void check_config(const bl_config *c) {
     if (!c->U_inlet || !c->V_inlet || !c->W_inlet || !c->U_top || !c->V_top || !c->W_top)
         bad("bl_create: missing boundary tables");
     if (c->scheme != 2 && c->scheme != 3) bad("bl_create: scheme must be 2 (RK2) or 3 (RK3)");
-    if (c->les != 0 && c->les != 2) bad("bl_create: les must be 0 (DNS) or 2 (dynamic Smagorinsky)");
+    if (c->les != 0 && c->les != 2 && c->les != 4)
+        bad("bl_create: les must be 0 (DNS), 2 (dynamic Smagorinsky) or 4 (learned closure with DSM fallback)");
     if (c->precision != BL_PREC_FP64 && c->precision != BL_PREC_MIXED)
         bad("bl_create: precision must be BL_PREC_FP64 or BL_PREC_MIXED");
     if (c->precision == BL_PREC_MIXED && c->les != 0)
@@ -367,7 +369,7 @@ void build(Handle &h, const bl_config *c) {
     h.P = falloc();
     h.phi = falloc();
     h.nu_t = falloc();
-    if (h.les == 2) {
+    if (h.les == 2 || h.les == 4) {
         h.dsm = dmalloc(nf * kDsmArrays);
         BL_HIP(hipMemsetAsync(h.dsm, 0, nf * kDsmArrays * sizeof(double), h.stream));
     }
@@ -488,7 +490,7 @@ void free_all(Handle &h) {
                     (void *)h.nu_t, (void *)h.T1, (void *)h.S, (void *)h.dct_w1, (void *)h.dct_w2,
                     (void *)h.dct_wN, (void *)h.dct_rot, (void *)h.zc, (void *)h.zs, (void *)h.lu_dl, (void *)h.lu_dp, (void *)h.lu_du,
                     (void *)h.scal, (void *)h.red, (void *)h.dsm, h.mem_grid, h.mem_gridf, (void *)h.mx_tab,
-                    (void *)h.p00})
+                    (void *)h.p00, (void *)h.cl_w, (void *)h.cl_b, (void *)h.cl_cnt})
         f(p);
     if (h.stream) (void)hipStreamSynchronize(h.stream);
     prof_free(h);
@@ -529,7 +531,8 @@ void do_step_t(Handle &h) {
     for (int s = 1; s <= nst; ++s) {
         const int nxt = order3[s - 1];
         // eddy viscosity of the stage's input fields (time_integration.f90:341-346)
-        if (h.les == 2) launch_dsm(h, h.buf[cur]);
+        if (h.les == 2 || h.les == 4) launch_dsm(h, h.buf[cur]);
+        if (h.les == 4) launch_closure(h, h.buf[cur]);
         Fld3<T> next = as<T>(h.buf[nxt]);
         launch_rk_stage(h, as<T>(h.buf[cur]), old, next, s, dt);
         const double ts = to + (nst == 3 ? h.rk_t[s] : h.rk2_t[s]) * dt;
@@ -774,9 +777,10 @@ int bl_synchronize(bl_handle *hh) {
 int bl_eddy_viscosity(bl_handle *hh) {
     return guarded([&] {
         Handle &h = H(hh);
-        if (h.les == 2)
+        if (h.les == 2 || h.les == 4) {
             launch_dsm(h, h.buf[h.icur]);
-        else
+            if (h.les == 4) launch_closure(h, h.buf[h.icur]);
+        } else
             BL_HIP(hipMemsetAsync(h.nu_t, 0, (size_t)h.d.sxy * h.d.nzg * elem_size(h), h.stream));
     });
 }
@@ -917,6 +921,44 @@ int bl_poisson(bl_handle *hh, double *rhs) {
         copy_to_device(h, h.phi, rhs, n1, n2, n3);
         launch_poisson(h, h.phi);
         copy_to_host(h, rhs, h.phi, n1, n2, n3);
+    });
+}
+
+int bl_closure_samples(bl_handle *fine, bl_handle *coarse, bl_mlp *m, int fx, int fy, int fz, double delta99,
+                       double smin) {
+    return guarded([&] {
+        if (!m || !(delta99 > 0.0)) {
+            last_error() = "bl_closure_samples: need a closure handle and delta99 > 0";
+            throw ArgError();
+        }
+        closure_samples(H(fine), H(coarse), *m, fx, fy, fz, delta99, smin);
+    });
+}
+
+int bl_set_closure(bl_handle *hh, bl_mlp *m, double delta99, double nsigma) {
+    return guarded([&] {
+        if (!m || !(delta99 > 0.0) || !(nsigma >= 0.0)) {
+            last_error() = "bl_set_closure: need a closure handle, delta99 > 0 and nsigma >= 0";
+            throw ArgError();
+        }
+        closure_install(H(hh), *m, delta99, nsigma);
+    });
+}
+
+int bl_closure_fallback(bl_handle *hh, long long *cells, long long *fallback) {
+    return guarded([&] {
+        if (!cells || !fallback) throw ArgError();
+        closure_fallback(H(hh), cells, fallback);
+    });
+}
+
+int bl_closure_infer(bl_mlp *m, const float *x, long long n, float *y) {
+    return guarded([&] {
+        if (!m || !x || !y || n < 1) {
+            last_error() = "bl_closure_infer: need a closure handle, n >= 1 and buffers";
+            throw ArgError();
+        }
+        closure_infer_rows(*m, x, n, y);
     });
 }
 

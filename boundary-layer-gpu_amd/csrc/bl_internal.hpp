@@ -80,6 +80,7 @@ enum KernelId {
     KID_DSM_VFILTER,
     KID_DSM_FILTER,
     KID_DSM_COLUMN,
+    KID_CLOSURE,
     KID_COUNT
 };
 const char *kernel_name(int id);
@@ -107,7 +108,14 @@ struct Handle {
     int icur;       // index of the current (most recent) buffer
     Field3 F1;      // first-stage RHS (RK3)
     double *P, *phi, *nu_t;  // float in mixed precision
-    double *dsm;  // dynamic Smagorinsky work arrays (LES = 2 only), kDsmArrays field-sized slots
+    double *dsm;  // dynamic Smagorinsky work arrays (LES = 2 and 4), kDsmArrays field-sized slots
+    // learned closure (LES = 4): BF16 weights, FP32 biases, input range
+    unsigned short *cl_w;
+    float *cl_b, *cl_lo, *cl_hi;
+    double cl_inv_d99;
+    double *cl_cnt;  // fallback cells per block of the last evaluation
+    int cl_nblk;
+    int cl_ready;
     double *T1, *S;  // transform intermediates: [m][j][kx] and [q][j][kx]
     int dct_n1;                      // x-DCT factorisation N = dct_n1 * (N / dct_n1)
     int lda_zt;                      // leading dimension of the z cos/sin tables
@@ -172,6 +180,8 @@ void launch_poisson_mixed(Handle &h, float *phi, const Fld3<float> &f);
 // dynamic Smagorinsky eddy viscosity of f into h.nu_t (bl_dsm.hip)
 constexpr int kDsmArrays = 15;
 void launch_dsm(Handle &h, const Field3 &f);
+// learned closure over the DSM values in h.nu_t (bl_closure.hip)
+void launch_closure(Handle &h, const Field3 &f);
 
 std::string &last_error();
 

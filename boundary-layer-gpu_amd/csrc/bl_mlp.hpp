@@ -32,6 +32,11 @@ void mlp_free(Mlp &m);
 void mlp_load_batch(Mlp &m, const float *x, const float *t, int n);
 void mlp_fwd_bwd(Mlp &m, int B, bool backward);
 void mlp_train(Mlp &m, int steps, int B, float lr, unsigned long long seed, float *loss_hist);
+// bl_closure.hip
+void closure_samples(Handle &fine, Handle &coarse, Mlp &m, int fx, int fy, int fz, double delta99, double smin);
+void closure_install(Handle &h, Mlp &m, double delta99, double nsigma);
+void closure_fallback(Handle &h, long long *cells, long long *fallback);
+void closure_infer_rows(Mlp &m, const float *x_host, long long n, float *y_host);
 
 }  // namespace blgpu
 

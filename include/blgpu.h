@@ -42,7 +42,8 @@ enum { BL_PREC_FP64 = 0, BL_PREC_MIXED = 1 };
  * LU / twiddle precompute (initialization.f90:313-320, 427-469): the library
  * derives metrics, transform matrices and LU factors itself from the grids.
  *  x(nx), y(ny), z(nz): face coordinates (x_global, y_global, z_global).
- *  scheme: RKscheme (2 or 3), les: LES flag (0 DNS, 2 dynamic Smagorinsky),
+ *  scheme: RKscheme (2 or 3), les: LES flag (0 DNS, 2 dynamic Smagorinsky,
+ *  4 learned closure with dynamic Smagorinsky fallback),
  *  cfl: .turbb CFL (negative = fixed dt = -cfl).
  *  Boundary tables as built by compute_blasius_solution_for_bc
  *  (boundary_conditions.f90:1452-1797): U_inlet(nyg) V_inlet(ny) W_inlet(nyg)
@@ -175,6 +176,25 @@ int bl_mlp_clear(bl_mlp *m);
  * drawn by a counter hash of (seed, step, slot); loss_hist[steps] optional:
  * the weighted batch loss before each update */
 int bl_mlp_train(bl_mlp *m, int steps, int batch, float lr, unsigned long long seed, float *loss_hist);
+
+/* ---- learned closure in the LES (les = 4; no counterpart in the reference) ----
+ * Training rows from a DNS (fine, FP64) filtered onto an LES grid (coarse,
+ * FP64) refined from it by (fx, fy, fz) cells with coinciding faces: box
+ * (top-hat) averages of the DNS faces give the filtered staggered field,
+ * which is left in the coarse solver's state; per interior LES cell the row
+ * holds the features the LES computes at run time and the least-squares eddy
+ * viscosity of the exact SGS stress.  Rows with |S| <= smin or next to the x
+ * boundaries get weight 0. */
+int bl_closure_samples(bl_handle *fine, bl_handle *coarse, bl_mlp *m, int fx, int fy, int fz, double delta99,
+                       double smin);
+/* install the trained network in an les = 4 solver (BF16 weights) with the
+ * input range [min - nsigma sd, max + nsigma sd] of the weighted rows; cells
+ * outside it keep the dynamic Smagorinsky value */
+int bl_set_closure(bl_handle *h, bl_mlp *m, double delta99, double nsigma);
+/* cells evaluated and cells that fell back to DSM in the last evaluation */
+int bl_closure_fallback(bl_handle *h, long long *cells, long long *fallback);
+/* the BF16-MFMA inference kernel on feature rows x[n][10] (parity checks) */
+int bl_closure_infer(bl_mlp *m, const float *x, long long n, float *y);
 
 /* ---- host-side set-up helpers (not on the per-step path) ---- */
 

@@ -117,3 +117,179 @@ def test_training_lowers_the_loss(net):
     say("loss first", first, "last", last)
     assert np.all(np.isfinite(hist))
     assert last < 0.2 * first
+
+
+# ---------------------------------------------------------------------------
+# data path: DNS filtered onto an LES grid, features, targets, inference
+# ---------------------------------------------------------------------------
+from tests import cases, closure_ref as R  # noqa: E402
+from tests import orc  # noqa: E402
+
+FX, FY, FZ = 4, 2, 4
+NC = (12, 10, 6)  # LES grid (nx, ny, nz)
+
+
+def fine_and_coarse(seed=21, noise=0.05):
+    nx, ny, nz = NC
+    nxf, nyf, nzf = FX * (nx - 1) + 1, FY * (ny - 1) + 1, FZ * (nz - 2) + 2
+    Lx, Ly, Lz = 0.08, 0.015, 0.01
+    xf, yf, zf = cases.tanh_grid(nxf, nyf, nzf, Lx, Ly, Lz, 2.0)
+    xc, yc, zc = cases.tanh_grid(nx, ny, nz, Lx, Ly, Lz, 2.0)
+    eta, f, df = cases.load_blasius()
+    Uf, Vf, Wf = cases.blasius_fields(xf, yf, zf, 4.898844e-6, eta, f, df)
+    cases.add_noise(Uf, Vf, Wf, noise, seed)
+    for A in (Uf, Vf):
+        A[:, :, 0] = A[:, :, -3]
+        A[:, :, -2] = A[:, :, 1]
+        A[:, :, -1] = A[:, :, 2]
+    Wf[:, :, 0] = Wf[:, :, -2]
+    Wf[:, :, -1] = Wf[:, :, 1]
+    return (xf, yf, zf, Uf, Vf, Wf), (xc, yc, zc)
+
+
+def solvers(fine, coarse, les=0):
+    xf, yf, zf, Uf, Vf, Wf = fine
+    xc, yc, zc = coarse
+    eta, f, df = cases.load_blasius()
+    nu = 4.898844e-6
+    sf = blgpu.Solver(xf, yf, zf, nu, orc.blasius_tables(xf, yf, nu, eta, f, df))
+    sf.set_fields(Uf, Vf, Wf)
+    sc = blgpu.Solver(xc, yc, zc, nu, orc.blasius_tables(xc, yc, nu, eta, f, df), les=les)
+    return sf, sc
+
+
+def test_samples_match_reference():
+    fine, coarse = fine_and_coarse()
+    xf, yf, zf, Uf, Vf, Wf = fine
+    xc, yc, zc = coarse
+    sf, sc = solvers(fine, coarse)
+    c = blgpu.Closure(capacity=100000, batch_max=1024)
+    d99 = 0.01
+    c.samples_from(sf, sc, FX, FY, FZ, d99, smin=1e-3)
+    Uc, Vc, Wc = sc.fields()
+    Ur, Vr, Wr = R.coarsen(Uf, Vf, Wf, yf, FX, FY, FZ, NC)
+    for name, a, b in zip("UVW", (Uc, Vc, Wc), (Ur, Vr, Wr)):
+        e = np.abs(a - b).max()
+        say("coarse", name, e)
+        assert e < 1e-13, name
+    rows = c.rows()
+    nxg, nyg, nzg = NC[0] + 1, NC[1] + 1, NC[2] + 1
+    assert rows.shape[0] == (nxg - 2) * (nyg - 2) * (nzg - 3)
+    rows = rows.reshape(nzg - 3, nyg - 2, nxg - 2, -1)  # slot = I-1 + (nxg-2)((J-1) + (nyg-2)(K-1))
+    f, S, smag, scale = R.features(xc, yc, zc, Ur, Vr, Wr, d99)
+    tau = R.sgs_stress(Uf, Vf, Wf, yf, FX, FY, FZ, NC)
+    tgt = R.target(S[:, :, :, : nzg - 3], tau, scale[:, :, : nzg - 3])
+    for q in range(10):
+        np.testing.assert_allclose(rows[..., q], f[q][:, :, : nzg - 3].transpose(2, 1, 0), atol=2e-6, err_msg=f"f{q}")
+    for q in range(6):
+        np.testing.assert_allclose(rows[..., 13 + q], S[q][:, :, : nzg - 3].transpose(2, 1, 0), rtol=1e-5,
+                                   atol=1e-6 * np.abs(S).max())
+        np.testing.assert_allclose(rows[..., 19 + q], tau[q].transpose(2, 1, 0), rtol=1e-5,
+                                   atol=1e-6 * np.abs(tau).max())
+    wmask = rows[..., 11] > 0
+    assert wmask.sum() > 0
+    np.testing.assert_allclose(rows[..., 10][wmask], tgt.transpose(2, 1, 0)[wmask], rtol=1e-4,
+                               atol=1e-5 * np.abs(tgt.transpose(2, 1, 0)[wmask]).max())
+    say("samples", rows.shape, "weighted", int(wmask.sum()))
+    c.close()
+
+
+def test_bf16_inference_close_to_fp32(net):
+    c, theta = net
+    X, _ = data(1000, seed=9)
+    y32 = c.forward(X)
+    y16 = c.infer_bf16(X)
+    e = np.abs(y16 - y32).max() / np.abs(y32).max()
+    say("bf16 vs fp32 inference", e)
+    assert e < 3e-2
+
+
+@pytest.mark.parametrize("nsigma", [1e30, 0.0])
+def test_les_closure_nu_t(nsigma):
+    """LES = 4 eddy viscosity: the network's value where the features are in
+    range, the dynamic Smagorinsky value (oracle) elsewhere."""
+    fine, coarse = fine_and_coarse(seed=22)
+    xc, yc, zc = coarse
+    sf, sc = solvers(fine, coarse, les=4)
+    c = blgpu.Closure(capacity=100000, batch_max=2048)
+    d99 = 0.01
+    c.samples_from(sf, sc, FX, FY, FZ, d99, smin=1e-3)
+    c.params = M.init_params(3).astype(np.float32)
+    c.train(50, 512, lr=1e-3, seed=2)
+    c.install(sc, d99, nsigma)
+    sc.eddy_viscosity()
+    got = sc.download(4)
+    Uc, Vc, Wc = sc.fields()
+    f, S, smag, scale = R.features(xc, yc, zc, Uc, Vc, Wc, d99)
+    X = f.reshape(10, -1, order="F").T.astype(np.float32)
+    y = c.forward(X).reshape(f.shape[1:], order="F")
+    nu_net = np.maximum(y, 0) * scale
+    o = orc.Oracle(xc, yc, zc, 4.898844e-6, les=2)
+    o.set_fields(Uc, Vc, Wc)
+    nu_dsm = o.eddy_viscosity()[1:-1, 1:-1, 1:-1]
+    cells, fb = sc.closure_fallback()
+    say("nsigma", nsigma, "cells", cells, "fallback", fb)
+    inner = got[1:-1, 1:-1, 1:-1]
+    if nsigma > 1e20:
+        assert fb == 0
+        e = np.abs(inner - nu_net).max() / np.abs(nu_net).max()
+        say("learned nu_t vs fp32 network", e)
+        assert e < 5e-2
+    else:
+        assert 0 < fb <= cells
+        rows = c.rows()
+        lo = rows[rows[:, 11] > 0, :10].min(0)
+        hi = rows[rows[:, 11] > 0, :10].max(0)
+        lo4, hi4 = lo[:, None, None, None], hi[:, None, None, None]
+        f32 = f.astype(np.float32)  # the device compares FP32 features
+        out = ((f32 < lo4) | (f32 > hi4)).any(0)
+        clear = ((f < lo4 - 1e-5) | (f > hi4 + 1e-5)).any(0)  # out by more than FP32 rounding
+        assert clear.sum() > 0
+        np.testing.assert_allclose(inner[clear], nu_dsm[clear], rtol=1e-9, atol=1e-12 * np.abs(nu_dsm).max())
+        assert abs(int(out.sum()) - fb) <= max(2, fb // 100)  # float rounding at the range edges
+    c.close()
+
+
+def test_trained_network_gains_a_priori():
+    """Train on filtered snapshots of a small noisy-Blasius DNS, judge on a
+    held-out snapshot: the trained network's SGS-stress error is below the
+    untrained network's and below no model; the dynamic Smagorinsky and
+    constant-coefficient errors are reported beside it."""
+    import importlib.util
+    import os
+
+    spec = importlib.util.spec_from_file_location(
+        "closure_train", os.path.join(os.path.dirname(blgpu.__file__), "closure_train.py"))
+    T = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(T)
+    nc, fac = (24, 16, 10), (4, 2, 4)
+    box = (0.1, 0.012, 0.01)
+    (xf, yf, zf), (xc, yc, zc) = T.lesgrid_pair(blgpu, nc, fac, box)
+    eta, f, df = cases.load_blasius()
+    nu = 4.898844e-6
+    Uf, Vf, Wf = blgpu.blasius_field(xf, yf, zf, nu, eta, f, df)
+    blgpu.add_noise(Uf, Vf, Wf, 0.08, 3)
+    fine = blgpu.Solver(xf, yf, zf, nu, blgpu.blasius_tables(xf, yf, nu, eta, f, df), cfl=-2e-4)
+    fine.set_fields(Uf, Vf, Wf)
+    coarse = blgpu.Solver(xc, yc, zc, nu, blgpu.blasius_tables(xc, yc, nu, eta, f, df), les=2)
+    c = blgpu.Closure(capacity=200000, batch_max=4096)
+    d99 = 0.006
+    T.collect(fine, coarse, c, fac, d99, snapshots=5, every=20, smin=1e-2)
+    rows = c.rows()
+    per = rows.shape[0] // 5
+    held = rows[4 * per:]
+    nu_dsm = T.dsm_at_rows(coarse)  # the coarse solver holds the last (held-out) snapshot
+    c.clear()
+    c.add_rows(rows[:4 * per])
+    c.params = M.init_params(11).astype(np.float32)
+    before = T.apriori(c, held, nu_dsm)
+    hist = c.train(1500, 2048, lr=1e-3, seed=4)
+    after = T.apriori(c, held, nu_dsm)
+    say("weighted held-out rows", int((held[:, 11] > 0).sum()))
+    say("loss", hist[:10].mean(), "->", hist[-10:].mean())
+    for k in after:
+        say("a priori", k, after[k], "(untrained" if k == "learned" else "", before[k] if k == "learned" else "")
+    assert hist[-10:].mean() < hist[:10].mean()
+    assert after["learned"]["rel_error"] < before["learned"]["rel_error"]
+    assert after["learned"]["rel_error"] < after["no_model"]["rel_error"]
+    c.close()

@@ -5,7 +5,8 @@ FP32 accumulation, FP64 Thomas solves, FP64 (0,0) pressure mode, FP64
 mass-flux correction and FP64 statistics.  Error budget (relative L2 against
 the FP64 oracle): single operators 1e-5 (RHS 2e-5: FP32 differences of
 nearly equal neighbours), one projection 1e-5 on the velocities, and 1e-4 on
-u, v, w after twenty RK3 steps (the stated target for the mixed mode).
+u, v, w after twenty RK3 steps (the stated target for the mixed mode), 1e-3
+on the mean-free pressure.
 """
 import sys
 
@@ -82,11 +83,12 @@ def test_mixed_projection(case):
         e = np.linalg.norm(a - b) / np.linalg.norm(U)
         say("projection", name, e)
         assert e < 1e-5, name
-    # the divergence left by the mixed projection, relative to max|u|/min(dy)
+    # the divergence left by one projection (before the second boundary
+    # update) is the FP64 path's to FP32 accuracy, relative to max|u|/min(dy)
     scale = np.abs(U).max() / np.diff(case.y).min()
     dg, do = s.max_divergence(), o.max_divergence()
     say("max div mixed", dg / scale, "fp64", do / scale)
-    assert dg / scale < 1e-5
+    assert abs(dg - do) / scale < 1e-6
 
 
 def test_mixed_steps(case):
@@ -102,8 +104,13 @@ def test_mixed_steps(case):
         e = np.linalg.norm(a - b) / np.linalg.norm(U)
         say("20 steps", name, e)
         assert e < 1e-4, name
-    P_o, P_g = o.get(3), s.download(3)
-    say("20 steps P", rel_l2(P_g, P_o))
+    # P is fixed only up to the constant that the pinned (0,0) mode picks from
+    # the rhs's compatibility residual (round-off of the field), so compare the
+    # pressure with its interior mean removed
+    P_o, P_g = o.get(3)[1:-1, 1:-1, 1:-2], s.download(3)[1:-1, 1:-1, 1:-2]
+    e = rel_l2(P_g - P_g.mean(), P_o - P_o.mean())
+    say("20 steps P (mean removed)", e, "raw", rel_l2(P_g, P_o))
+    assert e < 1e-3
     cf_o, cf_g = o.wall_stats(), s.wall_stats()
     e = rel_l2(cf_g, cf_o)
     say("20 steps Cf", e)
