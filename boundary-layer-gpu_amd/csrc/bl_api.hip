@@ -146,8 +146,7 @@ void check_config(const bl_config *c) {
         bad("bl_create: les must be 0 (DNS), 2 (dynamic Smagorinsky) or 4 (learned closure with DSM fallback)");
     if (c->precision != BL_PREC_FP64 && c->precision != BL_PREC_MIXED)
         bad("bl_create: precision must be BL_PREC_FP64 or BL_PREC_MIXED");
-    if (c->precision == BL_PREC_MIXED && c->les != 0)
-        bad("bl_create: the mixed-precision path runs DNS (les = 0) only");
+
     if (c->n_modes < 0 || c->m_modes < 0) bad("bl_create: negative mode count");
     if (c->n_modes > 0 && (c->m_modes <= 0 || !c->zmode || !c->tmode || !c->qu_re || !c->qu_im ||
                            !c->qv_re || !c->qv_im || !c->qw_re || !c->qw_im))

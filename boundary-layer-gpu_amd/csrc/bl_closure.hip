@@ -425,11 +425,12 @@ __global__ __launch_bounds__(256) void k_infer_rows(const unsigned short *__rest
 
 // LES = 4: per interior cell, the network's eddy viscosity or the DSM value
 // already in nu_t when a feature is out of range; fallback counts per block.
-__global__ __launch_bounds__(256) void k_closure(Dims d, Metrics m, const double *__restrict__ U,
-                                                 const double *__restrict__ V, const double *__restrict__ W,
+template <class T>
+__global__ __launch_bounds__(256) void k_closure(Dims d, Metrics m, const T *__restrict__ U, const T *__restrict__ V,
+                                                 const T *__restrict__ W,
                                                  const unsigned short *__restrict__ wbf, const float *__restrict__ bias,
                                                  const float *__restrict__ lo, const float *__restrict__ hi,
-                                                 double inv_d99, double *__restrict__ nu_t, double *__restrict__ cnt) {
+                                                 double inv_d99, T *__restrict__ nu_t, double *__restrict__ cnt) {
     __shared__ InferLds L;
     __shared__ unsigned short act[4][64 * kH];
     __shared__ float outb[4][64];
@@ -456,7 +457,7 @@ __global__ __launch_bounds__(256) void k_closure(Dims d, Metrics m, const double
     for (int q = 0; q < kMlpIn; ++q) A[lane * kH + q] = to_bf16(cf.f[q]);
     __builtin_amdgcn_wave_barrier();
     const float c = infer_wave(L, A, outb[wave]);
-    if (live && !fb) nu_t[fidx(d, i, j, k)] = fmax((double)c, 0.0) * cf.scale;
+    if (live && !fb) nu_t[fidx(d, i, j, k)] = (T)(fmax((double)c, 0.0) * cf.scale);
     // fallback count of the block
     const unsigned long long bal = __ballot(live && fb);
     if (lane == 0) nfb[wave] = __popcll(bal);
@@ -466,11 +467,12 @@ __global__ __launch_bounds__(256) void k_closure(Dims d, Metrics m, const double
 
 // y ghost rows of nu_t after the interior changed (as apply_Neumann_bc_y /
 // apply_Dirichlet_bc_y on nu_t, subgrid.f90:406-411)
-__global__ __launch_bounds__(256) void k_nut_yghost(Dims d, int dirichlet, double *__restrict__ nu_t) {
+template <class T>
+__global__ __launch_bounds__(256) void k_nut_yghost(Dims d, int dirichlet, T *__restrict__ nu_t) {
     const int i = blockIdx.x * 256 + threadIdx.x;
     const int k = blockIdx.y;
     if (i >= d.nxg) return;
-    const double sg = dirichlet == 1 ? -1.0 : 1.0;
+    const T sg = dirichlet == 1 ? (T)-1 : (T)1;
     nu_t[fidx(d, i, 0, k)] = sg * nu_t[fidx(d, i, 1, k)];
     nu_t[fidx(d, i, d.nyg - 1, k)] = sg * nu_t[fidx(d, i, d.nyg - 2, k)];
 }
@@ -597,10 +599,20 @@ void launch_closure(Handle &h, const Field3 &f) {
     }
     const Dims &d = h.d;
     ProfScope ps(h, KID_CLOSURE);
-    hipLaunchKernelGGL(k_closure, dim3(h.cl_nblk), dim3(256), 0, h.stream, d, h.m, f.u, f.v, f.w, h.cl_w, h.cl_b,
-                       h.cl_lo, h.cl_hi, h.cl_inv_d99, h.nu_t, h.cl_cnt);
-    check();
-    hipLaunchKernelGGL(k_nut_yghost, dim3((d.nxg + 255) / 256, d.nzg), dim3(256), 0, h.stream, d, h.nut_bc, h.nu_t);
+    const dim3 gy((d.nxg + 255) / 256, d.nzg);
+    if (h.precision == BL_PREC_MIXED) {
+        const Fld3<float> ff = as<float>(f);
+        float *nut = reinterpret_cast<float *>(h.nu_t);
+        hipLaunchKernelGGL(k_closure<float>, dim3(h.cl_nblk), dim3(256), 0, h.stream, d, h.m, ff.u, ff.v, ff.w, h.cl_w,
+                           h.cl_b, h.cl_lo, h.cl_hi, h.cl_inv_d99, nut, h.cl_cnt);
+        check();
+        hipLaunchKernelGGL(k_nut_yghost<float>, gy, dim3(256), 0, h.stream, d, h.nut_bc, nut);
+    } else {
+        hipLaunchKernelGGL(k_closure<double>, dim3(h.cl_nblk), dim3(256), 0, h.stream, d, h.m, f.u, f.v, f.w, h.cl_w,
+                           h.cl_b, h.cl_lo, h.cl_hi, h.cl_inv_d99, h.nu_t, h.cl_cnt);
+        check();
+        hipLaunchKernelGGL(k_nut_yghost<double>, gy, dim3(256), 0, h.stream, d, h.nut_bc, h.nu_t);
+    }
     check();
 }
 

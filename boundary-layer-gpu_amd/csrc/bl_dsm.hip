@@ -23,6 +23,7 @@
 
 #include <cmath>
 
+#include "../../include/blgpu.h"
 #include "bl_internal.hpp"
 
 namespace blgpu {
@@ -52,25 +53,28 @@ __device__ __forceinline__ int zmap_f(const Dims &d, int k) { return k == 0 ? d.
 
 // compute_Sij at interior cell (i, j, k): diagonal terms at the centre, the
 // off-diagonal terms averaged from the four surrounding edges; returns |S|.
-__device__ double strain(const Dims &d, const Metrics &m, const double *__restrict__ U,
-                         const double *__restrict__ V, const double *__restrict__ W, int i, int j, int k,
-                         double s[6]) {
+template <class T>
+__device__ double strain(const Dims &d, const Metrics &m, const T *__restrict__ U, const T *__restrict__ V,
+                         const T *__restrict__ W, int i, int j, int k, double s[6]) {
     const long long sx = d.sx, sxy = d.sxy;
     const long long p = fidx(d, i, j, k);
-    s[0] = (U[p] - U[p - 1]) * m.rdx[i];
-    s[1] = (V[p] - V[p - sx]) * m.rdy[j];
-    s[2] = (W[p] - W[p - sxy]) * m.rdz[k];
+    s[0] = ((double)U[p] - (double)U[p - 1]) * m.rdx[i];
+    s[1] = ((double)V[p] - (double)V[p - sx]) * m.rdy[j];
+    s[2] = ((double)W[p] - (double)W[p - sxy]) * m.rdz[k];
     auto e4 = [&](int a, int b, int c) {
         const long long q = fidx(d, a, b, c);
-        return 0.5 * ((U[q - 1] - U[q - 1 - sx]) * m.rdyg[b - 1] + (V[q - sx] - V[q - 1 - sx]) * m.rdxg[a - 1]);
+        return 0.5 * (((double)U[q - 1] - (double)U[q - 1 - sx]) * m.rdyg[b - 1] +
+                      ((double)V[q - sx] - (double)V[q - 1 - sx]) * m.rdxg[a - 1]);
     };
     auto e5 = [&](int a, int b, int c) {
         const long long q = fidx(d, a, b, c);
-        return 0.5 * ((U[q - 1] - U[q - 1 - sxy]) * m.rdzg[c - 1] + (W[q - sxy] - W[q - 1 - sxy]) * m.rdxg[a - 1]);
+        return 0.5 * (((double)U[q - 1] - (double)U[q - 1 - sxy]) * m.rdzg[c - 1] +
+                      ((double)W[q - sxy] - (double)W[q - 1 - sxy]) * m.rdxg[a - 1]);
     };
     auto e6 = [&](int a, int b, int c) {
         const long long q = fidx(d, a, b, c);
-        return 0.5 * ((V[q - sx] - V[q - sx - sxy]) * m.rdzg[c - 1] + (W[q - sxy] - W[q - sx - sxy]) * m.rdyg[b - 1]);
+        return 0.5 * (((double)V[q - sx] - (double)V[q - sx - sxy]) * m.rdzg[c - 1] +
+                      ((double)W[q - sxy] - (double)W[q - sx - sxy]) * m.rdyg[b - 1]);
     };
     s[3] = 0.25 * (e4(i, j, k) + e4(i + 1, j, k) + e4(i, j + 1, k) + e4(i + 1, j + 1, k));
     s[4] = 0.25 * (e5(i, j, k) + e5(i + 1, j, k) + e5(i, j, k + 1) + e5(i + 1, j, k + 1));
@@ -88,15 +92,16 @@ __device__ __forceinline__ double fil(int a, int c) {
 
 // y part of filter_xzy at row j of a section spanning rows jlo..jhi
 // (one-sided at the section ends, subgrid.f90:179-181); q indexes row j.
-__device__ __forceinline__ double yfilt(const double *__restrict__ F, long long q, long long sx, int j, int jlo,
-                                        int jhi) {
-    if (j == jlo) return 2.0 / 3.0 * F[q] + 1.0 / 3.0 * F[q + sx];
-    if (j == jhi) return 1.0 / 3.0 * F[q - sx] + 2.0 / 3.0 * F[q];
-    return 1.0 / 6.0 * F[q - sx] + 2.0 / 3.0 * F[q] + 1.0 / 6.0 * F[q + sx];
+template <class T>
+__device__ __forceinline__ double yfilt(const T *__restrict__ F, long long q, long long sx, int j, int jlo, int jhi) {
+    if (j == jlo) return 2.0 / 3.0 * (double)F[q] + 1.0 / 3.0 * (double)F[q + sx];
+    if (j == jhi) return 1.0 / 3.0 * (double)F[q - sx] + 2.0 / 3.0 * (double)F[q];
+    return 1.0 / 6.0 * (double)F[q - sx] + 2.0 / 3.0 * (double)F[q] + 1.0 / 6.0 * (double)F[q + sx];
 }
 
 // full filter_xzy value centred on (ic, j, kc) of a face array
-__device__ double filt_face(const Dims &d, const double *__restrict__ F, int ic, int j, int kc, int jhi) {
+template <class T>
+__device__ double filt_face(const Dims &d, const T *__restrict__ F, int ic, int j, int kc, int jhi) {
     double acc = 0.0;
 #pragma unroll
     for (int a = -1; a <= 1; ++a)
@@ -106,19 +111,19 @@ __device__ double filt_face(const Dims &d, const double *__restrict__ F, int ic,
     return acc;
 }
 
-__global__ __launch_bounds__(256) void k_dsm_cells(Dims d, Metrics m, const double *__restrict__ U,
-                                                   const double *__restrict__ V, const double *__restrict__ W,
-                                                   double *__restrict__ ws, long long nf) {
+template <class T>
+__global__ __launch_bounds__(256) void k_dsm_cells(Dims d, Metrics m, const T *__restrict__ U, const T *__restrict__ V,
+                                                   const T *__restrict__ W, double *__restrict__ ws, long long nf) {
     const int i = blockIdx.x * 64 + (threadIdx.x & 63);
     const int j = blockIdx.y * 4 + (threadIdx.x >> 6);
     const int k = blockIdx.z;
     if (i >= d.nxg || j >= d.nyg) return;
     const long long p = fidx(d, i, j, k);
     const int ic = clampi(i, 1, d.nxg - 2);
-    ws[D_CU * nf + p] = 0.5 * (U[fidx(d, ic - 1, j, k)] + U[fidx(d, ic, j, k)]);
-    ws[D_CV * nf + p] = (j >= 1 && j <= d.nyg - 2) ? 0.5 * (V[p - d.sx] + V[p]) : 0.0;
+    ws[D_CU * nf + p] = 0.5 * ((double)U[fidx(d, ic - 1, j, k)] + (double)U[fidx(d, ic, j, k)]);
+    ws[D_CV * nf + p] = (j >= 1 && j <= d.nyg - 2) ? 0.5 * ((double)V[p - d.sx] + (double)V[p]) : 0.0;
     const int kc = zmap_c(d, k);
-    ws[D_CW * nf + p] = 0.5 * (W[fidx(d, i, j, kc - 1)] + W[fidx(d, i, j, kc)]);
+    ws[D_CW * nf + p] = 0.5 * ((double)W[fidx(d, i, j, kc - 1)] + (double)W[fidx(d, i, j, kc)]);
     if (i >= 1 && i <= d.nxg - 2 && j >= 1 && j <= d.nyg - 2 && k >= 1 && k <= d.nzg - 2) {
         double s[6];
         const double mag = strain(d, m, U, V, W, i, j, k, s);
@@ -128,9 +133,9 @@ __global__ __launch_bounds__(256) void k_dsm_cells(Dims d, Metrics m, const doub
     }
 }
 
-__global__ __launch_bounds__(256) void k_dsm_vfilter(Dims d, const double *__restrict__ U,
-                                                     const double *__restrict__ V, const double *__restrict__ W,
-                                                     double *__restrict__ ws, long long nf) {
+template <class T>
+__global__ __launch_bounds__(256) void k_dsm_vfilter(Dims d, const T *__restrict__ U, const T *__restrict__ V,
+                                                     const T *__restrict__ W, double *__restrict__ ws, long long nf) {
     const int i = blockIdx.x * 64 + (threadIdx.x & 63);
     const int j = blockIdx.y * 4 + (threadIdx.x >> 6);
     const int k = blockIdx.z;
@@ -210,8 +215,9 @@ __global__ __launch_bounds__(256) void k_dsm_filter(Dims d, Metrics m, double *_
     ws[D_MM * nf + p] = M[0] * M[0] + M[1] * M[1] + M[2] * M[2] + 2.0 * (M[3] * M[3] + M[4] * M[4] + M[5] * M[5]);
 }
 
+template <class T>
 __global__ __launch_bounds__(256) void k_dsm_column(Dims d, const double *__restrict__ ws, long long nf,
-                                                    int dirichlet, double *__restrict__ nu_t) {
+                                                    int dirichlet, T *__restrict__ nu_t) {
     const int i = blockIdx.x * 64 + (threadIdx.x & 63);
     const int j = blockIdx.y * 4 + (threadIdx.x >> 6);
     if (i >= d.nxg || j >= d.nyg) return;
@@ -219,7 +225,7 @@ __global__ __launch_bounds__(256) void k_dsm_column(Dims d, const double *__rest
     const int js = j == 0 ? 1 : (j == d.nyg - 1 ? d.nyg - 2 : j);
     const double sgn = (ghost_y && dirichlet == 1) ? -1.0 : 1.0;
     if (i == 0 || i == d.nxg - 1) {
-        for (int k = 0; k < d.nzg; ++k) nu_t[fidx(d, i, j, k)] = sgn * 0.0;
+        for (int k = 0; k < d.nzg; ++k) nu_t[fidx(d, i, j, k)] = (T)(sgn * 0.0);
         return;
     }
     const double *LM = ws + D_LM * nf, *MM = ws + D_MM * nf, *S = ws + D_S * nf;
@@ -231,10 +237,10 @@ __global__ __launch_bounds__(256) void k_dsm_column(Dims d, const double *__rest
     }
     const double nz3 = (double)(d.nzg - 3);
     const double avg = (slm / nz3) / (smm / nz3);
-    nu_t[fidx(d, i, j, 0)] = sgn * 0.0;
-    nu_t[fidx(d, i, j, d.nzg - 1)] = sgn * 0.0;
+    nu_t[fidx(d, i, j, 0)] = (T)(sgn * 0.0);
+    nu_t[fidx(d, i, j, d.nzg - 1)] = (T)(sgn * 0.0);
     for (int k = 1; k <= d.nzg - 2; ++k)
-        nu_t[fidx(d, i, j, k)] = sgn * fmax(-0.5 * avg * S[fidx(d, i, js, k)], 0.0);
+        nu_t[fidx(d, i, j, k)] = (T)(sgn * fmax(-0.5 * avg * S[fidx(d, i, js, k)], 0.0));
 }
 
 void check() {
@@ -245,21 +251,20 @@ void check() {
     }
 }
 
-}  // namespace
-
-void launch_dsm(Handle &h, const Field3 &f) {
+template <class T>
+void dsm_t(Handle &h, const Fld3<T> &f, T *nu_t) {
     const Dims &d = h.d;
     const long long nf = d.sxy * d.nzg;
     const dim3 block(256);
     const dim3 gfull((d.nxg + 63) / 64, (d.nyg + 3) / 4, d.nzg);
     {
         ProfScope ps(h, KID_DSM_CELLS);
-        hipLaunchKernelGGL(k_dsm_cells, gfull, block, 0, h.stream, d, h.m, f.u, f.v, f.w, h.dsm, nf);
+        hipLaunchKernelGGL(k_dsm_cells<T>, gfull, block, 0, h.stream, d, h.m, f.u, f.v, f.w, h.dsm, nf);
         check();
     }
     {
         ProfScope ps(h, KID_DSM_VFILTER);
-        hipLaunchKernelGGL(k_dsm_vfilter, gfull, block, 0, h.stream, d, f.u, f.v, f.w, h.dsm, nf);
+        hipLaunchKernelGGL(k_dsm_vfilter<T>, gfull, block, 0, h.stream, d, f.u, f.v, f.w, h.dsm, nf);
         check();
     }
     {
@@ -271,9 +276,18 @@ void launch_dsm(Handle &h, const Field3 &f) {
     {
         ProfScope ps(h, KID_DSM_COLUMN);
         const dim3 g((d.nxg + 63) / 64, (d.nyg + 3) / 4);
-        hipLaunchKernelGGL(k_dsm_column, g, block, 0, h.stream, d, h.dsm, nf, h.nut_bc, h.nu_t);
+        hipLaunchKernelGGL(k_dsm_column<T>, g, block, 0, h.stream, d, h.dsm, nf, h.nut_bc, nu_t);
         check();
     }
+}
+
+}  // namespace
+
+void launch_dsm(Handle &h, const Field3 &f) {
+    if (h.precision == BL_PREC_MIXED)
+        dsm_t(h, as<float>(f), reinterpret_cast<float *>(h.nu_t));
+    else
+        dsm_t(h, as<double>(f), h.nu_t);
 }
 
 }  // namespace blgpu

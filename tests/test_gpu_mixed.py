@@ -4,9 +4,17 @@ FP32 fields, FP32 stencils and RK updates, split-BF16x3 MFMA transforms with
 FP32 accumulation, FP64 Thomas solves, FP64 (0,0) pressure mode, FP64
 mass-flux correction and FP64 statistics.  Error budget (relative L2 against
 the FP64 oracle): single operators 1e-5 (RHS 2e-5: FP32 differences of
-nearly equal neighbours), one projection 1e-5 on the velocities, and 1e-4 on
-u, v, w after twenty RK3 steps (the stated target for the mixed mode), 1e-3
-on the mean-free pressure.
+nearly equal neighbours), one projection 1e-5 on the velocities and 1e-4 on
+the pressure it records, and 1e-4 on u, v, w after twenty RK3 steps (the
+stated target for the mixed mode).
+
+The pressure after many steps is a different matter.  P = phi / (a11 dt)
+(projection.f90:75-85) is the solution of a Poisson problem whose source is
+the divergence of the RHS; in these laminar fields that divergence is ~1e-3 of
+the RHS's own size, so the FP32 rounding of the RHS (which the FP64 path does
+not make) reaches P amplified by ~1e3-1e4.  After twenty steps P is checked
+against that amplification computed on the CPU: the pressure response of the
+oracle's Poisson solve to an RHS perturbation of FP32 epsilon.
 """
 import sys
 
@@ -44,6 +52,32 @@ def case(request):
 
 def mixed_solver(case, o):
     return solver_for(case, o, precision=blgpu.PREC_MIXED)[0]
+
+
+def divergence(case, o, U, V, W):
+    """the pseudo-pressure rhs (projection.f90:38-47) of a field, in the
+    oracle's P layout"""
+    x, y, z = case.x, case.y, case.z
+    I, J, K = np.arange(1, x.size), np.arange(1, y.size), np.arange(1, z.size)
+    ix = np.ix_
+    d = ((U[ix(I, J, K)] - U[ix(I - 1, J, K)]) / (x[I] - x[I - 1])[:, None, None] +
+         (V[ix(I, J, K)] - V[ix(I, J - 1, K)]) / (y[J] - y[J - 1])[None, :, None] +
+         (W[ix(I, J, K)] - W[ix(I, J, K - 1)]) / (z[K] - z[K - 1])[None, None, :])
+    rhs = np.zeros(o.shapes[3], order="F")
+    rhs[ix(I, J, K)] = d
+    return rhs
+
+
+def fp32_pressure_response(case, o, seed=7):
+    """L2 norm of the (mean-free) interior P that white noise of 2^-24
+    max|RHS| per RHS component produces.  The first sub-step's predictor
+    carries dt a11 dR, the projection turns it into phi = L^-1 div(dt a11 dR),
+    and P = phi / (dt a11) = L^-1 div(dR): dt and a11 cancel."""
+    rng = np.random.default_rng(seed)
+    r = o.rhs()
+    dR = [np.asfortranarray(2.0 ** -24 * np.abs(a).max() * rng.standard_normal(a.shape)) for a in r]
+    phi = o.poisson(divergence(case, o, *dR))[1:-1, 1:-1, 1:-2]
+    return np.linalg.norm(phi - phi.mean())
 
 
 def test_mixed_rhs(case):
@@ -91,6 +125,32 @@ def test_mixed_projection(case):
     assert abs(dg - do) / scale < 1e-6
 
 
+def test_mixed_pressure(case):
+    """P recorded by a first sub-step projection (projection.f90:75-85) on a
+    field both paths hold exactly (FP32-representable): the predictor of a
+    field after nineteen steps, whose divergence is the stage's alone."""
+    o = case.oracle()
+    for n in range(19):
+        o.compute_dt()
+        o.step()
+    o.compute_dt()
+    dt = o.dt
+    r = o.rhs()
+    us = [(a + dt * b).astype(np.float32).astype(np.float64) for a, b in zip(o.fields(), r)]
+    o.set_fields(*us)
+    s = mixed_solver(case, o)
+    o.projection(1, dt)
+    s.projection(1, dt)
+    P_o, P_g = o.get(3)[1:-1, 1:-1, 1:-2], s.download(3)[1:-1, 1:-1, 1:-2]
+    e = rel_l2(P_g - P_g.mean(), P_o - P_o.mean())
+    say("one projection P (mean removed)", e, "raw", rel_l2(P_g, P_o))
+    assert e < 1e-4
+    for name, a, b in zip("UVW", s.fields(), o.fields()):
+        e = np.linalg.norm(a - b) / np.linalg.norm(o.fields()[0])
+        say("one projection", name, e)
+        assert e < 1e-6, name
+
+
 def test_mixed_steps(case):
     o = case.oracle()
     s = mixed_solver(case, o)
@@ -105,14 +165,39 @@ def test_mixed_steps(case):
         say("20 steps", name, e)
         assert e < 1e-4, name
     # P is fixed only up to the constant that the pinned (0,0) mode picks from
-    # the rhs's compatibility residual (round-off of the field), so compare the
-    # pressure with its interior mean removed
+    # the rhs's compatibility residual, so compare it with the interior mean
+    # removed, against the P response to FP32-epsilon noise in the RHS
     P_o, P_g = o.get(3)[1:-1, 1:-1, 1:-2], s.download(3)[1:-1, 1:-1, 1:-2]
-    e = rel_l2(P_g - P_g.mean(), P_o - P_o.mean())
-    say("20 steps P (mean removed)", e, "raw", rel_l2(P_g, P_o))
-    assert e < 1e-3
+    dP = (P_g - P_g.mean()) - (P_o - P_o.mean())
+    bound = fp32_pressure_response(case, o)
+    e = np.linalg.norm(dP)
+    say("20 steps P (mean removed) abs", e, "fp32 response", bound, "rel", rel_l2(P_g - P_g.mean(), P_o - P_o.mean()))
+    assert e < bound
     cf_o, cf_g = o.wall_stats(), s.wall_stats()
     e = rel_l2(cf_g, cf_o)
     say("20 steps Cf", e)
     assert e < 1e-4
     assert np.isclose(s.t, o.t, rtol=1e-14)
+
+
+@pytest.mark.parametrize("les", [2])
+def test_mixed_les_steps(les):
+    """Mixed-precision LES (FP32 fields, FP64 DSM arithmetic) against the
+    FP64 oracle's LES after ten RK3 steps."""
+    case = cases.laminar_small(nx=24, ny=16, nz=10, noise=0.05, seed=11, scheme=3, les=les)
+    o = case.oracle()
+    s = mixed_solver(case, o)
+    for n in range(10):
+        o.compute_dt()
+        s.compute_dt()
+        o.step()
+        s.step()
+    U = o.fields()[0]
+    for name, a, b in zip("UVW", s.fields(), o.fields()):
+        e = np.linalg.norm(a - b) / np.linalg.norm(U)
+        say("mixed LES", name, e)
+        assert e < 1e-4, name
+    nu_o, nu_g = o.get(4), s.download(4)
+    e = np.abs(nu_g - nu_o).max() / np.abs(nu_o).max()
+    say("mixed LES nu_t", e)
+    assert e < 1e-2
