@@ -62,14 +62,16 @@ def kernel_units(name, nx, ny, nz):
     """Algorithmic work of one launch: ('flop', F) or ('byte', B)."""
     N, M, NM = nx - 2, nz - 2, ny - 1
     cells = nx * ny * nz
-    if name in ("xform_x_fwd", "xform_x_inv"):
-        return "flop", 2.0 * N * N * NM * M
-    if name in ("xform_z_fwd", "xform_z_inv"):
-        return "flop", 2.0 * M * M * N * NM
+    # the FP64 transforms are FFT/folded-DFT kernels on the vector ALUs and
+    # LDS, not GEMMs: their algorithmic work is the HBM traffic of the lines
     per_cell_passes = {
         # one pass = one read or write of an N_cell FP64 array
         "rk_stage": 11.0,  # mean over RK3 stages of (read U,V,W[,Uo,F1]; write U*[,F1])
         "divergence": 4.0,
+        "xform_x_fwd": 4.0,  # read U, V, W (divergence fused), write the spectrum lines
+        "xform_x_inv": 2.0,
+        "xform_z_fwd": 2.0,
+        "xform_z_inv": 2.0,
         "thomas": 3.0,
         "project": 7.0,
         "pressure": 2.0,

@@ -283,7 +283,11 @@ __global__ __launch_bounds__(256) void k_samples(Coarsen cz, Metrics mc, const d
 #pragma unroll
     for (int q = 0; q < kMlpIn; ++q) row[q] = cf.f[q];
     row[kRowTarget] = ok ? (float)(nus / cf.scale) : 0.f;
-    row[kRowWeight] = ok ? 1.f : 0.f;
+    // stress-space weight: the deviatoric stress error of a model c at this
+    // cell is 4 (Delta^2 |S|)^2 (S:S) (c - target)^2 + const, so weighting the
+    // squared error in c by (Delta^2 |S|)^2 S:S makes the regression minimise
+    // the SGS stress error itself (training rescales weights to mean 1)
+    row[kRowWeight] = ok ? (float)(cf.scale * cf.scale * ss) : 0.f;
     row[kRowScale] = (float)cf.scale;
 #pragma unroll
     for (int q = 0; q < 6; ++q) {
@@ -331,7 +335,9 @@ __device__ void load_infer_weights(InferLds &L, const unsigned short *__restrict
 }
 
 __device__ __forceinline__ unsigned short to_bf16(float x) { return __builtin_bit_cast(unsigned short, (__bf16)x); }
-__device__ __forceinline__ float silu(float z) { return z / (1.f + expf(-z)); }
+// SiLU from the hardware exp and reciprocal (~1 ulp each; the result is
+// rounded to BF16 next); finite for all finite z: exp(-z) = inf gives z * 0
+__device__ __forceinline__ float silu(float z) { return z * __builtin_amdgcn_rcpf(1.f + __expf(-z)); }
 
 // one hidden layer of a wave's 64 cells: act[cell][0..K) (BF16) times W^T
 // (W stored [64 outputs][K]), bias, SiLU, written back to act[cell][0..64)
@@ -347,7 +353,9 @@ __device__ __forceinline__ void hidden_layer(const unsigned short *Wt, const flo
             a[cb][ks] = __builtin_bit_cast(
                 bf16x8, *reinterpret_cast<const u16x8 *>(act + (cb * 16 + l16) * kH + ks * 32 + 8 * lq));
     __builtin_amdgcn_wave_barrier();
-#pragma unroll
+    // one 16-output block at a time keeps the register count low enough for
+    // two waves per SIMD (the LDS use allows two blocks per CU)
+#pragma unroll 1
     for (int ob = 0; ob < 4; ++ob) {
         f32x4 acc[4];
 #pragma unroll
@@ -430,7 +438,8 @@ __global__ __launch_bounds__(256) void k_closure(Dims d, Metrics m, const T *__r
                                                  const T *__restrict__ W,
                                                  const unsigned short *__restrict__ wbf, const float *__restrict__ bias,
                                                  const float *__restrict__ lo, const float *__restrict__ hi,
-                                                 double inv_d99, T *__restrict__ nu_t, double *__restrict__ cnt) {
+                                                 double inv_d99, int dirichlet, T *__restrict__ nu_t,
+                                                 double *__restrict__ cnt) {
     __shared__ InferLds L;
     __shared__ unsigned short act[4][64 * kH];
     __shared__ float outb[4][64];
@@ -457,24 +466,21 @@ __global__ __launch_bounds__(256) void k_closure(Dims d, Metrics m, const T *__r
     for (int q = 0; q < kMlpIn; ++q) A[lane * kH + q] = to_bf16(cf.f[q]);
     __builtin_amdgcn_wave_barrier();
     const float c = infer_wave(L, A, outb[wave]);
-    if (live && !fb) nu_t[fidx(d, i, j, k)] = (T)(fmax((double)c, 0.0) * cf.scale);
+    if (live && !fb) {
+        // the cell and, next to a y boundary, its ghost (apply_Neumann_bc_y /
+        // apply_Dirichlet_bc_y on nu_t, subgrid.f90:406-411); DSM cells keep
+        // the ghosts the DSM pass wrote
+        const T v = (T)(fmax((double)c, 0.0) * cf.scale);
+        const T sg = dirichlet == 1 ? (T)-1 : (T)1;
+        nu_t[fidx(d, i, j, k)] = v;
+        if (j == 1) nu_t[fidx(d, i, 0, k)] = sg * v;
+        if (j == d.nyg - 2) nu_t[fidx(d, i, d.nyg - 1, k)] = sg * v;
+    }
     // fallback count of the block
     const unsigned long long bal = __ballot(live && fb);
     if (lane == 0) nfb[wave] = __popcll(bal);
     __syncthreads();
     if (threadIdx.x == 0) cnt[blockIdx.x] = (double)(nfb[0] + nfb[1] + nfb[2] + nfb[3]);
-}
-
-// y ghost rows of nu_t after the interior changed (as apply_Neumann_bc_y /
-// apply_Dirichlet_bc_y on nu_t, subgrid.f90:406-411)
-template <class T>
-__global__ __launch_bounds__(256) void k_nut_yghost(Dims d, int dirichlet, T *__restrict__ nu_t) {
-    const int i = blockIdx.x * 256 + threadIdx.x;
-    const int k = blockIdx.y;
-    if (i >= d.nxg) return;
-    const T sg = dirichlet == 1 ? (T)-1 : (T)1;
-    nu_t[fidx(d, i, 0, k)] = sg * nu_t[fidx(d, i, 1, k)];
-    nu_t[fidx(d, i, d.nyg - 1, k)] = sg * nu_t[fidx(d, i, d.nyg - 2, k)];
 }
 
 // per-feature min / max / sum / sum of squares over weighted rows, per block
@@ -599,19 +605,14 @@ void launch_closure(Handle &h, const Field3 &f) {
     }
     const Dims &d = h.d;
     ProfScope ps(h, KID_CLOSURE);
-    const dim3 gy((d.nxg + 255) / 256, d.nzg);
     if (h.precision == BL_PREC_MIXED) {
         const Fld3<float> ff = as<float>(f);
         float *nut = reinterpret_cast<float *>(h.nu_t);
         hipLaunchKernelGGL(k_closure<float>, dim3(h.cl_nblk), dim3(256), 0, h.stream, d, h.m, ff.u, ff.v, ff.w, h.cl_w,
-                           h.cl_b, h.cl_lo, h.cl_hi, h.cl_inv_d99, nut, h.cl_cnt);
-        check();
-        hipLaunchKernelGGL(k_nut_yghost<float>, gy, dim3(256), 0, h.stream, d, h.nut_bc, nut);
+                           h.cl_b, h.cl_lo, h.cl_hi, h.cl_inv_d99, h.nut_bc, nut, h.cl_cnt);
     } else {
         hipLaunchKernelGGL(k_closure<double>, dim3(h.cl_nblk), dim3(256), 0, h.stream, d, h.m, f.u, f.v, f.w, h.cl_w,
-                           h.cl_b, h.cl_lo, h.cl_hi, h.cl_inv_d99, h.nu_t, h.cl_cnt);
-        check();
-        hipLaunchKernelGGL(k_nut_yghost<double>, gy, dim3(256), 0, h.stream, d, h.nut_bc, h.nu_t);
+                           h.cl_b, h.cl_lo, h.cl_hi, h.cl_inv_d99, h.nut_bc, h.nu_t, h.cl_cnt);
     }
     check();
 }

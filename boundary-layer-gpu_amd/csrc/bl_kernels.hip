@@ -714,220 +714,6 @@ __global__ __launch_bounds__(256) void k_dct_inv(DctArgs a, const double *__rest
     }
 }
 
-// ------------------------------------------------------------------------
-// The same transforms with every twiddle of the two short DFTs uniform
-// across a wave: lanes hold (line, n2) or (line, k1) pairs and each wave owns
-// a slice of the output frequencies, so the factor e^{-2 pi i q/N1} (or /N2)
-// of an inner-loop step is one scalar load shared by the wave, and each value
-// read from LDS feeds up to kWU outputs held in registers (the LDS-bandwidth
-// bound of the per-output loops above).  LBW lines per block, four waves;
-// dct_launch checks the shape constraints and otherwise uses the kernels above.
-// ------------------------------------------------------------------------
-constexpr int kWU = 16;
-
-template <int LBW, bool DIV>
-__global__ __launch_bounds__(256) void k_dct_fwd_w(DctArgs a, Dims d, Metrics mt, const double *__restrict__ in,
-                                                   const double *__restrict__ in2, const double *__restrict__ in3,
-                                                   double *__restrict__ out) {
-    extern __shared__ double smem[];
-    const int N = a.N, N1 = a.N1, N2 = a.N2, K1 = a.K1h + 1;
-    double *v = smem;                                       // LBW*N
-    double2 *B = reinterpret_cast<double2 *>(v + LBW * N);  // LBW*K1*N2
-    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
-    const int line0 = blockIdx.x * LBW;
-    for (int idx = tid; idx < LBW * N; idx += 256) {
-        const int l = idx / N, n = idx - l * N;
-        const int c = min(line0 + l, a.C - 1);
-        const int c1 = c % a.C1, c2 = c / a.C1;
-        const long long p = a.in_base + c1 * a.in_c1 + c2 * a.in_c2 + n;
-        double f;
-        if (DIV) {
-            f = (in[p] - in[p - 1]) * mt.rdx[n + 1] + (in2[p] - in2[p - d.sx]) * mt.rdy[c1 + 1] +
-                (in3[p] - in3[p - d.sxy]) * mt.rdz[c2 + 1];
-        } else {
-            f = in[p];
-        }
-        v[l * N + ((n & 1) ? N - 1 - (n >> 1) : (n >> 1))] = f;
-    }
-    __syncthreads();
-    // step A, B[k1][n2] = W_N^{n2 k1} sum_n1 v[N2 n1 + n2] W_N1^{n1 k1}:
-    // waves 0, 1 take k1 < h, waves 2, 3 the rest; lanes (l, n2)
-    {
-        const int h = (K1 + 1) / 2;
-        const int kb = wave < 2 ? 0 : h, nk = wave < 2 ? h : K1 - h;
-        const int t = (wave & 1) * 64 + lane;
-        if (t < LBW * N2) {
-            const int l = t / N2, n2 = t - l * N2;
-            double ar[kWU], ai[kWU];
-#pragma unroll
-            for (int q = 0; q < kWU; ++q) ar[q] = ai[q] = 0.0;
-            const double *vl = v + l * N + n2;
-            for (int n1 = 0; n1 < N1; ++n1) {
-                const double x = vl[n1 * N2];
-#pragma unroll
-                for (int q = 0; q < kWU; ++q) {
-                    if (q < nk) {
-                        const int w = __builtin_amdgcn_readfirstlane((n1 * (kb + q)) % N1);
-                        const double2 tw = a.w1[w];
-                        ar[q] = fma(x, tw.x, ar[q]);
-                        ai[q] = fma(x, tw.y, ai[q]);
-                    }
-                }
-            }
-#pragma unroll
-            for (int q = 0; q < kWU; ++q) {
-                if (q < nk) {
-                    const int k1 = kb + q;
-                    B[(l * K1 + k1) * N2 + n2] = cmul(make_double2(ar[q], ai[q]), a.wN[(n2 * k1) % N]);
-                }
-            }
-        }
-    }
-    __syncthreads();
-    // step C, V[k1 + N1 k2] = sum_n2 B[k1][n2] W_N2^{n2 k2}, X[k] = 2 Re(rot_k V),
-    // X[N-k] = -2 Im(rot_k V): wave w takes a quarter of k2; lanes (l, k1)
-    {
-        const int qs = (N2 + 3) / 4;
-        const int kb = wave * qs, nk = max(0, min(N2, kb + qs) - kb);
-        if (lane < LBW * K1 && nk > 0) {
-            const int l = lane / K1, k1 = lane - l * K1;
-            double ar[kWU], ai[kWU];
-#pragma unroll
-            for (int q = 0; q < kWU; ++q) ar[q] = ai[q] = 0.0;
-            const double2 *Bl = B + (l * K1 + k1) * N2;
-            for (int n2 = 0; n2 < N2; ++n2) {
-                const double2 b = Bl[n2];
-#pragma unroll
-                for (int q = 0; q < kWU; ++q) {
-                    if (q < nk) {
-                        const int w = __builtin_amdgcn_readfirstlane((n2 * (kb + q)) % N2);
-                        const double2 tw = a.w2[w];
-                        ar[q] = fma(b.x, tw.x, fma(-b.y, tw.y, ar[q]));
-                        ai[q] = fma(b.x, tw.y, fma(b.y, tw.x, ai[q]));
-                    }
-                }
-            }
-#pragma unroll
-            for (int q = 0; q < kWU; ++q) {
-                if (q < nk) {
-                    const int k = k1 + N1 * (kb + q);
-                    const double2 wv = cmul(a.rot[k], make_double2(ar[q], ai[q]));
-                    v[l * N + k] = 2.0 * wv.x;
-                    if (k > 0) v[l * N + N - k] = -2.0 * wv.y;
-                }
-            }
-        }
-    }
-    __syncthreads();
-    for (int idx = tid; idx < LBW * N; idx += 256) {
-        const int l = idx / N, kx = idx - l * N;
-        const int c = line0 + l;
-        if (c >= a.C) continue;
-        const int c1 = c % a.C1, c2 = c / a.C1;
-        out[a.out_base + c1 * a.out_c1 + c2 * a.out_c2 + kx] = v[l * N + kx];
-    }
-}
-
-template <int LBW>
-__global__ __launch_bounds__(256) void k_dct_inv_w(DctArgs a, const double *__restrict__ in, double *__restrict__ out) {
-    extern __shared__ double smem[];
-    const int N = a.N, N1 = a.N1, N2 = a.N2, K1h = a.K1h, K1 = K1h + 1;
-    // only V[k1 + N1 k2] with k1 <= N1/2 is used: Vs[l][k1][k2]; the area is
-    // reused for the output lines (LBW*N doubles <= 2*LBW*K1*N2)
-    double2 *Vs = reinterpret_cast<double2 *>(smem);  // LBW*K1*N2
-    double2 *C = Vs + LBW * K1 * N2;                  // LBW*K1*N2
-    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
-    const int line0 = blockIdx.x * LBW;
-    for (int idx = tid; idx < LBW * K1 * N2; idx += 256) {
-        const int l = idx / (K1 * N2), r = idx - l * K1 * N2;
-        const int k1 = r / N2, k2 = r - k1 * N2;
-        const int k = k1 + N1 * k2;
-        const int c = min(line0 + l, a.C - 1);
-        const int c1 = c % a.C1, c2 = c / a.C1;
-        const double *gl = in + a.in_base + c1 * a.in_c1 + c2 * a.in_c2;
-        const double gk = gl[k], gm = (k == 0) ? 0.0 : gl[N - k];
-        Vs[idx] = cmulc(make_double2(0.5 * gk, -0.5 * gm), a.rot[k]);
-    }
-    __syncthreads();
-    // step C', C~[k1][n2] = W_N^{-n2 k1} sum_k2 V[k1 + N1 k2] W_N2^{-n2 k2}:
-    // wave w takes a quarter of n2; lanes (l, k1)
-    {
-        const int qs = (N2 + 3) / 4;
-        const int nb = wave * qs, nn = max(0, min(N2, nb + qs) - nb);
-        if (lane < LBW * K1 && nn > 0) {
-            const int l = lane / K1, k1 = lane - l * K1;
-            double ar[kWU], ai[kWU];
-#pragma unroll
-            for (int q = 0; q < kWU; ++q) ar[q] = ai[q] = 0.0;
-            const double2 *Vl = Vs + (l * K1 + k1) * N2;
-            for (int k2 = 0; k2 < N2; ++k2) {
-                const double2 V = Vl[k2];
-#pragma unroll
-                for (int q = 0; q < kWU; ++q) {
-                    if (q < nn) {
-                        const int w = __builtin_amdgcn_readfirstlane(((nb + q) * k2) % N2);
-                        const double2 tw = a.w2[w];  // use conj(tw)
-                        ar[q] = fma(V.x, tw.x, fma(V.y, tw.y, ar[q]));
-                        ai[q] = fma(V.y, tw.x, fma(-V.x, tw.y, ai[q]));
-                    }
-                }
-            }
-#pragma unroll
-            for (int q = 0; q < kWU; ++q) {
-                if (q < nn) {
-                    const int n2 = nb + q;
-                    C[(l * K1 + k1) * N2 + n2] = cmulc(make_double2(ar[q], ai[q]), a.wN[(n2 * k1) % N]);
-                }
-            }
-        }
-    }
-    __syncthreads();
-    // step A', v[N2 n1 + n2] = (1/N)(Re C~[0] + sum_{k1>=1} c_k1 Re(C~[k1] W_N1^{-n1 k1})):
-    // waves 0, 1 take n1 < h, waves 2, 3 the rest; lanes (l, n2)
-    double *o = smem;  // LBW*N doubles inside the V area, which is no longer read
-    {
-        const int h = (N1 + 1) / 2;
-        const int nb = wave < 2 ? 0 : h, nn = wave < 2 ? h : N1 - h;
-        const int t = (wave & 1) * 64 + lane;
-        if (t < LBW * N2) {
-            const int l = t / N2, n2 = t - l * N2;
-            double acc[kWU];
-            const double2 *Cl = C + l * K1 * N2 + n2;
-            const double c0 = Cl[0].x;
-#pragma unroll
-            for (int q = 0; q < kWU; ++q) acc[q] = c0;
-            for (int k1 = 1; k1 <= K1h; ++k1) {
-                const double2 c = Cl[k1 * N2];
-                const double f = ((N1 % 2 == 0) && k1 == K1h) ? 1.0 : 2.0;
-#pragma unroll
-                for (int q = 0; q < kWU; ++q) {
-                    if (q < nn) {
-                        const int w = __builtin_amdgcn_readfirstlane(((nb + q) * k1) % N1);
-                        const double2 tw = a.w1[w];
-                        acc[q] = fma(f, c.x * tw.x + c.y * tw.y, acc[q]);  // Re(c * conj(w))
-                    }
-                }
-            }
-#pragma unroll
-            for (int q = 0; q < kWU; ++q) {
-                if (q < nn) {
-                    const int n = N2 * (nb + q) + n2;
-                    const int pos = (n < (N + 1) / 2) ? 2 * n : 2 * (N - 1 - n) + 1;
-                    o[l * N + pos] = acc[q] * a.scale;
-                }
-            }
-        }
-    }
-    __syncthreads();
-    for (int idx = tid; idx < LBW * N; idx += 256) {
-        const int l = idx / N, n = idx - l * N;
-        const int c = line0 + l;
-        if (c >= a.C) continue;
-        const int c1 = c % a.C1, c2 = c / a.C1;
-        out[a.out_base + c1 * a.out_c1 + c2 * a.out_c2 + n] = o[idx];
-    }
-}
-
 // Thomas sweep with precomputed factors (projection.f90:137-150).  One real
 // system (kx, q) per lane: q = 0 is the kz = 0 row, q = 2k-1 / 2k the real /
 // imaginary rows of wavenumber k, which share the factors of k.  The sweep is
@@ -1170,37 +956,10 @@ void zdft_launch(Handle &h, int kid, bool forward, const double *in, double *out
     check_launch();
 }
 
-constexpr int kDctLinesW = 4;
-
-// the wave-uniform kernels fit when a wave slice holds at most kWU outputs
-bool dct_wave_fits(const DctArgs &a) {
-    const int K1 = a.K1h + 1;
-    return kDctLinesW * a.N2 <= 128 && kDctLinesW * K1 <= 64 && (K1 + 1) / 2 <= kWU && (a.N2 + 3) / 4 <= kWU &&
-           (a.N1 + 1) / 2 <= kWU;
-}
-
 void dct_launch(Handle &h, int kid, const DctArgs &a, bool forward, const double *in, double *out,
                 const Field3 *div = nullptr) {
     ProfScope ps(h, kid);
     const int K1 = a.K1h + 1;
-    if (dct_wave_fits(a)) {
-        const size_t ldsw = forward ? sizeof(double) * (size_t)kDctLinesW * a.N +
-                                          sizeof(double2) * (size_t)kDctLinesW * K1 * a.N2
-                                    : 2 * sizeof(double2) * (size_t)kDctLinesW * K1 * a.N2;
-        if (ldsw <= 65536) {
-            const dim3 grid((a.C + kDctLinesW - 1) / kDctLinesW);
-            if (forward && div)
-                hipLaunchKernelGGL((k_dct_fwd_w<kDctLinesW, true>), grid, dim3(256), ldsw, h.stream, a, h.d, h.m,
-                                   div->u, div->v, div->w, out);
-            else if (forward)
-                hipLaunchKernelGGL((k_dct_fwd_w<kDctLinesW, false>), grid, dim3(256), ldsw, h.stream, a, h.d, h.m,
-                                   in, nullptr, nullptr, out);
-            else
-                hipLaunchKernelGGL(k_dct_inv_w<kDctLinesW>, grid, dim3(256), ldsw, h.stream, a, in, out);
-            check_launch();
-            return;
-        }
-    }
     const size_t lds = (forward ? sizeof(double) : sizeof(double2)) * (size_t)kDctLines * a.N +
                        sizeof(double2) * ((size_t)kDctLines * K1 * a.N2 + a.N1 + a.N2);
     if (lds > 65536) {

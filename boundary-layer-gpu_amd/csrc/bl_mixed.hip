@@ -45,11 +45,17 @@ __device__ __forceinline__ float bf16_float(unsigned short u) {
 
 // ------------------------------------------------------------------------
 // C[r][c] = sum_k A[r][k] B[k][c] (+ add[c % o_C1]) with A pre-split into
-// BF16 hi/lo rows of length Kp (zero padded past K) and B split on the fly.
-// Workgroup tile BM x BN, K step 32, four waves of 64x64 each; MFMA
-// v_mfma_f32_16x16x32_bf16: lane l holds A[l%16][8(l/16)+e],
+// BF16 hi/lo rows of length Kp (zero padded past K, rows padded to a whole
+// number of BM tiles) and B split on the fly (b = b_hi + b_lo, FP32 in HBM).
+// Workgroup tile BM x BN = (64 WR) x (64 WC), one 64 x 64 tile per wave,
+// K step 32.  MFMA v_mfma_f32_16x16x32_bf16: lane l holds A[l%16][8(l/16)+e],
 // B[8(l/16)+e][l%16] (e < 8) and C[4(l/16)+r][l%16] (r < 4).
-// B layout: KCONTIG -> B[c*ldb + k], else B[k*ldb + c].
+// Software pipeline: the B tile of step s+1 is loaded into registers and the
+// A fragments of step s+1 are fetched while step s's MFMAs run out of one
+// LDS buffer; the converted B tile then goes to the other buffer (one
+// barrier per step).
+// B layout: KCONTIG -> B[c*ldb + k] (needs Kp <= ldb: the row tail past K is
+// read and masked), else B[k*ldb + c] (needs C % 4 == 0, ldb % 4 == 0).
 // Output layout: RCONTIG -> out[o_base + (c%o_C1)*o_c1 + (c/o_C1)*o_c2 + r],
 // else out[r*ldc + c].
 // ------------------------------------------------------------------------
@@ -66,10 +72,129 @@ struct GemmArgs {
 
 constexpr int kBK = 32, kLDK = kBK + 8;  // LDS row of one B column, padded
 
-template <int BM, int BN, bool KCONTIG, bool RCONTIG>
-__global__ __launch_bounds__(256) void k_gemm_bf16x3(GemmArgs g, int tiles_r, int nblk) {
-    static_assert((BM / 64) * (BN / 64) == 4, "four 64x64 wave tiles");
-    __shared__ unsigned short sBh[BN * kLDK], sBl[BN * kLDK];
+// float4 loads of the B tile per thread
+template <int WR, int WC>
+constexpr int gemm_bloads() {
+    return (kBK * 64 * WC / 4) / (64 * WR * WC);
+}
+
+// B tile loads: unconditional (addresses clamped into the operand) so that
+// the loads of the next step stay in flight across the MFMAs; the masks
+// (columns past C, k past K) are applied when the tile is split and stored.
+template <int WR, int WC, bool KCONTIG>
+__device__ __forceinline__ void gemm_load_b(const GemmArgs &g, int c0, int kk, int tid,
+                                            float4 (&v)[gemm_bloads<WR, WC>()]) {
+    constexpr int NT = 64 * WR * WC, BN = 64 * WC, NL = gemm_bloads<WR, WC>();
+    if constexpr (KCONTIG) {
+        // 8 threads cover the 32 k of one column; NT / 8 columns per pass
+#pragma unroll
+        for (int p = 0; p < NL; ++p) {
+            const int c = min(c0 + p * (NT / 8) + tid / 8, g.C - 1), k = kk + 4 * (tid % 8);
+            v[p] = *reinterpret_cast<const float4 *>(g.B + (long long)c * g.ldb + k);
+        }
+    } else {
+        // 4 consecutive columns per thread, BN / 4 threads per k row
+        static_assert(NL % 2 == 0, "k pairs");
+        const int kq = tid / (BN / 4), cg = tid % (BN / 4);
+        const int c = min(c0 + 4 * cg, g.C - 4);
+#pragma unroll
+        for (int p = 0; p < NL; ++p) {
+            const int k = min(kk + kq * NL + p, g.K - 1);
+            v[p] = *reinterpret_cast<const float4 *>(g.B + (long long)k * g.ldb + c);
+        }
+    }
+}
+
+__device__ __forceinline__ void split2(float x, unsigned short &h, unsigned short &l) {
+    h = bf16_bits(x);
+    l = bf16_bits(x - bf16_float(h));
+}
+
+template <int WR, int WC, bool KCONTIG>
+__device__ __forceinline__ void gemm_store_b(const GemmArgs &g, int c0, int kk, int tid,
+                                             const float4 (&v)[gemm_bloads<WR, WC>()], unsigned short *sBh,
+                                             unsigned short *sBl) {
+    constexpr int NT = 64 * WR * WC, BN = 64 * WC, NL = gemm_bloads<WR, WC>();
+    if constexpr (KCONTIG) {
+        const int k = 4 * (tid % 8);
+        const int kg = kk + k;
+#pragma unroll
+        for (int p = 0; p < NL; ++p) {
+            const int col = p * (NT / 8) + tid / 8;
+            const bool cin = c0 + col < g.C;
+            const float e[4] = {cin && kg < g.K ? v[p].x : 0.f, cin && kg + 1 < g.K ? v[p].y : 0.f,
+                                cin && kg + 2 < g.K ? v[p].z : 0.f, cin && kg + 3 < g.K ? v[p].w : 0.f};
+            unsigned short h[4], l[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) split2(e[q], h[q], l[q]);
+            const int o = col * kLDK + k;
+            *reinterpret_cast<uint2 *>(&sBh[o]) =
+                make_uint2((unsigned)h[0] | ((unsigned)h[1] << 16), (unsigned)h[2] | ((unsigned)h[3] << 16));
+            *reinterpret_cast<uint2 *>(&sBl[o]) =
+                make_uint2((unsigned)l[0] | ((unsigned)l[1] << 16), (unsigned)l[2] | ((unsigned)l[3] << 16));
+        }
+    } else {
+        const int kq = tid / (BN / 4), cg = tid % (BN / 4);
+        const bool cin = c0 + 4 * cg < g.C;
+#pragma unroll
+        for (int p = 0; p < NL; p += 2) {
+            const int k = kq * NL + p;
+            const bool k0 = cin && kk + k < g.K, k1 = cin && kk + k + 1 < g.K;
+            const float a0[4] = {v[p].x, v[p].y, v[p].z, v[p].w};
+            const float a1[4] = {v[p + 1].x, v[p + 1].y, v[p + 1].z, v[p + 1].w};
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                unsigned short h0, l0, h1, l1;
+                split2(k0 ? a0[e] : 0.f, h0, l0);
+                split2(k1 ? a1[e] : 0.f, h1, l1);
+                const int o = (4 * cg + e) * kLDK + k;
+                *reinterpret_cast<unsigned int *>(&sBh[o]) = (unsigned)h0 | ((unsigned)h1 << 16);
+                *reinterpret_cast<unsigned int *>(&sBl[o]) = (unsigned)l0 | ((unsigned)l1 << 16);
+            }
+        }
+    }
+}
+
+// A tile loads: BM rows x 32 k of hi and of lo as 16-byte chunks of 8 k
+// (rows are padded to whole tiles and k to Kp, so no masks)
+template <int WR, int WC>
+constexpr int gemm_aloads() {
+    return 8 / WC;  // (BM * 4 chunks * 2) / NT
+}
+
+template <int WR, int WC>
+__device__ __forceinline__ void gemm_load_a(const GemmArgs &g, int r0, int kk, int tid,
+                                            u16x8 (&va)[gemm_aloads<WR, WC>()]) {
+    constexpr int NT = 64 * WR * WC, NA = gemm_aloads<WR, WC>();
+#pragma unroll
+    for (int p = 0; p < NA; ++p) {
+        const int q = p * NT + tid;  // chunk id over [hi, lo][row][4 chunks]
+        const int half = q / (64 * WR * 4), rc = q % (64 * WR * 4);
+        const int row = rc / 4, ch = rc % 4;
+        const unsigned short *src = (half ? g.Alo : g.Ahi) + (long long)(r0 + row) * g.Kp + kk + 8 * ch;
+        va[p] = *reinterpret_cast<const u16x8 *>(src);
+    }
+}
+
+template <int WR, int WC>
+__device__ __forceinline__ void gemm_store_a(int tid, const u16x8 (&va)[gemm_aloads<WR, WC>()],
+                                             unsigned short *sAh, unsigned short *sAl) {
+    constexpr int NT = 64 * WR * WC, NA = gemm_aloads<WR, WC>();
+#pragma unroll
+    for (int p = 0; p < NA; ++p) {
+        const int q = p * NT + tid;
+        const int half = q / (64 * WR * 4), rc = q % (64 * WR * 4);
+        const int row = rc / 4, ch = rc % 4;
+        *reinterpret_cast<u16x8 *>((half ? sAl : sAh) + row * kLDK + 8 * ch) = va[p];
+    }
+}
+
+template <int WR, int WC, bool KCONTIG, bool RCONTIG>
+__global__ __launch_bounds__(64 * WR * WC) void k_gemm_bf16x3(GemmArgs g, int tiles_r, int nblk) {
+    constexpr int BM = 64 * WR, BN = 64 * WC, NL = gemm_bloads<WR, WC>(), NA = gemm_aloads<WR, WC>();
+    // one buffer of each operand: [row or column][k], rows padded to kLDK
+    __shared__ unsigned short sA[2][BM * kLDK], sB[2][BN * kLDK];  // [hi, lo]
+    static_assert(sizeof(unsigned short) * 2 * (BM + BN) * kLDK <= 65536, "LDS per workgroup");
     // XCD-aware order: blocks that share a column tile run on one XCD (the
     // hardware deals consecutive block ids round-robin over 8 XCDs)
     const int b = blockIdx.x;
@@ -79,7 +204,7 @@ __global__ __launch_bounds__(256) void k_gemm_bf16x3(GemmArgs g, int tiles_r, in
     const int tr = lb % tiles_r, tc = lb / tiles_r;
     const int r0 = tr * BM, c0 = tc * BN;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int wr = (wave % (BM / 64)) * 64, wc = (wave / (BM / 64)) * 64;
+    const int wr = (wave % WR) * 64, wc = (wave / WR) * 64;
     const int l16 = lane & 15, lq = lane >> 4;
 
     f32x4 acc[4][4];
@@ -88,70 +213,52 @@ __global__ __launch_bounds__(256) void k_gemm_bf16x3(GemmArgs g, int tiles_r, in
 #pragma unroll
         for (int c = 0; c < 4; ++c) acc[a][c] = f32x4{0.f, 0.f, 0.f, 0.f};
 
+    float4 vb[NL];
+    u16x8 va[NA];
+    gemm_load_b<WR, WC, KCONTIG>(g, c0, 0, tid, vb);
+    gemm_load_a<WR, WC>(g, r0, 0, tid, va);
+    gemm_store_b<WR, WC, KCONTIG>(g, c0, 0, tid, vb, sB[0], sB[1]);
+    gemm_store_a<WR, WC>(tid, va, sA[0], sA[1]);
+    __syncthreads();
     for (int kk = 0; kk < g.Kp; kk += kBK) {
-        // ---- stage the B tile (32 x BN) as BF16 hi/lo, [column][k] in LDS
-        if constexpr (KCONTIG) {
-            constexpr int TPC = 256 / BN;         // threads per column
-            constexpr int KPT = kBK / TPC;        // k values per thread
-            const int col = tid / TPC, k0 = (tid % TPC) * KPT;
-            const int c = c0 + col;
-            float v[KPT];
-#pragma unroll
-            for (int e = 0; e < KPT; ++e) {
-                const int k = kk + k0 + e;
-                v[e] = (c < g.C && k < g.K) ? g.B[(long long)c * g.ldb + k] : 0.f;
-            }
-#pragma unroll
-            for (int e = 0; e < KPT; e += 8) {
-                u16x8 hi, lo;
-#pragma unroll
-                for (int q = 0; q < 8; ++q) {
-                    const unsigned short h = bf16_bits(v[e + q]);
-                    hi[q] = h;
-                    lo[q] = bf16_bits(v[e + q] - bf16_float(h));
-                }
-                *reinterpret_cast<u16x8 *>(&sBh[col * kLDK + k0 + e]) = hi;
-                *reinterpret_cast<u16x8 *>(&sBl[col * kLDK + k0 + e]) = lo;
-            }
-        } else {
-            // thread: two consecutive k rows, CPT consecutive columns
-            constexpr int CPT = BN * kBK / 2 / 256;
-            const int kp = tid / (BN / CPT), cc0 = (tid % (BN / CPT)) * CPT;
-            const int ka = kk + 2 * kp;
-#pragma unroll
-            for (int e = 0; e < CPT; ++e) {
-                const int c = c0 + cc0 + e;
-                const float x0 = (c < g.C && ka < g.K) ? g.B[(long long)ka * g.ldb + c] : 0.f;
-                const float x1 = (c < g.C && ka + 1 < g.K) ? g.B[(long long)(ka + 1) * g.ldb + c] : 0.f;
-                const unsigned short h0 = bf16_bits(x0), h1 = bf16_bits(x1);
-                const unsigned short l0 = bf16_bits(x0 - bf16_float(h0)), l1 = bf16_bits(x1 - bf16_float(h1));
-                const int o = (cc0 + e) * kLDK + 2 * kp;
-                *reinterpret_cast<unsigned int *>(&sBh[o]) = (unsigned int)h0 | ((unsigned int)h1 << 16);
-                *reinterpret_cast<unsigned int *>(&sBl[o]) = (unsigned int)l0 | ((unsigned int)l1 << 16);
-            }
+        const bool more = kk + kBK < g.Kp;
+        // the next step's tiles stay in flight during this step's MFMAs
+        if (more) {
+            gemm_load_b<WR, WC, KCONTIG>(g, c0, kk + kBK, tid, vb);
+            gemm_load_a<WR, WC>(g, r0, kk + kBK, tid, va);
         }
-        __syncthreads();
-        // ---- 4 x 4 blocks of 16 x 16, three MFMAs each
-        bf16x8 bh[4], bl[4];
+        bf16x8 ah[4], al[4], bh[4], bl[4];
 #pragma unroll
-        for (int cb = 0; cb < 4; ++cb) {
-            const int o = (wc + cb * 16 + l16) * kLDK + 8 * lq;
-            bh[cb] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const u16x8 *>(&sBh[o]));
-            bl[cb] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const u16x8 *>(&sBl[o]));
+        for (int q = 0; q < 4; ++q) {
+            const int oa = (wr + q * 16 + l16) * kLDK + 8 * lq, ob = (wc + q * 16 + l16) * kLDK + 8 * lq;
+            ah[q] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const u16x8 *>(&sA[0][oa]));
+            al[q] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const u16x8 *>(&sA[1][oa]));
+            bh[q] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const u16x8 *>(&sB[0][ob]));
+            bl[q] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const u16x8 *>(&sB[1][ob]));
         }
+        // ---- 4 x 4 blocks of 16 x 16, three products each (small terms
+        // first), 16 independent accumulators between dependent MFMAs
 #pragma unroll
-        for (int rb = 0; rb < 4; ++rb) {
-            const long long ao = (long long)(r0 + wr + rb * 16 + l16) * g.Kp + kk + 8 * lq;
-            const bf16x8 ah = __builtin_bit_cast(bf16x8, *reinterpret_cast<const u16x8 *>(g.Ahi + ao));
-            const bf16x8 al = __builtin_bit_cast(bf16x8, *reinterpret_cast<const u16x8 *>(g.Alo + ao));
+        for (int rb = 0; rb < 4; ++rb)
 #pragma unroll
-            for (int cb = 0; cb < 4; ++cb) {
-                acc[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh[cb], acc[rb][cb], 0, 0, 0);
-                acc[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl[cb], acc[rb][cb], 0, 0, 0);
-                acc[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh[cb], acc[rb][cb], 0, 0, 0);
-            }
+            for (int cb = 0; cb < 4; ++cb)
+                acc[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[rb], bh[cb], acc[rb][cb], 0, 0, 0);
+#pragma unroll
+        for (int rb = 0; rb < 4; ++rb)
+#pragma unroll
+            for (int cb = 0; cb < 4; ++cb)
+                acc[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[rb], bl[cb], acc[rb][cb], 0, 0, 0);
+#pragma unroll
+        for (int rb = 0; rb < 4; ++rb)
+#pragma unroll
+            for (int cb = 0; cb < 4; ++cb)
+                acc[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[rb], bh[cb], acc[rb][cb], 0, 0, 0);
+        if (more) {
+            __syncthreads();  // every wave has read this step's tiles
+            gemm_store_b<WR, WC, KCONTIG>(g, c0, kk + kBK, tid, vb, sB[0], sB[1]);
+            gemm_store_a<WR, WC>(tid, va, sA[0], sA[1]);
+            __syncthreads();
         }
-        __syncthreads();
     }
     // ---- epilogue
 #pragma unroll
@@ -176,17 +283,25 @@ __global__ __launch_bounds__(256) void k_gemm_bf16x3(GemmArgs g, int tiles_r, in
     }
 }
 
-template <int BM, int BN, bool KCONTIG, bool RCONTIG>
+void check();
+
+template <int WR, int WC, bool KCONTIG, bool RCONTIG>
 void gemm(Handle &h, int kid, const GemmArgs &g) {
     ProfScope ps(h, kid);
-    const int tr = (g.R + BM - 1) / BM, tc = (g.C + BN - 1) / BN;
-    const int nblk = tr * tc;
-    hipLaunchKernelGGL((k_gemm_bf16x3<BM, BN, KCONTIG, RCONTIG>), dim3(nblk), dim3(256), 0, h.stream, g, tr, nblk);
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) {
-        last_error() = std::string("kernel launch: ") + hipGetErrorString(e);
+    constexpr int BM = 64 * WR, BN = 64 * WC;
+    // the loaders' assumptions (see k_gemm_bf16x3)
+    const bool ok = g.Kp % kBK == 0 && g.Kp >= g.K &&
+                    g.C >= 4 && g.K >= 1 &&
+                    (KCONTIG ? (g.Kp <= g.ldb && g.ldb % 4 == 0) : (g.C % 4 == 0 && g.ldb % 4 == 0 && g.ldb >= g.C));
+    if (!ok) {
+        last_error() = "internal: GEMM operand layout does not fit the kernel";
         throw DeviceError();
     }
+    const int tr = (g.R + BM - 1) / BM, tc = (g.C + BN - 1) / BN;
+    const int nblk = tr * tc;
+    hipLaunchKernelGGL((k_gemm_bf16x3<WR, WC, KCONTIG, RCONTIG>), dim3(nblk), dim3(64 * WR * WC), 0, h.stream, g, tr,
+                       nblk);
+    check();
 }
 
 // ------------------------------------------------------------------------
@@ -214,30 +329,47 @@ __global__ __launch_bounds__(256) void k_div_mixed(Dims d, Metrics m, const floa
         part[(long long)jj * gridDim.x * gridDim.z + blockIdx.x + (long long)gridDim.x * mm] = v;
 }
 
-// FP64 (0,0) mode: rhs(j) = 2 sum over the plane (DCT-II x2 at kx = 0, DFT
-// at kz = 0), Thomas with the kx = kz = 0 factors (pinned row), and the
-// constant phi00(j) / (2 N M) the inverse transforms would add.
-__global__ __launch_bounds__(256) void k_solve00(Dims d, const double *__restrict__ part, int np,
-                                                 const double *__restrict__ dl, const double *__restrict__ dp,
+// FP64 (0,0) mode, in two kernels.  k_sum00: rhs(j) = 2 sum over the x-z
+// plane (DCT-II x2 at kx = 0, DFT at kz = 0) of the divergence partials, one
+// block per y row, fixed summation order.  k_solve00: Thomas with the
+// kx = kz = 0 factors (pinned row), run by one lane from LDS copies of the
+// factor columns, and the constant phi00(j) / (2 N M) that the inverse
+// transforms would add.
+__global__ __launch_bounds__(256) void k_sum00(const double *__restrict__ part, int np,
+                                               double *__restrict__ r00) {
+    __shared__ double w[4];
+    const int j = blockIdx.x;
+    const double *p = part + (long long)j * np;
+    double s = 0.0;
+    for (int q = threadIdx.x; q < np; q += 256) s += p[q];
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_down(s, o);
+    if ((threadIdx.x & 63) == 0) w[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) r00[j] = 2.0 * ((w[0] + w[1]) + (w[2] + w[3]));
+}
+
+__global__ __launch_bounds__(256) void k_solve00(Dims d, const double *__restrict__ dl,
+                                                 const double *__restrict__ dp,
                                                  const double *__restrict__ du, double *__restrict__ c00) {
-    __shared__ double r[1024];
+    __shared__ double r[1024], L[1024], D[1024], Uu[1024];
     const int NM = d.NM, N = d.N;
     for (int j = threadIdx.x; j < NM; j += 256) {
-        double s = 0.0;
-        for (int q = 0; q < np; ++q) s += part[(long long)j * np + q];
-        r[j] = 2.0 * s;
+        r[j] = c00[j];
+        L[j] = j > 0 ? dl[(long long)(j - 1) * N] : 0.0;
+        D[j] = dp[(long long)j * N];
+        Uu[j] = du[j];
     }
     __syncthreads();
     if (threadIdx.x == 0) {
         double x = r[0];
         for (int j = 1; j < NM; ++j) {
-            x = r[j] - dl[(long long)(j - 1) * N] * x;
+            x = r[j] - L[j] * x;
             r[j] = x;
         }
-        x = x * dp[(long long)(NM - 1) * N];
+        x = x * D[NM - 1];
         r[NM - 1] = x;
         for (int j = NM - 2; j >= 0; --j) {
-            x = (r[j] - du[j] * x) * dp[(long long)j * N];
+            x = (r[j] - Uu[j] * x) * D[j];
             r[j] = x;
         }
     }
@@ -348,7 +480,10 @@ void split_matrix(int rows, int K, int Rp, int Kp, F f, std::vector<uint16_t> &h
         }
 }
 
-constexpr int kBMx = 128, kBNx = 128, kBMz = 64, kBNz = 256;
+// x: 128 x 128 tiles (812 rows -> 7 row tiles); z: 256 x 128 tiles, all
+// 191 rows in one row tile so each B column block is read once
+constexpr int kWRx = 2, kWCx = 2, kWRz = 4, kWCz = 2;
+constexpr int kBMx = 64 * kWRx, kBMz = 64 * kWRz;
 
 }  // namespace
 
@@ -438,7 +573,7 @@ void launch_poisson_mixed(Handle &h, float *phi, const Fld3<float> &f) {
     g.o_c2 = 0;
     g.o_C1 = g.C;
     g.add = nullptr;
-    gemm<kBMx, kBNx, true, true>(h, KID_XFORM_X_FWD, g);
+    gemm<kWRx, kWCx, true, true>(h, KID_XFORM_X_FWD, g);
     // forward z: S[q][col] = sum_m Zf[q][m] T1[m][col], col = j*NP + kx
     GemmArgs z{};
     z.R = M;
@@ -452,15 +587,16 @@ void launch_poisson_mixed(Handle &h, float *phi, const Fld3<float> &f) {
     z.out = S;
     z.ldc = (long long)NM * NP;
     z.o_C1 = 1;
-    gemm<kBMz, kBNz, false, false>(h, KID_XFORM_Z_FWD, z);
+    gemm<kWRz, kWCz, false, false>(h, KID_XFORM_Z_FWD, z);
     // FP64 Thomas on the spectrum, FP64 (0,0) mode
     {
         ProfScope ps(h, KID_THOMAS);
         hipLaunchKernelGGL(k_thomas_mixed<8>, dim3((N + 255) / 256, M), dim3(256), 0, h.stream, d, NP, h.lu_dl,
                            h.lu_dp, h.lu_du, S, h.T1);
         check();
-        hipLaunchKernelGGL(k_solve00, dim3(1), dim3(256), 0, h.stream, d, h.p00, h.np00, h.lu_dl, h.lu_dp, h.lu_du,
-                           h.c00);
+        hipLaunchKernelGGL(k_sum00, dim3(NM), dim3(256), 0, h.stream, h.p00, h.np00, h.c00);
+        check();
+        hipLaunchKernelGGL(k_solve00, dim3(1), dim3(256), 0, h.stream, d, h.lu_dl, h.lu_dp, h.lu_du, h.c00);
         check();
     }
     // inverse z: T1[m][col] = sum_q Zi[m][q] S[q][col]
@@ -468,7 +604,7 @@ void launch_poisson_mixed(Handle &h, float *phi, const Fld3<float> &f) {
     z.Alo = h.zi_lo;
     z.B = S;
     z.out = T1;
-    gemm<kBMz, kBNz, false, false>(h, KID_XFORM_Z_INV, z);
+    gemm<kWRz, kWCz, false, false>(h, KID_XFORM_Z_INV, z);
     // inverse x into the field layout, plus the (0,0) constant of each y row
     g.Ahi = h.dxi_hi;
     g.Alo = h.dxi_lo;
@@ -479,7 +615,7 @@ void launch_poisson_mixed(Handle &h, float *phi, const Fld3<float> &f) {
     g.o_c1 = d.sx;
     g.o_c2 = d.sxy;
     g.add = h.c00;
-    gemm<kBMx, kBNx, true, true>(h, KID_XFORM_X_INV, g);
+    gemm<kWRx, kWCx, true, true>(h, KID_XFORM_X_INV, g);
 }
 
 }  // namespace blgpu

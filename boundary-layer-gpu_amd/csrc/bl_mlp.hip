@@ -46,8 +46,9 @@ __device__ __forceinline__ unsigned long long mix64(unsigned long long z) {
 
 // batch gather: sample b of step s is row hash(seed, s, b) mod n
 __global__ __launch_bounds__(256) void k_gather(const float *__restrict__ rows, long long n, int B,
-                                                unsigned long long seed, long long step, float *__restrict__ Xb,
-                                                float *__restrict__ tb, float *__restrict__ wb) {
+                                                unsigned long long seed, long long step, float wscale,
+                                                float *__restrict__ Xb, float *__restrict__ tb,
+                                                float *__restrict__ wb) {
     const int b = blockIdx.x * 256 + threadIdx.x;
     if (b >= B) return;
     const unsigned long long h = mix64(seed ^ mix64((unsigned long long)step * 0x100000001B3ull + (unsigned long long)b));
@@ -56,7 +57,33 @@ __global__ __launch_bounds__(256) void k_gather(const float *__restrict__ rows, 
 #pragma unroll
     for (int f = 0; f < kIn; ++f) Xb[(long long)f * B + b] = row[f];
     tb[b] = row[kIn];
-    wb[b] = row[kIn + 1];
+    wb[b] = row[kIn + 1] * wscale;
+}
+
+// sum and count of the positive row weights, per block (fixed order)
+__global__ __launch_bounds__(256) void k_wsum(const float *__restrict__ rows, long long n, double *__restrict__ part) {
+    __shared__ double sh[2][4];
+    double s = 0.0, c = 0.0;
+    for (long long r = (long long)blockIdx.x * 256 + threadIdx.x; r < n; r += (long long)gridDim.x * 256) {
+        const float w = rows[r * kMlpRow + kIn + 1];
+        if (w > 0.f) {
+            s += w;
+            c += 1.0;
+        }
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        s += __shfl_down(s, o);
+        c += __shfl_down(c, o);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        sh[0][threadIdx.x >> 6] = s;
+        sh[1][threadIdx.x >> 6] = c;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        part[2 * blockIdx.x] = (sh[0][0] + sh[0][1]) + (sh[0][2] + sh[0][3]);
+        part[2 * blockIdx.x + 1] = (sh[1][0] + sh[1][1]) + (sh[1][2] + sh[1][3]);
+    }
 }
 
 // stage the parameters in LDS
@@ -358,9 +385,29 @@ void mlp_train(Mlp &m, int steps, int B, float lr, unsigned long long seed, floa
         throw ArgError();
     }
     const dim3 blk(256);
+    // weights rescaled to mean 1 over the weighted rows of the dataset (the
+    // partial sums go through the gradient buffer, which each step rewrites)
+    constexpr int kWBlk = 128;
+    static_assert(2 * kWBlk * sizeof(double) <= kMlpParams * sizeof(float), "partials fit in g");
+    double *wpart = reinterpret_cast<double *>(m.g);
+    hipLaunchKernelGGL(k_wsum, dim3(kWBlk), blk, 0, m.stream, m.rows, m.n, wpart);
+    check();
+    double hp[2 * kWBlk];
+    BL_HIP(hipMemcpyAsync(hp, wpart, sizeof(hp), hipMemcpyDeviceToHost, m.stream));
+    BL_HIP(hipStreamSynchronize(m.stream));
+    double wsum = 0.0, wcnt = 0.0;
+    for (int q = 0; q < kWBlk; ++q) {
+        wsum += hp[2 * q];
+        wcnt += hp[2 * q + 1];
+    }
+    if (!(wsum > 0.0)) {
+        last_error() = "bl_mlp_train: the dataset has no row with a positive weight";
+        throw ArgError();
+    }
+    const float wscale = (float)(wcnt / wsum);
     for (int s = 0; s < steps; ++s) {
-        hipLaunchKernelGGL(k_gather, dim3((B + 255) / 256), blk, 0, m.stream, m.rows, m.n, B, seed, m.step, m.Xb,
-                           m.tb, m.wb);
+        hipLaunchKernelGGL(k_gather, dim3((B + 255) / 256), blk, 0, m.stream, m.rows, m.n, B, seed, m.step, wscale,
+                           m.Xb, m.tb, m.wb);
         check();
         mlp_fwd_bwd(m, B, true);
         m.step += 1;

@@ -173,8 +173,10 @@ int bl_mlp_get_rows(bl_mlp *m, long long first, long long n, float *rows);
 int bl_mlp_count(bl_mlp *m, long long *n);
 int bl_mlp_clear(bl_mlp *m);
 /* steps of Adam (lr; betas 0.9, 0.999; eps 1e-8) on batches of `batch` rows
- * drawn by a counter hash of (seed, step, slot); loss_hist[steps] optional:
- * the weighted batch loss before each update */
+ * drawn by a counter hash of (seed, step, slot), minimising the weighted
+ * loss (1/B) sum w (y - t)^2 with the row weights rescaled to mean 1 over the
+ * dataset's positive-weight rows; loss_hist[steps] optional: that loss before
+ * each update */
 int bl_mlp_train(bl_mlp *m, int steps, int batch, float lr, unsigned long long seed, float *loss_hist);
 
 /* ---- learned closure in the LES (les = 4; no counterpart in the reference) ----
@@ -183,8 +185,9 @@ int bl_mlp_train(bl_mlp *m, int steps, int batch, float lr, unsigned long long s
  * (top-hat) averages of the DNS faces give the filtered staggered field,
  * which is left in the coarse solver's state; per interior LES cell the row
  * holds the features the LES computes at run time and the least-squares eddy
- * viscosity of the exact SGS stress.  Rows with |S| <= smin or next to the x
- * boundaries get weight 0. */
+ * viscosity of the exact SGS stress.  Row weight: (Delta^2 |S|)^2 S:S, the
+ * factor that turns the squared error in c into the squared deviatoric SGS
+ * stress error; rows with |S| <= smin or next to the x boundaries get weight 0. */
 int bl_closure_samples(bl_handle *fine, bl_handle *coarse, bl_mlp *m, int fx, int fy, int fz, double delta99,
                        double smin);
 /* install the trained network in an les = 4 solver (BF16 weights) with the

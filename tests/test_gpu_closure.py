@@ -103,6 +103,32 @@ def test_one_adam_step_matches_reference(net):
     assert err < 1e-6
 
 
+def test_weighted_loss_is_normalised(net):
+    """Row weights enter the loss rescaled to mean 1 over the positive-weight
+    rows of the dataset: loss = (1/B) sum_b (w_b / mean w) (y_b - t_b)^2 on
+    the batch the device's hash draws (zero-weight rows count in the batch
+    but not in the mean)."""
+    c, theta = net
+    n, B, seed = 2000, 256, 77
+    X, t = data(n, seed=4)
+    rng = np.random.default_rng(8)
+    w = rng.uniform(0.1, 5.0, n) * 1e-3
+    w[rng.uniform(size=n) < 0.2] = 0.0
+    rows = np.zeros((n, blgpu.MLP_ROW), dtype=np.float32)
+    rows[:, :10] = X
+    rows[:, 10] = t
+    rows[:, 11] = w
+    c.add_rows(rows)
+    hist = c.train(1, B, lr=1e-3, seed=seed)
+    idx = batch_rows(n, B, seed, 0)
+    w32 = rows[:, 11].astype(np.float64)
+    wmean = w32[w32 > 0].mean()
+    y, _ = M.forward(theta, rows[idx, :10].astype(np.float64))
+    ref = np.mean(w32[idx] / wmean * (y - rows[idx, 10].astype(np.float64)) ** 2)
+    say("weighted loss", hist[0], "reference", ref)
+    assert hist[0] == pytest.approx(ref, rel=1e-5)
+
+
 def test_training_lowers_the_loss(net):
     c, _ = net
     n = 4096
@@ -190,6 +216,11 @@ def test_samples_match_reference():
     assert wmask.sum() > 0
     np.testing.assert_allclose(rows[..., 10][wmask], tgt.transpose(2, 1, 0)[wmask], rtol=1e-4,
                                atol=1e-5 * np.abs(tgt.transpose(2, 1, 0)[wmask]).max())
+    # the stress-space weight (Delta^2 |S|)^2 S:S of the weighted rows
+    Sr = S[:, :, :, : nzg - 3]
+    ss = Sr[0] ** 2 + Sr[1] ** 2 + Sr[2] ** 2 + 2 * (Sr[3] ** 2 + Sr[4] ** 2 + Sr[5] ** 2)
+    wr = (scale[:, :, : nzg - 3] ** 2 * ss).transpose(2, 1, 0)
+    np.testing.assert_allclose(rows[..., 11][wmask], wr[wmask], rtol=1e-5)
     say("samples", rows.shape, "weighted", int(wmask.sum()))
     c.close()
 
