@@ -128,6 +128,22 @@ def run_leg(blgpu, precision, grid, args, keep_fields=False):
     return out
 
 
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "fp64_814x257x193_hbm_traffic.txt")
+
+
+def measured_traffic(label, nx, ny, nz):
+    """HBM bytes per launch of a kernel label from the committed counter
+    profile (rocprofv3 FETCH_SIZE / WRITE_SIZE, calibrated against a copy of
+    known size); only for the workload the profile was taken on."""
+    if (nx, ny, nz) != (814, 257, 193) or not os.path.exists(TRAFFIC_PROFILE):
+        return None
+    for line in open(TRAFFIC_PROFILE):
+        f = line.split()
+        if len(f) == 3 and f[0] == "traffic" and f[1] == label:
+            return float(f[2])
+    return None
+
+
 def roofline_of(prof, nx, ny, nz):
     tot = {k: v[0] for k, v in prof.items() if v[1] > 0}
     gpu_ms = sum(tot.values())
@@ -142,7 +158,8 @@ def roofline_of(prof, nx, ny, nz):
     else:
         achieved = (units / avg_s / 1e9) if units else None
         roof = dict(bound="hbm", achieved=achieved, peak=HBM_PEAK_GBS, unit="GB/s",
-                    frac=(achieved / HBM_PEAK_GBS) if achieved else None, traffic=None)
+                    frac=(achieved / HBM_PEAK_GBS) if achieved else None,
+                    traffic=measured_traffic(dom, nx, ny, nz), algorithmic_bytes=units)
     roof.update(kernel=dom, avg_launch_ms=dom_ms / dom_n, share_of_gpu_time=dom_ms / gpu_ms)
     return roof
 
@@ -332,7 +349,7 @@ def main():
                          kernels={k: dict(ms_per_step=v[0] / args.steps, launches_per_step=v[1] / args.steps)
                                   for k, v in rmx["prof"].items() if v[1] > 0})
         del rmx
-    del r64["fields"]
+    r64.pop("fields", None)
 
     les_out = None
     if not args.no_les:

@@ -103,7 +103,29 @@ __global__ __launch_bounds__(256) void k_rk_march(
     commit(kb % 3);
     fetch(kb + 1);
     const int i = i0 + tx;
+    // a first stage starts from the field it differentiates: the old values
+    // are then the LDS centre values and are not read again
+    const bool old_is_cur = MODE == 0 && Uo == U && Vo == V && Wo == W;
     for (int k = kb; k < ke; ++k) {
+        // this plane's step-start and first-stage values, issued before the
+        // barrier so that their latency overlaps the stencil work
+        T uo[2][3] = {}, fo[2][3] = {};
+        if (MODE == 1 || (MODE == 0 && !old_is_cur)) {
+#pragma unroll
+            for (int r = 0; r < 2; ++r) {
+                const int j = j0 + 2 * ty + r;
+                if (i > d.nxg - 2 || j > d.nyg - 2) continue;
+                const long long p = fidx(d, i, j, k);
+                uo[r][0] = Uo[p];
+                uo[r][1] = Vo[p];
+                uo[r][2] = Wo[p];
+                if constexpr (MODE == 1) {
+                    fo[r][0] = F1u[p];
+                    fo[r][1] = F1v[p];
+                    fo[r][2] = F1w[p];
+                }
+            }
+        }
         commit((k + 1) % 3);
         __syncthreads();
         if (k + 2 <= ke) fetch(k + 2);
@@ -155,7 +177,8 @@ __global__ __launch_bounds__(256) void k_rk_march(
                     Fou[p] = rr;
                 } else {
                     if (WRITE_F1) Fou[p] = rr;
-                    Un[p] = (MODE == 0) ? Uo[p] + c0 * rr : Uo[p] + dt * (c1 * F1u[p] + c0 * rr);
+                    Un[p] = (MODE == 0) ? (old_is_cur ? sU[sc][e] : uo[r][0]) + c0 * rr
+                                        : uo[r][0] + dt * (c1 * fo[r][0] + c0 * rr);
                 }
             }
             // ---- v (equations.f90:183-230, 287-292)
@@ -194,7 +217,8 @@ __global__ __launch_bounds__(256) void k_rk_march(
                     Fov[p] = rr;
                 } else {
                     if (WRITE_F1) Fov[p] = rr;
-                    Vn[p] = (MODE == 0) ? Vo[p] + c0 * rr : Vo[p] + dt * (c1 * F1v[p] + c0 * rr);
+                    Vn[p] = (MODE == 0) ? (old_is_cur ? sV[sc][e] : uo[r][1]) + c0 * rr
+                                        : uo[r][1] + dt * (c1 * fo[r][1] + c0 * rr);
                 }
             }
             // ---- w (equations.f90:323-366, 408-413)
@@ -235,7 +259,8 @@ __global__ __launch_bounds__(256) void k_rk_march(
                     Fow[p] = rr;
                 } else {
                     if (WRITE_F1) Fow[p] = rr;
-                    Wn[p] = (MODE == 0) ? Wo[p] + c0 * rr : Wo[p] + dt * (c1 * F1w[p] + c0 * rr);
+                    Wn[p] = (MODE == 0) ? (old_is_cur ? sW[sc][e] : uo[r][2]) + c0 * rr
+                                        : uo[r][2] + dt * (c1 * fo[r][2] + c0 * rr);
                 }
             }
         }
