@@ -414,6 +414,71 @@ void build(Handle &h, const bl_config *c) {
         upload_vec(h.dct_w2, w2, h.stream);
         upload_vec(h.dct_wN, wN, h.stream);
         upload_vec(h.dct_rot, rot, h.stream);
+        // FP64 MFMA operand tables of the two short DFTs (k_dct_fwd_m /
+        // k_dct_inv_m), one double per lane l per (row tile, k step):
+        // entry A[16 rt + l % 16][4 ks + l / 16].  A tile row R = q + 4 r
+        // (q = R % 4) of a paired table is (Re, Im) = (r % 2) of index
+        // 8 rt + q + 4 (r / 2).
+        const int k1n = n1 / 2 + 1;
+        h.dct_mfma = n1 <= 32 && k1n <= 16 && n2 <= 32;
+        if (h.dct_mfma) {
+            auto frag = [](int tiles, int ks_n, auto f) {
+                std::vector<double> t((size_t)tiles * ks_n * 64, 0.0);
+                for (int rt = 0; rt < tiles; ++rt)
+                    for (int ks = 0; ks < ks_n; ++ks)
+                        for (int l = 0; l < 64; ++l) t[((size_t)rt * ks_n + ks) * 64 + l] = f(rt, l % 16, 4 * ks + l / 16);
+                return t;
+            };
+            auto pair_row = [](int rt, int R, int &idx, int &part) {
+                idx = 8 * rt + R % 4 + 4 * ((R / 4) / 2);
+                part = (R / 4) % 2;
+            };
+            const int ksa = (n1 + 3) / 4, ksc = (2 * n2 + 3) / 4, ksi = (2 * k1n + 3) / 4;
+            // forward step A: B[k1][n2] = sum_n1 v W1^{n1 k1}, W1^q = (w1[2q], w1[2q+1])
+            std::vector<double> fA = frag(2, ksa, [&](int rt, int R, int k) {
+                int k1, part;
+                pair_row(rt, R, k1, part);
+                if (k1 >= k1n || k >= n1) return 0.0;
+                const int w = (int)(((long long)k * k1) % n1);
+                return w1[2 * w + part];
+            });
+            // forward step C: V = sum_n2 B' W2^{n2 k2}; column k = (n2, re/im)
+            std::vector<double> fC = frag(4, ksc, [&](int rt, int R, int k) {
+                int k2, part;
+                pair_row(rt, R, k2, part);
+                const int nn = k / 2, pin = k % 2;
+                if (k2 >= n2 || nn >= n2) return 0.0;
+                const int w = (int)(((long long)nn * k2) % n2);
+                const double wr = w2[2 * w], wi = w2[2 * w + 1];
+                return part == 0 ? (pin == 0 ? wr : -wi) : (pin == 0 ? wi : wr);
+            });
+            // inverse step C': sum_k2 V conj(W2^{n2 k2}); column k = (k2, re/im)
+            std::vector<double> iC = frag(4, ksc, [&](int rt, int R, int k) {
+                int nn, part;
+                pair_row(rt, R, nn, part);
+                const int k2 = k / 2, pin = k % 2;
+                if (nn >= n2 || k2 >= n2) return 0.0;
+                const int w = (int)(((long long)nn * k2) % n2);
+                const double wr = w2[2 * w], wi = w2[2 * w + 1];
+                return part == 0 ? (pin == 0 ? wr : wi) : (pin == 0 ? -wi : wr);
+            });
+            // inverse step A': v = sum_k1 c_k1 Re(C~ conj(W1^{n1 k1})), rows n1
+            std::vector<double> iA = frag(2, ksi, [&](int rt, int R, int k) {
+                const int nn1 = 16 * rt + R, k1 = k / 2, pin = k % 2;
+                if (nn1 >= n1 || k1 >= k1n) return 0.0;
+                const double c = (k1 == 0 || (n1 % 2 == 0 && k1 == n1 / 2)) ? 1.0 : 2.0;
+                const int w = (int)(((long long)nn1 * k1) % n1);
+                return c * w1[2 * w + pin];
+            });
+            h.dct_mf = dmalloc(fA.size() + fC.size() + iC.size() + iA.size());
+            std::vector<double> all;
+            for (auto *t : {&fA, &fC, &iC, &iA}) all.insert(all.end(), t->begin(), t->end());
+            upload_vec(h.dct_mf, all, h.stream);
+            h.dct_mf_off[0] = 0;
+            h.dct_mf_off[1] = fA.size();
+            h.dct_mf_off[2] = fA.size() + fC.size();
+            h.dct_mf_off[3] = fA.size() + fC.size() + iC.size();
+        }
     }
     // z cos/sin tables, symmetric in (k, m): lda covers the padded row blocks
     {
@@ -487,7 +552,7 @@ void free_all(Handle &h) {
     }
     for (void *p : {(void *)h.F1.u, (void *)h.F1.v, (void *)h.F1.w, (void *)h.P, (void *)h.phi,
                     (void *)h.nu_t, (void *)h.T1, (void *)h.S, (void *)h.dct_w1, (void *)h.dct_w2,
-                    (void *)h.dct_wN, (void *)h.dct_rot, (void *)h.zc, (void *)h.zs, (void *)h.lu_dl, (void *)h.lu_dp, (void *)h.lu_du,
+                    (void *)h.dct_wN, (void *)h.dct_rot, (void *)h.dct_mf, h.mx2, (void *)h.zc, (void *)h.zs, (void *)h.lu_dl, (void *)h.lu_dp, (void *)h.lu_du,
                     (void *)h.scal, (void *)h.red, (void *)h.dsm, h.mem_grid, h.mem_gridf, (void *)h.mx_tab,
                     (void *)h.p00, (void *)h.cl_w, (void *)h.cl_b, (void *)h.cl_cnt})
         f(p);

@@ -43,8 +43,6 @@ typedef unsigned short u16x8 __attribute__((ext_vector_type(8)));
 
 constexpr int kH = kMlpHidden;
 
-__device__ __forceinline__ int clampi(int i, int lo, int hi) { return i < lo ? lo : (i > hi ? hi : i); }
-
 // velocity-gradient tensor at interior cell (i, j, k): g[3a + b] = d u_a / d x_b
 template <class T>
 __device__ void grad_tensor(const Dims &d, const Metrics &m, const T *__restrict__ U, const T *__restrict__ V,

@@ -121,6 +121,9 @@ struct Handle {
     int lda_zt;                      // leading dimension of the z cos/sin tables
     double *zc, *zs;                 // z tables cos/sin(2 pi k m / M), [m][k]
     double *dct_w1, *dct_w2, *dct_wN, *dct_rot;  // x-DCT twiddle tables (complex)
+    bool dct_mfma = false;                       // x-DCT on the FP64 matrix cores (shape fits)
+    double *dct_mf = nullptr;                    // their operand tables: fA, fC, iC, iA
+    size_t dct_mf_off[4] = {0, 0, 0, 0};
     double *lu_dl, *lu_dp, *lu_du;  // Thomas factors [k][j][kx], [k][j][kx], [j]
     double *scal;                   // device scalars (mass-flux correction, reductions)
     double *red;                    // reduction scratch
@@ -132,6 +135,10 @@ struct Handle {
     // mixed precision: BF16 hi/lo transform matrices, FP64 (0,0) mode
     int NP;                         // kx stride of the FP32 spectral arrays (N rounded up to 32)
     int kpx, kpz;                   // GEMM depths N and M rounded up to 32
+    bool mx_two = false;            // x transform as two short DFTs (k_xfwd_mx / k_xinv_mx)
+    void *mx2 = nullptr;            // their operand and twiddle tables (one allocation)
+    const void *x2_fA = nullptr, *x2_fC = nullptr, *x2_iC = nullptr, *x2_iA = nullptr;
+    const float *x2_tw = nullptr;   // wN[N], rot[N] as float2
     unsigned short *mx_tab;         // one allocation for the eight matrices below
     const unsigned short *dxf_hi, *dxf_lo, *dxi_hi, *dxi_lo, *zf_hi, *zf_lo, *zi_hi, *zi_lo;
     double *p00;                    // FP64 plane partials of the divergence

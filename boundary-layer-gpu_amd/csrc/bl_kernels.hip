@@ -34,13 +34,20 @@ __device__ __forceinline__ T sq(T a) {
 // MODE 0: new = old + c0*F              (first stage, and RK2 second stage)
 // MODE 1: new = old + dt*(c1*F1 + c0*F) (RK3 stages 2 and 3)
 // MODE 2: write F only (operator test)  A block owns a
-// 64 (x) x 8 (y) column of cells and a chunk of z planes; U, V, W of planes
-// k-1, k, k+1 live in an LDS ring with a one-cell halo in x and y, so every
-// velocity value is read from HBM once per chunk (plus halo) instead of once
-// per neighbour.  Loads are x-fastest and 512-byte aligned rows.
-// Thread (tx, ty) computes cells (i0+tx, j0+2ty) and (i0+tx, j0+2ty+1).
+// 64 (x) x 4 RPT (y) column of cells and a chunk of z planes; U, V, W of
+// planes k-1, k, k+1 live in an LDS ring with a one-cell halo in x and y, so
+// every velocity value is read from HBM once per chunk (plus halo) instead of
+// once per neighbour.  Loads are x-fastest and 512-byte aligned rows.
+// Thread (tx, ty) computes cells (i0+tx, j0+RPT ty+r), r < RPT: RPT = 2 in
+// FP64 (47.5 KB of LDS per block), 4 in FP32 (taller tiles: less y halo and
+// twice the loads in flight per barrier at the same LDS size).
 // ------------------------------------------------------------------------
-constexpr int kTX = 64, kTY = 8, kHX = kTX + 2, kHY = kTY + 2, kPlane = kHX * kHY;
+constexpr int kTX = 64, kHX = kTX + 2;
+template <class T>
+struct MarchTile {
+    static constexpr int RPT = sizeof(T) == 4 ? 4 : 2;  // rows per thread
+    static constexpr int TY = 4 * RPT, HY = TY + 2, Plane = kHX * HY;
+};
 
 // LES: the viscous terms use nu + nu_t (equations.f90:97-132, 231-270,
 // 367-404); nu_t is staged in a fourth LDS ring.
@@ -53,6 +60,7 @@ __global__ __launch_bounds__(256) void k_rk_march(
     const T *__restrict__ F1u, const T *__restrict__ F1v, const T *__restrict__ F1w,
     T *__restrict__ Un, T *__restrict__ Vn, T *__restrict__ Wn, T *__restrict__ Fou,
     T *__restrict__ Fov, T *__restrict__ Fow, T c0, T c1, T dt) {
+    constexpr int RPT = MarchTile<T>::RPT, kTY = MarchTile<T>::TY, kPlane = MarchTile<T>::Plane;
     __shared__ T sU[3][kPlane], sV[3][kPlane], sW[3][kPlane];
     __shared__ T sN[LES ? 3 : 1][LES ? kPlane : 1];
     const int tid = threadIdx.x;
@@ -109,11 +117,11 @@ __global__ __launch_bounds__(256) void k_rk_march(
     for (int k = kb; k < ke; ++k) {
         // this plane's step-start and first-stage values, issued before the
         // barrier so that their latency overlaps the stencil work
-        T uo[2][3] = {}, fo[2][3] = {};
+        T uo[RPT][3] = {}, fo[RPT][3] = {};
         if (MODE == 1 || (MODE == 0 && !old_is_cur)) {
 #pragma unroll
-            for (int r = 0; r < 2; ++r) {
-                const int j = j0 + 2 * ty + r;
+            for (int r = 0; r < RPT; ++r) {
+                const int j = j0 + RPT * ty + r;
                 if (i > d.nxg - 2 || j > d.nyg - 2) continue;
                 const long long p = fidx(d, i, j, k);
                 uo[r][0] = Uo[p];
@@ -132,8 +140,8 @@ __global__ __launch_bounds__(256) void k_rk_march(
         const int sm = (k - 1) % 3, sc = k % 3, sp = (k + 1) % 3;
         const T rdzk = m.rdz[k], rdzgk = m.rdzg[k], rdzgm = m.rdzg[k - 1];
 #pragma unroll
-        for (int r = 0; r < 2; ++r) {
-            const int jt = 2 * ty + r;       // row in the tile
+        for (int r = 0; r < RPT; ++r) {
+            const int jt = RPT * ty + r;     // row in the tile
             const int j = j0 + jt;
             if (i > d.nxg - 2 || j > d.nyg - 2) continue;
             const int e = (jt + 1) * kHX + (tx + 1);  // LDS index of (i, j)
@@ -578,6 +586,9 @@ struct DctArgs {
     long long out_base, out_c1, out_c2;
     double scale;  // inverse only: 1/(N*M)
     const double2 *w1, *w2, *wN, *rot;
+    // FP64 MFMA operand tables of the two short DFTs, in lane order (see
+    // k_dct_fwd_m); null when the shape does not fit those kernels
+    const double *fA, *fC, *iC, *iA;
 };
 
 __device__ __forceinline__ double2 cmul(double2 a, double2 b) {
@@ -731,6 +742,239 @@ __global__ __launch_bounds__(256) void k_dct_inv(DctArgs a, const double *__rest
     }
     __syncthreads();
     for (int idx = tid; idx < LB * N; idx += 256) {
+        const int l = idx / N, n = idx - l * N;
+        const int c = line0 + l;
+        if (c >= a.C) continue;
+        const int c1 = c % a.C1, c2 = c / a.C1;
+        out[a.out_base + c1 * a.out_c1 + c2 * a.out_c2 + n] = o[idx];
+    }
+}
+
+// ------------------------------------------------------------------------
+// The same transforms with both short DFTs on the FP64 matrix cores
+// (v_mfma_f64_16x16x4_f64; lane l holds A[l%16][l/16], B[l/16][l%16] and
+// D[l/16 + 4r][l%16], r < 4).  kMfL lines per block, four waves.
+// Step A / A' use 2 kMfL column tiles (kMfL / 2 per wave), step C / C' kMfL.
+// Forward, step A: [Re, Im of B[k1][n2]] = WA [N1 x (line, n2)] with the
+//   rows of WA ordered so that a lane's D rows r = 0, 1 are Re, Im of one k1
+//   and r = 2, 3 those of k1 + 4; the twiddle W_N^{n2 k1} is applied in
+//   registers and B' goes to LDS as complex.
+// Forward, step C: [Re, Im of V[k1 + N1 k2]] = WC [(n2, re/im) x (line, k1)],
+//   rows paired the same way over k2; the post-rotation and the Makhoul
+//   placement write the output line into LDS.
+// Inverse, step C': the conjugate DFT over k2, rows paired over n2, then the
+//   conjugate twiddle; step A': the real output v[N2 n1 + n2] as one
+//   real matrix WA' [n1 x (k1, re/im)] with the Hermitian weights (1, 2, .., 1).
+// The operand tables are built on the host in lane order (one double per
+// lane per (row tile, k step)); shapes: N1 <= 32, N1/2 + 1 <= 16, N2 <= 32.
+// ------------------------------------------------------------------------
+typedef double dbl4 __attribute__((ext_vector_type(4)));
+constexpr int kMfL = 2;                   // lines per block (2 or 4)
+constexpr int kMfKA = 8, kMfKC = 16;      // k steps: step A (N1 <= 32), step C (2 N2 <= 64)
+
+__device__ __forceinline__ dbl4 mfma64(double a, double b, dbl4 c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+
+template <bool DIV>
+__global__ __launch_bounds__(256) void k_dct_fwd_m(DctArgs a, Dims d, Metrics mt, const double *__restrict__ in,
+                                                   const double *__restrict__ in2, const double *__restrict__ in3,
+                                                   double *__restrict__ out) {
+    extern __shared__ double smem[];
+    const int N = a.N, N1 = a.N1, N2 = a.N2, K1 = a.K1h + 1;
+    double *v = smem;                                         // kMfL*N
+    double2 *Bc = reinterpret_cast<double2 *>(v + kMfL * N);  // kMfL*K1*N2
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63, q = lane >> 4, c16 = lane & 15;
+    const int line0 = blockIdx.x * kMfL;
+    // load and reorder: v[n/2] = f[n] (n even), v[N-1-(n-1)/2] = f[n] (n odd)
+    for (int idx = tid; idx < kMfL * N; idx += 256) {
+        const int l = idx / N, n = idx - l * N;
+        const int c = min(line0 + l, a.C - 1);
+        const int c1 = c % a.C1, c2 = c / a.C1;
+        const long long p = a.in_base + c1 * a.in_c1 + c2 * a.in_c2 + n;
+        double f;
+        if (DIV) {
+            f = (in[p] - in[p - 1]) * mt.rdx[n + 1] + (in2[p] - in2[p - d.sx]) * mt.rdy[c1 + 1] +
+                (in3[p] - in3[p - d.sxy]) * mt.rdz[c2 + 1];
+        } else {
+            f = in[p];
+        }
+        v[l * N + ((n & 1) ? N - 1 - (n >> 1) : (n >> 1))] = f;
+    }
+    const int KSA = (N1 + 3) / 4, KSC = (2 * N2 + 3) / 4;
+    double fa[2][kMfKA];
+#pragma unroll
+    for (int ks = 0; ks < kMfKA; ++ks) {
+        fa[0][ks] = ks < KSA ? a.fA[(0 * KSA + ks) * 64 + lane] : 0.0;
+        fa[1][ks] = ks < KSA ? a.fA[(1 * KSA + ks) * 64 + lane] : 0.0;
+    }
+    __syncthreads();
+    // ---- step A: wave w owns kMfL / 2 column tiles (columns (line, n2))
+    for (int ct = wave * (kMfL / 2); ct < (wave + 1) * (kMfL / 2); ++ct) {
+        const int col = ct * 16 + c16;
+        const int ln = col / N2, n2 = col - ln * N2;
+        const bool cok = col < kMfL * N2;
+        dbl4 acc0 = {0.0, 0.0, 0.0, 0.0}, acc1 = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int ks = 0; ks < kMfKA; ++ks) {
+            if (ks < KSA) {
+                const int n1 = 4 * ks + q;
+                const double b = (cok && n1 < N1) ? v[ln * N + N2 * n1 + n2] : 0.0;
+                acc0 = mfma64(fa[0][ks], b, acc0);
+                acc1 = mfma64(fa[1][ks], b, acc1);
+            }
+        }
+        if (cok) {
+#pragma unroll
+            for (int h = 0; h < 4; ++h) {  // (row tile, pair): k1 = 8 (h/2) + q + 4 (h%2)
+                const dbl4 &acc = (h < 2) ? acc0 : acc1;
+                const int k1 = 8 * (h / 2) + q + 4 * (h % 2);
+                if (k1 < K1) {
+                    const double re = acc[2 * (h % 2)], im = acc[2 * (h % 2) + 1];
+                    Bc[(ln * K1 + k1) * N2 + n2] = cmul(make_double2(re, im), a.wN[(n2 * k1) % N]);
+                }
+            }
+        }
+    }
+    __syncthreads();
+    // ---- step C: wave w owns row tile w (k2 = 8w + q, 8w + q + 4), all
+    // kMfL column tiles (columns (line, k1))
+    {
+        double fc[kMfKC];
+#pragma unroll
+        for (int ks = 0; ks < kMfKC; ++ks) fc[ks] = ks < KSC ? a.fC[(wave * KSC + ks) * 64 + lane] : 0.0;
+        for (int ct = 0; ct < kMfL; ++ct) {
+            const int col = ct * 16 + c16;
+            const int ln = col / K1, k1 = col - ln * K1;
+            const bool cok = col < kMfL * K1;
+            const double *Bl = reinterpret_cast<const double *>(Bc + (ln * K1 + k1) * N2);
+            dbl4 acc = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+            for (int ks = 0; ks < kMfKC; ++ks) {
+                if (ks < KSC) {
+                    const int kk = 4 * ks + q;  // (n2, re/im) = (kk / 2, kk % 2)
+                    const double b = (cok && (kk >> 1) < N2) ? Bl[kk] : 0.0;
+                    acc = mfma64(fc[ks], b, acc);
+                }
+            }
+            if (cok) {
+#pragma unroll
+                for (int h = 0; h < 2; ++h) {
+                    const int k2 = 8 * wave + q + 4 * h;
+                    if (k2 < N2) {
+                        const int k = k1 + N1 * k2;
+                        const double2 wv = cmul(a.rot[k], make_double2(acc[2 * h], acc[2 * h + 1]));
+                        v[ln * N + k] = 2.0 * wv.x;
+                        if (k > 0) v[ln * N + N - k] = -2.0 * wv.y;
+                    }
+                }
+            }
+        }
+    }
+    __syncthreads();
+    for (int idx = tid; idx < kMfL * N; idx += 256) {
+        const int l = idx / N, kx = idx - l * N;
+        const int c = line0 + l;
+        if (c >= a.C) continue;
+        const int c1 = c % a.C1, c2 = c / a.C1;
+        out[a.out_base + c1 * a.out_c1 + c2 * a.out_c2 + kx] = v[l * N + kx];
+    }
+}
+
+__global__ __launch_bounds__(256) void k_dct_inv_m(DctArgs a, const double *__restrict__ in, double *__restrict__ out) {
+    extern __shared__ double smem[];
+    const int N = a.N, N1 = a.N1, N2 = a.N2, K1h = a.K1h, K1 = K1h + 1;
+    double2 *Vs = reinterpret_cast<double2 *>(smem);  // kMfL*K1*N2, later the output lines (kMfL*N doubles)
+    double2 *Cs = Vs + kMfL * K1 * N2;                // kMfL*K1*N2
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63, q = lane >> 4, c16 = lane & 15;
+    const int line0 = blockIdx.x * kMfL;
+    // V[k1 + N1 k2] = conj(rot_k) (g[k] - i g[N-k]) / 2 for k1 <= N1/2: Vs[l][k1][k2]
+    for (int idx = tid; idx < kMfL * K1 * N2; idx += 256) {
+        const int l = idx / (K1 * N2), r = idx - l * K1 * N2;
+        const int k1 = r / N2, k2 = r - k1 * N2;
+        const int k = k1 + N1 * k2;
+        const int c = min(line0 + l, a.C - 1);
+        const int c1 = c % a.C1, c2 = c / a.C1;
+        const double *gl = in + a.in_base + c1 * a.in_c1 + c2 * a.in_c2;
+        const double gk = gl[k], gm = (k == 0) ? 0.0 : gl[N - k];
+        Vs[idx] = cmulc(make_double2(0.5 * gk, -0.5 * gm), a.rot[k]);
+    }
+    const int KSC = (2 * N2 + 3) / 4, KSA = (2 * K1 + 3) / 4;
+    __syncthreads();
+    // ---- step C': wave w owns row tile w (n2 = 8w + q, 8w + q + 4), all
+    // kMfL column tiles (columns (line, k1))
+    {
+        double fc[kMfKC];
+#pragma unroll
+        for (int ks = 0; ks < kMfKC; ++ks) fc[ks] = ks < KSC ? a.iC[(wave * KSC + ks) * 64 + lane] : 0.0;
+        for (int ct = 0; ct < kMfL; ++ct) {
+            const int col = ct * 16 + c16;
+            const int ln = col / K1, k1 = col - ln * K1;
+            const bool cok = col < kMfL * K1;
+            const double *Vl = reinterpret_cast<const double *>(Vs + (ln * K1 + k1) * N2);
+            dbl4 acc = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+            for (int ks = 0; ks < kMfKC; ++ks) {
+                if (ks < KSC) {
+                    const int kk = 4 * ks + q;  // (k2, re/im) = (kk / 2, kk % 2)
+                    const double b = (cok && (kk >> 1) < N2) ? Vl[kk] : 0.0;
+                    acc = mfma64(fc[ks], b, acc);
+                }
+            }
+            if (cok) {
+#pragma unroll
+                for (int h = 0; h < 2; ++h) {
+                    const int n2 = 8 * wave + q + 4 * h;
+                    if (n2 < N2)
+                        Cs[(ln * K1 + k1) * N2 + n2] =
+                            cmulc(make_double2(acc[2 * h], acc[2 * h + 1]), a.wN[(n2 * k1) % N]);
+                }
+            }
+        }
+    }
+    __syncthreads();
+    // ---- step A': v[N2 n1 + n2] = sum over (k1, re/im) of WA'[n1][.] C~;
+    // wave w owns kMfL / 2 column tiles (columns (line, n2)), both row tiles
+    double *o = smem;  // kMfL*N doubles inside the Vs area, which is no longer read
+    {
+        double fa[2][kMfKA];
+#pragma unroll
+        for (int ks = 0; ks < kMfKA; ++ks) {
+            fa[0][ks] = ks < KSA ? a.iA[(0 * KSA + ks) * 64 + lane] : 0.0;
+            fa[1][ks] = ks < KSA ? a.iA[(1 * KSA + ks) * 64 + lane] : 0.0;
+        }
+        for (int ct = wave * (kMfL / 2); ct < (wave + 1) * (kMfL / 2); ++ct) {
+            const int col = ct * 16 + c16;
+            const int ln = col / N2, n2 = col - ln * N2;
+            const bool cok = col < kMfL * N2;
+            dbl4 acc0 = {0.0, 0.0, 0.0, 0.0}, acc1 = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+            for (int ks = 0; ks < kMfKA; ++ks) {
+                if (ks < KSA) {
+                    const int kk = 4 * ks + q;  // (k1, re/im)
+                    const int k1 = kk >> 1;
+                    const double b =
+                        (cok && k1 < K1) ? reinterpret_cast<const double *>(Cs + (ln * K1 + k1) * N2 + n2)[kk & 1]
+                                         : 0.0;
+                    acc0 = mfma64(fa[0][ks], b, acc0);
+                    acc1 = mfma64(fa[1][ks], b, acc1);
+                }
+            }
+            if (cok) {
+#pragma unroll
+                for (int r = 0; r < 8; ++r) {  // n1 = 16 (r / 4) + q + 4 (r % 4)
+                    const int n1 = 16 * (r / 4) + q + 4 * (r % 4);
+                    if (n1 < N1) {
+                        const int n = N2 * n1 + n2;
+                        const int pos = (n < (N + 1) / 2) ? 2 * n : 2 * (N - 1 - n) + 1;
+                        o[ln * N + pos] = (r < 4 ? acc0[r % 4] : acc1[r % 4]) * a.scale;
+                    }
+                }
+            }
+        }
+    }
+    __syncthreads();
+    for (int idx = tid; idx < kMfL * N; idx += 256) {
         const int l = idx / N, n = idx - l * N;
         const int c = line0 + l;
         if (c >= a.C) continue;
@@ -985,6 +1229,26 @@ void dct_launch(Handle &h, int kid, const DctArgs &a, bool forward, const double
                 const Field3 *div = nullptr) {
     ProfScope ps(h, kid);
     const int K1 = a.K1h + 1;
+    if (a.fA) {
+        // matrix-core kernels (the shape was checked with the tables)
+        const size_t lds = forward ? sizeof(double) * (size_t)kMfL * a.N + sizeof(double2) * (size_t)kMfL * K1 * a.N2
+                                   : 2 * sizeof(double2) * (size_t)kMfL * K1 * a.N2;
+        if (lds > 163840) {
+            last_error() = "x transform length too large for the LDS-resident DCT (N = " + std::to_string(a.N) + ")";
+            throw ArgError();
+        }
+        const dim3 grid((a.C + kMfL - 1) / kMfL);
+        if (forward && div)
+            hipLaunchKernelGGL(k_dct_fwd_m<true>, grid, dim3(256), lds, h.stream, a, h.d, h.m, div->u, div->v,
+                               div->w, out);
+        else if (forward)
+            hipLaunchKernelGGL(k_dct_fwd_m<false>, grid, dim3(256), lds, h.stream, a, h.d, h.m, in, nullptr,
+                               nullptr, out);
+        else
+            hipLaunchKernelGGL(k_dct_inv_m, grid, dim3(256), lds, h.stream, a, in, out);
+        check_launch();
+        return;
+    }
     const size_t lds = (forward ? sizeof(double) : sizeof(double2)) * (size_t)kDctLines * a.N +
                        sizeof(double2) * ((size_t)kDctLines * K1 * a.N2 + a.N1 + a.N2);
     if (lds > 65536) {
@@ -1028,7 +1292,9 @@ const MetricsT<float> &metrics<float>(const Handle &h) {
 }
 
 
+template <class T>
 dim3 march_grid(const Dims &d) {
+    constexpr int kTY = MarchTile<T>::TY;
     return dim3((d.nxg - 2 + kTX - 1) / kTX, (d.nyg - 2 + kTY - 1) / kTY, (d.nzg - 2 + kChunk - 1) / kChunk);
 }
 
@@ -1036,7 +1302,7 @@ template <int MODE, bool WRITE_F1, class T>
 void march(Handle &h, const Fld3<T> &cur, const Fld3<T> *old, const Fld3<T> *f1, Fld3<T> *nxt, Fld3<T> *fo,
            double c0, double c1, double dt) {
     const Dims &d = h.d;
-    const dim3 grid = march_grid(d), block(256);
+    const dim3 grid = march_grid<T>(d), block(256);
     const T *o[3] = {old ? old->u : nullptr, old ? old->v : nullptr, old ? old->w : nullptr};
     const T *f[3] = {f1 ? f1->u : nullptr, f1 ? f1->v : nullptr, f1 ? f1->w : nullptr};
     T *n[3] = {nxt ? nxt->u : nullptr, nxt ? nxt->v : nullptr, nxt ? nxt->w : nullptr};
@@ -1125,6 +1391,12 @@ void launch_poisson(Handle &h, double *phi, const Field3 *div) {
     da.w2 = reinterpret_cast<const double2 *>(h.dct_w2);
     da.wN = reinterpret_cast<const double2 *>(h.dct_wN);
     da.rot = reinterpret_cast<const double2 *>(h.dct_rot);
+    if (h.dct_mfma) {
+        da.fA = h.dct_mf + h.dct_mf_off[0];
+        da.fC = h.dct_mf + h.dct_mf_off[1];
+        da.iC = h.dct_mf + h.dct_mf_off[2];
+        da.iA = h.dct_mf + h.dct_mf_off[3];
+    }
     da.in_base = fidx(d, 1, 1, 1);
     da.in_c1 = d.sx;
     da.in_c2 = d.sxy;

@@ -70,26 +70,28 @@ struct GemmArgs {
     const double *add;  // optional per-(c % o_C1) addend
 };
 
-constexpr int kBK = 32, kLDK = kBK + 8;  // LDS row of one B column, padded
+// K step BK (a multiple of 32); an LDS row of one column holds BK + 8 values
+constexpr int kGemmLds = 163840;  // LDS bytes per workgroup on gfx950
 
 // float4 loads of the B tile per thread
-template <int WR, int WC>
+template <int WR, int WC, int BK>
 constexpr int gemm_bloads() {
-    return (kBK * 64 * WC / 4) / (64 * WR * WC);
+    return (BK * 64 * WC / 4) / (64 * WR * WC);
 }
 
 // B tile loads: unconditional (addresses clamped into the operand) so that
 // the loads of the next step stay in flight across the MFMAs; the masks
 // (columns past C, k past K) are applied when the tile is split and stored.
-template <int WR, int WC, bool KCONTIG>
+template <int WR, int WC, int BK, bool KCONTIG>
 __device__ __forceinline__ void gemm_load_b(const GemmArgs &g, int c0, int kk, int tid,
-                                            float4 (&v)[gemm_bloads<WR, WC>()]) {
-    constexpr int NT = 64 * WR * WC, BN = 64 * WC, NL = gemm_bloads<WR, WC>();
+                                            float4 (&v)[gemm_bloads<WR, WC, BK>()]) {
+    constexpr int NT = 64 * WR * WC, BN = 64 * WC, NL = gemm_bloads<WR, WC, BK>();
     if constexpr (KCONTIG) {
-        // 8 threads cover the 32 k of one column; NT / 8 columns per pass
+        // BK / 4 threads cover the BK k of one column
+        constexpr int TPC = BK / 4;
 #pragma unroll
         for (int p = 0; p < NL; ++p) {
-            const int c = min(c0 + p * (NT / 8) + tid / 8, g.C - 1), k = kk + 4 * (tid % 8);
+            const int c = min(c0 + p * (NT / TPC) + tid / TPC, g.C - 1), k = kk + 4 * (tid % TPC);
             v[p] = *reinterpret_cast<const float4 *>(g.B + (long long)c * g.ldb + k);
         }
     } else {
@@ -110,24 +112,25 @@ __device__ __forceinline__ void split2(float x, unsigned short &h, unsigned shor
     l = bf16_bits(x - bf16_float(h));
 }
 
-template <int WR, int WC, bool KCONTIG>
+template <int WR, int WC, int BK, bool KCONTIG>
 __device__ __forceinline__ void gemm_store_b(const GemmArgs &g, int c0, int kk, int tid,
-                                             const float4 (&v)[gemm_bloads<WR, WC>()], unsigned short *sBh,
+                                             const float4 (&v)[gemm_bloads<WR, WC, BK>()], unsigned short *sBh,
                                              unsigned short *sBl) {
-    constexpr int NT = 64 * WR * WC, BN = 64 * WC, NL = gemm_bloads<WR, WC>();
+    constexpr int NT = 64 * WR * WC, BN = 64 * WC, NL = gemm_bloads<WR, WC, BK>(), LDK = BK + 8;
     if constexpr (KCONTIG) {
-        const int k = 4 * (tid % 8);
+        constexpr int TPC = BK / 4;
+        const int k = 4 * (tid % TPC);
         const int kg = kk + k;
 #pragma unroll
         for (int p = 0; p < NL; ++p) {
-            const int col = p * (NT / 8) + tid / 8;
+            const int col = p * (NT / TPC) + tid / TPC;
             const bool cin = c0 + col < g.C;
             const float e[4] = {cin && kg < g.K ? v[p].x : 0.f, cin && kg + 1 < g.K ? v[p].y : 0.f,
                                 cin && kg + 2 < g.K ? v[p].z : 0.f, cin && kg + 3 < g.K ? v[p].w : 0.f};
             unsigned short h[4], l[4];
 #pragma unroll
             for (int q = 0; q < 4; ++q) split2(e[q], h[q], l[q]);
-            const int o = col * kLDK + k;
+            const int o = col * LDK + k;
             *reinterpret_cast<uint2 *>(&sBh[o]) =
                 make_uint2((unsigned)h[0] | ((unsigned)h[1] << 16), (unsigned)h[2] | ((unsigned)h[3] << 16));
             *reinterpret_cast<uint2 *>(&sBl[o]) =
@@ -147,7 +150,7 @@ __device__ __forceinline__ void gemm_store_b(const GemmArgs &g, int c0, int kk, 
                 unsigned short h0, l0, h1, l1;
                 split2(k0 ? a0[e] : 0.f, h0, l0);
                 split2(k1 ? a1[e] : 0.f, h1, l1);
-                const int o = (4 * cg + e) * kLDK + k;
+                const int o = (4 * cg + e) * LDK + k;
                 *reinterpret_cast<unsigned int *>(&sBh[o]) = (unsigned)h0 | ((unsigned)h1 << 16);
                 *reinterpret_cast<unsigned int *>(&sBl[o]) = (unsigned)l0 | ((unsigned)l1 << 16);
             }
@@ -155,46 +158,48 @@ __device__ __forceinline__ void gemm_store_b(const GemmArgs &g, int c0, int kk, 
     }
 }
 
-// A tile loads: BM rows x 32 k of hi and of lo as 16-byte chunks of 8 k
+// A tile loads: BM rows x BK k of hi and of lo as 16-byte chunks of 8 k
 // (rows are padded to whole tiles and k to Kp, so no masks)
-template <int WR, int WC>
+template <int WR, int WC, int BK>
 constexpr int gemm_aloads() {
-    return 8 / WC;  // (BM * 4 chunks * 2) / NT
+    return BK / (4 * WC);  // (BM * BK / 8 chunks * 2) / NT
 }
 
-template <int WR, int WC>
+template <int WR, int WC, int BK>
 __device__ __forceinline__ void gemm_load_a(const GemmArgs &g, int r0, int kk, int tid,
-                                            u16x8 (&va)[gemm_aloads<WR, WC>()]) {
-    constexpr int NT = 64 * WR * WC, NA = gemm_aloads<WR, WC>();
+                                            u16x8 (&va)[gemm_aloads<WR, WC, BK>()]) {
+    constexpr int NT = 64 * WR * WC, NA = gemm_aloads<WR, WC, BK>(), kCPR = BK / 8;
 #pragma unroll
     for (int p = 0; p < NA; ++p) {
-        const int q = p * NT + tid;  // chunk id over [hi, lo][row][4 chunks]
-        const int half = q / (64 * WR * 4), rc = q % (64 * WR * 4);
-        const int row = rc / 4, ch = rc % 4;
+        const int q = p * NT + tid;  // chunk id over [hi, lo][row][kCPR chunks]
+        const int half = q / (64 * WR * kCPR), rc = q % (64 * WR * kCPR);
+        const int row = rc / kCPR, ch = rc % kCPR;
         const unsigned short *src = (half ? g.Alo : g.Ahi) + (long long)(r0 + row) * g.Kp + kk + 8 * ch;
         va[p] = *reinterpret_cast<const u16x8 *>(src);
     }
 }
 
-template <int WR, int WC>
-__device__ __forceinline__ void gemm_store_a(int tid, const u16x8 (&va)[gemm_aloads<WR, WC>()],
+template <int WR, int WC, int BK>
+__device__ __forceinline__ void gemm_store_a(int tid, const u16x8 (&va)[gemm_aloads<WR, WC, BK>()],
                                              unsigned short *sAh, unsigned short *sAl) {
-    constexpr int NT = 64 * WR * WC, NA = gemm_aloads<WR, WC>();
+    constexpr int NT = 64 * WR * WC, NA = gemm_aloads<WR, WC, BK>(), kCPR = BK / 8, LDK = BK + 8;
 #pragma unroll
     for (int p = 0; p < NA; ++p) {
         const int q = p * NT + tid;
-        const int half = q / (64 * WR * 4), rc = q % (64 * WR * 4);
-        const int row = rc / 4, ch = rc % 4;
-        *reinterpret_cast<u16x8 *>((half ? sAl : sAh) + row * kLDK + 8 * ch) = va[p];
+        const int half = q / (64 * WR * kCPR), rc = q % (64 * WR * kCPR);
+        const int row = rc / kCPR, ch = rc % kCPR;
+        *reinterpret_cast<u16x8 *>((half ? sAl : sAh) + row * LDK + 8 * ch) = va[p];
     }
 }
 
-template <int WR, int WC, bool KCONTIG, bool RCONTIG>
+template <int WR, int WC, int BK, bool KCONTIG, bool RCONTIG>
 __global__ __launch_bounds__(64 * WR * WC) void k_gemm_bf16x3(GemmArgs g, int tiles_r, int nblk) {
-    constexpr int BM = 64 * WR, BN = 64 * WC, NL = gemm_bloads<WR, WC>(), NA = gemm_aloads<WR, WC>();
+    constexpr int BM = 64 * WR, BN = 64 * WC, NL = gemm_bloads<WR, WC, BK>(), NA = gemm_aloads<WR, WC, BK>();
+    constexpr int kLDK = BK + 8;
+    static_assert(BK % 32 == 0, "K step");
     // one buffer of each operand: [row or column][k], rows padded to kLDK
     __shared__ unsigned short sA[2][BM * kLDK], sB[2][BN * kLDK];  // [hi, lo]
-    static_assert(sizeof(unsigned short) * 2 * (BM + BN) * kLDK <= 65536, "LDS per workgroup");
+    static_assert(sizeof(unsigned short) * 2 * (BM + BN) * kLDK <= kGemmLds, "LDS per workgroup");
     // XCD-aware order: blocks that share a column tile run on one XCD (the
     // hardware deals consecutive block ids round-robin over 8 XCDs)
     const int b = blockIdx.x;
@@ -215,48 +220,52 @@ __global__ __launch_bounds__(64 * WR * WC) void k_gemm_bf16x3(GemmArgs g, int ti
 
     float4 vb[NL];
     u16x8 va[NA];
-    gemm_load_b<WR, WC, KCONTIG>(g, c0, 0, tid, vb);
-    gemm_load_a<WR, WC>(g, r0, 0, tid, va);
-    gemm_store_b<WR, WC, KCONTIG>(g, c0, 0, tid, vb, sB[0], sB[1]);
-    gemm_store_a<WR, WC>(tid, va, sA[0], sA[1]);
+    gemm_load_b<WR, WC, BK, KCONTIG>(g, c0, 0, tid, vb);
+    gemm_load_a<WR, WC, BK>(g, r0, 0, tid, va);
+    gemm_store_b<WR, WC, BK, KCONTIG>(g, c0, 0, tid, vb, sB[0], sB[1]);
+    gemm_store_a<WR, WC, BK>(tid, va, sA[0], sA[1]);
     __syncthreads();
-    for (int kk = 0; kk < g.Kp; kk += kBK) {
-        const bool more = kk + kBK < g.Kp;
+    for (int kk = 0; kk < g.Kp; kk += BK) {
+        const bool more = kk + BK < g.Kp;
         // the next step's tiles stay in flight during this step's MFMAs
         if (more) {
-            gemm_load_b<WR, WC, KCONTIG>(g, c0, kk + kBK, tid, vb);
-            gemm_load_a<WR, WC>(g, r0, kk + kBK, tid, va);
+            gemm_load_b<WR, WC, BK, KCONTIG>(g, c0, kk + BK, tid, vb);
+            gemm_load_a<WR, WC, BK>(g, r0, kk + BK, tid, va);
         }
-        bf16x8 ah[4], al[4], bh[4], bl[4];
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            const int oa = (wr + q * 16 + l16) * kLDK + 8 * lq, ob = (wc + q * 16 + l16) * kLDK + 8 * lq;
-            ah[q] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const u16x8 *>(&sA[0][oa]));
-            al[q] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const u16x8 *>(&sA[1][oa]));
-            bh[q] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const u16x8 *>(&sB[0][ob]));
-            bl[q] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const u16x8 *>(&sB[1][ob]));
+        for (int ks = 0; ks < BK / 32; ++ks) {
+            bf16x8 ah[4], al[4], bh[4], bl[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int oa = (wr + q * 16 + l16) * kLDK + 32 * ks + 8 * lq;
+                const int ob = (wc + q * 16 + l16) * kLDK + 32 * ks + 8 * lq;
+                ah[q] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const u16x8 *>(&sA[0][oa]));
+                al[q] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const u16x8 *>(&sA[1][oa]));
+                bh[q] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const u16x8 *>(&sB[0][ob]));
+                bl[q] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const u16x8 *>(&sB[1][ob]));
+            }
+            // ---- 4 x 4 blocks of 16 x 16, three products each (small terms
+            // first), 16 independent accumulators between dependent MFMAs
+#pragma unroll
+            for (int rb = 0; rb < 4; ++rb)
+#pragma unroll
+                for (int cb = 0; cb < 4; ++cb)
+                    acc[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[rb], bh[cb], acc[rb][cb], 0, 0, 0);
+#pragma unroll
+            for (int rb = 0; rb < 4; ++rb)
+#pragma unroll
+                for (int cb = 0; cb < 4; ++cb)
+                    acc[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[rb], bl[cb], acc[rb][cb], 0, 0, 0);
+#pragma unroll
+            for (int rb = 0; rb < 4; ++rb)
+#pragma unroll
+                for (int cb = 0; cb < 4; ++cb)
+                    acc[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[rb], bh[cb], acc[rb][cb], 0, 0, 0);
         }
-        // ---- 4 x 4 blocks of 16 x 16, three products each (small terms
-        // first), 16 independent accumulators between dependent MFMAs
-#pragma unroll
-        for (int rb = 0; rb < 4; ++rb)
-#pragma unroll
-            for (int cb = 0; cb < 4; ++cb)
-                acc[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[rb], bh[cb], acc[rb][cb], 0, 0, 0);
-#pragma unroll
-        for (int rb = 0; rb < 4; ++rb)
-#pragma unroll
-            for (int cb = 0; cb < 4; ++cb)
-                acc[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[rb], bl[cb], acc[rb][cb], 0, 0, 0);
-#pragma unroll
-        for (int rb = 0; rb < 4; ++rb)
-#pragma unroll
-            for (int cb = 0; cb < 4; ++cb)
-                acc[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[rb], bh[cb], acc[rb][cb], 0, 0, 0);
         if (more) {
             __syncthreads();  // every wave has read this step's tiles
-            gemm_store_b<WR, WC, KCONTIG>(g, c0, kk + kBK, tid, vb, sB[0], sB[1]);
-            gemm_store_a<WR, WC>(tid, va, sA[0], sA[1]);
+            gemm_store_b<WR, WC, BK, KCONTIG>(g, c0, kk + BK, tid, vb, sB[0], sB[1]);
+            gemm_store_a<WR, WC, BK>(tid, va, sA[0], sA[1]);
             __syncthreads();
         }
     }
@@ -285,12 +294,12 @@ __global__ __launch_bounds__(64 * WR * WC) void k_gemm_bf16x3(GemmArgs g, int ti
 
 void check();
 
-template <int WR, int WC, bool KCONTIG, bool RCONTIG>
+template <int WR, int WC, int BK, bool KCONTIG, bool RCONTIG>
 void gemm(Handle &h, int kid, const GemmArgs &g) {
     ProfScope ps(h, kid);
     constexpr int BM = 64 * WR, BN = 64 * WC;
     // the loaders' assumptions (see k_gemm_bf16x3)
-    const bool ok = g.Kp % kBK == 0 && g.Kp >= g.K &&
+    const bool ok = g.Kp % BK == 0 && g.Kp >= g.K &&
                     g.C >= 4 && g.K >= 1 &&
                     (KCONTIG ? (g.Kp <= g.ldb && g.ldb % 4 == 0) : (g.C % 4 == 0 && g.ldb % 4 == 0 && g.ldb >= g.C));
     if (!ok) {
@@ -299,9 +308,263 @@ void gemm(Handle &h, int kid, const GemmArgs &g) {
     }
     const int tr = (g.R + BM - 1) / BM, tc = (g.C + BN - 1) / BN;
     const int nblk = tr * tc;
-    hipLaunchKernelGGL((k_gemm_bf16x3<WR, WC, KCONTIG, RCONTIG>), dim3(nblk), dim3(64 * WR * WC), 0, h.stream, g, tr,
+    hipLaunchKernelGGL((k_gemm_bf16x3<WR, WC, BK, KCONTIG, RCONTIG>), dim3(nblk), dim3(64 * WR * WC), 0, h.stream, g, tr,
                        nblk);
     check();
+}
+
+// ------------------------------------------------------------------------
+// x transform as two short DFTs on the matrix cores (N = N1 N2, Makhoul's
+// reordering, k1 <= N1/2 only; the same factorisation as the FP64 kernels),
+// each DFT a small GEMM with split-BF16x3 products accumulated in FP32.
+// Forward, step A: [Re, Im of B[k1][n2]] = WA [n1 x (line, n2)], K = N1 <= 32
+//   (one K step); step C: [Re, Im of V[k1 + N1 k2]] = WC [(n2, re/im) x
+//   (line, k1)], K = 2 N2 <= 64 (two K steps).  Inverse: the conjugate DFT
+//   over k2 (step C'), then the real output over (k1, re/im) with the
+//   Hermitian weights (step A').
+// Operand tables (host, lane order): per (row tile, K step) and lane l the
+// BF16 hi and lo of A[l%16][8(l/16) + e], e < 8.  In a paired table a tile
+// row R is (Re, Im) = R % 2 of index 8 rt + 2 (R/4) + (R%4)/2, so that a
+// lane's four accumulator rows 4(l/16) + r hold (Re, Im) of two indices.
+// ------------------------------------------------------------------------
+struct XArgs {
+    int N, N1, N2, K1h, NP, C1, C, M;
+    const u16x8 *fA, *fC, *iC, *iA;  // [(rt * KS + ks) * 64 + lane][hi, lo]
+    const float2 *wN, *rot;          // e^{-2 pi i q/N}, e^{-i pi k/(2N)}
+    float scale;                     // inverse: 1/(N M)
+};
+
+constexpr int kXL = 2;  // lines per block (even)
+
+__device__ __forceinline__ float2 cmulf(float2 a, float2 b) {
+    return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ float2 cmulcf(float2 a, float2 b) {  // a * conj(b)
+    return make_float2(a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y);
+}
+
+// split 8 FP32 values into BF16 hi / lo fragments
+__device__ __forceinline__ void split8(const float (&x)[8], bf16x8 &hi, bf16x8 &lo) {
+    u16x8 h, l;
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+        const unsigned short b = bf16_bits(x[e]);
+        h[e] = b;
+        l[e] = bf16_bits(x[e] - bf16_float(b));
+    }
+    hi = __builtin_bit_cast(bf16x8, h);
+    lo = __builtin_bit_cast(bf16x8, l);
+}
+
+// a b with a = a_hi + a_lo, b = b_hi + b_lo as the three split-BF16 products
+// a_lo b_hi + a_hi b_lo + a_hi b_hi (small terms first), FP32 accumulation
+__device__ __forceinline__ f32x4 mfma_split(const u16x8 *A, bf16x8 bh, bf16x8 bl, f32x4 acc) {
+    const bf16x8 ah = __builtin_bit_cast(bf16x8, A[0]), al = __builtin_bit_cast(bf16x8, A[1]);
+    acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh, acc, 0, 0, 0);
+    acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl, acc, 0, 0, 0);
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh, acc, 0, 0, 0);
+}
+
+// forward: the pseudo-pressure rhs (projection.f90:38-47) of kXL lines
+// (cells i = 1..N of row j, plane m; line c = j-1 + NM (m-1)) formed in FP64,
+// its FP64 line sums for the (0,0) mode (part[(j-1) M + m-1]), then the
+// transform into T1[c NP + kx]
+__global__ __launch_bounds__(256) void k_xfwd_mx(Dims d, Metrics mt, XArgs a, const float *__restrict__ U,
+                                                 const float *__restrict__ V, const float *__restrict__ W,
+                                                 float *__restrict__ out, double *__restrict__ part) {
+    extern __shared__ float smx[];
+    const int N = a.N, N1 = a.N1, N2 = a.N2, K1 = a.K1h + 1;
+    float *v = smx;                                         // kXL*N
+    float2 *Bc = reinterpret_cast<float2 *>(v + kXL * N);   // kXL*K1*N2 (8-byte aligned: kXL*N even)
+    __shared__ double lsum[kXL][4];
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63, q = lane >> 4, c16 = lane & 15;
+    const int line0 = blockIdx.x * kXL;
+    double ps[kXL];
+#pragma unroll
+    for (int l = 0; l < kXL; ++l) ps[l] = 0.0;
+#pragma unroll
+    for (int l = 0; l < kXL; ++l) {
+        const int c = min(line0 + l, a.C - 1);
+        const int c1 = c % a.C1, c2 = c / a.C1;
+        const long long base = fidx(d, 1, c1 + 1, c2 + 1);
+        const double ry = mt.rdy[c1 + 1], rz = mt.rdz[c2 + 1];
+        for (int n = tid; n < N; n += 256) {
+            const long long p = base + n;
+            const double dv = ((double)U[p] - (double)U[p - 1]) * mt.rdx[n + 1] +
+                              ((double)V[p] - (double)V[p - d.sx]) * ry + ((double)W[p] - (double)W[p - d.sxy]) * rz;
+            ps[l] += dv;
+            v[l * N + ((n & 1) ? N - 1 - (n >> 1) : (n >> 1))] = (float)dv;
+        }
+    }
+#pragma unroll
+    for (int l = 0; l < kXL; ++l) {
+        double t = ps[l];
+        for (int o = 32; o > 0; o >>= 1) t += __shfl_down(t, o);
+        if (lane == 0) lsum[l][wave] = t;
+    }
+    __syncthreads();
+    if (tid < kXL && line0 + tid < a.C) {
+        const int c = line0 + tid, c1 = c % a.C1, c2 = c / a.C1;
+        part[(long long)c1 * a.M + c2] = (lsum[tid][0] + lsum[tid][1]) + (lsum[tid][2] + lsum[tid][3]);
+    }
+    // ---- step A: wave w owns column tiles (line, n2) kXL/2 w .. kXL/2 (w+1)
+    for (int ct = wave * (kXL / 2); ct < (wave + 1) * (kXL / 2); ++ct) {
+        const int col = ct * 16 + c16;
+        const int ln = col / N2, n2 = col - ln * N2;
+        const bool cok = col < kXL * N2;
+        float x[8];
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            const int n1 = 8 * q + e;
+            x[e] = (cok && n1 < N1) ? v[ln * N + N2 * n1 + n2] : 0.f;
+        }
+        bf16x8 bh, bl;
+        split8(x, bh, bl);
+        const f32x4 z = {0.f, 0.f, 0.f, 0.f};
+        const f32x4 acc0 = mfma_split(a.fA + (0 * 64 + lane) * 2, bh, bl, z);
+        const f32x4 acc1 = mfma_split(a.fA + (1 * 64 + lane) * 2, bh, bl, z);
+        if (cok) {
+#pragma unroll
+            for (int h = 0; h < 4; ++h) {  // (row tile, pair): k1 = 8 (h/2) + 2 q + h%2
+                const f32x4 &acc = (h < 2) ? acc0 : acc1;
+                const int k1 = 8 * (h / 2) + 2 * q + (h % 2);
+                if (k1 < K1)
+                    Bc[(ln * K1 + k1) * N2 + n2] =
+                        cmulf(make_float2(acc[2 * (h % 2)], acc[2 * (h % 2) + 1]), a.wN[(n2 * k1) % N]);
+            }
+        }
+    }
+    __syncthreads();
+    // ---- step C: wave w owns row tile w (k2 = 8w + 2q, 8w + 2q + 1), all kXL
+    // column tiles (line, k1)
+    for (int ct = 0; ct < kXL; ++ct) {
+        const int col = ct * 16 + c16;
+        const int ln = col / K1, k1 = col - ln * K1;
+        const bool cok = col < kXL * K1;
+        const float *Bl = reinterpret_cast<const float *>(Bc + (ln * K1 + k1) * N2);
+        f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int ks = 0; ks < 2; ++ks) {
+            float x[8];
+#pragma unroll
+            for (int e = 0; e < 8; ++e) {
+                const int kk = 32 * ks + 8 * q + e;  // (n2, re/im) = (kk/2, kk%2)
+                x[e] = (cok && (kk >> 1) < N2) ? Bl[kk] : 0.f;
+            }
+            bf16x8 bh, bl;
+            split8(x, bh, bl);
+            acc = mfma_split(a.fC + ((wave * 2 + ks) * 64 + lane) * 2, bh, bl, acc);
+        }
+        if (cok) {
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                const int k2 = 8 * wave + 2 * q + h;
+                if (k2 < N2) {
+                    const int k = k1 + N1 * k2;
+                    const float2 wv = cmulf(a.rot[k], make_float2(acc[2 * h], acc[2 * h + 1]));
+                    v[ln * N + k] = 2.f * wv.x;
+                    if (k > 0) v[ln * N + N - k] = -2.f * wv.y;
+                }
+            }
+        }
+    }
+    __syncthreads();
+    for (int idx = tid; idx < kXL * N; idx += 256) {
+        const int l = idx / N, kx = idx - l * N;
+        const int c = line0 + l;
+        if (c < a.C) out[(long long)c * a.NP + kx] = v[idx];
+    }
+}
+
+// inverse: T1 lines (c NP + kx) -> phi cells (1..N, j, m) plus the (0,0)
+// constant add[j-1] of each y row
+__global__ __launch_bounds__(256) void k_xinv_mx(Dims d, XArgs a, const float *__restrict__ in,
+                                                 const double *__restrict__ add, float *__restrict__ phi) {
+    extern __shared__ float smx[];
+    const int N = a.N, N1 = a.N1, N2 = a.N2, K1h = a.K1h, K1 = K1h + 1;
+    float2 *Vs = reinterpret_cast<float2 *>(smx);  // kXL*K1*N2, later the output lines (kXL*N floats)
+    float2 *Cs = Vs + kXL * K1 * N2;               // kXL*K1*N2
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63, q = lane >> 4, c16 = lane & 15;
+    const int line0 = blockIdx.x * kXL;
+    for (int idx = tid; idx < kXL * K1 * N2; idx += 256) {
+        const int l = idx / (K1 * N2), r = idx - l * K1 * N2;
+        const int k1 = r / N2, k2 = r - k1 * N2;
+        const int k = k1 + N1 * k2;
+        const int c = min(line0 + l, a.C - 1);
+        const float *gl = in + (long long)c * a.NP;
+        const float gk = gl[k], gm = (k == 0) ? 0.f : gl[N - k];
+        Vs[idx] = cmulcf(make_float2(0.5f * gk, -0.5f * gm), a.rot[k]);
+    }
+    __syncthreads();
+    // ---- step C': wave w owns row tile w (n2 = 8w + 2q, +1), all kXL column
+    // tiles (line, k1)
+    for (int ct = 0; ct < kXL; ++ct) {
+        const int col = ct * 16 + c16;
+        const int ln = col / K1, k1 = col - ln * K1;
+        const bool cok = col < kXL * K1;
+        const float *Vl = reinterpret_cast<const float *>(Vs + (ln * K1 + k1) * N2);
+        f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int ks = 0; ks < 2; ++ks) {
+            float x[8];
+#pragma unroll
+            for (int e = 0; e < 8; ++e) {
+                const int kk = 32 * ks + 8 * q + e;  // (k2, re/im)
+                x[e] = (cok && (kk >> 1) < N2) ? Vl[kk] : 0.f;
+            }
+            bf16x8 bh, bl;
+            split8(x, bh, bl);
+            acc = mfma_split(a.iC + ((wave * 2 + ks) * 64 + lane) * 2, bh, bl, acc);
+        }
+        if (cok) {
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                const int n2 = 8 * wave + 2 * q + h;
+                if (n2 < N2)
+                    Cs[(ln * K1 + k1) * N2 + n2] =
+                        cmulcf(make_float2(acc[2 * h], acc[2 * h + 1]), a.wN[(n2 * k1) % N]);
+            }
+        }
+    }
+    __syncthreads();
+    // ---- step A': wave w owns column tiles (line, n2) kXL/2 w .. kXL/2 (w+1),
+    // both row tiles (n1 = 16 rt + 4 q + r)
+    float *o = smx;  // kXL*N floats inside the Vs area, which is no longer read
+    for (int ct = wave * (kXL / 2); ct < (wave + 1) * (kXL / 2); ++ct) {
+        const int col = ct * 16 + c16;
+        const int ln = col / N2, n2 = col - ln * N2;
+        const bool cok = col < kXL * N2;
+        float x[8];
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            const int kk = 8 * q + e, k1 = kk >> 1;  // (k1, re/im)
+            x[e] = (cok && k1 < K1) ? reinterpret_cast<const float *>(Cs + (ln * K1 + k1) * N2 + n2)[kk & 1] : 0.f;
+        }
+        bf16x8 bh, bl;
+        split8(x, bh, bl);
+        const f32x4 z = {0.f, 0.f, 0.f, 0.f};
+        const f32x4 acc0 = mfma_split(a.iA + (0 * 64 + lane) * 2, bh, bl, z);
+        const f32x4 acc1 = mfma_split(a.iA + (1 * 64 + lane) * 2, bh, bl, z);
+        if (cok) {
+#pragma unroll
+            for (int r = 0; r < 8; ++r) {
+                const int n1 = 16 * (r / 4) + 4 * q + (r % 4);
+                if (n1 < N1) {
+                    const int n = N2 * n1 + n2;
+                    const int pos = (n < (N + 1) / 2) ? 2 * n : 2 * (N - 1 - n) + 1;
+                    o[ln * N + pos] = (r < 4 ? acc0[r % 4] : acc1[r % 4]) * a.scale;
+                }
+            }
+        }
+    }
+    __syncthreads();
+    for (int idx = tid; idx < kXL * N; idx += 256) {
+        const int l = idx / N, n = idx - l * N;
+        const int c = line0 + l;
+        if (c >= a.C) continue;
+        const int c1 = c % a.C1, c2 = c / a.C1;
+        phi[fidx(d, n + 1, c1 + 1, c2 + 1)] = o[idx] + (float)add[c1];
+    }
 }
 
 // ------------------------------------------------------------------------
@@ -480,9 +743,10 @@ void split_matrix(int rows, int K, int Rp, int Kp, F f, std::vector<uint16_t> &h
         }
 }
 
-// x: 128 x 128 tiles (812 rows -> 7 row tiles); z: 256 x 128 tiles, all
-// 191 rows in one row tile so each B column block is read once
-constexpr int kWRx = 2, kWCx = 2, kWRz = 4, kWCz = 2;
+// x: 128 x 128 tiles (812 rows -> 7 row tiles), K step 32 (two waves per
+// SIMD); z: 256 x 128 tiles, all 191 rows in one row tile so each B column
+// block is read once, K step 64 (110.6 KB of LDS, one block per CU)
+constexpr int kWRx = 2, kWCx = 2, kBKx = 32, kWRz = 4, kWCz = 2, kBKz = 64;
 constexpr int kBMx = 64 * kWRx, kBMz = 64 * kWRz;
 
 }  // namespace
@@ -496,8 +760,8 @@ void build_mixed(Handle &h) {
     }
     const long double pi = 3.141592653589793238462643383279502884L;
     h.NP = (N + 31) / 32 * 32;
-    h.kpx = (N + kBK - 1) / kBK * kBK;
-    h.kpz = (M + kBK - 1) / kBK * kBK;
+    h.kpx = (N + kBKx - 1) / kBKx * kBKx;
+    h.kpz = (M + kBKz - 1) / kBKz * kBKz;
     const int rpx = (N + kBMx - 1) / kBMx * kBMx, rpz = (M + kBMz - 1) / kBMz * kBMz;
     // exact integer argument reduction before the long double cosine
     auto cosx = [&](int k, int n) {  // cos(pi k (2n+1) / (2N))
@@ -538,8 +802,96 @@ void build_mixed(Handle &h) {
         *dst[q] = h.mx_tab + off;
         off += t[q].size();
     }
-    // FP64 (0,0) mode scratch: plane partials of the divergence, then phi00/(2NM)
-    h.np00 = ((N + 63) / 64) * M;
+    // two-factor x transform (k_xfwd_mx / k_xinv_mx) when N = N1 N2 fits it
+    {
+        // N = N1 N2 with N1 the largest divisor <= sqrt(N), as for the FP64 tables
+        int n1 = 1;
+        for (int a = 1; (long long)a * a <= N; ++a)
+            if (N % a == 0) n1 = a;
+        h.dct_n1 = n1;
+        const int n2 = N / n1, k1n = n1 / 2 + 1;
+        h.mx_two = n1 <= 32 && k1n <= 16 && n2 <= 32;
+        if (h.mx_two) {
+            auto cis = [&](long long num, long long den) {  // e^{-2 pi i num/den}, exact reduction
+                const long double ang = -2.0L * pi * (long double)(num % den) / (long double)den;
+                return std::pair<long double, long double>(cosl(ang), sinl(ang));
+            };
+            // one table: tiles x ks x 64 lanes x (hi, lo) x 8 k values
+            auto frag = [&](int tiles, int ks_n, auto f) {
+                std::vector<uint16_t> t((size_t)tiles * ks_n * 64 * 16, 0);
+                for (int rt = 0; rt < tiles; ++rt)
+                    for (int ks = 0; ks < ks_n; ++ks)
+                        for (int l = 0; l < 64; ++l)
+                            for (int e = 0; e < 8; ++e) {
+                                const float v = (float)f(rt, l % 16, 32 * ks + 8 * (l / 16) + e);
+                                const uint16_t hb = host_bf16(v);
+                                const size_t o = (((size_t)rt * ks_n + ks) * 64 + l) * 16;
+                                t[o + e] = hb;
+                                t[o + 8 + e] = host_bf16(v - host_bf16_float(hb));
+                            }
+                return t;
+            };
+            auto pair_row = [](int rt, int R, int &idx, int &part) {
+                idx = 8 * rt + 2 * (R / 4) + (R % 4) / 2;
+                part = R % 2;
+            };
+            std::vector<uint16_t> fA = frag(2, 1, [&](int rt, int R, int k) -> long double {
+                int k1, part;
+                pair_row(rt, R, k1, part);
+                if (k1 >= k1n || k >= n1) return 0.0L;
+                const auto w = cis((long long)k * k1, n1);
+                return part == 0 ? w.first : w.second;
+            });
+            std::vector<uint16_t> fC = frag(4, 2, [&](int rt, int R, int k) -> long double {
+                int k2, part;
+                pair_row(rt, R, k2, part);
+                const int nn = k / 2, pin = k % 2;
+                if (k2 >= n2 || nn >= n2) return 0.0L;
+                const auto w = cis((long long)nn * k2, n2);
+                return part == 0 ? (pin == 0 ? w.first : -w.second) : (pin == 0 ? w.second : w.first);
+            });
+            std::vector<uint16_t> iC = frag(4, 2, [&](int rt, int R, int k) -> long double {
+                int nn, part;
+                pair_row(rt, R, nn, part);
+                const int k2 = k / 2, pin = k % 2;
+                if (nn >= n2 || k2 >= n2) return 0.0L;
+                const auto w = cis((long long)nn * k2, n2);
+                return part == 0 ? (pin == 0 ? w.first : w.second) : (pin == 0 ? -w.second : w.first);
+            });
+            std::vector<uint16_t> iA = frag(2, 1, [&](int rt, int R, int k) -> long double {
+                const int nn1 = 16 * rt + R, k1 = k / 2, pin = k % 2;
+                if (nn1 >= n1 || k1 >= k1n) return 0.0L;
+                const long double c = (k1 == 0 || (n1 % 2 == 0 && k1 == n1 / 2)) ? 1.0L : 2.0L;
+                const auto w = cis((long long)nn1 * k1, n1);
+                return c * (pin == 0 ? w.first : w.second);
+            });
+            std::vector<float> tw(4 * (size_t)N);
+            for (int q = 0; q < N; ++q) {
+                const auto w = cis(q, N), r = cis(q, 4LL * N);
+                tw[2 * q] = (float)w.first;
+                tw[2 * q + 1] = (float)w.second;
+                tw[2 * N + 2 * q] = (float)r.first;
+                tw[2 * N + 2 * q + 1] = (float)r.second;
+            }
+            const size_t nb = (fA.size() + fC.size() + iC.size() + iA.size()) * sizeof(uint16_t) + tw.size() * 4;
+            BL_HIP(hipMalloc(&h.mx2, nb));
+            char *base = reinterpret_cast<char *>(h.mx2);
+            size_t off = 0;
+            const void **dst[4] = {(const void **)&h.x2_fA, (const void **)&h.x2_fC, (const void **)&h.x2_iC,
+                                   (const void **)&h.x2_iA};
+            std::vector<uint16_t> *src[4] = {&fA, &fC, &iC, &iA};
+            for (int t = 0; t < 4; ++t) {
+                BL_HIP(hipMemcpyAsync(base + off, src[t]->data(), src[t]->size() * 2, hipMemcpyHostToDevice, h.stream));
+                *dst[t] = base + off;
+                off += src[t]->size() * 2;
+            }
+            BL_HIP(hipMemcpyAsync(base + off, tw.data(), tw.size() * 4, hipMemcpyHostToDevice, h.stream));
+            h.x2_tw = reinterpret_cast<const float *>(base + off);
+        }
+    }
+    // FP64 (0,0) mode scratch: per-line (two-factor path) or per-block plane
+    // partials of the divergence, then phi00/(2NM)
+    h.np00 = h.mx_two ? M : ((N + 63) / 64) * M;
     BL_HIP(hipMalloc(&h.p00, sizeof(double) * ((size_t)h.np00 * d.NM + d.NM)));
     h.c00 = h.p00 + (size_t)h.np00 * d.NM;
     BL_HIP(hipStreamSynchronize(h.stream));
@@ -550,8 +902,35 @@ void launch_poisson_mixed(Handle &h, float *phi, const Fld3<float> &f) {
     const int N = d.N, M = d.M, NM = d.NM, NP = h.NP;
     float *S = reinterpret_cast<float *>(h.S);
     float *T1 = reinterpret_cast<float *>(h.T1);
+    XArgs xa{};
+    if (h.mx_two) {
+        xa.N = N;
+        xa.N1 = h.dct_n1;
+        xa.N2 = N / h.dct_n1;
+        xa.K1h = h.dct_n1 / 2;
+        xa.NP = NP;
+        xa.C1 = NM;
+        xa.C = NM * M;
+        xa.M = M;
+        xa.fA = reinterpret_cast<const u16x8 *>(h.x2_fA);
+        xa.fC = reinterpret_cast<const u16x8 *>(h.x2_fC);
+        xa.iC = reinterpret_cast<const u16x8 *>(h.x2_iC);
+        xa.iA = reinterpret_cast<const u16x8 *>(h.x2_iA);
+        xa.wN = reinterpret_cast<const float2 *>(h.x2_tw);
+        xa.rot = reinterpret_cast<const float2 *>(h.x2_tw) + N;
+        xa.scale = (float)(1.0 / ((double)N * (double)M));
+    }
+    const int K1 = h.dct_n1 / 2 + 1, N2 = N / h.dct_n1;
+    const dim3 xgrid((NM * M + kXL - 1) / kXL);
+    if (h.mx_two) {
+        // divergence, FP64 line sums and forward x transform in one kernel
+        ProfScope ps(h, KID_XFORM_X_FWD);
+        const size_t lds = sizeof(float) * (size_t)kXL * N + sizeof(float2) * (size_t)kXL * K1 * N2;
+        hipLaunchKernelGGL(k_xfwd_mx, xgrid, dim3(256), lds, h.stream, d, h.m, xa, f.u, f.v, f.w, T1, h.p00);
+        check();
+    }
     // rhs: divergence lines into S (as scratch), FP64 plane partials
-    {
+    if (!h.mx_two) {
         ProfScope ps(h, KID_DIVERGENCE);
         hipLaunchKernelGGL(k_div_mixed, dim3((N + 63) / 64, (NM + 3) / 4, M), dim3(256), 0, h.stream, d, h.m,
                            f.u, f.v, f.w, NP, S, h.p00);
@@ -573,7 +952,7 @@ void launch_poisson_mixed(Handle &h, float *phi, const Fld3<float> &f) {
     g.o_c2 = 0;
     g.o_C1 = g.C;
     g.add = nullptr;
-    gemm<kWRx, kWCx, true, true>(h, KID_XFORM_X_FWD, g);
+    if (!h.mx_two) gemm<kWRx, kWCx, kBKx, true, true>(h, KID_XFORM_X_FWD, g);
     // forward z: S[q][col] = sum_m Zf[q][m] T1[m][col], col = j*NP + kx
     GemmArgs z{};
     z.R = M;
@@ -587,7 +966,7 @@ void launch_poisson_mixed(Handle &h, float *phi, const Fld3<float> &f) {
     z.out = S;
     z.ldc = (long long)NM * NP;
     z.o_C1 = 1;
-    gemm<kWRz, kWCz, false, false>(h, KID_XFORM_Z_FWD, z);
+    gemm<kWRz, kWCz, kBKz, false, false>(h, KID_XFORM_Z_FWD, z);
     // FP64 Thomas on the spectrum, FP64 (0,0) mode
     {
         ProfScope ps(h, KID_THOMAS);
@@ -604,7 +983,7 @@ void launch_poisson_mixed(Handle &h, float *phi, const Fld3<float> &f) {
     z.Alo = h.zi_lo;
     z.B = S;
     z.out = T1;
-    gemm<kWRz, kWCz, false, false>(h, KID_XFORM_Z_INV, z);
+    gemm<kWRz, kWCz, kBKz, false, false>(h, KID_XFORM_Z_INV, z);
     // inverse x into the field layout, plus the (0,0) constant of each y row
     g.Ahi = h.dxi_hi;
     g.Alo = h.dxi_lo;
@@ -615,7 +994,14 @@ void launch_poisson_mixed(Handle &h, float *phi, const Fld3<float> &f) {
     g.o_c1 = d.sx;
     g.o_c2 = d.sxy;
     g.add = h.c00;
-    gemm<kWRx, kWCx, true, true>(h, KID_XFORM_X_INV, g);
+    if (h.mx_two) {
+        ProfScope ps(h, KID_XFORM_X_INV);
+        const size_t lds = 2 * sizeof(float2) * (size_t)kXL * K1 * N2;
+        hipLaunchKernelGGL(k_xinv_mx, xgrid, dim3(256), lds, h.stream, d, xa, T1, h.c00, phi);
+        check();
+    } else {
+        gemm<kWRx, kWCx, kBKx, true, true>(h, KID_XFORM_X_INV, g);
+    }
 }
 
 }  // namespace blgpu

