@@ -164,20 +164,38 @@ def roofline_of(prof, nx, ny, nz):
     return roof
 
 
-def gemm_mfma(prof, nx, ny, nz):
-    """MFMA rate of the mixed-precision x-transform GEMM: BF16 flops issued
-    (three products per term, padded tile extents) per launch / launch time."""
-    name = "xform_x_fwd"
-    if name not in prof or prof[name][1] == 0:
-        return None
+def dct_factor(N):
+    """N = N1 N2 with N1 the largest divisor <= sqrt(N) (the library's choice)."""
+    n1 = max(a for a in range(1, int(N ** 0.5) + 1) if N % a == 0)
+    return n1, N // n1
+
+
+def mixed_mfma(prof, nx, ny, nz):
+    """MFMA rates of the mixed path's transform kernels: BF16 flops issued
+    (16x16x32 MFMAs x 16384 flop, three per split product, padded tiles) per
+    launch / launch time.  x: two short DFTs per line pair (k_xfwd_mx, 2 lines
+    per block: step A 4 column tiles x 2 row tiles x 1 K step, step C 4 row
+    tiles x 2 column tiles x 2 K steps; the divergence is fused in); z: the
+    dense 191-point DFT GEMM (256 x 128 tiles, K padded to 64)."""
     N, M, NM = nx - 2, nz - 2, ny - 1
-    rp, kp = -(-N // 128) * 128, -(-N // 32) * 32
-    cols = -(-(NM * M) // 128) * 128
-    issued = 3 * 2.0 * rp * kp * cols
-    avg_s = prof[name][0] / prof[name][1] * 1e-3
-    ach = issued / avg_s / 1e12
-    return dict(kernel=name, bf16_tflops_issued=ach, peak=BF16_PEAK_TFLOPS, frac=ach / BF16_PEAK_TFLOPS,
-                avg_launch_ms=prof[name][0] / prof[name][1])
+    NP = -(-N // 32) * 32
+    out = {}
+    n1, n2 = dct_factor(N)
+    if n1 <= 32 and n1 // 2 + 1 <= 16 and n2 <= 32:
+        blocks = -(-(NM * M) // 2)
+        mfmas = blocks * (4 * 2 * 1 + 4 * 2 * 2) * 3
+        x_issued = mfmas * 16384.0
+    else:
+        x_issued = 3 * 2.0 * (-(-N // 128) * 128) * NP * (-(-(NM * M) // 128) * 128)
+    z_issued = 3 * 2.0 * 256 * (-(-M // 64) * 64) * (-(-(NM * NP) // 128) * 128)
+    for name, issued in (("xform_x_fwd", x_issued), ("xform_z_fwd", z_issued)):
+        if name not in prof or prof[name][1] == 0:
+            continue
+        avg_s = prof[name][0] / prof[name][1] * 1e-3
+        ach = issued / avg_s / 1e12
+        out[name] = dict(bf16_tflops_issued=ach, peak=BF16_PEAK_TFLOPS, frac=ach / BF16_PEAK_TFLOPS,
+                         avg_launch_ms=prof[name][0] / prof[name][1])
+    return out
 
 
 def les_leg(blgpu, args, log):
@@ -345,7 +363,7 @@ def main():
         mixed_out = dict(value=v_mx, unit="ns/cell/step", ms_per_step=rmx["el"] / args.steps * 1e3,
                          speedup_vs_fp64=value / v_mx,
                          rel_l2_vs_fp64_after_steps=err, steps_compared=args.warmup + args.steps,
-                         max_divergence=rmx["div"], mfma=gemm_mfma(rmx["prof"], nx, ny, nz),
+                         max_divergence=rmx["div"], mfma=mixed_mfma(rmx["prof"], nx, ny, nz),
                          kernels={k: dict(ms_per_step=v[0] / args.steps, launches_per_step=v[1] / args.steps)
                                   for k, v in rmx["prof"].items() if v[1] > 0})
         del rmx
