@@ -20,6 +20,11 @@ timed on this host for one RK3 step of a z-slab of the same workload.
 MFMA transforms, FP64 Thomas / mass flux / statistics): its ns/cell/step, the
 MFMA rate of its x-transform GEMM, and the relative L2 difference of u, v, w,
 p and Cf from the FP64 run after the same warm-up + timed steps.
+`les` is the learned-closure LES against dynamic Smagorinsky (both in the
+mixed-precision form) on an LES grid that a DNS refines by (4, 2, 4): the
+a-priori SGS-stress errors of the trained network and the other models on a
+held-out filtered snapshot, the ns/cell/step of both LES, their Cf against
+the DNS and against each other, and their mean U(y) against the filtered DNS.
 """
 import argparse
 import json
@@ -244,7 +249,8 @@ def les_leg(blgpu, args, log):
     t_start = judge.t
     runs = {}
     for name, les in (("dynamic_smagorinsky", 2), ("learned", 4)):
-        s = blgpu.Solver(xc, yc, zc, NU, tab_c, les=les, cfl=CFL)
+        # the LES runs in the mixed-precision form (FP32 fields)
+        s = blgpu.Solver(xc, yc, zc, NU, tab_c, les=les, cfl=CFL, precision=blgpu.PREC_MIXED)
         s.set_fields(Uc, Vc, Wc)
         s.t = t_start
         if les == 4:
@@ -265,6 +271,12 @@ def les_leg(blgpu, args, log):
         fine.compute_dt()
         fine.step()
     cf_dns = fine.wall_stats()[:: fac[0]]
+    # the DNS filtered onto the LES grid at the same time (into the judge
+    # solver), for the mean streamwise-velocity profile U(y) averaged over
+    # the interior x stations and the z planes
+    T.collect(fine, judge, c, fac, d99, snapshots=1, every=0, smin=1e-3 / THETA)
+    prof = lambda U: U[1:-1, 1:-1, 1:-2].mean(axis=(0, 2))
+    u_dns = prof(judge.fields()[0])
     cells_c = nc[0] * nc[1] * nc[2]
     out = dict(grid_les=list(nc), grid_dns=[xf.size, yf.size, zf.size], les_steps=args.les_steps,
                time_span=t_end - t_start, dns_time_matched=fine.t,
@@ -278,13 +290,20 @@ def les_leg(blgpu, args, log):
         inner = slice(2, -2)
         err = float(np.linalg.norm(cf[inner] - cf_dns[inner]) / np.linalg.norm(cf_dns[inner]))
         nt = args.les_steps - 1
+        u_les = prof(s.fields()[0])
         d = dict(ns_per_cell_step=r["el"] / nt / cells_c * 1e9, ms_per_step=r["el"] / nt * 1e3,
-                 cf_rel_l2_vs_dns=err, dt=r["dt"])
+                 cf_rel_l2_vs_dns=err,
+                 mean_u_profile_rel_l2_vs_filtered_dns=float(np.linalg.norm(u_les - u_dns) / np.linalg.norm(u_dns)),
+                 dt=r["dt"])
+        r["cf"] = cf
         if name == "learned":
             d["fallback_cells"], d["cells"] = s.closure_fallback()[1], s.closure_fallback()[0]
         out[name] = d
         s.close()
     out["cost_ratio_learned_over_dsm"] = out["learned"]["ns_per_cell_step"] / out["dynamic_smagorinsky"]["ns_per_cell_step"]
+    cl, cd = runs["learned"]["cf"][2:-2], runs["dynamic_smagorinsky"]["cf"][2:-2]
+    out["cf_rel_l2_learned_vs_dsm"] = float(np.linalg.norm(cl - cd) / np.linalg.norm(cd))
+    out["precision"] = "mixed (FP32 fields)"
     c.close()
     judge.close()
     fine.close()

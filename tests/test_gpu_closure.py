@@ -281,6 +281,40 @@ def test_les_closure_nu_t(nsigma):
     c.close()
 
 
+def test_les_closure_nu_t_mixed():
+    """LES = 4 in the mixed-precision solver (FP32 fields): the network's
+    eddy viscosity from the FP32 field, against the FP32 network evaluated on
+    the same field's features (no fallback)."""
+    fine, coarse = fine_and_coarse(seed=23)
+    xc, yc, zc = coarse
+    sf, sc = solvers(fine, coarse)
+    c = blgpu.Closure(capacity=100000, batch_max=2048)
+    d99 = 0.01
+    c.samples_from(sf, sc, FX, FY, FZ, d99, smin=1e-3)
+    c.params = M.init_params(4).astype(np.float32)
+    c.train(50, 512, lr=1e-3, seed=3)
+    Uc, Vc, Wc = [a.astype(np.float32).astype(np.float64) for a in sc.fields()]
+    eta, f, df = cases.load_blasius()
+    nu = 4.898844e-6
+    sm = blgpu.Solver(xc, yc, zc, nu, orc.blasius_tables(xc, yc, nu, eta, f, df), les=4,
+                      precision=blgpu.PREC_MIXED)
+    sm.set_fields(Uc, Vc, Wc)
+    c.install(sm, d99, 1e30)
+    sm.eddy_viscosity()
+    got = sm.download(4)[1:-1, 1:-1, 1:-1]
+    feat, S, smag, scale = R.features(xc, yc, zc, Uc, Vc, Wc, d99)
+    X = feat.reshape(10, -1, order="F").T.astype(np.float32)
+    y = c.forward(X).reshape(feat.shape[1:], order="F")
+    nu_net = np.maximum(y, 0) * scale
+    cells, fb = sm.closure_fallback()
+    assert fb == 0
+    e = np.abs(got - nu_net).max() / np.abs(nu_net).max()
+    say("mixed LES=4 nu_t vs fp32 network", e)
+    assert e < 5e-2
+    sm.close()
+    c.close()
+
+
 def test_trained_network_gains_a_priori():
     """Train on filtered snapshots of a small noisy-Blasius DNS, judge on a
     held-out snapshot: the trained network's SGS-stress error is below the
