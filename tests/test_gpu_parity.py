@@ -161,3 +161,44 @@ def test_laminar_blasius_cf():
     ratio = cf[3:-5] / (0.664 / np.sqrt(rex[3:-5]))
     assert np.abs(ratio - 1).max() < 5e-3, ratio
     assert s.max_divergence() < 1e-12 / np.diff(y).min()
+
+
+def test_cfg1_laminar_2000_steps():
+    """The laminar Blasius configuration (192x32x8, Lx = 100 theta,
+    Ly = 10 theta, Lz = 4 theta, tanh alpha = 2, RK3, CFL = 0.5, DNS) for 2000
+    steps from the Blasius field extended in x, against the oracle after the
+    same steps: u, v, w, p, the divergence after every step of the last
+    hundred, and Cf against 0.664/sqrt(Re_x) away from the ends."""
+    nu = 4.898844e-6
+    theta = 1.469653e-3
+    x, y, z = blgpu.grid_tanh(192, 32, 8, 100 * theta, 10 * theta, 4 * theta, 2.0, 1.0)
+    eta, f, df = blgpu.blasius_profile()
+    t = blgpu.blasius_tables(x, y, nu, eta, f, df)
+    U, V, W = blgpu.blasius_field(x, y, z, nu, eta, f, df)
+    s = blgpu.Solver(x, y, z, nu, t, scheme=3, cfl=0.5)
+    s.set_fields(U, V, W)
+    o = orc.Oracle(x, y, z, nu, 0.0, 0.0, 3, 0, 0.5)
+    o.set_bc_tables(t)
+    o.set_fields(U, V, W)
+    dmax = 0.0
+    for n in range(2000):
+        o.compute_dt()
+        o.step()
+        s.compute_dt()
+        s.step()
+        if n >= 1900:
+            dmax = max(dmax, s.max_divergence())
+    assert s.t == o.t
+    for c, (a, b) in enumerate(zip(s.fields(), o.fields())):
+        e = rel_l2(a, b)
+        print("cfg1 field", c, e, flush=True)
+        assert e < 1e-11, c
+    ep = rel_l2(s.download(blgpu.FIELD_P), o.get(3))
+    print("cfg1 P", ep, "max div", dmax, flush=True)
+    assert ep < 1e-9
+    assert dmax < 1e-12 / np.diff(y).min()
+    cf = s.wall_stats()
+    rex = x / nu
+    ratio = cf[3:-5] / (0.664 / np.sqrt(rex[3:-5]))  # ends excluded as in the reference postprocess
+    print("cfg1 Cf / Blasius", ratio.min(), ratio.max(), flush=True)
+    assert np.abs(ratio - 1).max() < 5e-3
