@@ -439,17 +439,27 @@ __device__ __forceinline__ long long phi_at(const Dims &d, int i, int j, int k) 
     return fidx(d, i, j, k);
 }
 
+// a thread walks kProjZ planes of one (i, j) column, so phi(k+1) of a plane
+// is phi(k) of the next
+constexpr int kProjZ = 4;
+
 template <class T>
 __global__ __launch_bounds__(256) void k_project(Dims d, MetricsT<T> m, T *U, T *V, T *W, const T *phi) {
     const int i = blockIdx.x * kWave + threadIdx.x + 1;
     const int j = blockIdx.y * 4 + threadIdx.y + 1;
-    const int k = blockIdx.z + 1;
+    const int k0 = blockIdx.z * kProjZ + 1;
     if (i > d.nxg - 2 || j > d.nyg - 2) return;
-    const long long p = fidx(d, i, j, k);
-    const T pc = phi[phi_at(d, i, j, k)];
-    if (i <= d.nx - 2) U[p] = U[p] - (phi[phi_at(d, i + 1, j, k)] - pc) * m.rdxg[i];
-    if (j <= d.ny - 2) V[p] = V[p] - (phi[phi_at(d, i, j + 1, k)] - pc) * m.rdyg[j];
-    if (k <= d.nz - 2) W[p] = W[p] - (phi[phi_at(d, i, j, k + 1)] - pc) * m.rdzg[k];
+    const int k1 = min(k0 + kProjZ, d.nzg - 1);  // exclusive
+    const T rdxg = m.rdxg[i], rdyg = m.rdyg[j];
+    T pc = phi[phi_at(d, i, j, k0)];
+    for (int k = k0; k < k1; ++k) {
+        const long long p = fidx(d, i, j, k);
+        const T pk = phi[phi_at(d, i, j, k + 1)];
+        if (i <= d.nx - 2) U[p] = U[p] - (phi[phi_at(d, i + 1, j, k)] - pc) * rdxg;
+        if (j <= d.ny - 2) V[p] = V[p] - (phi[phi_at(d, i, j + 1, k)] - pc) * rdyg;
+        if (k <= d.nz - 2) W[p] = W[p] - (pk - pc) * m.rdzg[k];
+        pc = pk;
+    }
 }
 
 template <class T>
@@ -1430,8 +1440,8 @@ template <class T>
 void launch_project(Handle &h, Fld3<T> &f, const T *phi) {
     const Dims &d = h.d;
     ProfScope ps(h, KID_PROJECT);
-    hipLaunchKernelGGL(k_project<T>, interior_grid(d), dim3(kWave, 4), 0, h.stream, d, metrics<T>(h), f.u, f.v,
-                       f.w, phi);
+    const dim3 grid((d.nxg - 2 + kWave - 1) / kWave, (d.nyg - 2 + 3) / 4, (d.nzg - 2 + kProjZ - 1) / kProjZ);
+    hipLaunchKernelGGL(k_project<T>, grid, dim3(kWave, 4), 0, h.stream, d, metrics<T>(h), f.u, f.v, f.w, phi);
     check_launch();
 }
 
