@@ -38,15 +38,16 @@ __device__ __forceinline__ T sq(T a) {
 // planes k-1, k, k+1 live in an LDS ring with a one-cell halo in x and y, so
 // every velocity value is read from HBM once per chunk (plus halo) instead of
 // once per neighbour.  Loads are x-fastest and 512-byte aligned rows.
-// Thread (tx, ty) computes cells (i0+tx, j0+RPT ty+r), r < RPT: RPT = 2 in
-// FP64 (47.5 KB of LDS per block), 4 in FP32 (taller tiles: less y halo and
-// twice the loads in flight per barrier at the same LDS size).
+// Thread (tx, ty) computes cells (i0+tx, j0+RPT ty+r), r < RPT.  FP64: 64 x 8
+// tiles (RPT = 2, 47.5 KB of LDS); FP32: 64 x 16 tiles (RPT = 4: less y halo
+// and twice the loads in flight per barrier at a similar LDS size).  128-wide
+// FP32 tiles (fewer x-halo lines) measured slower.
 // ------------------------------------------------------------------------
-constexpr int kTX = 64, kHX = kTX + 2;
 template <class T>
 struct MarchTile {
-    static constexpr int RPT = sizeof(T) == 4 ? 4 : 2;  // rows per thread
-    static constexpr int TY = 4 * RPT, HY = TY + 2, Plane = kHX * HY;
+    static constexpr int TX = 64;                          // tile width (x)
+    static constexpr int RPT = sizeof(T) == 4 ? 4 : 2;    // rows per thread
+    static constexpr int TY = (256 / TX) * RPT, HX = TX + 2, HY = TY + 2, Plane = HX * HY;
 };
 
 // LES: the viscous terms use nu + nu_t (equations.f90:97-132, 231-270,
@@ -61,10 +62,11 @@ __global__ __launch_bounds__(256) void k_rk_march(
     T *__restrict__ Un, T *__restrict__ Vn, T *__restrict__ Wn, T *__restrict__ Fou,
     T *__restrict__ Fov, T *__restrict__ Fow, T c0, T c1, T dt) {
     constexpr int RPT = MarchTile<T>::RPT, kTY = MarchTile<T>::TY, kPlane = MarchTile<T>::Plane;
+    constexpr int kTX = MarchTile<T>::TX, kHX = MarchTile<T>::HX;
     __shared__ T sU[3][kPlane], sV[3][kPlane], sW[3][kPlane];
     __shared__ T sN[LES ? 3 : 1][LES ? kPlane : 1];
     const int tid = threadIdx.x;
-    const int tx = tid & 63, ty = tid >> 6;
+    const int tx = tid % kTX, ty = tid / kTX;
     const int i0 = blockIdx.x * kTX + 1, j0 = blockIdx.y * kTY + 1;
     const int kb = 1 + blockIdx.z * kchunk;
     const int ke = min(kb + kchunk, d.nzg - 1);  // exclusive
@@ -1304,7 +1306,7 @@ const MetricsT<float> &metrics<float>(const Handle &h) {
 
 template <class T>
 dim3 march_grid(const Dims &d) {
-    constexpr int kTY = MarchTile<T>::TY;
+    constexpr int kTX = MarchTile<T>::TX, kTY = MarchTile<T>::TY;
     return dim3((d.nxg - 2 + kTX - 1) / kTX, (d.nyg - 2 + kTY - 1) / kTY, (d.nzg - 2 + kChunk - 1) / kChunk);
 }
 
