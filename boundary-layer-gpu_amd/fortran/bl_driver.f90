@@ -102,6 +102,9 @@ module bl_state
   integer :: nx, ny, nz, nxg, nyg, nzg, nxm, nym, nzm
   integer :: nsteps = 0, nsave = 1, nstats = 1, nmonitor = 1, itime_step = 3, les_model = 0
   integer :: nut_bc = 0   ! Dirichlet_nu_t (initialization.f90:586, params.f90:156)
+  ! optional key of this build (absent from the reference's files): 0 = FP64,
+  ! 1 = mixed precision (FP32 fields, split-BF16x3 transforms, FP64 Thomas)
+  integer :: iprecision = 0
   integer :: iwall_model = 0, inflow_flag = 1, top_flag = 0, random_init = 1
   integer :: nstep_init = 0, nstep_init_input = -45, istep = 0
   real(real64) :: nu = 0d0, cfl = -1d-4, dPdx = 0d0, dPdz = 0d0, t = 0d0, dt = 0d0
@@ -170,6 +173,7 @@ contains
     call get_dbl('dPdz', dPdz, f)
     call get_int('LES', les_model, f)
     call get_int('nutBC', nut_bc, f)
+    call get_int('precision', iprecision, f)
     call get_int('WM', iwall_model, f)
     call get_int('init_step', nstep_init_input, f)
     call get_int('init_rand', random_init, f)
@@ -188,6 +192,7 @@ contains
     if (itime_step /= 2 .and. itime_step /= 3) stop 'Error: only RKscheme = 2 or 3 is built'
     if (iwall_model /= 0) stop 'Error: only WM = 0 is built'
     if (les_model /= 0 .and. les_model /= 2) stop 'Error: only LES = 0 or 2 is built'
+    if (iprecision /= 0 .and. iprecision /= 1) stop 'Error: precision must be 0 (FP64) or 1 (mixed)'
     if (inflow_flag == 3 .or. inflow_flag == 5) stop 'Error: turbulent (Lund) inflow is not built'
   end subroutine read_parameters
 
@@ -577,7 +582,7 @@ program bl_driver
   cfg%nx = nx; cfg%ny = ny; cfg%nz = nz
   cfg%x = c_loc(x); cfg%y = c_loc(y); cfg%z = c_loc(z)
   cfg%nu = nu; cfg%dPdx = dPdx; cfg%dPdz = dPdz; cfg%cfl = cfl
-  cfg%scheme = itime_step; cfg%les = les_model; cfg%precision = BL_PREC_FP64
+  cfg%scheme = itime_step; cfg%les = les_model; cfg%precision = merge(BL_PREC_MIXED, BL_PREC_FP64, iprecision == 1)
   cfg%nut_bc = nut_bc
   cfg%U_inlet = c_loc(U_inlet); cfg%V_inlet = c_loc(V_inlet); cfg%W_inlet = c_loc(W_inlet)
   cfg%U_top = c_loc(U_top); cfg%V_top = c_loc(V_top); cfg%W_top = c_loc(W_top)

@@ -116,6 +116,28 @@ def test_driver_fresh_start_matches_oracle(tmp_path):
 
 @pytest.mark.gpu
 @needs_driver
+def test_driver_mixed_precision(tmp_path):
+    """The optional `precision = 1` key runs the mixed-precision path: the
+    snapshot after 20 steps agrees with the FP64 oracle to FP32 accuracy."""
+    d = str(tmp_path)
+    write_case(d, precision="1")
+    r = run_driver(d)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    case = Case(40, 20, 8, 0.2, 0.02, 0.01, 1e-5, alpha=3.0, grid="sinh", scheme=3, cfl=-2e-4)
+    o = case.oracle()
+    for istep in range(1, 21):
+        o.compute_dt()
+        o.step()
+    snap = read_snapshot(os.path.join(d, "data", "BL_lam.00000020"))
+    U = o.fields()[0]
+    for name, ref in zip("UVW", o.fields()):
+        e = np.linalg.norm(snap[name] - ref[:, :, :-1]) / np.linalg.norm(U[:, :, :-1])
+        print("driver mixed", name, e, flush=True)
+        assert e < 1e-4, name
+
+
+@pytest.mark.gpu
+@needs_driver
 def test_driver_restart_rk2_temporal_modes(tmp_path):
     """Restart from a full-plane input file (init_rand = 0), step taken from
     the file (init_step = -45), RK2, temporal inflow modes read from the
@@ -155,6 +177,7 @@ def test_driver_requires_input_file(tmp_path):
     ("WM", "1", "only WM = 0"),
     ("LES", "1", "only LES = 0 or 2"),
     ("inflow_flag", "3", "Lund"),
+    ("precision", "2", "precision must be 0"),
 ])
 def test_driver_rejects_unbuilt_options(tmp_path, key, val, msg):
     d = str(tmp_path)
