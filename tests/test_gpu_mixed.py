@@ -45,6 +45,8 @@ CASES = {
     "tanh_even": dict(nx=24, ny=16, nz=8, noise=0.01, seed=1),
     "tanh_odd": dict(nx=37, ny=21, nz=11, noise=0.01, seed=2),
     "modes": dict(nx=40, ny=24, nz=12, Ly=0.02, Lz=0.1083, modes=True, noise=0.002, seed=4),
+    # N = 47 (prime, longer than the two-factor kernels take): dense x GEMM
+    "prime_x": dict(nx=49, ny=12, nz=9, noise=0.01, seed=5),
 }
 
 
