@@ -504,11 +504,16 @@ void build(Handle &h, const bl_config *c) {
         kzz[k] = 2.0 * (std::cos(2.0 * kPiD * (double)k / (double)M) - 1.0) / std::pow(g.dzmin, 2.0);
     std::vector<double> dd, dlo, dup;
     build_dyy(g, d.nyg, dd, dlo, dup);
+    // partitioned kernel for NM <= 32 * kThMaxP rows: row coefficients only
+    h.th_p = (NM + kThSeg - 1) / kThSeg;
+    if (h.th_p > kThMaxP) h.th_p = 0;
     const size_t plane = (size_t)N * NM;
-    std::vector<double> fdl(plane * Kz, 0.0), fdp(plane * Kz, 0.0);
+    const bool tables = h.th_p == 0;
+    std::vector<double> fdl(tables ? plane * Kz : 0, 0.0), fdp(tables ? plane * Kz : 0, 0.0);
+    std::vector<double> lu00(2 * (size_t)NM, 0.0);
     std::vector<double> piv(NM), low(NM);
-    for (int kk = 0; kk < Kz; ++kk)
-        for (int kx = 0; kx < N; ++kx) {
+    for (int kk = 0; kk < (tables ? Kz : 1); ++kk)
+        for (int kx = 0; kx < (tables ? N : 1); ++kx) {
             for (int j = 0; j < NM; ++j) piv[j] = dd[j] + kxx[kx] + kzz[kk];
             for (int j = 0; j < NM - 1; ++j) low[j] = dlo[j];
             if (kx == 0 && kk == 0) piv[0] = 3.0 / 2.0 * piv[0];
@@ -517,16 +522,38 @@ void build(Handle &h, const bl_config *c) {
                 low[j - 1] = m;
                 piv[j] = piv[j] - m * dup[j - 1];
             }
-            for (int j = 0; j < NM; ++j) {
-                fdp[(size_t)kk * plane + (size_t)j * N + kx] = 1.0 / piv[j];
-                if (j < NM - 1) fdl[(size_t)kk * plane + (size_t)j * N + kx] = low[j];
-            }
+            if (kx == 0 && kk == 0)
+                for (int j = 0; j < NM; ++j) {
+                    lu00[NM + j] = 1.0 / piv[j];
+                    if (j < NM - 1) lu00[j] = low[j];
+                }
+            if (tables)
+                for (int j = 0; j < NM; ++j) {
+                    fdp[(size_t)kk * plane + (size_t)j * N + kx] = 1.0 / piv[j];
+                    if (j < NM - 1) fdl[(size_t)kk * plane + (size_t)j * N + kx] = low[j];
+                }
         }
-    h.lu_dl = dmalloc(fdl.size());
-    h.lu_dp = dmalloc(fdp.size());
+    if (tables) {
+        h.lu_dl = dmalloc(fdl.size());
+        h.lu_dp = dmalloc(fdp.size());
+        upload_vec(h.lu_dl, fdl, h.stream);
+        upload_vec(h.lu_dp, fdp, h.stream);
+    } else {
+        const int L = kThSeg * h.th_p;
+        std::vector<double> co(3 * (size_t)L + N + Kz, 0.0);
+        for (int j = 0; j < L; ++j) {
+            co[j] = (j >= 1 && j < NM) ? dlo[j - 1] : 0.0;
+            co[L + j] = j < NM ? dd[j] : 1.0;
+            co[2 * L + j] = j < NM - 1 ? dup[j] : 0.0;
+        }
+        for (int i = 0; i < N; ++i) co[3 * L + i] = kxx[i];
+        for (int k = 0; k < Kz; ++k) co[3 * L + N + k] = kzz[k];
+        h.th_co = dmalloc(co.size());
+        upload_vec(h.th_co, co, h.stream);
+    }
+    h.lu00 = dmalloc(lu00.size());
+    upload_vec(h.lu00, lu00, h.stream);
     h.lu_du = dmalloc(NM);
-    upload_vec(h.lu_dl, fdl, h.stream);
-    upload_vec(h.lu_dp, fdp, h.stream);
     upload_vec(h.lu_du, dup, h.stream);
 
     h.scal = dmalloc(64);
@@ -553,6 +580,7 @@ void free_all(Handle &h) {
     for (void *p : {(void *)h.F1.u, (void *)h.F1.v, (void *)h.F1.w, (void *)h.P, (void *)h.phi,
                     (void *)h.nu_t, (void *)h.T1, (void *)h.S, (void *)h.dct_w1, (void *)h.dct_w2,
                     (void *)h.dct_wN, (void *)h.dct_rot, (void *)h.dct_mf, h.mx2, (void *)h.zc, (void *)h.zs, (void *)h.lu_dl, (void *)h.lu_dp, (void *)h.lu_du,
+                    (void *)h.lu00, (void *)h.th_co,
                     (void *)h.scal, (void *)h.red, (void *)h.dsm, h.mem_grid, h.mem_gridf, (void *)h.mx_tab,
                     (void *)h.p00, (void *)h.cl_w, (void *)h.cl_b, (void *)h.cl_cnt})
         f(p);

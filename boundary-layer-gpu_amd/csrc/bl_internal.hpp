@@ -27,6 +27,46 @@ __host__ __device__ inline long long fidx(const Dims &d, int i, int j, int k) {
     return (long long)i + (long long)d.sx * ((long long)j + (long long)d.sy * k);
 }
 
+// XCD-aware tile order: the T tiles of a launch are cut into 8 contiguous
+// runs, one per XCD (the hardware deals consecutive block ids round-robin
+// over the 8 XCDs), so that tiles sharing halo lines share an L2.  The launch
+// has xcd_blocks(T) blocks; a block whose tile is >= T returns at once.
+__host__ __device__ inline int xcd_blocks(int T) { return 8 * ((T + 7) / 8); }
+__host__ __device__ inline int xcd_tile(int b, int T) { return (b % 8) * ((T + 7) / 8) + b / 8; }
+
+// Partitioned Thomas solve: rows per segment (held in registers) and
+// segments per system, so systems up to NM = 256 rows take that kernel.
+constexpr int kThSeg = 32, kThMaxP = 8;
+
+// Block order of the Thomas sweeps (nkb kx blocks times M spectral rows, one
+// block id b each).  Rows q = 2k-1 and 2k (real and imaginary parts of z
+// wavenumber k) read the same factors, so their blocks for one kx range go
+// to one XCD back to back: the hardware deals consecutive block ids
+// round-robin over 8 XCDs, and b = 16 g + s pairs with b + 8 (s < 8).  The
+// unpaired rows (q = 0, and q = M-1 when M is even) come last.
+__host__ __device__ inline void thomas_block(int b, int nkb, int M, int &kxb, int &q) {
+    const int P = nkb * ((M - 1) / 2);  // (kx block, k) pairs
+    if (b < 2 * P) {
+        const int G = P / 8;
+        int p, member;
+        if (b < 16 * G) {
+            const int s = b % 16;
+            p = 8 * (b / 16) + s % 8;
+            member = s / 8;
+        } else {  // the last, partial group: its pairs one after the other
+            const int s = b - 16 * G;
+            p = 8 * G + s / 2;
+            member = s % 2;
+        }
+        kxb = p % nkb;
+        q = 2 * (p / nkb + 1) - 1 + member;
+    } else {
+        const int s = b - 2 * P;
+        kxb = s % nkb;
+        q = s < nkb ? 0 : M - 1;
+    }
+}
+
 // Metric arrays (0-based, reciprocal spacings) living on the device, in the
 // arithmetic type of the kernels that read them.
 template <class T>
@@ -124,7 +164,10 @@ struct Handle {
     bool dct_mfma = false;                       // x-DCT on the FP64 matrix cores (shape fits)
     double *dct_mf = nullptr;                    // their operand tables: fA, fC, iC, iA
     size_t dct_mf_off[4] = {0, 0, 0, 0};
-    double *lu_dl, *lu_dp, *lu_du;  // Thomas factors [k][j][kx], [k][j][kx], [j]
+    double *lu_dl, *lu_dp, *lu_du;  // Thomas factors [k][j][kx], [k][j][kx], [j] (table kernel only)
+    double *lu00;                   // the pinned (0,0) system: multipliers [NM], reciprocal pivots [NM]
+    int th_p = 0;                   // segments of the partitioned Thomas kernel (0: table kernel)
+    double *th_co = nullptr;        // its coefficients: a, b, c [32 th_p] each, kxx [N], kzz [Kz]
     double *scal;                   // device scalars (mass-flux correction, reductions)
     double *red;                    // reduction scratch
     void *mem_grid;                 // one allocation for metrics and tables
@@ -169,6 +212,14 @@ void launch_bc(Handle &h, Fld3<T> &f, const Fld3<T> &old, int rk_step, double dt
 void launch_poisson(Handle &h, double *phi_inout, const Field3 *div = nullptr);
 template <class T>
 void launch_project(Handle &h, Fld3<T> &f, const T *phi);
+// the y solves of all spectral systems S[q][j][kx] (row stride ldj, q stride
+// ldq) by the partitioned kernel; false if the handle uses the table kernel.
+// The (0,0) system is left untouched (zero00 = false) or zeroed.
+template <class T>
+bool launch_thomas_seg(Handle &h, T *S, int ldj, long long ldq, bool zero00);
+// the pinned (0,0) system of x (row stride ldj) in place, times scale (NM <= kMax00)
+constexpr int kMax00 = 1024;
+void launch_solve00(Handle &h, double *x, int ldj, double scale);
 template <class T>
 void launch_pressure(Handle &h, const T *phi, T *P, double dt);
 template <class T>

@@ -594,10 +594,9 @@ __global__ __launch_bounds__(256) void k_div_mixed(Dims d, Metrics m, const floa
 
 // FP64 (0,0) mode, in two kernels.  k_sum00: rhs(j) = 2 sum over the x-z
 // plane (DCT-II x2 at kx = 0, DFT at kz = 0) of the divergence partials, one
-// block per y row, fixed summation order.  k_solve00: Thomas with the
-// kx = kz = 0 factors (pinned row), run by one lane from LDS copies of the
-// factor columns, and the constant phi00(j) / (2 N M) that the inverse
-// transforms would add.
+// block per y row, fixed summation order.  Then the pinned kx = kz = 0
+// system (launch_solve00) turns it into the constant phi00(j) / (2 N M) that
+// the inverse transforms would add.
 __global__ __launch_bounds__(256) void k_sum00(const double *__restrict__ part, int np,
                                                double *__restrict__ r00) {
     __shared__ double w[4];
@@ -611,47 +610,18 @@ __global__ __launch_bounds__(256) void k_sum00(const double *__restrict__ part, 
     if (threadIdx.x == 0) r00[j] = 2.0 * ((w[0] + w[1]) + (w[2] + w[3]));
 }
 
-__global__ __launch_bounds__(256) void k_solve00(Dims d, const double *__restrict__ dl,
-                                                 const double *__restrict__ dp,
-                                                 const double *__restrict__ du, double *__restrict__ c00) {
-    __shared__ double r[1024], L[1024], D[1024], Uu[1024];
-    const int NM = d.NM, N = d.N;
-    for (int j = threadIdx.x; j < NM; j += 256) {
-        r[j] = c00[j];
-        L[j] = j > 0 ? dl[(long long)(j - 1) * N] : 0.0;
-        D[j] = dp[(long long)j * N];
-        Uu[j] = du[j];
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        double x = r[0];
-        for (int j = 1; j < NM; ++j) {
-            x = r[j] - L[j] * x;
-            r[j] = x;
-        }
-        x = x * D[NM - 1];
-        r[NM - 1] = x;
-        for (int j = NM - 2; j >= 0; --j) {
-            x = (r[j] - Uu[j] * x) * D[j];
-            r[j] = x;
-        }
-    }
-    __syncthreads();
-    const double sc = 1.0 / (2.0 * (double)N * (double)d.M);
-    for (int j = threadIdx.x; j < NM; j += 256) c00[j] = r[j] * sc;
-}
-
 // Thomas sweep in FP64 on the FP32 spectrum (kx stride NP): the forward
 // values are kept in FP64 in X ([q][j][kx], kx stride N) for the back
-// substitution.  The (0,0) system is left to k_solve00 and zeroed here.
+// substitution.  The (0,0) system is zeroed here (launch_solve00 solves it).
 template <int TJ>
 __global__ __launch_bounds__(256) void k_thomas_mixed(Dims d, int NP, const double *__restrict__ dl,
                                                       const double *__restrict__ dp,
                                                       const double *__restrict__ du, float *S,
                                                       double *__restrict__ X) {
-    const int kx = blockIdx.x * blockDim.x + threadIdx.x;
-    const int q = blockIdx.y;
     const int N = d.N, NM = d.NM;
+    int kxb, q;
+    thomas_block(blockIdx.x, (N + 255) / 256, d.M, kxb, q);
+    const int kx = kxb * 256 + threadIdx.x;
     if (kx >= N) return;
     const long long sp = (long long)NP * NM, fp = (long long)N * NM;
     float *a = S + q * sp + kx;
@@ -754,7 +724,7 @@ constexpr int kBMx = 64 * kWRx, kBMz = 64 * kWRz;
 void build_mixed(Handle &h) {
     const Dims &d = h.d;
     const int N = d.N, M = d.M;
-    if (d.NM > 1024) {
+    if (d.NM > kMax00) {
         last_error() = "bl_create: the mixed-precision path supports ny <= 1025";
         throw ArgError();
     }
@@ -970,13 +940,14 @@ void launch_poisson_mixed(Handle &h, float *phi, const Fld3<float> &f) {
     // FP64 Thomas on the spectrum, FP64 (0,0) mode
     {
         ProfScope ps(h, KID_THOMAS);
-        hipLaunchKernelGGL(k_thomas_mixed<8>, dim3((N + 255) / 256, M), dim3(256), 0, h.stream, d, NP, h.lu_dl,
-                           h.lu_dp, h.lu_du, S, h.T1);
-        check();
+        if (!launch_thomas_seg<float>(h, S, NP, (long long)NP * NM, true)) {
+            hipLaunchKernelGGL(k_thomas_mixed<8>, dim3((N + 255) / 256 * M), dim3(256), 0, h.stream, d, NP,
+                               h.lu_dl, h.lu_dp, h.lu_du, S, h.T1);
+            check();
+        }
         hipLaunchKernelGGL(k_sum00, dim3(NM), dim3(256), 0, h.stream, h.p00, h.np00, h.c00);
         check();
-        hipLaunchKernelGGL(k_solve00, dim3(1), dim3(256), 0, h.stream, d, h.lu_dl, h.lu_dp, h.lu_du, h.c00);
-        check();
+        launch_solve00(h, h.c00, 1, 1.0 / (2.0 * (double)N * (double)M));
     }
     // inverse z: T1[m][col] = sum_q Zi[m][q] S[q][col]
     z.Ahi = h.zi_hi;

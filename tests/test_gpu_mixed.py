@@ -131,6 +131,23 @@ def test_mixed_projection(case):
     assert abs(dg - do) / scale < 1e-6
 
 
+@pytest.mark.parametrize("ny", [201, 257, 301])
+def test_mixed_projection_tall(ny):
+    """the FP64 Thomas solves on y systems of 200, 256 (partitioned kernel)
+    and 300 rows (factor-table kernel) inside a mixed-precision projection"""
+    c = cases.laminar_small(nx=14, ny=ny, nz=7, noise=0.01, seed=12)
+    say("mixed tall", c.x.size, c.y.size, c.z.size)
+    o = c.oracle()
+    s = mixed_solver(c, o)
+    o.projection(2, 2e-4)
+    s.projection(2, 2e-4)
+    U = o.fields()[0]
+    for name, a, b in zip("UVW", s.fields(), o.fields()):
+        e = np.linalg.norm(a - b) / np.linalg.norm(U)
+        say("mixed tall projection", name, e)
+        assert e < 1e-5, name
+
+
 def test_mixed_pressure(case):
     """P recorded by a first sub-step projection (projection.f90:75-85) on a
     field both paths hold exactly (FP32-representable): the predictor of a

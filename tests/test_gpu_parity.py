@@ -76,6 +76,31 @@ def test_poisson_parity(case):
     assert rel_l2(pg, po) < 1e-12
 
 
+@pytest.mark.parametrize("ny,tol", [(201, 2e-11), (257, 2e-11), (301, 1e-12)])
+def test_poisson_parity_tall(ny, tol):
+    """y systems of 200, 256 and 300 rows: the partitioned Thomas kernel with
+    7 segments (the last one mostly padding) and with 8, and the factor-table
+    kernel past 256 rows.  On these grids the kx = 1, kz = 0 system is
+    ill-conditioned (diagonal from -4e7 to -1.6e10 against a shift of -512):
+    against an 80-bit solve, the oracle's FP64 Thomas elimination is off by
+    4.5e-12 (ny = 201) and 7.2e-12 (ny = 257) of max|x| in that mode, and the
+    partitioned elimination by 4.7e-12 and 6.5e-14.  Two FP64 eliminations of
+    the same system therefore agree to ~1e-11 there, not to 1e-12; the table
+    kernel repeats the oracle's elimination order and keeps 1e-12."""
+    c = cases.laminar_small(nx=14, ny=ny, nz=7, noise=0.01, seed=12)
+    say("tall poisson", c.x.size, c.y.size, c.z.size)
+    o = c.oracle()
+    s, _ = solver_for(c, o)
+    rng = np.random.default_rng(8)
+    rhs = np.zeros(o.shapes[3], order="F")
+    rhs[1:-2, 1:-1, 1:-2] = rng.standard_normal(rhs[1:-2, 1:-1, 1:-2].shape)
+    po = o.poisson(rhs)[1:-2, 1:-1, 1:-2]
+    pg = s.poisson(rhs)[1:-2, 1:-1, 1:-2]
+    e = rel_l2(pg, po)
+    say("tall poisson rel", e)
+    assert e < tol
+
+
 def test_projection_parity(case):
     o = case.oracle()
     s, _ = solver_for(case, o)
