@@ -495,18 +495,31 @@ __global__ __launch_bounds__(256) void k_pressure(Dims d, const T *phi, T *P, do
 // stab[m][k] = sin(2 pi k m/M), m, k < lda rows/cols (zero padded).
 // Columns are c = kx + N*j, contiguous; stride between z planes is `plane`.
 // ------------------------------------------------------------------------
-template <int RW>
+// A block is 256 columns (four waves of 64) times one group of RW output
+// rows, so that its four waves read the same table rows.  Tiles run in
+// XCD-contiguous order with the row group fastest: the row groups of one
+// column block follow each other on one XCD and share its L2 for the input.
+// The next PF inputs of a column are loaded ahead of their use.
+__device__ __forceinline__ void zdft_tile(int M, int C, int RW, int &k0, int &c0, bool &live) {
+    const int ng = (M / 2 + 1 + RW - 1) / RW, nt = ng * ((C + 255) / 256);
+    const int t = xcd_tile(blockIdx.x, nt);
+    live = t < nt;
+    k0 = (t % ng) * RW;
+    c0 = (t / ng) * 256;
+}
+
+template <int RW, int PF>
 __global__ __launch_bounds__(256) void k_zdft_fwd(int M, int C, long long plane, int lda,
                                                   const double *__restrict__ ctab,
                                                   const double *__restrict__ stab,
                                                   const double *__restrict__ in,
                                                   double *__restrict__ out) {
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int k0 = __builtin_amdgcn_readfirstlane((blockIdx.y * 4 + wave) * RW);
+    int k0, c0;
+    bool live;
+    zdft_tile(M, C, RW, k0, c0, live);
+    const int c = c0 + threadIdx.x;
+    if (!live || c >= C) return;
     const int Kc = M / 2 + 1;  // cos rows k = 0..M/2
-    if (k0 >= Kc) return;
-    const int c = blockIdx.x * 64 + lane;
-    if (c >= C) return;
     const double *src = in + c;
     const double t0 = src[0];
     double ac[RW], as[RW];
@@ -516,17 +529,34 @@ __global__ __launch_bounds__(256) void k_zdft_fwd(int M, int C, long long plane,
         as[r] = 0.0;
     }
     const int hm = M / 2;
-    for (int m = 1; m <= hm; ++m) {
-        const double tp = src[(long long)m * plane];
-        const double tm = src[(long long)(M - m) * plane];
-        const double e = (2 * m == M) ? tp : tp + tm;
-        const double o = tp - tm;  // zero for the middle element of even M
-        const double *cr = ctab + (long long)m * lda + k0;
-        const double *sr = stab + (long long)m * lda + k0;
+    if (hm >= 1) {
+        double rp[PF], rm[PF];
 #pragma unroll
-        for (int r = 0; r < RW; ++r) {
-            ac[r] = fma(e, cr[r], ac[r]);
-            as[r] = fma(o, sr[r], as[r]);
+        for (int u = 0; u < PF; ++u) {
+            const int m = min(1 + u, hm);
+            rp[u] = src[(long long)m * plane];
+            rm[u] = src[(long long)(M - m) * plane];
+        }
+        for (int m0 = 1; m0 <= hm; m0 += PF) {
+#pragma unroll
+            for (int u = 0; u < PF; ++u) {
+                const int m = m0 + u;
+                const double tp = rp[u], tm = rm[u];
+                const int mn = min(m + PF, hm);
+                rp[u] = src[(long long)mn * plane];
+                rm[u] = src[(long long)(M - mn) * plane];
+                if (m <= hm) {
+                    const double e = (2 * m == M) ? tp : tp + tm;
+                    const double o = tp - tm;  // zero for the middle element of even M
+                    const double *cr = ctab + (long long)m * lda + k0;
+                    const double *sr = stab + (long long)m * lda + k0;
+#pragma unroll
+                    for (int r = 0; r < RW; ++r) {
+                        ac[r] = fma(e, cr[r], ac[r]);
+                        as[r] = fma(o, sr[r], as[r]);
+                    }
+                }
+            }
         }
     }
     double *dst = out + c;
@@ -541,19 +571,19 @@ __global__ __launch_bounds__(256) void k_zdft_fwd(int M, int C, long long plane,
 
 // inverse: g[m] = S0 + sum_k c_k (Re_k cos - Im_k sin)(2 pi k m/M), c_k = 2
 // (1 for the Nyquist row of even M); outputs m and M-m come from the same
-// cos and sin sums.
-template <int RW>
+// cos and sin sums.  Row groups cover the outputs m = 0..M/2.
+template <int RW, int PF>
 __global__ __launch_bounds__(256) void k_zdft_inv(int M, int C, long long plane, int lda,
                                                   const double *__restrict__ ctab,
                                                   const double *__restrict__ stab,
                                                   const double *__restrict__ in,
                                                   double *__restrict__ out) {
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int m0 = __builtin_amdgcn_readfirstlane((blockIdx.y * 4 + wave) * RW);
+    int m0, c0;
+    bool live;
+    zdft_tile(M, C, RW, m0, c0, live);
+    const int c = c0 + threadIdx.x;
+    if (!live || c >= C) return;
     const int hm = M / 2;  // outputs m = 0..M/2 (paired with M-m)
-    if (m0 > hm) return;
-    const int c = blockIdx.x * 64 + lane;
-    if (c >= C) return;
     const double *src = in + c;
     const double s0 = src[0];
     double cc[RW], ss[RW];
@@ -563,19 +593,38 @@ __global__ __launch_bounds__(256) void k_zdft_inv(int M, int C, long long plane,
         ss[r] = 0.0;
     }
     const int Kc = M / 2 + 1;
-    for (int k = 1; k < Kc; ++k) {
-        const double re = src[(long long)(2 * k - 1) * plane];
-        const bool nyq = (2 * k == M);
-        const double im = nyq ? 0.0 : src[(long long)(2 * k) * plane];
-        const double cw = nyq ? re : 2.0 * re;
-        const double sw = 2.0 * im;
-        // tables indexed [k][m] here: ctab is symmetric in (k, m)
-        const double *cr = ctab + (long long)k * lda + m0;
-        const double *sr = stab + (long long)k * lda + m0;
+    if (Kc >= 2) {
+        // Re_k = row 2k-1, Im_k = row 2k (absent for the Nyquist row of even M)
+        double rr[PF], ri[PF];
 #pragma unroll
-        for (int r = 0; r < RW; ++r) {
-            cc[r] = fma(cw, cr[r], cc[r]);
-            ss[r] = fma(sw, sr[r], ss[r]);
+        for (int u = 0; u < PF; ++u) {
+            const int k = min(1 + u, Kc - 1);
+            rr[u] = src[(long long)(2 * k - 1) * plane];
+            ri[u] = src[(long long)min(2 * k, M - 1) * plane];
+        }
+        for (int kb = 1; kb < Kc; kb += PF) {
+#pragma unroll
+            for (int u = 0; u < PF; ++u) {
+                const int k = kb + u;
+                const double re = rr[u], imr = ri[u];
+                const int kn = min(k + PF, Kc - 1);
+                rr[u] = src[(long long)(2 * kn - 1) * plane];
+                ri[u] = src[(long long)min(2 * kn, M - 1) * plane];
+                if (k < Kc) {
+                    const bool nyq = (2 * k == M);
+                    const double im = nyq ? 0.0 : imr;
+                    const double cw = nyq ? re : 2.0 * re;
+                    const double sw = 2.0 * im;
+                    // tables indexed [k][m] here: ctab is symmetric in (k, m)
+                    const double *cr = ctab + (long long)k * lda + m0;
+                    const double *sr = stab + (long long)k * lda + m0;
+#pragma unroll
+                    for (int r = 0; r < RW; ++r) {
+                        cc[r] = fma(cw, cr[r], cc[r]);
+                        ss[r] = fma(sw, sr[r], ss[r]);
+                    }
+                }
+            }
         }
     }
     double *dst = out + c;
@@ -1405,7 +1454,7 @@ void check_launch() {
 }
 
 constexpr int kDctLines = 2;
-constexpr int kZRows = 16;
+constexpr int kZRows = 16, kZPre = 4;
 
 void zdft_launch(Handle &h, int kid, bool forward, const double *in, double *out) {
     ProfScope ps(h, kid);
@@ -1413,12 +1462,12 @@ void zdft_launch(Handle &h, int kid, bool forward, const double *in, double *out
     const int C = d.N * d.NM;
     const long long plane = (long long)C;
     const int rows = d.M / 2 + 1;
-    const dim3 grid((C + 63) / 64, (rows + 4 * kZRows - 1) / (4 * kZRows));
+    const dim3 grid(xcd_blocks((rows + kZRows - 1) / kZRows * ((C + 255) / 256)));
     if (forward)
-        hipLaunchKernelGGL(k_zdft_fwd<kZRows>, grid, dim3(256), 0, h.stream, d.M, C, plane, h.lda_zt,
+        hipLaunchKernelGGL((k_zdft_fwd<kZRows, kZPre>), grid, dim3(256), 0, h.stream, d.M, C, plane, h.lda_zt,
                            h.zc, h.zs, in, out);
     else
-        hipLaunchKernelGGL(k_zdft_inv<kZRows>, grid, dim3(256), 0, h.stream, d.M, C, plane, h.lda_zt,
+        hipLaunchKernelGGL((k_zdft_inv<kZRows, kZPre>), grid, dim3(256), 0, h.stream, d.M, C, plane, h.lda_zt,
                            h.zc, h.zs, in, out);
     check_launch();
 }
