@@ -839,7 +839,8 @@ __global__ __launch_bounds__(256) void k_dct_inv(DctArgs a, const double *__rest
 // lane per (row tile, k step)); shapes: N1 <= 32, N1/2 + 1 <= 16, N2 <= 32.
 // ------------------------------------------------------------------------
 typedef double dbl4 __attribute__((ext_vector_type(4)));
-constexpr int kMfL = 2;                   // lines per block (2 or 4)
+constexpr int kMfL = 2;                   // lines per block
+constexpr int kLd = 4;                    // loads per thread in flight in the line load phases
 constexpr int kMfKA = 8, kMfKC = 16;      // k steps: step A (N1 <= 32), step C (2 N2 <= 64)
 
 __device__ __forceinline__ dbl4 mfma64(double a, double b, dbl4 c) {
@@ -856,20 +857,41 @@ __global__ __launch_bounds__(256) void k_dct_fwd_m(DctArgs a, Dims d, Metrics mt
     double2 *Bc = reinterpret_cast<double2 *>(v + kMfL * N);  // kMfL*K1*N2
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63, q = lane >> 4, c16 = lane & 15;
     const int line0 = blockIdx.x * kMfL;
-    // load and reorder: v[n/2] = f[n] (n even), v[N-1-(n-1)/2] = f[n] (n odd)
-    for (int idx = tid; idx < kMfL * N; idx += 256) {
-        const int l = idx / N, n = idx - l * N;
+    // load and reorder: v[n/2] = f[n] (n even), v[N-1-(n-1)/2] = f[n] (n odd);
+    // kLd elements per thread are loaded before the first one is used
+    static_assert(kMfL == 2, "line index below assumes two lines per block");
+    long long lb[kMfL];
+    double ry[kMfL], rz[kMfL];
+#pragma unroll
+    for (int l = 0; l < kMfL; ++l) {
         const int c = min(line0 + l, a.C - 1);
         const int c1 = c % a.C1, c2 = c / a.C1;
-        const long long p = a.in_base + c1 * a.in_c1 + c2 * a.in_c2 + n;
-        double f;
-        if (DIV) {
-            f = (in[p] - in[p - 1]) * mt.rdx[n + 1] + (in2[p] - in2[p - d.sx]) * mt.rdy[c1 + 1] +
-                (in3[p] - in3[p - d.sxy]) * mt.rdz[c2 + 1];
-        } else {
-            f = in[p];
+        lb[l] = a.in_base + c1 * a.in_c1 + c2 * a.in_c2;
+        ry[l] = DIV ? mt.rdy[c1 + 1] : 0.0;
+        rz[l] = DIV ? mt.rdz[c2 + 1] : 0.0;
+    }
+    for (int i0 = tid; i0 < kMfL * N; i0 += 256 * kLd) {
+        double f[kLd];
+#pragma unroll
+        for (int u = 0; u < kLd; ++u) {
+            const int idx = min(i0 + 256 * u, kMfL * N - 1);
+            const int l = idx >= N ? 1 : 0, n = idx - l * N;
+            const long long p = lb[l] + n;
+            if (DIV) {
+                f[u] = (in[p] - in[p - 1]) * mt.rdx[n + 1] + (in2[p] - in2[p - d.sx]) * ry[l] +
+                       (in3[p] - in3[p - d.sxy]) * rz[l];
+            } else {
+                f[u] = in[p];
+            }
         }
-        v[l * N + ((n & 1) ? N - 1 - (n >> 1) : (n >> 1))] = f;
+#pragma unroll
+        for (int u = 0; u < kLd; ++u) {
+            const int idx = i0 + 256 * u;
+            if (idx < kMfL * N) {
+                const int l = idx >= N ? 1 : 0, n = idx - l * N;
+                v[l * N + ((n & 1) ? N - 1 - (n >> 1) : (n >> 1))] = f[u];
+            }
+        }
     }
     const int KSA = (N1 + 3) / 4, KSC = (2 * N2 + 3) / 4;
     double fa[2][kMfKA];
@@ -958,16 +980,36 @@ __global__ __launch_bounds__(256) void k_dct_inv_m(DctArgs a, const double *__re
     double2 *Cs = Vs + kMfL * K1 * N2;                // kMfL*K1*N2
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63, q = lane >> 4, c16 = lane & 15;
     const int line0 = blockIdx.x * kMfL;
-    // V[k1 + N1 k2] = conj(rot_k) (g[k] - i g[N-k]) / 2 for k1 <= N1/2: Vs[l][k1][k2]
-    for (int idx = tid; idx < kMfL * K1 * N2; idx += 256) {
-        const int l = idx / (K1 * N2), r = idx - l * K1 * N2;
-        const int k1 = r / N2, k2 = r - k1 * N2;
-        const int k = k1 + N1 * k2;
+    // V[k1 + N1 k2] = conj(rot_k) (g[k] - i g[N-k]) / 2 for k1 <= N1/2: Vs[l][k1][k2];
+    // kLd elements per thread are loaded before the first one is used
+    static_assert(kMfL == 2, "line index below assumes two lines per block");
+    const double *gls[kMfL];
+#pragma unroll
+    for (int l = 0; l < kMfL; ++l) {
         const int c = min(line0 + l, a.C - 1);
         const int c1 = c % a.C1, c2 = c / a.C1;
-        const double *gl = in + a.in_base + c1 * a.in_c1 + c2 * a.in_c2;
-        const double gk = gl[k], gm = (k == 0) ? 0.0 : gl[N - k];
-        Vs[idx] = cmulc(make_double2(0.5 * gk, -0.5 * gm), a.rot[k]);
+        gls[l] = in + a.in_base + c1 * a.in_c1 + c2 * a.in_c2;
+    }
+    const int nv = kMfL * K1 * N2;
+    for (int i0 = tid; i0 < nv; i0 += 256 * kLd) {
+        double gk[kLd], gm[kLd];
+        double2 rt[kLd];
+#pragma unroll
+        for (int u = 0; u < kLd; ++u) {
+            const int idx = min(i0 + 256 * u, nv - 1);
+            const int l = idx >= K1 * N2 ? 1 : 0, r = idx - l * K1 * N2;
+            const int k1 = r / N2, k2 = r - k1 * N2;
+            const int k = k1 + N1 * k2;
+            gk[u] = gls[l][k];
+            gm[u] = gls[l][k == 0 ? 0 : N - k];
+            rt[u] = a.rot[k];
+            if (k == 0) gm[u] = 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < kLd; ++u) {
+            const int idx = i0 + 256 * u;
+            if (idx < nv) Vs[idx] = cmulc(make_double2(0.5 * gk[u], -0.5 * gm[u]), rt[u]);
+        }
     }
     const int KSC = (2 * N2 + 3) / 4, KSA = (2 * K1 + 3) / 4;
     __syncthreads();

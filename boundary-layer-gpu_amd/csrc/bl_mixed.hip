@@ -334,7 +334,8 @@ struct XArgs {
     float scale;                     // inverse: 1/(N M)
 };
 
-constexpr int kXL = 2;  // lines per block (even)
+constexpr int kXL = 2;   // lines per block (even)
+constexpr int kLdx = 4;  // loads per thread in flight in the inverse's gather
 
 __device__ __forceinline__ float2 cmulf(float2 a, float2 b) {
     return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
@@ -486,14 +487,31 @@ __global__ __launch_bounds__(256) void k_xinv_mx(Dims d, XArgs a, const float *_
     float2 *Cs = Vs + kXL * K1 * N2;               // kXL*K1*N2
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63, q = lane >> 4, c16 = lane & 15;
     const int line0 = blockIdx.x * kXL;
-    for (int idx = tid; idx < kXL * K1 * N2; idx += 256) {
-        const int l = idx / (K1 * N2), r = idx - l * K1 * N2;
-        const int k1 = r / N2, k2 = r - k1 * N2;
-        const int k = k1 + N1 * k2;
-        const int c = min(line0 + l, a.C - 1);
-        const float *gl = in + (long long)c * a.NP;
-        const float gk = gl[k], gm = (k == 0) ? 0.f : gl[N - k];
-        Vs[idx] = cmulcf(make_float2(0.5f * gk, -0.5f * gm), a.rot[k]);
+    // kLdx elements per thread loaded before the first one is used
+    static_assert(kXL == 2, "line index below assumes two lines per block");
+    const float *gls[kXL];
+#pragma unroll
+    for (int l = 0; l < kXL; ++l) gls[l] = in + (long long)min(line0 + l, a.C - 1) * a.NP;
+    const int nv = kXL * K1 * N2;
+    for (int i0 = tid; i0 < nv; i0 += 256 * kLdx) {
+        float gk[kLdx], gm[kLdx];
+        float2 rt[kLdx];
+#pragma unroll
+        for (int u = 0; u < kLdx; ++u) {
+            const int idx = min(i0 + 256 * u, nv - 1);
+            const int l = idx >= K1 * N2 ? 1 : 0, r = idx - l * K1 * N2;
+            const int k1 = r / N2, k2 = r - k1 * N2;
+            const int k = k1 + N1 * k2;
+            gk[u] = gls[l][k];
+            gm[u] = gls[l][k == 0 ? 0 : N - k];
+            rt[u] = a.rot[k];
+            if (k == 0) gm[u] = 0.f;
+        }
+#pragma unroll
+        for (int u = 0; u < kLdx; ++u) {
+            const int idx = i0 + 256 * u;
+            if (idx < nv) Vs[idx] = cmulcf(make_float2(0.5f * gk[u], -0.5f * gm[u]), rt[u]);
+        }
     }
     __syncthreads();
     // ---- step C': wave w owns row tile w (n2 = 8w + 2q, +1), all kXL column
