@@ -491,16 +491,32 @@ __global__ __launch_bounds__(256) void k_project(Dims d, MetricsT<T> m, T *U, T 
     }
 }
 
+// P = phi / (dt a11) on every cell, ghosts from the nearest interior cell;
+// 64 x 4 cells per block, kPresZ planes per thread, loads issued first
+constexpr int kPresZ = 4;
+
 template <class T>
 __global__ __launch_bounds__(256) void k_pressure(Dims d, const T *phi, T *P, double den) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    const int j = blockIdx.y;
-    const int k = blockIdx.z;
-    if (i >= d.nxg) return;
+    const int nbx = (d.nxg + kWave - 1) / kWave, nby = (d.nyg + 3) / 4;
+    const int ntile = nbx * nby * ((d.nzg + kPresZ - 1) / kPresZ);
+    const int t = sizeof(T) == 4 ? xcd_tile(blockIdx.x, ntile) : blockIdx.x;
+    if (t >= ntile) return;
+    const int i = (t % nbx) * kWave + threadIdx.x;
+    const int j = ((t / nbx) % nby) * 4 + threadIdx.y;
+    const int k0 = (t / (nbx * nby)) * kPresZ;
+    if (i >= d.nxg || j >= d.nyg) return;
     const int ci = i < 1 ? 1 : (i > d.nxg - 2 ? d.nxg - 2 : i);
     const int cj = j < 1 ? 1 : (j > d.nyg - 2 ? d.nyg - 2 : j);
-    const int ck = k < 1 ? 1 : (k > d.nzg - 2 ? d.nzg - 2 : k);
-    P[fidx(d, i, j, k)] = (T)((double)phi[phi_at(d, ci, cj, ck)] / den);
+    T v[kPresZ];
+#pragma unroll
+    for (int r = 0; r < kPresZ; ++r) {
+        const int k = min(k0 + r, d.nzg - 1);
+        const int ck = k < 1 ? 1 : (k > d.nzg - 2 ? d.nzg - 2 : k);
+        v[r] = phi[phi_at(d, ci, cj, ck)];
+    }
+#pragma unroll
+    for (int r = 0; r < kPresZ; ++r)
+        if (k0 + r < d.nzg) P[fidx(d, i, j, k0 + r)] = (T)((double)v[r] / den);
 }
 
 // ------------------------------------------------------------------------
@@ -1773,8 +1789,8 @@ void launch_pressure(Handle &h, const T *phi, T *P, double dt) {
     const Dims &d = h.d;
     ProfScope ps(h, KID_PRESSURE);
     const double den = dt * h.rk2_coef[1][1];
-    hipLaunchKernelGGL(k_pressure<T>, dim3((d.nxg + 255) / 256, d.nyg, d.nzg), dim3(256), 0, h.stream, d, phi, P,
-                       den);
+    const dim3 grid(xcd_blocks((d.nxg + kWave - 1) / kWave * ((d.nyg + 3) / 4) * ((d.nzg + kPresZ - 1) / kPresZ)));
+    hipLaunchKernelGGL(k_pressure<T>, grid, dim3(kWave, 4), 0, h.stream, d, phi, P, den);
     check_launch();
 }
 
