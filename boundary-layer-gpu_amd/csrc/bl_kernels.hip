@@ -67,15 +67,18 @@ __global__ __launch_bounds__(256) void k_rk_march(
     __shared__ T sN[LES ? 3 : 1][LES ? kPlane : 1];
     const int tid = threadIdx.x;
     const int tx = tid % kTX, ty = tid / kTX;
-    const int nbx = (d.nxg - 2 + kTX - 1) / kTX, nby = (d.nyg - 2 + kTY - 1) / kTY;
+    const int nbx = (d.nxg - 1 + kTX - 1) / kTX, nby = (d.nyg - 2 + kTY - 1) / kTY;
     const int ntile = nbx * nby * ((d.nzg - 2 + kchunk - 1) / kchunk);
     const int t = xcd_tile(blockIdx.x, ntile);
     if (t >= ntile) return;
-    const int i0 = (t % nbx) * kTX + 1, j0 = ((t / nbx) % nby) * kTY + 1;
+    // the tile's cells are i0 .. i0 + kTX - 1 with i0 a multiple of kTX, so
+    // that the per-cell rows start on a 128-byte line (cell 0 is idle)
+    const int i0 = (t % nbx) * kTX, j0 = ((t / nbx) % nby) * kTY + 1;
     const int kb = 1 + (t / (nbx * nby)) * kchunk;
     const int ke = min(kb + kchunk, d.nzg - 1);  // exclusive
     const long long sx = d.sx, sxy = d.sxy;
-    // halo tile origin (i0-1, j0-1); loads clamped into the allocation
+    // halo tile origin (i0-1, j0-1); loads clamped into the allocation (the
+    // column i0-1 = -1 of the first tile feeds only the idle cell 0)
     // each thread moves up to kPer halo elements of a plane; they are fetched
     // into registers one iteration ahead and committed to LDS after the sync
     constexpr int kPer = (kPlane + 255) / 256;
@@ -85,7 +88,7 @@ __global__ __launch_bounds__(256) void k_rk_march(
         const int e = tid + 256 * q;
         const int ee = e < kPlane ? e : kPlane - 1;
         const int hy = ee / kHX, hx = ee - hy * kHX;
-        const int ii = min(i0 - 1 + hx, d.sx - 1), jj = min(j0 - 1 + hy, d.nyg - 1);
+        const int ii = max(0, min(i0 - 1 + hx, d.sx - 1)), jj = min(j0 - 1 + hy, d.nyg - 1);
         off[q] = (long long)jj * sx + ii;
     }
     T ru[kPer], rv[kPer], rw[kPer], rn[LES ? kPer : 1];
@@ -128,7 +131,7 @@ __global__ __launch_bounds__(256) void k_rk_march(
 #pragma unroll
             for (int r = 0; r < RPT; ++r) {
                 const int j = j0 + RPT * ty + r;
-                if (i > d.nxg - 2 || j > d.nyg - 2) continue;
+                if (i < 1 || i > d.nxg - 2 || j > d.nyg - 2) continue;
                 const long long p = fidx(d, i, j, k);
                 uo[r][0] = Uo[p];
                 uo[r][1] = Vo[p];
@@ -149,7 +152,7 @@ __global__ __launch_bounds__(256) void k_rk_march(
         for (int r = 0; r < RPT; ++r) {
             const int jt = RPT * ty + r;     // row in the tile
             const int j = j0 + jt;
-            if (i > d.nxg - 2 || j > d.nyg - 2) continue;
+            if (i < 1 || i > d.nxg - 2 || j > d.nyg - 2) continue;
             const int e = (jt + 1) * kHX + (tx + 1);  // LDS index of (i, j)
             const long long p = fidx(d, i, j, k);
             // ---- u (equations.f90:47-94, 149-154)
@@ -447,21 +450,23 @@ __device__ __forceinline__ long long phi_at(const Dims &d, int i, int j, int k) 
 
 // a thread walks kProjZ planes of one (i, j) column, so phi(k+1) of a plane
 // is phi(k) of the next; all loads of its planes are issued before the first
-// update (0.81 -> 0.71 ms/step in FP32 at 814x257x193).  Tiles in
-// XCD-contiguous order in FP32 only: at that size it is 3 % faster there and
-// 4 % slower in FP64 than plain order.
+// update (0.81 -> 0.71 ms/step in FP32 at 814x257x193).  Tiles cover cells
+// 64 bx .. 64 bx + 63 (cell 0 idle), so that a wave's row segment starts on
+// a 128-byte line (-5 % FP64, -7 % FP32).  Tiles in XCD-contiguous order in
+// FP32 only: at that size it is 3 % faster there and 4 % slower in FP64 than
+// plain order.
 constexpr int kProjZ = 4;
 
 template <class T>
 __global__ __launch_bounds__(256) void k_project(Dims d, MetricsT<T> m, T *U, T *V, T *W, const T *phi) {
-    const int nbx = (d.nxg - 2 + kWave - 1) / kWave, nby = (d.nyg - 2 + 3) / 4;
+    const int nbx = (d.nxg - 1 + kWave - 1) / kWave, nby = (d.nyg - 2 + 3) / 4;
     const int ntile = nbx * nby * ((d.nzg - 2 + kProjZ - 1) / kProjZ);
     const int t = sizeof(T) == 4 ? xcd_tile(blockIdx.x, ntile) : blockIdx.x;
     if (t >= ntile) return;
-    const int i = (t % nbx) * kWave + threadIdx.x + 1;
+    const int i = (t % nbx) * kWave + threadIdx.x;
     const int j = ((t / nbx) % nby) * 4 + threadIdx.y + 1;
     const int k0 = (t / (nbx * nby)) * kProjZ + 1;
-    if (i > d.nxg - 2 || j > d.nyg - 2) return;
+    if (i < 1 || i > d.nxg - 2 || j > d.nyg - 2) return;
     const int k1 = min(k0 + kProjZ, d.nzg - 1);  // exclusive
     const T rdxg = m.rdxg[i], rdyg = m.rdyg[j];
     // every load of the kProjZ planes first (planes past the last one clamped
@@ -1618,7 +1623,7 @@ const MetricsT<float> &metrics<float>(const Handle &h) {
 template <class T>
 dim3 march_grid(const Dims &d) {
     constexpr int kTX = MarchTile<T>::TX, kTY = MarchTile<T>::TY;
-    return dim3(xcd_blocks((d.nxg - 2 + kTX - 1) / kTX * ((d.nyg - 2 + kTY - 1) / kTY) *
+    return dim3(xcd_blocks((d.nxg - 1 + kTX - 1) / kTX * ((d.nyg - 2 + kTY - 1) / kTY) *
                            ((d.nzg - 2 + kChunk - 1) / kChunk)));
 }
 
@@ -1778,7 +1783,7 @@ template <class T>
 void launch_project(Handle &h, Fld3<T> &f, const T *phi) {
     const Dims &d = h.d;
     ProfScope ps(h, KID_PROJECT);
-    const dim3 grid(xcd_blocks((d.nxg - 2 + kWave - 1) / kWave * ((d.nyg - 2 + 3) / 4) *
+    const dim3 grid(xcd_blocks((d.nxg - 1 + kWave - 1) / kWave * ((d.nyg - 2 + 3) / 4) *
                                ((d.nzg - 2 + kProjZ - 1) / kProjZ)));
     hipLaunchKernelGGL(k_project<T>, grid, dim3(kWave, 4), 0, h.stream, d, metrics<T>(h), f.u, f.v, f.w, phi);
     check_launch();
