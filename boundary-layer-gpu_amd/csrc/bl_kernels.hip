@@ -1396,10 +1396,10 @@ template <class T>
 __global__ __launch_bounds__(256) void k_maxdiv(Dims d, Metrics m, const T *U, const T *V, const T *W,
                                                 double *out) {
     __shared__ double sh[4];
-    const int i = blockIdx.x * kWave + threadIdx.x + 1;
+    const int i = blockIdx.x * kWave + threadIdx.x;
     const int j = blockIdx.y * 4 + threadIdx.y + 1;
     double v = 0.0;
-    if (i <= d.nxg - 3 && j <= d.nyg - 2)
+    if (i >= 1 && i <= d.nxg - 3 && j <= d.nyg - 2)
         for (int k = 1 + blockIdx.z; k <= d.nzg - 2; k += gridDim.z) {
             const long long p = fidx(d, i, j, k);
             v = fmax(v, fabs(((double)U[p] - (double)U[p - 1]) * m.rdx[i] +
@@ -1415,16 +1415,32 @@ template <class T>
 __global__ __launch_bounds__(256) void k_dtconv(Dims d, Metrics m, const T *U, const T *V, const T *W,
                                                 double *out) {
     __shared__ double sh[3][4];
-    const int i = blockIdx.x * kWave + threadIdx.x + 1;
+    const int i = blockIdx.x * kWave + threadIdx.x;
     const int j = blockIdx.y * 4 + threadIdx.y + 1;
     double a = 0.0, b = 0.0, c = 0.0;
-    if (i <= d.nxg - 2 && j <= d.nyg - 2)
-        for (int k = 1 + blockIdx.z; k <= d.nzg - 2; k += gridDim.z) {
-            const long long p = fidx(d, i, j, k);
-            a = fmax(a, (m.xg[i + 1] - m.xg[i]) / fabs((double)U[p]));
-            b = fmax(b, (m.yg[j + 1] - m.yg[j]) / fabs((double)V[p]));
-            c = fmax(c, (m.zg[k + 1] - m.zg[k]) / fabs((double)W[p]));
+    if (i >= 1 && i <= d.nxg - 2 && j <= d.nyg - 2) {
+        const double dxi = m.xg[i + 1] - m.xg[i], dyj = m.yg[j + 1] - m.yg[j];
+        const int gz = gridDim.z;
+        // four of the thread's planes loaded before the first is used
+        for (int k0 = 1 + blockIdx.z; k0 <= d.nzg - 2; k0 += 4 * gz) {
+            T u[4], v[4], w[4];
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const long long p = fidx(d, i, j, min(k0 + r * gz, d.nzg - 2));
+                u[r] = U[p];
+                v[r] = V[p];
+                w[r] = W[p];
+            }
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int k = k0 + r * gz;
+                if (k > d.nzg - 2) break;
+                a = fmax(a, dxi / fabs((double)u[r]));
+                b = fmax(b, dyj / fabs((double)v[r]));
+                c = fmax(c, (m.zg[k + 1] - m.zg[k]) / fabs((double)w[r]));
+            }
         }
+    }
     a = block_max_256(a, sh[0]);
     b = block_max_256(b, sh[1]);
     c = block_max_256(c, sh[2]);
@@ -1520,10 +1536,11 @@ __global__ __launch_bounds__(256) void k_monitor(Dims d, const T *U, const T *V,
 }
 
 // reductions: each thread walks z planes with stride kRedZ, so the partials
-// per launch are kRedZ times fewer than the interior blocks
+// per launch are kRedZ times fewer than the interior blocks; x tiles cover
+// cells 64 bx .. 64 bx + 63 (cell 0 idle) so that rows start on a 128-byte line
 constexpr int kRedZ = 12;
 dim3 reduce_grid(const Dims &d) {
-    return dim3((d.nxg - 2 + kWave - 1) / kWave, (d.nyg - 2 + 3) / 4, std::min(d.nzg - 2, kRedZ));
+    return dim3((d.nxg - 1 + kWave - 1) / kWave, (d.nyg - 2 + 3) / 4, std::min(d.nzg - 2, kRedZ));
 }
 
 void check_launch() {
