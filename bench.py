@@ -12,10 +12,14 @@ algorithm diverges within ~15 steps.  The bench therefore runs CFL = 0.5; the
 work per step does not depend on dt.  Inputs are resident
 in HBM before the timed region.  One "step" = compute_dt + one RK3 step.
 
-Prints one JSON line.  `value` is the FP64 path.  `roofline` describes the
-kernel with the largest share of GPU time, measured with HIP events on the
-solver's stream during the timed steps; `cpu_baseline` is the FP64 C oracle
-timed on this host for one RK3 step of a z-slab of the same workload.
+Prints one JSON line.  `value` is the FP64 path.  The last --prof-steps
+warm-up steps are a breakdown pass that brackets every kernel class with HIP
+events on the solver's stream; it gives `kernels` and picks the class with
+the largest share of GPU time.  In the timed steps only that
+class is bracketed (the events of every class would add ~0.3 ms to a step),
+and `roofline` is its average launch time there.  `cpu_baseline` is the
+FP64 C oracle timed on this host for one RK3 step of a z-slab of the same
+workload.
 `mixed` is the same workload in mixed precision (FP32 fields, split-BF16x3
 MFMA transforms, FP64 Thomas / mass flux / statistics): its ns/cell/step, the
 MFMA rate of its x-transform GEMM, and the relative L2 difference of u, v, w,
@@ -108,16 +112,27 @@ def cpu_baseline(nz_sample):
 
 
 def run_leg(blgpu, precision, grid, args, keep_fields=False):
-    """Warm-up, then the timed steps with the HIP-event profile on."""
+    """Warm-up, whose last args.prof_steps steps are a breakdown pass with
+    every kernel class bracketed by HIP events; then the timed steps, in
+    which only the class that took the most time in the breakdown is
+    bracketed (the events of all classes would add ~0.3 ms to a step) and
+    gives the roofline."""
     x, y, z, tables, (U, V, W) = grid
     s = blgpu.Solver(x, y, z, NU, tables, scheme=3, les=0, cfl=CFL, precision=precision)
     s.set_fields(U, V, W)
     dt = s.compute_dt()
-    for _ in range(args.warmup):
+    nprof = min(args.prof_steps, args.warmup)
+    for n in range(args.warmup):
+        if n == args.warmup - nprof:
+            s.synchronize()
+            s.profile(True)
         s.compute_dt()
         s.step()
     s.synchronize()
-    s.profile(True)
+    breakdown = s.profile_read()
+    s.profile(False)
+    dom = max((k for k, v in breakdown.items() if v[1] > 0), key=lambda k: breakdown[k][0])
+    s.profile(True, only=[dom])
     t0 = time.perf_counter()
     for _ in range(args.steps):
         s.compute_dt()
@@ -126,7 +141,7 @@ def run_leg(blgpu, precision, grid, args, keep_fields=False):
     el = time.perf_counter() - t0
     prof = s.profile_read()
     s.profile(False)
-    out = dict(el=el, dt=dt, prof=prof, div=s.max_divergence(), cf=s.wall_stats())
+    out = dict(el=el, dt=dt, prof=prof, breakdown=breakdown, div=s.max_divergence(), cf=s.wall_stats())
     if keep_fields:
         out["fields"] = s.fields() + (s.download(3),)
     s.close()
@@ -149,8 +164,11 @@ def measured_traffic(label, nx, ny, nz):
     return None
 
 
-def roofline_of(prof, nx, ny, nz):
-    tot = {k: v[0] for k, v in prof.items() if v[1] > 0}
+def roofline_of(prof, breakdown, nx, ny, nz):
+    """Roofline of the kernel class bracketed in the timed steps (`prof`, the
+    class with the most time in the breakdown pass); its share of GPU time
+    from the breakdown pass."""
+    tot = {k: v[0] for k, v in breakdown.items() if v[1] > 0}
     gpu_ms = sum(tot.values())
     dom = max(tot, key=tot.get)
     dom_ms, dom_n = prof[dom]
@@ -165,7 +183,7 @@ def roofline_of(prof, nx, ny, nz):
         roof = dict(bound="hbm", achieved=achieved, peak=HBM_PEAK_GBS, unit="GB/s",
                     frac=(achieved / HBM_PEAK_GBS) if achieved else None,
                     traffic=measured_traffic(dom, nx, ny, nz), algorithmic_bytes=units)
-    roof.update(kernel=dom, avg_launch_ms=dom_ms / dom_n, share_of_gpu_time=dom_ms / gpu_ms)
+    roof.update(kernel=dom, avg_launch_ms=dom_ms / dom_n, share_of_gpu_time=tot[dom] / gpu_ms)
     return roof
 
 
@@ -315,6 +333,9 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--prof-steps", type=int, default=20,
+                    help="the last PROF_STEPS warm-up steps are the per-kernel breakdown pass "
+                         "(all classes bracketed by events)")
     ap.add_argument("--nx", type=int, default=814)
     ap.add_argument("--ny", type=int, default=257)
     ap.add_argument("--nz", type=int, default=193)
@@ -332,6 +353,8 @@ def main():
     ap.add_argument("--les-train-steps", type=int, default=2000)
     ap.add_argument("--les-steps", type=int, default=100)
     args = ap.parse_args()
+    if args.warmup < 1 or args.prof_steps < 1:
+        ap.error("--warmup and --prof-steps must be at least 1 (the breakdown pass is the end of the warm-up)")
     if args.gpus != 1:
         print("bench.py: this build measures one GPU; multi-GPU runs are not part of it "
               f"(asked for --gpus {args.gpus})", file=sys.stderr)
@@ -361,9 +384,10 @@ def main():
         raise RuntimeError("non-finite Cf after the timed steps")
     ms_per_step = r64["el"] / args.steps * 1e3
     value = r64["el"] / args.steps / cells * 1e9
-    roof = roofline_of(r64["prof"], nx, ny, nz)
-    kernels = {k: dict(ms_per_step=v[0] / args.steps, launches_per_step=v[1] / args.steps)
-               for k, v in r64["prof"].items() if v[1] > 0}
+    roof = roofline_of(r64["prof"], r64["breakdown"], nx, ny, nz)
+    nprof = min(args.prof_steps, args.warmup)
+    kernels = {k: dict(ms_per_step=v[0] / nprof, launches_per_step=v[1] / nprof)
+               for k, v in r64["breakdown"].items() if v[1] > 0}
 
     mixed_out = None
     if mixed:
@@ -381,10 +405,11 @@ def main():
         v_mx = rmx["el"] / args.steps / cells * 1e9
         mixed_out = dict(value=v_mx, unit="ns/cell/step", ms_per_step=rmx["el"] / args.steps * 1e3,
                          speedup_vs_fp64=value / v_mx,
-                         rel_l2_vs_fp64_after_steps=err, steps_compared=args.warmup + args.steps,
-                         max_divergence=rmx["div"], mfma=mixed_mfma(rmx["prof"], nx, ny, nz),
-                         kernels={k: dict(ms_per_step=v[0] / args.steps, launches_per_step=v[1] / args.steps)
-                                  for k, v in rmx["prof"].items() if v[1] > 0})
+                         rel_l2_vs_fp64_after_steps=err,
+                         steps_compared=args.warmup + args.steps,
+                         max_divergence=rmx["div"], mfma=mixed_mfma(rmx["breakdown"], nx, ny, nz),
+                         kernels={k: dict(ms_per_step=v[0] / nprof, launches_per_step=v[1] / nprof)
+                                  for k, v in rmx["breakdown"].items() if v[1] > 0})
         del rmx
     r64.pop("fields", None)
 

@@ -52,6 +52,7 @@ SIGNATURES = {
     "bl_projection": (_I, [_P, _I, _D]),
     "bl_poisson": (_I, [_P, _P]),
     "bl_profile_enable": (_I, [_P, _I]),
+    "bl_profile_select": (_I, [_P, ctypes.c_ulonglong]),
     "bl_profile_read": (_I, [_P, _P, _P, _I]),
     "bl_profile_count": (_I, []),
     "bl_profile_name": (ctypes.c_char_p, [_I]),
@@ -288,7 +289,16 @@ class Solver:
         _check(lib().bl_closure_fallback(self.h, ctypes.byref(c), ctypes.byref(f)), "bl_closure_fallback")
         return c.value, f.value
 
-    def profile(self, on=True):
+    def profile(self, on=True, only=None):
+        """Bracket kernel launches with HIP events; `only`: the class names to
+        bracket (default all)."""
+        mask = (1 << 64) - 1
+        if only is not None:
+            names = [lib().bl_profile_name(i).decode() for i in range(lib().bl_profile_count())]
+            mask = 0
+            for n in only:
+                mask |= 1 << names.index(n)
+        _check(lib().bl_profile_select(self.h, mask), "bl_profile_select")
         _check(lib().bl_profile_enable(self.h, 1 if on else 0), "bl_profile_enable")
 
     def profile_read(self):

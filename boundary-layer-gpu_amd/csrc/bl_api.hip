@@ -695,7 +695,7 @@ void prof_free(Handle &h) {
 }  // namespace
 
 void prof_begin(Handle &h, int id) {
-    if (!h.prof) return;
+    if (!h.prof || !((h.prof_mask >> id) & 1ull)) return;
     hipEvent_t e = h.prof->take();
     BL_HIP(hipEventRecord(e, h.stream));
     h.prof->open_ev[id] = e;
@@ -976,6 +976,10 @@ int bl_profile_enable(bl_handle *hh, int on) {
             h.prof = nullptr;
         }
     });
+}
+
+int bl_profile_select(bl_handle *hh, unsigned long long mask) {
+    return guarded([&] { H(hh).prof_mask = mask; });
 }
 
 int bl_profile_read(bl_handle *hh, double *ms, long long *launches, int n) {

@@ -188,6 +188,7 @@ struct Handle {
     double *c00;                    // phi00(j) / (2 N M)
     int np00;                       // partials per y row
     ProfState *prof;                // null unless profiling is enabled
+    unsigned long long prof_mask = ~0ull;  // kernel classes the profile brackets (bit = class id)
 };
 
 // Records HIP events around one launch on the handle's stream when
