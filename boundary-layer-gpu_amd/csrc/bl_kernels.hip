@@ -778,7 +778,8 @@ __global__ __launch_bounds__(256) void k_dct_fwd(DctArgs a, Dims d, Metrics mt,
         const int k = k1 + N1 * k2;
         const double2 wv = cmul(a.rot[k], make_double2(re, im));
         v[l * N + k] = 2.0 * wv.x;
-        if (k > 0) v[l * N + N - k] = -2.0 * wv.y;
+        // the mirror N - k of k1 = 0 and k1 = N1/2 is written directly (k1' = k1)
+        if (k1 != 0 && 2 * k1 != N1) v[l * N + N - k] = -2.0 * wv.y;
     }
     __syncthreads();
     for (int idx = tid; idx < LB * N; idx += 256) {
@@ -996,7 +997,9 @@ __global__ __launch_bounds__(256) void k_dct_fwd_m(DctArgs a, Dims d, Metrics mt
                         const int k = k1 + N1 * k2;
                         const double2 wv = cmul(a.rot[k], make_double2(acc[2 * h], acc[2 * h + 1]));
                         v[ln * N + k] = 2.0 * wv.x;
-                        if (k > 0) v[ln * N + N - k] = -2.0 * wv.y;
+                        // the mirror N - k of k1 = 0 and k1 = N1/2 is written
+                        // directly by another lane (k1' = k1): one writer each
+                        if (k1 != 0 && 2 * k1 != N1) v[ln * N + N - k] = -2.0 * wv.y;
                     }
                 }
             }

@@ -464,7 +464,9 @@ __global__ __launch_bounds__(256) void k_xfwd_mx(Dims d, Metrics mt, XArgs a, co
                     const int k = k1 + N1 * k2;
                     const float2 wv = cmulf(a.rot[k], make_float2(acc[2 * h], acc[2 * h + 1]));
                     v[ln * N + k] = 2.f * wv.x;
-                    if (k > 0) v[ln * N + N - k] = -2.f * wv.y;
+                    // the mirror N - k of k1 = 0 and k1 = N1/2 is written
+                    // directly by another lane (k1' = k1): one writer each
+                    if (k1 != 0 && 2 * k1 != N1) v[ln * N + N - k] = -2.f * wv.y;
                 }
             }
         }
