@@ -1386,7 +1386,8 @@ __global__ __launch_bounds__(256) void k_solve00(int NM, int ldj, const double *
 // ------------------------------------------------------------------------
 // Reductions
 // ------------------------------------------------------------------------
-// block max of non-negative values; one atomic per block
+// block max of non-negative values over 4 waves; the caller stores one
+// partial per block and k_reduce_max reduces the partials
 __device__ __forceinline__ double block_max_256(double v, double *sh) {
     for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_down(v, o));
     const int t = threadIdx.x + blockDim.x * threadIdx.y;
