@@ -213,14 +213,27 @@ void launch_bc(Handle &h, Fld3<T> &f, const Fld3<T> &old, int rk_step, double dt
 void launch_poisson(Handle &h, double *phi_inout, const Field3 *div = nullptr);
 template <class T>
 void launch_project(Handle &h, Fld3<T> &f, const T *phi);
-// the y solves of all spectral systems S[q][j][kx] (row stride ldj, q stride
-// ldq) by the partitioned kernel; false if the handle uses the table kernel.
-// The (0,0) system is left untouched (zero00 = false) or zeroed.
-template <class T>
-bool launch_thomas_seg(Handle &h, T *S, int ldj, long long ldq, bool zero00);
-// the pinned (0,0) system of x (row stride ldj) in place, times scale (NM <= kMax00)
+// The pinned (0,0) system (NM <= kMax00 rows) solved in place in x (row
+// stride ldx) and scaled; with part set, its right-hand side is first formed
+// from the FP64 plane sums part[j][0..npart) (mixed path).
 constexpr int kMax00 = 1024;
-void launch_solve00(Handle &h, double *x, int ldj, double scale);
+struct Solve00 {
+    const double *L00, *D00, *du;  // multipliers, reciprocal pivots, super-diagonal
+    const double *part;            // null: the rhs is x itself
+    int npart;
+    double *x;
+    int ldx;
+    double scale;
+};
+Solve00 solve00_args(const Handle &h, double *x, int ldx, double scale, const double *part, int npart);
+void launch_solve00(Handle &h, const Solve00 &z);
+// the y solves of all spectral systems S[q][j][kx] (row stride ldj, q stride
+// ldq) by the partitioned kernel, with the (0,0) system z solved by one more
+// block of the same launch; false if the handle uses the table kernel.  The
+// (0,0) column of S is left alone (z.part null: z solves it in place) or
+// zeroed (z.part set: z writes its own output).
+template <class T>
+bool launch_thomas_seg(Handle &h, T *S, int ldj, long long ldq, const Solve00 &z);
 template <class T>
 void launch_pressure(Handle &h, const T *phi, T *P, double dt);
 template <class T>

@@ -1223,6 +1223,9 @@ __device__ __forceinline__ cdptr opaque(const double *p) {
     return (cdptr)p;
 }
 
+template <int NMAX>
+__device__ void solve00_block(const Solve00 &z, int NM);
+
 // Partitioned Thomas solve held in registers (the systems of
 // projection.f90:137-150, factors formed on the fly instead of read from
 // tables).  A block is P waves; wave p holds rows [32 p, 32 p + 32) of the 64
@@ -1237,17 +1240,23 @@ __device__ __forceinline__ cdptr opaque(const double *p) {
 // rows (a = c = 0, b = 1).  co = {a[L], b[L], c[L], kxx[N], kzz[Kz]},
 // L = 32 P: a[j] couples x(j-1), c[j] couples x(j+1), b[j] + kxx + kzz is
 // the diagonal.  The (0,0) system (kx = 0, q = 0, pinned first pivot) is
-// skipped (zero00 = 0) or zeroed (zero00 = 1) for a separate solve.
+// skipped (zero00 = 0) or zeroed (zero00 = 1); block 0 solves it (z) while
+// the other blocks run.
 template <class T>
 __global__ __launch_bounds__(64 * kThMaxP) void k_thomas_seg(int N, int NM, int P, int ldj, long long ldq,
                                                              const double *__restrict__ co, int zero00,
-                                                             T *__restrict__ S) {
+                                                             T *__restrict__ S, Solve00 z) {
     __shared__ double xs[6][kThMaxP][64];
+    if (blockIdx.x == 0) {  // the (0,0) system, alongside the other blocks
+        solve00_block<kThSeg * kThMaxP>(z, NM);
+        return;
+    }
     const int lane = threadIdx.x & 63;
     const int p = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int nkb = (N + 63) / 64;
-    const int q = blockIdx.x / nkb;
-    const int kx = (blockIdx.x - q * nkb) * 64 + lane;
+    const int bid = blockIdx.x - 1;
+    const int q = bid / nkb;
+    const int kx = (bid - q * nkb) * 64 + lane;
     const bool live = kx < N;
     const int kxc = live ? kx : N - 1;
     const int L = P * kThSeg;
@@ -1347,23 +1356,36 @@ __global__ __launch_bounds__(64 * kThMaxP) void k_thomas_seg(int N, int NM, int 
         if (r0 + i < NM) col[(long long)(r0 + i) * ldj] = (T)dv[i];
 }
 
-// The pinned (0,0) system (kx = 0, kz = 0) of x (row stride ldj) in place,
-// scaled by `scale`: one lane, in the elimination order of the table kernel,
-// from LDS copies of the stored multipliers L00 and reciprocal pivots D00.
-// The sweeps read one array and write another, so that the LDS reads of the
-// unrolled iterations are issued ahead of the dependent chain.
-__global__ __launch_bounds__(256) void k_solve00(int NM, int ldj, const double *__restrict__ L00,
-                                                 const double *__restrict__ D00,
-                                                 const double *__restrict__ du, double *x, double scale) {
-    __shared__ double r[kMax00], o[kMax00], L[kMax00], D[kMax00], U[kMax00];
-    for (int j = threadIdx.x; j < NM; j += 256) {
-        r[j] = x[(long long)j * ldj];
-        L[j] = j > 0 ? L00[j - 1] : 0.0;
-        D[j] = D00[j];
-        U[j] = du[j];
+// The pinned (0,0) system (kx = 0, kz = 0) in place in z.x (row stride
+// z.ldx), scaled by z.scale: one lane, in the elimination order of the
+// table kernel, from LDS copies of the stored multipliers L00 and reciprocal
+// pivots D00.  With z.part set, the right-hand side is first formed as
+// r(j) = 2 sum_q part[j][q] (the FP64 plane sums of the mixed path's
+// divergence), one wave per row.  The sweeps read one array and write
+// another, so that the LDS reads of the unrolled iterations are issued ahead
+// of the dependent chain.  Any block size that is a multiple of 64.
+template <int NMAX>
+__device__ void solve00_block(const Solve00 &z, int NM) {
+    __shared__ double r[NMAX], o[NMAX], L[NMAX], D[NMAX], U[NMAX];
+    const int tid = threadIdx.x, nt = blockDim.x;
+    if (z.part) {
+        const int lane = tid & 63;
+        for (int j = tid >> 6; j < NM; j += nt >> 6) {
+            double v = 0.0;
+            for (int q = lane; q < z.npart; q += 64) v += z.part[(long long)j * z.npart + q];
+            for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off);
+            if (lane == 0) r[j] = 2.0 * v;
+        }
+    } else {
+        for (int j = tid; j < NM; j += nt) r[j] = z.x[(long long)j * z.ldx];
+    }
+    for (int j = tid; j < NM; j += nt) {
+        L[j] = j > 0 ? z.L00[j - 1] : 0.0;
+        D[j] = z.D00[j];
+        U[j] = z.du[j];
     }
     __syncthreads();
-    if (threadIdx.x == 0) {
+    if (tid == 0) {
         double v = r[0];
         o[0] = v;
 #pragma unroll 8
@@ -1380,8 +1402,10 @@ __global__ __launch_bounds__(256) void k_solve00(int NM, int ldj, const double *
         }
     }
     __syncthreads();
-    for (int j = threadIdx.x; j < NM; j += 256) x[(long long)j * ldj] = r[j] * scale;
+    for (int j = tid; j < NM; j += nt) z.x[(long long)j * z.ldx] = r[j] * z.scale;
 }
+
+__global__ __launch_bounds__(256) void k_solve00(int NM, Solve00 z) { solve00_block<kMax00>(z, NM); }
 
 // ------------------------------------------------------------------------
 // Reductions
@@ -1759,9 +1783,8 @@ void launch_poisson(Handle &h, double *phi, const Field3 *div) {
     zdft_launch(h, KID_XFORM_Z_FWD, true, h.T1, h.S);
     {
     ProfScope ps(h, KID_THOMAS);
-    if (launch_thomas_seg<double>(h, h.S, N, (long long)plane, false)) {
-        launch_solve00(h, h.S, N, 1.0);
-    } else {
+    // the (0,0) column of S in place (the partitioned kernel leaves it alone)
+    if (!launch_thomas_seg<double>(h, h.S, N, (long long)plane, solve00_args(h, h.S, N, 1.0, nullptr, 0))) {
         hipLaunchKernelGGL(k_thomas<8>, dim3((N + 255) / 256 * M), dim3(256), 0, h.stream, d, h.lu_dl,
                            h.lu_dp, h.lu_du, h.S);
     }
@@ -1781,22 +1804,34 @@ void launch_poisson(Handle &h, double *phi, const Field3 *div) {
 }
 
 template <class T>
-bool launch_thomas_seg(Handle &h, T *S, int ldj, long long ldq, bool zero00) {
+bool launch_thomas_seg(Handle &h, T *S, int ldj, long long ldq, const Solve00 &z) {
     const Dims &d = h.d;
     if (h.th_p == 0) return false;
-    hipLaunchKernelGGL(k_thomas_seg<T>, dim3((d.N + 63) / 64 * d.M), dim3(64 * h.th_p), 0, h.stream, d.N, d.NM,
-                       h.th_p, ldj, ldq, (const double *)h.th_co, zero00 ? 1 : 0, S);
+    hipLaunchKernelGGL(k_thomas_seg<T>, dim3(1 + (d.N + 63) / 64 * d.M), dim3(64 * h.th_p), 0, h.stream, d.N,
+                       d.NM, h.th_p, ldj, ldq, (const double *)h.th_co, z.part ? 1 : 0, S, z);
     check_launch();
     return true;
 }
 
-void launch_solve00(Handle &h, double *x, int ldj, double scale) {
+Solve00 solve00_args(const Handle &h, double *x, int ldx, double scale, const double *part, int npart) {
     if (h.d.NM > kMax00) {
         last_error() = "internal: the (0,0) system is longer than the solve kernel holds";
         throw ArgError();
     }
-    hipLaunchKernelGGL(k_solve00, dim3(1), dim3(256), 0, h.stream, h.d.NM, ldj, h.lu00, h.lu00 + h.d.NM, h.lu_du,
-                       x, scale);
+    Solve00 z;
+    z.L00 = h.lu00;
+    z.D00 = h.lu00 + h.d.NM;
+    z.du = h.lu_du;
+    z.part = part;
+    z.npart = npart;
+    z.x = x;
+    z.ldx = ldx;
+    z.scale = scale;
+    return z;
+}
+
+void launch_solve00(Handle &h, const Solve00 &z) {
+    hipLaunchKernelGGL(k_solve00, dim3(1), dim3(256), 0, h.stream, h.d.NM, z);
     check_launch();
 }
 
@@ -1889,7 +1924,7 @@ void monitor(Handle &h, const Fld3<T> &f, double *out) {
     template void launch_rhs_only<T>(Handle &, const Fld3<T> &, Fld3<T> &);                                \
     template void launch_bc<T>(Handle &, Fld3<T> &, const Fld3<T> &, int, double, double);                 \
     template void launch_project<T>(Handle &, Fld3<T> &, const T *);                                      \
-    template bool launch_thomas_seg<T>(Handle &, T *, int, long long, bool);                              \
+    template bool launch_thomas_seg<T>(Handle &, T *, int, long long, const Solve00 &);                   \
     template void launch_pressure<T>(Handle &, const T *, T *, double);                                   \
     template double max_divergence<T>(Handle &, const Fld3<T> &);                                         \
     template double dt_convective<T>(Handle &, const Fld3<T> &);                                          \

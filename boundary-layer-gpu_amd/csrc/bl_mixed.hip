@@ -960,14 +960,15 @@ void launch_poisson_mixed(Handle &h, float *phi, const Fld3<float> &f) {
     // FP64 Thomas on the spectrum, FP64 (0,0) mode
     {
         ProfScope ps(h, KID_THOMAS);
-        if (!launch_thomas_seg<float>(h, S, NP, (long long)NP * NM, true)) {
+        const double s00 = 1.0 / (2.0 * (double)N * (double)M);
+        if (!launch_thomas_seg<float>(h, S, NP, (long long)NP * NM, solve00_args(h, h.c00, 1, s00, h.p00, h.np00))) {
             hipLaunchKernelGGL(k_thomas_mixed<8>, dim3((N + 255) / 256 * M), dim3(256), 0, h.stream, d, NP,
                                h.lu_dl, h.lu_dp, h.lu_du, S, h.T1);
             check();
+            hipLaunchKernelGGL(k_sum00, dim3(NM), dim3(256), 0, h.stream, h.p00, h.np00, h.c00);
+            check();
+            launch_solve00(h, solve00_args(h, h.c00, 1, s00, nullptr, 0));
         }
-        hipLaunchKernelGGL(k_sum00, dim3(NM), dim3(256), 0, h.stream, h.p00, h.np00, h.c00);
-        check();
-        launch_solve00(h, h.c00, 1, 1.0 / (2.0 * (double)N * (double)M));
     }
     // inverse z: T1[m][col] = sum_q Zi[m][q] S[q][col]
     z.Ahi = h.zi_hi;
