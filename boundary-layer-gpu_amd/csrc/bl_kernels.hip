@@ -39,14 +39,15 @@ __device__ __forceinline__ T sq(T a) {
 // every velocity value is read from HBM once per chunk (plus halo) instead of
 // once per neighbour.  Loads are x-fastest and 512-byte aligned rows.
 // Thread (tx, ty) computes cells (i0+tx, j0+RPT ty+r), r < RPT.  FP64: 64 x 8
-// tiles (RPT = 2, 47.5 KB of LDS); FP32: 64 x 16 tiles (RPT = 4: less y halo
-// and twice the loads in flight per barrier at a similar LDS size).  128-wide
-// FP32 tiles (fewer x-halo lines) measured slower.
+// tiles (RPT = 2, 47.5 KB of LDS, 164 VGPRs: 3 blocks per CU); FP32: 64 x 12
+// tiles (RPT = 3, 33 KB, 161 VGPRs: 3 blocks per CU; 64 x 16 tiles held 2
+// blocks per CU and took 3 % longer at 814x257x193).  128-wide FP32 tiles
+// (fewer x-halo lines) measured slower.
 // ------------------------------------------------------------------------
 template <class T>
 struct MarchTile {
     static constexpr int TX = 64;                          // tile width (x)
-    static constexpr int RPT = sizeof(T) == 4 ? 4 : 2;    // rows per thread
+    static constexpr int RPT = sizeof(T) == 4 ? 3 : 2;    // rows per thread
     static constexpr int TY = (256 / TX) * RPT, HX = TX + 2, HY = TY + 2, Plane = HX * HY;
 };
 
