@@ -894,6 +894,8 @@ __global__ __launch_bounds__(256) void k_dct_fwd_m(DctArgs a, Dims d, Metrics mt
                                                    double *__restrict__ out) {
     extern __shared__ double smem[];
     const int N = a.N, N1 = a.N1, N2 = a.N2, K1 = a.K1h + 1;
+    // twiddle indices n2 k1 stay below N (n2 < N2, k1 <= N1/2): no reduction
+    const float rN2 = 1.0f / (float)N2, rK1 = 1.0f / (float)K1;
     double *v = smem;                                         // kMfL*N
     double2 *Bc = reinterpret_cast<double2 *>(v + kMfL * N);  // kMfL*K1*N2
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63, q = lane >> 4, c16 = lane & 15;
@@ -945,7 +947,7 @@ __global__ __launch_bounds__(256) void k_dct_fwd_m(DctArgs a, Dims d, Metrics mt
     // ---- step A: wave w owns kMfL / 2 column tiles (columns (line, n2))
     for (int ct = wave * (kMfL / 2); ct < (wave + 1) * (kMfL / 2); ++ct) {
         const int col = ct * 16 + c16;
-        const int ln = col / N2, n2 = col - ln * N2;
+        const int ln = small_div(col, rN2), n2 = col - ln * N2;
         const bool cok = col < kMfL * N2;
         dbl4 acc0 = {0.0, 0.0, 0.0, 0.0}, acc1 = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
@@ -964,7 +966,7 @@ __global__ __launch_bounds__(256) void k_dct_fwd_m(DctArgs a, Dims d, Metrics mt
                 const int k1 = 8 * (h / 2) + q + 4 * (h % 2);
                 if (k1 < K1) {
                     const double re = acc[2 * (h % 2)], im = acc[2 * (h % 2) + 1];
-                    Bc[(ln * K1 + k1) * N2 + n2] = cmul(make_double2(re, im), a.wN[(n2 * k1) % N]);
+                    Bc[(ln * K1 + k1) * N2 + n2] = cmul(make_double2(re, im), a.wN[n2 * k1]);
                 }
             }
         }
@@ -978,7 +980,7 @@ __global__ __launch_bounds__(256) void k_dct_fwd_m(DctArgs a, Dims d, Metrics mt
         for (int ks = 0; ks < kMfKC; ++ks) fc[ks] = ks < KSC ? a.fC[(wave * KSC + ks) * 64 + lane] : 0.0;
         for (int ct = 0; ct < kMfL; ++ct) {
             const int col = ct * 16 + c16;
-            const int ln = col / K1, k1 = col - ln * K1;
+            const int ln = small_div(col, rK1), k1 = col - ln * K1;
             const bool cok = col < kMfL * K1;
             const double *Bl = reinterpret_cast<const double *>(Bc + (ln * K1 + k1) * N2);
             dbl4 acc = {0.0, 0.0, 0.0, 0.0};
@@ -1007,18 +1009,25 @@ __global__ __launch_bounds__(256) void k_dct_fwd_m(DctArgs a, Dims d, Metrics mt
         }
     }
     __syncthreads();
+    // per-line output bases, then one element per thread and step (kMfL = 2)
+    long long ob[kMfL];
+#pragma unroll
+    for (int l = 0; l < kMfL; ++l) {
+        const int c = min(line0 + l, a.C - 1);
+        ob[l] = a.out_base + (long long)(c % a.C1) * a.out_c1 + (long long)(c / a.C1) * a.out_c2;
+    }
     for (int idx = tid; idx < kMfL * N; idx += 256) {
-        const int l = idx / N, kx = idx - l * N;
-        const int c = line0 + l;
-        if (c >= a.C) continue;
-        const int c1 = c % a.C1, c2 = c / a.C1;
-        out[a.out_base + c1 * a.out_c1 + c2 * a.out_c2 + kx] = v[l * N + kx];
+        const int l = idx >= N ? 1 : 0, kx = idx - l * N;
+        if (line0 + l >= a.C) continue;
+        out[ob[l] + kx] = v[idx];
     }
 }
 
 __global__ __launch_bounds__(256) void k_dct_inv_m(DctArgs a, const double *__restrict__ in, double *__restrict__ out) {
     extern __shared__ double smem[];
     const int N = a.N, N1 = a.N1, N2 = a.N2, K1h = a.K1h, K1 = K1h + 1;
+    // twiddle indices n2 k1 stay below N (n2 < N2, k1 <= N1/2): no reduction
+    const float rN2 = 1.0f / (float)N2, rK1 = 1.0f / (float)K1;
     double2 *Vs = reinterpret_cast<double2 *>(smem);  // kMfL*K1*N2, later the output lines (kMfL*N doubles)
     double2 *Cs = Vs + kMfL * K1 * N2;                // kMfL*K1*N2
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63, q = lane >> 4, c16 = lane & 15;
@@ -1041,7 +1050,7 @@ __global__ __launch_bounds__(256) void k_dct_inv_m(DctArgs a, const double *__re
         for (int u = 0; u < kLd; ++u) {
             const int idx = min(i0 + 256 * u, nv - 1);
             const int l = idx >= K1 * N2 ? 1 : 0, r = idx - l * K1 * N2;
-            const int k1 = r / N2, k2 = r - k1 * N2;
+            const int k1 = small_div(r, rN2), k2 = r - k1 * N2;
             const int k = k1 + N1 * k2;
             gk[u] = gls[l][k];
             gm[u] = gls[l][k == 0 ? 0 : N - k];
@@ -1064,7 +1073,7 @@ __global__ __launch_bounds__(256) void k_dct_inv_m(DctArgs a, const double *__re
         for (int ks = 0; ks < kMfKC; ++ks) fc[ks] = ks < KSC ? a.iC[(wave * KSC + ks) * 64 + lane] : 0.0;
         for (int ct = 0; ct < kMfL; ++ct) {
             const int col = ct * 16 + c16;
-            const int ln = col / K1, k1 = col - ln * K1;
+            const int ln = small_div(col, rK1), k1 = col - ln * K1;
             const bool cok = col < kMfL * K1;
             const double *Vl = reinterpret_cast<const double *>(Vs + (ln * K1 + k1) * N2);
             dbl4 acc = {0.0, 0.0, 0.0, 0.0};
@@ -1082,7 +1091,7 @@ __global__ __launch_bounds__(256) void k_dct_inv_m(DctArgs a, const double *__re
                     const int n2 = 8 * wave + q + 4 * h;
                     if (n2 < N2)
                         Cs[(ln * K1 + k1) * N2 + n2] =
-                            cmulc(make_double2(acc[2 * h], acc[2 * h + 1]), a.wN[(n2 * k1) % N]);
+                            cmulc(make_double2(acc[2 * h], acc[2 * h + 1]), a.wN[n2 * k1]);
                 }
             }
         }
@@ -1100,7 +1109,7 @@ __global__ __launch_bounds__(256) void k_dct_inv_m(DctArgs a, const double *__re
         }
         for (int ct = wave * (kMfL / 2); ct < (wave + 1) * (kMfL / 2); ++ct) {
             const int col = ct * 16 + c16;
-            const int ln = col / N2, n2 = col - ln * N2;
+            const int ln = small_div(col, rN2), n2 = col - ln * N2;
             const bool cok = col < kMfL * N2;
             dbl4 acc0 = {0.0, 0.0, 0.0, 0.0}, acc1 = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
@@ -1129,12 +1138,17 @@ __global__ __launch_bounds__(256) void k_dct_inv_m(DctArgs a, const double *__re
         }
     }
     __syncthreads();
+    // per-line output bases, then one element per thread and step (kMfL = 2)
+    long long ob[kMfL];
+#pragma unroll
+    for (int l = 0; l < kMfL; ++l) {
+        const int c = min(line0 + l, a.C - 1);
+        ob[l] = a.out_base + (long long)(c % a.C1) * a.out_c1 + (long long)(c / a.C1) * a.out_c2;
+    }
     for (int idx = tid; idx < kMfL * N; idx += 256) {
-        const int l = idx / N, n = idx - l * N;
-        const int c = line0 + l;
-        if (c >= a.C) continue;
-        const int c1 = c % a.C1, c2 = c / a.C1;
-        out[a.out_base + c1 * a.out_c1 + c2 * a.out_c2 + n] = o[idx];
+        const int l = idx >= N ? 1 : 0, n = idx - l * N;
+        if (line0 + l >= a.C) continue;
+        out[ob[l] + n] = o[idx];
     }
 }
 

@@ -375,6 +375,8 @@ __global__ __launch_bounds__(256) void k_xfwd_mx(Dims d, Metrics mt, XArgs a, co
                                                  float *__restrict__ out, double *__restrict__ part) {
     extern __shared__ float smx[];
     const int N = a.N, N1 = a.N1, N2 = a.N2, K1 = a.K1h + 1;
+    // twiddle indices n2 k1 stay below N (n2 < N2, k1 <= N1/2): no reduction
+    const float rN2 = 1.0f / (float)N2, rK1 = 1.0f / (float)K1;
     float *v = smx;                                         // kXL*N
     float2 *Bc = reinterpret_cast<float2 *>(v + kXL * N);   // kXL*K1*N2 (8-byte aligned: kXL*N even)
     __shared__ double lsum[kXL][4];
@@ -411,7 +413,7 @@ __global__ __launch_bounds__(256) void k_xfwd_mx(Dims d, Metrics mt, XArgs a, co
     // ---- step A: wave w owns column tiles (line, n2) kXL/2 w .. kXL/2 (w+1)
     for (int ct = wave * (kXL / 2); ct < (wave + 1) * (kXL / 2); ++ct) {
         const int col = ct * 16 + c16;
-        const int ln = col / N2, n2 = col - ln * N2;
+        const int ln = small_div(col, rN2), n2 = col - ln * N2;
         const bool cok = col < kXL * N2;
         float x[8];
 #pragma unroll
@@ -431,7 +433,7 @@ __global__ __launch_bounds__(256) void k_xfwd_mx(Dims d, Metrics mt, XArgs a, co
                 const int k1 = 8 * (h / 2) + 2 * q + (h % 2);
                 if (k1 < K1)
                     Bc[(ln * K1 + k1) * N2 + n2] =
-                        cmulf(make_float2(acc[2 * (h % 2)], acc[2 * (h % 2) + 1]), a.wN[(n2 * k1) % N]);
+                        cmulf(make_float2(acc[2 * (h % 2)], acc[2 * (h % 2) + 1]), a.wN[n2 * k1]);
             }
         }
     }
@@ -440,7 +442,7 @@ __global__ __launch_bounds__(256) void k_xfwd_mx(Dims d, Metrics mt, XArgs a, co
     // column tiles (line, k1)
     for (int ct = 0; ct < kXL; ++ct) {
         const int col = ct * 16 + c16;
-        const int ln = col / K1, k1 = col - ln * K1;
+        const int ln = small_div(col, rK1), k1 = col - ln * K1;
         const bool cok = col < kXL * K1;
         const float *Bl = reinterpret_cast<const float *>(Bc + (ln * K1 + k1) * N2);
         f32x4 acc = {0.f, 0.f, 0.f, 0.f};
@@ -473,7 +475,7 @@ __global__ __launch_bounds__(256) void k_xfwd_mx(Dims d, Metrics mt, XArgs a, co
     }
     __syncthreads();
     for (int idx = tid; idx < kXL * N; idx += 256) {
-        const int l = idx / N, kx = idx - l * N;
+        const int l = idx >= N ? 1 : 0, kx = idx - l * N;  // kXL = 2
         const int c = line0 + l;
         if (c < a.C) out[(long long)c * a.NP + kx] = v[idx];
     }
@@ -485,6 +487,8 @@ __global__ __launch_bounds__(256) void k_xinv_mx(Dims d, XArgs a, const float *_
                                                  const double *__restrict__ add, float *__restrict__ phi) {
     extern __shared__ float smx[];
     const int N = a.N, N1 = a.N1, N2 = a.N2, K1h = a.K1h, K1 = K1h + 1;
+    // twiddle indices n2 k1 stay below N (n2 < N2, k1 <= N1/2): no reduction
+    const float rN2 = 1.0f / (float)N2, rK1 = 1.0f / (float)K1;
     float2 *Vs = reinterpret_cast<float2 *>(smx);  // kXL*K1*N2, later the output lines (kXL*N floats)
     float2 *Cs = Vs + kXL * K1 * N2;               // kXL*K1*N2
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63, q = lane >> 4, c16 = lane & 15;
@@ -502,7 +506,7 @@ __global__ __launch_bounds__(256) void k_xinv_mx(Dims d, XArgs a, const float *_
         for (int u = 0; u < kLdx; ++u) {
             const int idx = min(i0 + 256 * u, nv - 1);
             const int l = idx >= K1 * N2 ? 1 : 0, r = idx - l * K1 * N2;
-            const int k1 = r / N2, k2 = r - k1 * N2;
+            const int k1 = small_div(r, rN2), k2 = r - k1 * N2;
             const int k = k1 + N1 * k2;
             gk[u] = gls[l][k];
             gm[u] = gls[l][k == 0 ? 0 : N - k];
@@ -520,7 +524,7 @@ __global__ __launch_bounds__(256) void k_xinv_mx(Dims d, XArgs a, const float *_
     // tiles (line, k1)
     for (int ct = 0; ct < kXL; ++ct) {
         const int col = ct * 16 + c16;
-        const int ln = col / K1, k1 = col - ln * K1;
+        const int ln = small_div(col, rK1), k1 = col - ln * K1;
         const bool cok = col < kXL * K1;
         const float *Vl = reinterpret_cast<const float *>(Vs + (ln * K1 + k1) * N2);
         f32x4 acc = {0.f, 0.f, 0.f, 0.f};
@@ -542,7 +546,7 @@ __global__ __launch_bounds__(256) void k_xinv_mx(Dims d, XArgs a, const float *_
                 const int n2 = 8 * wave + 2 * q + h;
                 if (n2 < N2)
                     Cs[(ln * K1 + k1) * N2 + n2] =
-                        cmulcf(make_float2(acc[2 * h], acc[2 * h + 1]), a.wN[(n2 * k1) % N]);
+                        cmulcf(make_float2(acc[2 * h], acc[2 * h + 1]), a.wN[n2 * k1]);
             }
         }
     }
@@ -552,7 +556,7 @@ __global__ __launch_bounds__(256) void k_xinv_mx(Dims d, XArgs a, const float *_
     float *o = smx;  // kXL*N floats inside the Vs area, which is no longer read
     for (int ct = wave * (kXL / 2); ct < (wave + 1) * (kXL / 2); ++ct) {
         const int col = ct * 16 + c16;
-        const int ln = col / N2, n2 = col - ln * N2;
+        const int ln = small_div(col, rN2), n2 = col - ln * N2;
         const bool cok = col < kXL * N2;
         float x[8];
 #pragma unroll
@@ -578,12 +582,20 @@ __global__ __launch_bounds__(256) void k_xinv_mx(Dims d, XArgs a, const float *_
         }
     }
     __syncthreads();
+    // per-line output bases and (0,0) constants, then one element per thread
+    // and step (kXL = 2)
+    long long ob[kXL];
+    float ad[kXL];
+#pragma unroll
+    for (int l = 0; l < kXL; ++l) {
+        const int c = min(line0 + l, a.C - 1), c1 = c % a.C1, c2 = c / a.C1;
+        ob[l] = fidx(d, 1, c1 + 1, c2 + 1);
+        ad[l] = (float)add[c1];
+    }
     for (int idx = tid; idx < kXL * N; idx += 256) {
-        const int l = idx / N, n = idx - l * N;
-        const int c = line0 + l;
-        if (c >= a.C) continue;
-        const int c1 = c % a.C1, c2 = c / a.C1;
-        phi[fidx(d, n + 1, c1 + 1, c2 + 1)] = o[idx] + (float)add[c1];
+        const int l = idx >= N ? 1 : 0, n = idx - l * N;
+        if (line0 + l >= a.C) continue;
+        phi[ob[l] + n] = o[idx] + ad[l];
     }
 }
 
