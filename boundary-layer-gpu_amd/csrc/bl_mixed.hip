@@ -73,6 +73,21 @@ struct GemmArgs {
 // K step BK (a multiple of 32); an LDS row of one column holds BK + 8 values
 constexpr int kGemmLds = 163840;  // LDS bytes per workgroup on gfx950
 
+// Offset of column col of a B tile in LDS (bf16 units): rows of LDK = BK + 8
+// plus 16 bytes per group of 16 columns.  The transposed stores of a wave
+// (lane cg writes columns 4 cg .. 4 cg + 3) then meet each bank at most twice
+// instead of eight times (64 banks of 4 bytes), while the 16 columns that one
+// fragment read covers share one skew and stay conflict-free; each column's
+// k run stays 16-byte aligned and the runs do not overlap.
+template <int BK>
+__device__ __forceinline__ constexpr int gemm_bcol(int col) {
+    return col * (BK + 8) + (col >> 4) * 8;
+}
+template <int WC, int BK>
+constexpr int gemm_bsize() {
+    return 64 * WC * (BK + 8) + 4 * WC * 8;
+}
+
 // float4 loads of the B tile per thread
 template <int WR, int WC, int BK>
 constexpr int gemm_bloads() {
@@ -116,7 +131,7 @@ template <int WR, int WC, int BK, bool KCONTIG>
 __device__ __forceinline__ void gemm_store_b(const GemmArgs &g, int c0, int kk, int tid,
                                              const float4 (&v)[gemm_bloads<WR, WC, BK>()], unsigned short *sBh,
                                              unsigned short *sBl) {
-    constexpr int NT = 64 * WR * WC, BN = 64 * WC, NL = gemm_bloads<WR, WC, BK>(), LDK = BK + 8;
+    constexpr int NT = 64 * WR * WC, BN = 64 * WC, NL = gemm_bloads<WR, WC, BK>();
     if constexpr (KCONTIG) {
         constexpr int TPC = BK / 4;
         const int k = 4 * (tid % TPC);
@@ -130,7 +145,7 @@ __device__ __forceinline__ void gemm_store_b(const GemmArgs &g, int c0, int kk, 
             unsigned short h[4], l[4];
 #pragma unroll
             for (int q = 0; q < 4; ++q) split2(e[q], h[q], l[q]);
-            const int o = col * LDK + k;
+            const int o = gemm_bcol<BK>(col) + k;
             *reinterpret_cast<uint2 *>(&sBh[o]) =
                 make_uint2((unsigned)h[0] | ((unsigned)h[1] << 16), (unsigned)h[2] | ((unsigned)h[3] << 16));
             *reinterpret_cast<uint2 *>(&sBl[o]) =
@@ -150,7 +165,7 @@ __device__ __forceinline__ void gemm_store_b(const GemmArgs &g, int c0, int kk, 
                 unsigned short h0, l0, h1, l1;
                 split2(k0 ? a0[e] : 0.f, h0, l0);
                 split2(k1 ? a1[e] : 0.f, h1, l1);
-                const int o = (4 * cg + e) * LDK + k;
+                const int o = gemm_bcol<BK>(4 * cg + e) + k;
                 *reinterpret_cast<unsigned int *>(&sBh[o]) = (unsigned)h0 | ((unsigned)h1 << 16);
                 *reinterpret_cast<unsigned int *>(&sBl[o]) = (unsigned)l0 | ((unsigned)l1 << 16);
             }
@@ -198,8 +213,8 @@ __global__ __launch_bounds__(64 * WR * WC) void k_gemm_bf16x3(GemmArgs g, int ti
     constexpr int kLDK = BK + 8;
     static_assert(BK % 32 == 0, "K step");
     // one buffer of each operand: [row or column][k], rows padded to kLDK
-    __shared__ unsigned short sA[2][BM * kLDK], sB[2][BN * kLDK];  // [hi, lo]
-    static_assert(sizeof(unsigned short) * 2 * (BM + BN) * kLDK <= kGemmLds, "LDS per workgroup");
+    __shared__ unsigned short sA[2][BM * kLDK], sB[2][gemm_bsize<WC, BK>()];  // [hi, lo]
+    static_assert(sizeof(unsigned short) * 2 * (BM * kLDK + gemm_bsize<WC, BK>()) <= kGemmLds, "LDS per workgroup");
     // XCD-aware order: blocks that share a column tile run on one XCD (the
     // hardware deals consecutive block ids round-robin over 8 XCDs)
     const int b = blockIdx.x;
@@ -238,7 +253,7 @@ __global__ __launch_bounds__(64 * WR * WC) void k_gemm_bf16x3(GemmArgs g, int ti
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
                 const int oa = (wr + q * 16 + l16) * kLDK + 32 * ks + 8 * lq;
-                const int ob = (wc + q * 16 + l16) * kLDK + 32 * ks + 8 * lq;
+                const int ob = gemm_bcol<BK>(wc + q * 16 + l16) + 32 * ks + 8 * lq;
                 ah[q] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const u16x8 *>(&sA[0][oa]));
                 al[q] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const u16x8 *>(&sA[1][oa]));
                 bh[q] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const u16x8 *>(&sB[0][ob]));
