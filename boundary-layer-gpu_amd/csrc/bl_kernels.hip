@@ -889,15 +889,17 @@ __device__ __forceinline__ dbl4 mfma64(double a, double b, dbl4 c) {
 }
 
 template <bool DIV>
-__global__ __launch_bounds__(256) void k_dct_fwd_m(DctArgs a, Dims d, Metrics mt, const double *__restrict__ in,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k_dct_fwd_m(DctArgs a, Dims d, Metrics mt, const double *__restrict__ in,
                                                    const double *__restrict__ in2, const double *__restrict__ in3,
                                                    double *__restrict__ out) {
     extern __shared__ double smem[];
     const int N = a.N, N1 = a.N1, N2 = a.N2, K1 = a.K1h + 1;
     // twiddle indices n2 k1 stay below N (n2 < N2, k1 <= N1/2): no reduction
     const float rN2 = 1.0f / (float)N2, rK1 = 1.0f / (float)K1;
-    double *v = smem;                                         // kMfL*N
-    double2 *Bc = reinterpret_cast<double2 *>(v + kMfL * N);  // kMfL*K1*N2
+    // v (kMfL*N doubles) and Bc (kMfL*K1*N2 complex) share one LDS area: each
+    // step finishes reading its input before a barrier and then writes its output
+    double *v = smem;
+    double2 *Bc = reinterpret_cast<double2 *>(smem);
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63, q = lane >> 4, c16 = lane & 15;
     const int line0 = blockIdx.x * kMfL;
     // load and reorder: v[n/2] = f[n] (n even), v[N-1-(n-1)/2] = f[n] (n odd);
@@ -959,6 +961,8 @@ __global__ __launch_bounds__(256) void k_dct_fwd_m(DctArgs a, Dims d, Metrics mt
                 acc1 = mfma64(fa[1][ks], b, acc1);
             }
         }
+        static_assert(kMfL == 2, "one step-A column tile per wave");
+        __syncthreads();  // v is read; Bc overwrites it
         if (cok) {
 #pragma unroll
             for (int h = 0; h < 4; ++h) {  // (row tile, pair): k1 = 8 (h/2) + q + 4 (h%2)
@@ -978,6 +982,8 @@ __global__ __launch_bounds__(256) void k_dct_fwd_m(DctArgs a, Dims d, Metrics mt
         double fc[kMfKC];
 #pragma unroll
         for (int ks = 0; ks < kMfKC; ++ks) fc[ks] = ks < KSC ? a.fC[(wave * KSC + ks) * 64 + lane] : 0.0;
+        dbl4 accs[kMfL];
+#pragma unroll
         for (int ct = 0; ct < kMfL; ++ct) {
             const int col = ct * 16 + c16;
             const int ln = small_div(col, rK1), k1 = col - ln * K1;
@@ -992,6 +998,15 @@ __global__ __launch_bounds__(256) void k_dct_fwd_m(DctArgs a, Dims d, Metrics mt
                     acc = mfma64(fc[ks], b, acc);
                 }
             }
+            accs[ct] = acc;
+        }
+        __syncthreads();  // Bc is read; v overwrites it
+#pragma unroll
+        for (int ct = 0; ct < kMfL; ++ct) {
+            const int col = ct * 16 + c16;
+            const int ln = small_div(col, rK1), k1 = col - ln * K1;
+            const bool cok = col < kMfL * K1;
+            const dbl4 acc = accs[ct];
             if (cok) {
 #pragma unroll
                 for (int h = 0; h < 2; ++h) {
@@ -1023,13 +1038,16 @@ __global__ __launch_bounds__(256) void k_dct_fwd_m(DctArgs a, Dims d, Metrics mt
     }
 }
 
-__global__ __launch_bounds__(256) void k_dct_inv_m(DctArgs a, const double *__restrict__ in, double *__restrict__ out) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k_dct_inv_m(DctArgs a, const double *__restrict__ in, double *__restrict__ out) {
     extern __shared__ double smem[];
     const int N = a.N, N1 = a.N1, N2 = a.N2, K1h = a.K1h, K1 = K1h + 1;
     // twiddle indices n2 k1 stay below N (n2 < N2, k1 <= N1/2): no reduction
     const float rN2 = 1.0f / (float)N2, rK1 = 1.0f / (float)K1;
-    double2 *Vs = reinterpret_cast<double2 *>(smem);  // kMfL*K1*N2, later the output lines (kMfL*N doubles)
-    double2 *Cs = Vs + kMfL * K1 * N2;                // kMfL*K1*N2
+    // Vs, Cs (kMfL*K1*N2 complex each) and the output lines o (kMfL*N doubles)
+    // share one LDS area: each step finishes reading its input before a
+    // barrier and then writes its output
+    double2 *Vs = reinterpret_cast<double2 *>(smem);
+    double2 *Cs = Vs;
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63, q = lane >> 4, c16 = lane & 15;
     const int line0 = blockIdx.x * kMfL;
     // V[k1 + N1 k2] = conj(rot_k) (g[k] - i g[N-k]) / 2 for k1 <= N1/2: Vs[l][k1][k2];
@@ -1071,6 +1089,8 @@ __global__ __launch_bounds__(256) void k_dct_inv_m(DctArgs a, const double *__re
         double fc[kMfKC];
 #pragma unroll
         for (int ks = 0; ks < kMfKC; ++ks) fc[ks] = ks < KSC ? a.iC[(wave * KSC + ks) * 64 + lane] : 0.0;
+        dbl4 accs[kMfL];
+#pragma unroll
         for (int ct = 0; ct < kMfL; ++ct) {
             const int col = ct * 16 + c16;
             const int ln = small_div(col, rK1), k1 = col - ln * K1;
@@ -1085,6 +1105,15 @@ __global__ __launch_bounds__(256) void k_dct_inv_m(DctArgs a, const double *__re
                     acc = mfma64(fc[ks], b, acc);
                 }
             }
+            accs[ct] = acc;
+        }
+        __syncthreads();  // Vs is read; Cs overwrites it
+#pragma unroll
+        for (int ct = 0; ct < kMfL; ++ct) {
+            const int col = ct * 16 + c16;
+            const int ln = small_div(col, rK1), k1 = col - ln * K1;
+            const bool cok = col < kMfL * K1;
+            const dbl4 acc = accs[ct];
             if (cok) {
 #pragma unroll
                 for (int h = 0; h < 2; ++h) {
@@ -1099,7 +1128,7 @@ __global__ __launch_bounds__(256) void k_dct_inv_m(DctArgs a, const double *__re
     __syncthreads();
     // ---- step A': v[N2 n1 + n2] = sum over (k1, re/im) of WA'[n1][.] C~;
     // wave w owns kMfL / 2 column tiles (columns (line, n2)), both row tiles
-    double *o = smem;  // kMfL*N doubles inside the Vs area, which is no longer read
+    double *o = smem;  // kMfL*N doubles over Cs, written after Cs is read
     {
         double fa[2][kMfKA];
 #pragma unroll
@@ -1124,6 +1153,8 @@ __global__ __launch_bounds__(256) void k_dct_inv_m(DctArgs a, const double *__re
                     acc1 = mfma64(fa[1][ks], b, acc1);
                 }
             }
+            static_assert(kMfL == 2, "one step-A' column tile per wave");
+            __syncthreads();  // Cs is read; o overwrites it
             if (cok) {
 #pragma unroll
                 for (int r = 0; r < 8; ++r) {  // n1 = 16 (r / 4) + q + 4 (r % 4)
@@ -1619,8 +1650,9 @@ void dct_launch(Handle &h, int kid, const DctArgs &a, bool forward, const double
     const int K1 = a.K1h + 1;
     if (a.fA) {
         // matrix-core kernels (the shape was checked with the tables)
-        const size_t lds = forward ? sizeof(double) * (size_t)kMfL * a.N + sizeof(double2) * (size_t)kMfL * K1 * a.N2
-                                   : 2 * sizeof(double2) * (size_t)kMfL * K1 * a.N2;
+        // one shared area per block: the lines (kMfL*N doubles) and the
+        // intermediate spectra (kMfL*K1*N2 complex) in turn
+        const size_t lds = std::max(sizeof(double) * (size_t)kMfL * a.N, sizeof(double2) * (size_t)kMfL * K1 * a.N2);
         if (lds > 163840) {
             last_error() = "x transform length too large for the LDS-resident DCT (N = " + std::to_string(a.N) + ")";
             throw ArgError();
