@@ -188,6 +188,11 @@ void build_dyy(const HostSetup &g, int nyg, std::vector<double> &dd, std::vector
 }
 
 void build(Handle &h, const bl_config *c) {
+    {
+        int dev = 0;
+        BL_HIP(hipGetDevice(&dev));
+        BL_HIP(hipDeviceGetAttribute(&h.n_cu, hipDeviceAttributeMultiprocessorCount, dev));
+    }
     Dims &d = h.d;
     d.nx = c->nx;
     d.ny = c->ny;

@@ -172,6 +172,7 @@ struct Handle {
     double *lu_dl, *lu_dp, *lu_du;  // Thomas factors [k][j][kx], [k][j][kx], [j] (table kernel only)
     double *lu00;                   // the pinned (0,0) system: multipliers [NM], reciprocal pivots [NM]
     int th_p = 0;                   // segments of the partitioned Thomas kernel (0: table kernel)
+    int n_cu = 0;                   // compute units of the device (persistent launches)
     double *th_co = nullptr;        // its coefficients: a, b, c [32 th_p] each, kxx [N], kzz [Kz]
     double *scal;                   // device scalars (mass-flux correction, reductions)
     double *red;                    // reduction scratch

@@ -1287,9 +1287,10 @@ __device__ void solve00_block(const Solve00 &z, int NM);
 // L = 32 P: a[j] couples x(j-1), c[j] couples x(j+1), b[j] + kxx + kzz is
 // the diagonal.  The (0,0) system (kx = 0, q = 0, pinned first pivot) is
 // skipped (zero00 = 0) or zeroed (zero00 = 1); block 0 solves it (z) while
-// the other blocks run.
+// the other blocks run.  The other blocks (one resident per CU: 8 waves of
+// more than 128 VGPRs each) loop over the nq * ceil(N/64) tiles of 64 systems.
 template <class T>
-__global__ __launch_bounds__(64 * kThMaxP) void k_thomas_seg(int N, int NM, int P, int ldj, long long ldq,
+__global__ __launch_bounds__(64 * kThMaxP) void k_thomas_seg(int N, int NM, int P, int nq, int ldj, long long ldq,
                                                              const double *__restrict__ co, int zero00,
                                                              T *__restrict__ S, Solve00 z) {
     __shared__ double xs[6][kThMaxP][64];
@@ -1299,107 +1300,108 @@ __global__ __launch_bounds__(64 * kThMaxP) void k_thomas_seg(int N, int NM, int 
     }
     const int lane = threadIdx.x & 63;
     const int p = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const int nkb = (N + 63) / 64;
-    const int bid = blockIdx.x - 1;
-    const int q = bid / nkb;
-    const int kx = (bid - q * nkb) * 64 + lane;
-    const bool live = kx < N;
-    const int kxc = live ? kx : N - 1;
+    const int nkb = (N + 63) / 64, ntiles = nkb * nq, nblk = gridDim.x - 1;
     const int L = P * kThSeg;
     const double *ca = co, *cb = co + L, *cc = co + 2 * L;
-    const double lam = co[3 * L + kxc] + co[3 * L + N + (q + 1) / 2];
     const int r0 = p * kThSeg;
-    T *col = S + q * ldq + kxc;
-    double dv[kThSeg], ip[kThSeg];
-    {
-        // unconditional loads (rows past NM clamped, then zeroed), all issued
-        // before the first conversion waits on one
-        T raw[kThSeg];
+    // tiles (q, 64 kx) bid, bid + nblk, ... (holding the next tile's rows
+    // in registers during a solve spills them)
+    for (int tile = blockIdx.x - 1; tile < ntiles; tile += nblk) {
+        const int q = tile / nkb;
+        const int kx = (tile - q * nkb) * 64 + lane;
+        const bool live = kx < N;
+        const int kxc = live ? kx : N - 1;
+        const double lam = co[3 * L + kxc] + co[3 * L + N + (q + 1) / 2];
+        T *col = S + q * ldq + kxc;
+        double dv[kThSeg], ip[kThSeg];
+        {
+            // unconditional loads (rows past NM clamped, then zeroed), all
+            // issued before the first conversion waits on one
+            T raw[kThSeg];
 #pragma unroll
-        for (int i = 0; i < kThSeg; ++i) raw[i] = col[(long long)min(r0 + i, NM - 1) * ldj];
-        __builtin_amdgcn_sched_barrier(0);
+            for (int i = 0; i < kThSeg; ++i) raw[i] = col[(long long)min(r0 + i, NM - 1) * ldj];
+            __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-        for (int i = 0; i < kThSeg; ++i) dv[i] = (r0 + i < NM) ? (double)raw[i] : 0.0;
-    }
-    // Each sweep reads the coefficients through its own opaque copy of the
-    // pointer, so that a value loaded by one sweep is not kept in registers
-    // for the other sweeps' use of the same row.
-    const cdptr la = opaque(ca), lb = opaque(cb), lc = opaque(cc);
-    const cdptr ua = opaque(ca), ub = opaque(cb), uc = opaque(cc);
-    // LU (index i) and UL (index kThSeg-1-i) sweeps, interleaved
-    double dp = dv[0], s = 1.0;
-    ip[0] = rcp64(lb[r0] + (r0 < NM ? lam : 0.0));
-    const int rl = r0 + kThSeg - 1;
-    double iu = rcp64(ub[rl] + (rl < NM ? lam : 0.0)), du = dv[kThSeg - 1], t = 1.0;
+            for (int i = 0; i < kThSeg; ++i) dv[i] = (r0 + i < NM) ? (double)raw[i] : 0.0;
+        }
+        // Each sweep reads the coefficients through its own opaque copy of the
+        // pointer, so that a value loaded by one sweep is not kept in registers
+        // for the other sweeps' use of the same row.
+        const cdptr la = opaque(ca), lb = opaque(cb), lc = opaque(cc);
+        const cdptr ua = opaque(ca), ub = opaque(cb), uc = opaque(cc);
+        // LU (index i) and UL (index kThSeg-1-i) sweeps, interleaved
+        double dp = dv[0], s = 1.0;
+        ip[0] = rcp64(lb[r0] + (r0 < NM ? lam : 0.0));
+        const int rl = r0 + kThSeg - 1;
+        double iu = rcp64(ub[rl] + (rl < NM ? lam : 0.0)), du = dv[kThSeg - 1], t = 1.0;
 #pragma unroll
-    for (int i = 1; i < kThSeg; ++i) {
-        const int j = r0 + i;
-        const double m = la[j] * ip[i - 1];
-        ip[i] = rcp64(lb[j] + (j < NM ? lam : 0.0) - m * lc[j - 1]);
-        dp = dv[i] - m * dp;
-        s = -m * s;
-        const int iv = kThSeg - 1 - i, jv = r0 + iv;
-        const double mu = uc[jv] * iu;
-        iu = rcp64(ub[jv] + (jv < NM ? lam : 0.0) - mu * ua[jv + 1]);
-        du = dv[iv] - mu * du;
-        t = -mu * t;
-        if (i % 4 == 0) __builtin_amdgcn_sched_barrier(0);
-    }
-    const double aL = ca[r0], cR = cc[r0 + kThSeg - 1];
-    xs[0][p][lane] = du * iu;                        // y first
-    xs[1][p][lane] = -aL * iu;                       // d x_first / d xL
-    xs[2][p][lane] = -cR * t * iu;                   // d x_first / d xR
-    xs[3][p][lane] = dp * ip[kThSeg - 1];            // y last
-    xs[4][p][lane] = -aL * s * ip[kThSeg - 1];       // d x_last / d xL
-    xs[5][p][lane] = -cR * ip[kThSeg - 1];           // d x_last / d xR
-    __syncthreads();
-    // segment ends: A_p (first row), B_p (last row); B_{p-1} = u + v A_p
-    double g[kThMaxP], hh[kThMaxP];
-    double u = 0.0, v = 0.0, ul = 0.0, vl = 0.0;  // (ul, vl): B_{p-1} = ul + vl A_p
+        for (int i = 1; i < kThSeg; ++i) {
+            const int j = r0 + i;
+            const double m = la[j] * ip[i - 1];
+            ip[i] = rcp64(lb[j] + (j < NM ? lam : 0.0) - m * lc[j - 1]);
+            dp = dv[i] - m * dp;
+            s = -m * s;
+            const int iv = kThSeg - 1 - i, jv = r0 + iv;
+            const double mu = uc[jv] * iu;
+            iu = rcp64(ub[jv] + (jv < NM ? lam : 0.0) - mu * ua[jv + 1]);
+            du = dv[iv] - mu * du;
+            t = -mu * t;
+            if (i % 4 == 0) __builtin_amdgcn_sched_barrier(0);
+        }
+        const double aL = ca[r0], cR = cc[r0 + kThSeg - 1];
+        xs[0][p][lane] = du * iu;                        // y first
+        xs[1][p][lane] = -aL * iu;                       // d x_first / d xL
+        xs[2][p][lane] = -cR * t * iu;                   // d x_first / d xR
+        xs[3][p][lane] = dp * ip[kThSeg - 1];            // y last
+        xs[4][p][lane] = -aL * s * ip[kThSeg - 1];       // d x_last / d xL
+        xs[5][p][lane] = -cR * ip[kThSeg - 1];           // d x_last / d xR
+        __syncthreads();
+        // segment ends: A_p (first row), B_p (last row); B_{p-1} = u + v A_p
+        double g[kThMaxP], hh[kThMaxP];
+        double u = 0.0, v = 0.0, ul = 0.0, vl = 0.0;  // (ul, vl): B_{p-1} = ul + vl A_p
 #pragma unroll
-    for (int pp = 0; pp < kThMaxP; ++pp) {
-        if (pp < P) {
-            if (pp == p) {
-                ul = u;
-                vl = v;
+        for (int pp = 0; pp < kThMaxP; ++pp) {
+            if (pp < P) {
+                if (pp == p) {
+                    ul = u;
+                    vl = v;
+                }
+                const double a0 = xs[1][pp][lane], an = xs[4][pp][lane];
+                const double den = 1.0 / (1.0 - a0 * v);
+                g[pp] = (xs[0][pp][lane] + a0 * u) * den;
+                hh[pp] = xs[2][pp][lane] * den;
+                const double un = xs[3][pp][lane] + an * (u + v * g[pp]);
+                v = an * v * hh[pp] + xs[5][pp][lane];
+                u = un;
             }
-            const double a0 = xs[1][pp][lane], an = xs[4][pp][lane];
-            const double den = 1.0 / (1.0 - a0 * v);
-            g[pp] = (xs[0][pp][lane] + a0 * u) * den;
-            hh[pp] = xs[2][pp][lane] * den;
-            const double un = xs[3][pp][lane] + an * (u + v * g[pp]);
-            v = an * v * hh[pp] + xs[5][pp][lane];
-            u = un;
+        }
+        __syncthreads();  // xs is read; the next tile writes it
+        double Anext = 0.0, xL = 0.0, xR = 0.0;
+#pragma unroll
+        for (int pp = kThMaxP - 1; pp >= 0; --pp) {
+            if (pp < P) {
+                const double Ap = g[pp] + hh[pp] * Anext;
+                if (pp == p + 1) xR = Ap;
+                if (pp == p) xL = ul + vl * Ap;
+                Anext = Ap;
+            }
+        }
+        // the segment with its neighbour values on the right-hand side
+        const cdptr fa = opaque(ca), fc = opaque(cc);
+        dv[0] -= aL * xL;
+        dv[kThSeg - 1] -= cR * xR;
+#pragma unroll
+        for (int i = 1; i < kThSeg; ++i) dv[i] -= fa[r0 + i] * ip[i - 1] * dv[i - 1];
+        dv[kThSeg - 1] *= ip[kThSeg - 1];
+#pragma unroll
+        for (int i = kThSeg - 2; i >= 0; --i) dv[i] = (dv[i] - fc[r0 + i] * dv[i + 1]) * ip[i];
+        const bool is00 = q == 0 && kx == 0;
+        if (live && !(is00 && !zero00)) {
+#pragma unroll
+            for (int i = 0; i < kThSeg; ++i)
+                if (r0 + i < NM) col[(long long)(r0 + i) * ldj] = is00 ? (T)0 : (T)dv[i];
         }
     }
-    double Anext = 0.0, xL = 0.0, xR = 0.0;
-#pragma unroll
-    for (int pp = kThMaxP - 1; pp >= 0; --pp) {
-        if (pp < P) {
-            const double Ap = g[pp] + hh[pp] * Anext;
-            if (pp == p + 1) xR = Ap;
-            if (pp == p) xL = ul + vl * Ap;
-            Anext = Ap;
-        }
-    }
-    // the segment with its neighbour values on the right-hand side
-    const cdptr fa = opaque(ca), fc = opaque(cc);
-    dv[0] -= aL * xL;
-    dv[kThSeg - 1] -= cR * xR;
-#pragma unroll
-    for (int i = 1; i < kThSeg; ++i) dv[i] -= fa[r0 + i] * ip[i - 1] * dv[i - 1];
-    dv[kThSeg - 1] *= ip[kThSeg - 1];
-#pragma unroll
-    for (int i = kThSeg - 2; i >= 0; --i) dv[i] = (dv[i] - fc[r0 + i] * dv[i + 1]) * ip[i];
-    if (!live) return;
-    if (q == 0 && kx == 0) {
-        if (!zero00) return;
-#pragma unroll
-        for (int i = 0; i < kThSeg; ++i) dv[i] = 0.0;
-    }
-#pragma unroll
-    for (int i = 0; i < kThSeg; ++i)
-        if (r0 + i < NM) col[(long long)(r0 + i) * ldj] = (T)dv[i];
 }
 
 // The pinned (0,0) system (kx = 0, kz = 0) in place in z.x (row stride
@@ -1854,8 +1856,12 @@ template <class T>
 bool launch_thomas_seg(Handle &h, T *S, int ldj, long long ldq, const Solve00 &z) {
     const Dims &d = h.d;
     if (h.th_p == 0) return false;
-    hipLaunchKernelGGL(k_thomas_seg<T>, dim3(1 + (d.N + 63) / 64 * d.M), dim3(64 * h.th_p), 0, h.stream, d.N,
-                       d.NM, h.th_p, ldj, ldq, (const double *)h.th_co, z.part ? 1 : 0, S, z);
+    // one resident block per CU (the registers of 8 waves), each looping over
+    // its tiles, plus block 0 for the (0,0) system
+    const int ntiles = (d.N + 63) / 64 * d.M;
+    const int nblk = std::max(1, std::min(ntiles, h.n_cu > 1 ? h.n_cu - 1 : ntiles));
+    hipLaunchKernelGGL(k_thomas_seg<T>, dim3(1 + nblk), dim3(64 * h.th_p), 0, h.stream, d.N, d.NM, h.th_p, d.M,
+                       ldj, ldq, (const double *)h.th_co, z.part ? 1 : 0, S, z);
     check_launch();
     return true;
 }
