@@ -1702,8 +1702,6 @@ const char *kernel_name(int id) {
 // ------------------------------------------------------------------------
 // launch wrappers
 // ------------------------------------------------------------------------
-constexpr int kChunk = 16;
-
 template <>
 const MetricsT<double> &metrics<double>(const Handle &h) {
     return h.m;
@@ -1715,17 +1713,33 @@ const MetricsT<float> &metrics<float>(const Handle &h) {
 
 
 template <class T>
-dim3 march_grid(const Dims &d) {
+dim3 march_grid(const Dims &d, int chunk) {
     constexpr int kTX = MarchTile<T>::TX, kTY = MarchTile<T>::TY;
     return dim3(xcd_blocks((d.nxg - 1 + kTX - 1) / kTX * ((d.nyg - 2 + kTY - 1) / kTY) *
-                           ((d.nzg - 2 + kChunk - 1) / kChunk)));
+                           ((d.nzg - 2 + chunk - 1) / chunk)));
+}
+
+// z-chunk length of the march.  The tiles run in XCD-contiguous order with
+// the z chunk slowest, so with a multiple of 8 chunks each XCD holds whole
+// chunk layers and a tile's x and y neighbours (whose halos it shares) sit in
+// the same L2; the count is the multiple of 8 that gives chunks of about 24
+// planes.  At 814x257x193 (8 chunks of 24 planes) this measured 1 % (FP64)
+// and 2.4 % (FP32) faster than 12 chunks of 16, and 20-plane chunks (10
+// chunks) 4-5 % slower.  Below 128 planes, chunks of up to 16 planes (the
+// two re-read halo planes stay a small share).
+inline int march_chunk(const Dims &d) {
+    const int nz = d.nzg - 2;
+    const int nch = nz >= 8 * 16 ? 8 * std::max(1, (nz + 8 * 12) / (8 * 24))  // multiple of 8 nearest nz / 24
+                                 : (nz + 15) / 16;
+    return std::max(1, (nz + nch - 1) / nch);
 }
 
 template <int MODE, bool WRITE_F1, class T>
 void march(Handle &h, const Fld3<T> &cur, const Fld3<T> *old, const Fld3<T> *f1, Fld3<T> *nxt, Fld3<T> *fo,
            double c0, double c1, double dt) {
     const Dims &d = h.d;
-    const dim3 grid = march_grid<T>(d), block(256);
+    const int chunk = march_chunk(d);
+    const dim3 grid = march_grid<T>(d, chunk), block(256);
     const T *o[3] = {old ? old->u : nullptr, old ? old->v : nullptr, old ? old->w : nullptr};
     const T *f[3] = {f1 ? f1->u : nullptr, f1 ? f1->v : nullptr, f1 ? f1->w : nullptr};
     T *n[3] = {nxt ? nxt->u : nullptr, nxt ? nxt->v : nullptr, nxt ? nxt->w : nullptr};
@@ -1733,12 +1747,12 @@ void march(Handle &h, const Fld3<T> &cur, const Fld3<T> *old, const Fld3<T> *f1,
     const MetricsT<T> &m = metrics<T>(h);
     if (h.les != 0)
         hipLaunchKernelGGL((k_rk_march<MODE, WRITE_F1, true, T>), grid, block, 0, h.stream, d, m, (T)h.nu,
-                           (T)h.dPdx, (T)h.dPdz, kChunk, cur.u, cur.v, cur.w, reinterpret_cast<const T *>(h.nu_t),
+                           (T)h.dPdx, (T)h.dPdz, chunk, cur.u, cur.v, cur.w, reinterpret_cast<const T *>(h.nu_t),
                            o[0], o[1], o[2], f[0], f[1], f[2], n[0], n[1], n[2], r[0], r[1], r[2], (T)c0, (T)c1,
                            (T)dt);
     else
         hipLaunchKernelGGL((k_rk_march<MODE, WRITE_F1, false, T>), grid, block, 0, h.stream, d, m, (T)h.nu,
-                           (T)h.dPdx, (T)h.dPdz, kChunk, cur.u, cur.v, cur.w, nullptr, o[0], o[1], o[2], f[0],
+                           (T)h.dPdx, (T)h.dPdz, chunk, cur.u, cur.v, cur.w, nullptr, o[0], o[1], o[2], f[0],
                            f[1], f[2], n[0], n[1], n[2], r[0], r[1], r[2], (T)c0, (T)c1, (T)dt);
     check_launch();
 }
