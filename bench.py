@@ -75,13 +75,15 @@ def kernel_units(name, nx, ny, nz):
     # LDS, not GEMMs: their algorithmic work is the HBM traffic of the lines
     per_cell_passes = {
         # one pass = one read or write of an N_cell FP64 array
-        "rk_stage": 11.0,  # mean over RK3 stages of (read U,V,W[,Uo,F1]; write U*[,F1])
+        # mean over the FP64 RK3 stages: stage 1 reads U,V,W and writes U*, G2, G3
+        # (12); stages 2, 3 read U,V,W, G_s and write U* (9 each)
+        "rk_stage": 10.0,
         "divergence": 4.0,
         "xform_x_fwd": 4.0,  # read U, V, W (divergence fused), write the spectrum lines
         "xform_x_inv": 2.0,
         "xform_z_fwd": 2.0,
         "xform_z_inv": 2.0,
-        "thomas": 3.0,
+        "thomas": 2.0,  # read and write the spectrum
         "project": 7.0,
         "pressure": 2.0,
     }

@@ -370,6 +370,11 @@ void build(Handle &h, const bl_config *c) {
     h.F1.u = falloc();
     h.F1.v = falloc();
     h.F1.w = falloc();
+    if (h.precision != BL_PREC_MIXED && c->scheme == 3) {
+        h.G3.u = falloc();
+        h.G3.v = falloc();
+        h.G3.w = falloc();
+    }
     h.P = falloc();
     h.phi = falloc();
     h.nu_t = falloc();
@@ -582,7 +587,8 @@ void free_all(Handle &h) {
         f(h.buf[b].v);
         f(h.buf[b].w);
     }
-    for (void *p : {(void *)h.F1.u, (void *)h.F1.v, (void *)h.F1.w, (void *)h.P, (void *)h.phi,
+    for (void *p : {(void *)h.F1.u, (void *)h.F1.v, (void *)h.F1.w, (void *)h.G3.u, (void *)h.G3.v, (void *)h.G3.w,
+                    (void *)h.P, (void *)h.phi,
                     (void *)h.nu_t, (void *)h.T1, (void *)h.S, (void *)h.dct_w1, (void *)h.dct_w2,
                     (void *)h.dct_wN, (void *)h.dct_rot, (void *)h.dct_mf, h.mx2, (void *)h.zc, (void *)h.zs, (void *)h.lu_dl, (void *)h.lu_dp, (void *)h.lu_du,
                     (void *)h.lu00, (void *)h.th_co,

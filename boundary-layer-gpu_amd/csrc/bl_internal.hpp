@@ -151,7 +151,8 @@ struct Handle {
     Field3 buf[3];  // rotating state buffers
     int iold;       // index of the step-start buffer (Uo)
     int icur;       // index of the current (most recent) buffer
-    Field3 F1;      // first-stage RHS (RK3)
+    Field3 F1;      // first-stage RHS (RK3); FP64 RK3: G2 = Uo + dt a21 F1 instead
+    Field3 G3;      // FP64 RK3: G3 = Uo + dt a31 F1 (else null)
     double *P, *phi, *nu_t;  // float in mixed precision
     double *dsm;  // dynamic Smagorinsky work arrays (LES = 2 and 4), kDsmArrays field-sized slots
     // learned closure (LES = 4): BF16 weights, FP32 biases, input range

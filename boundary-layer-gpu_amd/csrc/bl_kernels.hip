@@ -31,9 +31,13 @@ __device__ __forceinline__ T sq(T a) {
 
 // ------------------------------------------------------------------------
 // Fused RHS + Runge-Kutta stage update, marching in z.
-// MODE 0: new = old + c0*F              (first stage, and RK2 second stage)
-// MODE 1: new = old + dt*(c1*F1 + c0*F) (RK3 stages 2 and 3)
-// MODE 2: write F only (operator test)  A block owns a
+// MODE 0: new = old + c0*F              (first stage, and RK2 second stage;
+//         FP64 RK3 stages 2 and 3 with old = G2 or G3, below)
+// MODE 1: new = old + dt*(c1*F1 + c0*F) (RK3 stages 2 and 3, FP32 fields)
+// MODE 2: write F only (operator test)
+// FOUT (first stage of RK3): 1 writes F1 = F; 2 writes G2 = old + g2*F and
+// G3 = old + g3*F instead (g_s = dt a_s1), so that stages 2 and 3 read one
+// field per component (G_s) instead of two (old and F1).  A block owns a
 // 64 (x) x 4 RPT (y) column of cells and a chunk of z planes; U, V, W of
 // planes k-1, k, k+1 live in an LDS ring with a one-cell halo in x and y, so
 // every velocity value is read from HBM once per chunk (plus halo) instead of
@@ -53,7 +57,7 @@ struct MarchTile {
 
 // LES: the viscous terms use nu + nu_t (equations.f90:97-132, 231-270,
 // 367-404); nu_t is staged in a fourth LDS ring.
-template <int MODE, bool WRITE_F1, bool LES, class T>
+template <int MODE, int FOUT, bool LES, class T>
 __global__ __launch_bounds__(256) void k_rk_march(
     Dims d, MetricsT<T> m, T nu, T dPdx, T dPdz, int kchunk,
     const T *__restrict__ U, const T *__restrict__ V, const T *__restrict__ W,
@@ -61,7 +65,8 @@ __global__ __launch_bounds__(256) void k_rk_march(
     const T *__restrict__ Uo, const T *__restrict__ Vo, const T *__restrict__ Wo,
     const T *__restrict__ F1u, const T *__restrict__ F1v, const T *__restrict__ F1w,
     T *__restrict__ Un, T *__restrict__ Vn, T *__restrict__ Wn, T *__restrict__ Fou,
-    T *__restrict__ Fov, T *__restrict__ Fow, T c0, T c1, T dt) {
+    T *__restrict__ Fov, T *__restrict__ Fow, T c0, T c1, T dt,
+    T *__restrict__ Gou, T *__restrict__ Gov, T *__restrict__ Gow, T g2, T g3) {
     constexpr int RPT = MarchTile<T>::RPT, kTY = MarchTile<T>::TY, kPlane = MarchTile<T>::Plane;
     constexpr int kTX = MarchTile<T>::TX, kHX = MarchTile<T>::HX;
     __shared__ T sU[3][kPlane], sV[3][kPlane], sW[3][kPlane];
@@ -124,11 +129,32 @@ __global__ __launch_bounds__(256) void k_rk_march(
     // a first stage starts from the field it differentiates: the old values
     // are then the LDS centre values and are not read again
     const bool old_is_cur = MODE == 0 && Uo == U && Vo == V && Wo == W;
+    // MODE 0 with a separate old field (G2 / G3, or the RK2 second stage):
+    // its three values per cell are loaded one plane ahead of their use
+    // (the registers fit without lowering occupancy; MODE 1's six do not)
+    constexpr bool kAheadOld = MODE == 0 && FOUT == 0;
+    const bool ahead = kAheadOld && !old_is_cur;
+    T nuo[kAheadOld ? RPT : 1][3] = {};
+    auto fetch_old = [&](int k) {
+#pragma unroll
+        for (int r = 0; r < (kAheadOld ? RPT : 1); ++r) {
+            const long long p = fidx(d, max(1, min(i0 + tx, d.nxg - 2)), min(j0 + RPT * ty + r, d.nyg - 2), k);
+            nuo[r][0] = Uo[p];
+            nuo[r][1] = Vo[p];
+            nuo[r][2] = Wo[p];
+        }
+    };
+    if (ahead) fetch_old(kb);
     for (int k = kb; k < ke; ++k) {
         // this plane's step-start and first-stage values, issued before the
         // barrier so that their latency overlaps the stencil work
         T uo[RPT][3] = {}, fo[RPT][3] = {};
-        if (MODE == 1 || (MODE == 0 && !old_is_cur)) {
+        if (ahead) {
+#pragma unroll
+            for (int r = 0; r < (kAheadOld ? RPT : 1); ++r)
+#pragma unroll
+                for (int c = 0; c < 3; ++c) uo[r][c] = nuo[r][c];
+        } else if (MODE == 1 || (MODE == 0 && !old_is_cur)) {
 #pragma unroll
             for (int r = 0; r < RPT; ++r) {
                 const int j = j0 + RPT * ty + r;
@@ -147,6 +173,7 @@ __global__ __launch_bounds__(256) void k_rk_march(
         commit((k + 1) % 3);
         __syncthreads();
         if (k + 2 <= ke) fetch(k + 2);
+        if (ahead && k + 1 < ke) fetch_old(k + 1);
         const int sm = (k - 1) % 3, sc = k % 3, sp = (k + 1) % 3;
         const T rdzk = m.rdz[k], rdzgk = m.rdzg[k], rdzgm = m.rdzg[k - 1];
 #pragma unroll
@@ -194,8 +221,13 @@ __global__ __launch_bounds__(256) void k_rk_march(
                 if (MODE == 2) {
                     Fou[p] = rr;
                 } else {
-                    if (WRITE_F1) Fou[p] = rr;
-                    Un[p] = (MODE == 0) ? (old_is_cur ? sU[sc][e] : uo[r][0]) + c0 * rr
+                    const T base = old_is_cur ? sU[sc][e] : uo[r][0];
+                    if (FOUT == 1) Fou[p] = rr;
+                    if (FOUT == 2) {
+                        Fou[p] = base + g2 * rr;
+                        Gou[p] = base + g3 * rr;
+                    }
+                    Un[p] = (MODE == 0) ? base + c0 * rr
                                         : uo[r][0] + dt * (c1 * fo[r][0] + c0 * rr);
                 }
             }
@@ -234,8 +266,13 @@ __global__ __launch_bounds__(256) void k_rk_march(
                 if (MODE == 2) {
                     Fov[p] = rr;
                 } else {
-                    if (WRITE_F1) Fov[p] = rr;
-                    Vn[p] = (MODE == 0) ? (old_is_cur ? sV[sc][e] : uo[r][1]) + c0 * rr
+                    const T base = old_is_cur ? sV[sc][e] : uo[r][1];
+                    if (FOUT == 1) Fov[p] = rr;
+                    if (FOUT == 2) {
+                        Fov[p] = base + g2 * rr;
+                        Gov[p] = base + g3 * rr;
+                    }
+                    Vn[p] = (MODE == 0) ? base + c0 * rr
                                         : uo[r][1] + dt * (c1 * fo[r][1] + c0 * rr);
                 }
             }
@@ -276,8 +313,13 @@ __global__ __launch_bounds__(256) void k_rk_march(
                 if (MODE == 2) {
                     Fow[p] = rr;
                 } else {
-                    if (WRITE_F1) Fow[p] = rr;
-                    Wn[p] = (MODE == 0) ? (old_is_cur ? sW[sc][e] : uo[r][2]) + c0 * rr
+                    const T base = old_is_cur ? sW[sc][e] : uo[r][2];
+                    if (FOUT == 1) Fow[p] = rr;
+                    if (FOUT == 2) {
+                        Fow[p] = base + g2 * rr;
+                        Gow[p] = base + g3 * rr;
+                    }
+                    Wn[p] = (MODE == 0) ? base + c0 * rr
                                         : uo[r][2] + dt * (c1 * fo[r][2] + c0 * rr);
                 }
             }
@@ -1734,9 +1776,9 @@ inline int march_chunk(const Dims &d) {
     return std::max(1, (nz + nch - 1) / nch);
 }
 
-template <int MODE, bool WRITE_F1, class T>
+template <int MODE, int FOUT, class T>
 void march(Handle &h, const Fld3<T> &cur, const Fld3<T> *old, const Fld3<T> *f1, Fld3<T> *nxt, Fld3<T> *fo,
-           double c0, double c1, double dt) {
+           double c0, double c1, double dt, Fld3<T> *go = nullptr, double g2 = 0.0, double g3 = 0.0) {
     const Dims &d = h.d;
     const int chunk = march_chunk(d);
     const dim3 grid = march_grid<T>(d, chunk), block(256);
@@ -1744,16 +1786,18 @@ void march(Handle &h, const Fld3<T> &cur, const Fld3<T> *old, const Fld3<T> *f1,
     const T *f[3] = {f1 ? f1->u : nullptr, f1 ? f1->v : nullptr, f1 ? f1->w : nullptr};
     T *n[3] = {nxt ? nxt->u : nullptr, nxt ? nxt->v : nullptr, nxt ? nxt->w : nullptr};
     T *r[3] = {fo ? fo->u : nullptr, fo ? fo->v : nullptr, fo ? fo->w : nullptr};
+    T *g[3] = {go ? go->u : nullptr, go ? go->v : nullptr, go ? go->w : nullptr};
     const MetricsT<T> &m = metrics<T>(h);
     if (h.les != 0)
-        hipLaunchKernelGGL((k_rk_march<MODE, WRITE_F1, true, T>), grid, block, 0, h.stream, d, m, (T)h.nu,
+        hipLaunchKernelGGL((k_rk_march<MODE, FOUT, true, T>), grid, block, 0, h.stream, d, m, (T)h.nu,
                            (T)h.dPdx, (T)h.dPdz, chunk, cur.u, cur.v, cur.w, reinterpret_cast<const T *>(h.nu_t),
                            o[0], o[1], o[2], f[0], f[1], f[2], n[0], n[1], n[2], r[0], r[1], r[2], (T)c0, (T)c1,
-                           (T)dt);
+                           (T)dt, g[0], g[1], g[2], (T)g2, (T)g3);
     else
-        hipLaunchKernelGGL((k_rk_march<MODE, WRITE_F1, false, T>), grid, block, 0, h.stream, d, m, (T)h.nu,
+        hipLaunchKernelGGL((k_rk_march<MODE, FOUT, false, T>), grid, block, 0, h.stream, d, m, (T)h.nu,
                            (T)h.dPdx, (T)h.dPdz, chunk, cur.u, cur.v, cur.w, nullptr, o[0], o[1], o[2], f[0],
-                           f[1], f[2], n[0], n[1], n[2], r[0], r[1], r[2], (T)c0, (T)c1, (T)dt);
+                           f[1], f[2], n[0], n[1], n[2], r[0], r[1], r[2], (T)c0, (T)c1, (T)dt, g[0], g[1], g[2],
+                           (T)g2, (T)g3);
     check_launch();
 }
 
@@ -1762,27 +1806,40 @@ void launch_rk_stage(Handle &h, const Fld3<T> &cur, const Fld3<T> &old, Fld3<T> 
     const bool rk3 = h.scheme == 3;
     Fld3<T> F1 = as<T>(h.F1);
     ProfScope ps(h, KID_RK_STAGE);
+    // FP64 RK3 carries G_s = Uo + dt a_s1 F1 (s = 2, 3) from the first stage
+    // (in the F1 buffers and G3); FP32 fields carry Uo and F1, since storing
+    // G_s in FP32 would round a velocity-sized value once more per stage
+    const bool gform = rk3 && sizeof(T) == 8 && h.G3.u != nullptr;
+    Fld3<T> G3 = as<T>(h.G3);
     if (stage == 1) {
         const double c0 = dt * (rk3 ? h.rk_coef[1][1] : h.rk2_coef[1][1]);
-        if (rk3)
-            march<0, true>(h, cur, &old, (const Fld3<T> *)nullptr, &nxt, &F1, c0, 0.0, dt);
+        if (gform)
+            march<0, 2>(h, cur, &old, (const Fld3<T> *)nullptr, &nxt, &F1, c0, 0.0, dt, &G3,
+                        dt * h.rk_coef[2][1], dt * h.rk_coef[3][1]);
+        else if (rk3)
+            march<0, 1>(h, cur, &old, (const Fld3<T> *)nullptr, &nxt, &F1, c0, 0.0, dt);
         else
-            march<0, false>(h, cur, &old, (const Fld3<T> *)nullptr, &nxt, (Fld3<T> *)nullptr, c0, 0.0, dt);
+            march<0, 0>(h, cur, &old, (const Fld3<T> *)nullptr, &nxt, (Fld3<T> *)nullptr, c0, 0.0, dt);
     } else if (!rk3) {
         // RK2 second stage: Uo + dt*(0*F1 + 1*F2) == Uo + dt*F2 exactly
-        march<0, false>(h, cur, &old, (const Fld3<T> *)nullptr, &nxt, (Fld3<T> *)nullptr, dt * h.rk2_coef[2][2],
-                        0.0, dt);
+        march<0, 0>(h, cur, &old, (const Fld3<T> *)nullptr, &nxt, (Fld3<T> *)nullptr, dt * h.rk2_coef[2][2],
+                    0.0, dt);
+    } else if (gform) {
+        // RK3 stages 2, 3: new = G_s + dt a_ss F (a_32 = 0 so F2 is never needed)
+        const Fld3<T> &gs = stage == 2 ? F1 : G3;
+        march<0, 0>(h, cur, &gs, (const Fld3<T> *)nullptr, &nxt, (Fld3<T> *)nullptr, dt * h.rk_coef[stage][stage],
+                    0.0, dt);
     } else {
         // RK3 stages 2, 3: a_32 = 0 so F2 is never needed
         const double c1 = h.rk_coef[stage][1], c0 = h.rk_coef[stage][stage];
-        march<1, false>(h, cur, &old, &F1, &nxt, (Fld3<T> *)nullptr, c0, c1, dt);
+        march<1, 0>(h, cur, &old, &F1, &nxt, (Fld3<T> *)nullptr, c0, c1, dt);
     }
 }
 
 template <class T>
 void launch_rhs_only(Handle &h, const Fld3<T> &cur, Fld3<T> &rhs) {
-    march<2, false>(h, cur, (const Fld3<T> *)nullptr, (const Fld3<T> *)nullptr, (Fld3<T> *)nullptr, &rhs, 0.0,
-                    0.0, 0.0);
+    march<2, 0>(h, cur, (const Fld3<T> *)nullptr, (const Fld3<T> *)nullptr, (Fld3<T> *)nullptr, &rhs, 0.0, 0.0,
+                0.0);
 }
 
 template <class T>
