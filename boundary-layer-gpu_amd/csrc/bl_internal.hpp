@@ -1,6 +1,6 @@
 // Internal state of a blgpu handle and the kernel launch interface.
 //
-// Device layout: every 3-D array (U, V, W, P, phi, F1, nu_t) uses one padded
+// Device layout: every 3-D array (U, V, W, P, phi, F1, G3, nu_t) uses one padded
 // x-fastest layout idx = i + sx*(j + sy*k) with 0-based (i, j, k),
 // sx = nxg rounded up to 16 doubles (128-byte rows), sy = nyg, nzg planes.
 // A staggered field simply leaves the unused tail of its shorter dimension
