@@ -49,8 +49,19 @@ FP64_PEAK_TFLOPS = 78.6
 BF16_PEAK_TFLOPS = 2500.0  # dense
 
 
+def load_blgpu():
+    """The product's ctypes binding, boundary-layer-gpu_amd/blgpu.py (the
+    package directory name is not an importable module name)."""
+    import importlib.util
+
+    spec = importlib.util.spec_from_file_location("blgpu", os.path.join(ROOT, "boundary-layer-gpu_amd", "blgpu.py"))
+    mod = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mod)
+    return mod
+
+
 def workload(nx, ny, nz, seed=0, noise=0.01):
-    from tests.blgpu_import import blgpu
+    blgpu = load_blgpu()
 
     x, y, z = blgpu.grid_tanh(nx, ny, nz, 400 * THETA, 40 * THETA, 40 * THETA, 2.0, 1.0)
     eta, f, df = blgpu.blasius_profile()
@@ -75,9 +86,9 @@ def kernel_units(name, nx, ny, nz):
     # LDS, not GEMMs: their algorithmic work is the HBM traffic of the lines
     per_cell_passes = {
         # one pass = one read or write of an N_cell FP64 array
-        # mean over the FP64 RK3 stages: stage 1 reads U,V,W and writes U*, G2, G3
-        # (12); stages 2, 3 read U,V,W, G_s and write U* (9 each)
-        "rk_stage": 10.0,
+        # every FP64 RK3 stage moves 9: stage 1 reads U,V,W and writes U* and
+        # G = Uo + dt a21 F1; stages 2, 3 read U,V,W, G and write U*
+        "rk_stage": 9.0,
         "divergence": 4.0,
         "xform_x_fwd": 4.0,  # read U, V, W (divergence fused), write the spectrum lines
         "xform_x_inv": 2.0,
@@ -362,7 +373,7 @@ def main():
               f"(asked for --gpus {args.gpus})", file=sys.stderr)
         sys.exit(2)
 
-    from tests.blgpu_import import blgpu
+    blgpu = load_blgpu()
 
     if args.calib_copy > 0:
         import torch

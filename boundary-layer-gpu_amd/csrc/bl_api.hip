@@ -142,8 +142,9 @@ void check_config(const bl_config *c) {
     if (!c->U_inlet || !c->V_inlet || !c->W_inlet || !c->U_top || !c->V_top || !c->W_top)
         bad("bl_create: missing boundary tables");
     if (c->scheme != 2 && c->scheme != 3) bad("bl_create: scheme must be 2 (RK2) or 3 (RK3)");
-    if (c->les != 0 && c->les != 2 && c->les != 4)
-        bad("bl_create: les must be 0 (DNS), 2 (dynamic Smagorinsky) or 4 (learned closure with DSM fallback)");
+    if (c->les < 0 || c->les > 4)
+        bad("bl_create: les must be 0 (DNS), 1 (Smagorinsky), 2 (dynamic Smagorinsky), 3 (dynamic Smagorinsky "
+            "without the z average) or 4 (learned closure with dynamic Smagorinsky fallback)");
     if (c->precision != BL_PREC_FP64 && c->precision != BL_PREC_MIXED)
         bad("bl_create: precision must be BL_PREC_FP64 or BL_PREC_MIXED");
 
@@ -236,6 +237,11 @@ void build(Handle &h, const bl_config *c) {
     h.rk_coef[3][1] = 1.0 / 4.0;
     h.rk_coef[3][2] = 0.0;
     h.rk_coef[3][3] = 3.0 / 4.0;
+    // the FP64 RK3 stages 2 and 3 share one carried field Uo + dt a_s1 F1
+    if (h.rk_coef[2][1] != h.rk_coef[3][1]) {
+        last_error() = "bl_create: RK3 tableau with a21 != a31";
+        throw ArgError();
+    }
 
     HostSetup g;
     g.x.assign(c->x, c->x + d.nx);
@@ -370,17 +376,30 @@ void build(Handle &h, const bl_config *c) {
     h.F1.u = falloc();
     h.F1.v = falloc();
     h.F1.w = falloc();
-    if (h.precision != BL_PREC_MIXED && c->scheme == 3) {
-        h.G3.u = falloc();
-        h.G3.v = falloc();
-        h.G3.w = falloc();
-    }
     h.P = falloc();
     h.phi = falloc();
     h.nu_t = falloc();
-    if (h.les == 2 || h.les == 4) {
+    if (h.les >= 2) {
         h.dsm = dmalloc(nf * kDsmArrays);
         BL_HIP(hipMemsetAsync(h.dsm, 0, nf * kDsmArrays * sizeof(double), h.stream));
+    }
+    if (h.les == 1) {
+        // sgs_Smagorinsky (subgrid.f90:609-662): Cs = 0.11, Delta_j =
+        // (dx dz dy_j)^(1/3) with dx, dz the smallest spacings, van Driest
+        // damping of the distance to the nearer wall (rows j < nyg/2 count
+        // from the bottom) in units nu/utau, utau = dPdx^0.5; f = 1 when
+        // nutBC = 0
+        std::vector<double> cj(d.nyg, 0.0);
+        const double utau = std::pow(c->dPdx, 0.5);
+        for (int j = 2; j <= d.nyg - 1; ++j) {
+            const double yw = j < d.nyg / 2 ? g.yg[j - 1] - g.y[0] : g.y[d.ny - 1] - g.yg[j - 1];
+            double f = 1.0 - std::exp(-yw / 25.0 / (c->nu / utau));
+            if (c->nut_bc == 0) f = 1.0;
+            const double delta = std::pow(g.dxmin * g.dzmin * (g.y[j - 1] - g.y[j - 2]), 1.0 / 3.0);
+            cj[j - 1] = std::pow(0.11 * delta * f, 2.0);
+        }
+        h.smag_c = dmalloc(cj.size());
+        upload_vec(h.smag_c, cj, h.stream);
     }
     const size_t nspec = (size_t)d.N * d.NM * d.M;
     if (h.precision == BL_PREC_MIXED) {
@@ -587,12 +606,11 @@ void free_all(Handle &h) {
         f(h.buf[b].v);
         f(h.buf[b].w);
     }
-    for (void *p : {(void *)h.F1.u, (void *)h.F1.v, (void *)h.F1.w, (void *)h.G3.u, (void *)h.G3.v, (void *)h.G3.w,
-                    (void *)h.P, (void *)h.phi,
+    for (void *p : {(void *)h.F1.u, (void *)h.F1.v, (void *)h.F1.w, (void *)h.P, (void *)h.phi,
                     (void *)h.nu_t, (void *)h.T1, (void *)h.S, (void *)h.dct_w1, (void *)h.dct_w2,
                     (void *)h.dct_wN, (void *)h.dct_rot, (void *)h.dct_mf, h.mx2, (void *)h.zc, (void *)h.zs, (void *)h.lu_dl, (void *)h.lu_dp, (void *)h.lu_du,
                     (void *)h.lu00, (void *)h.th_co,
-                    (void *)h.scal, (void *)h.red, (void *)h.dsm, h.mem_grid, h.mem_gridf, (void *)h.mx_tab,
+                    (void *)h.scal, (void *)h.red, (void *)h.dsm, (void *)h.smag_c, h.mem_grid, h.mem_gridf, (void *)h.mx_tab,
                     (void *)h.p00, (void *)h.cl_w, (void *)h.cl_b, (void *)h.cl_cnt})
         f(p);
     if (h.stream) (void)hipStreamSynchronize(h.stream);
@@ -634,7 +652,7 @@ void do_step_t(Handle &h) {
     for (int s = 1; s <= nst; ++s) {
         const int nxt = order3[s - 1];
         // eddy viscosity of the stage's input fields (time_integration.f90:341-346)
-        if (h.les == 2 || h.les == 4) launch_dsm(h, h.buf[cur]);
+        if (h.les != 0) launch_dsm(h, h.buf[cur]);
         if (h.les == 4) launch_closure(h, h.buf[cur]);
         Fld3<T> next = as<T>(h.buf[nxt]);
         launch_rk_stage(h, as<T>(h.buf[cur]), old, next, s, dt);
@@ -880,7 +898,7 @@ int bl_synchronize(bl_handle *hh) {
 int bl_eddy_viscosity(bl_handle *hh) {
     return guarded([&] {
         Handle &h = H(hh);
-        if (h.les == 2 || h.les == 4) {
+        if (h.les != 0) {
             launch_dsm(h, h.buf[h.icur]);
             if (h.les == 4) launch_closure(h, h.buf[h.icur]);
         } else

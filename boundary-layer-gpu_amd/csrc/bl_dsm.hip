@@ -1,13 +1,19 @@
-// Dynamic Smagorinsky eddy viscosity on the device (LES = 2, z-averaged),
-// the semantics of sgs_dynamic_Smagorinsky (subgrid.f90:261-414) as the
-// reference's CPU build runs it, with filter_xzy (:140-187) and compute_Sij
-// (:672-730).
+// Smagorinsky-type eddy viscosities on the device, the semantics of the
+// reference's CPU build:
+//   LES = 2  dynamic, z-averaged   sgs_dynamic_Smagorinsky (subgrid.f90:261-414)
+//   LES = 3  dynamic, per cell     sgs_dynamic_Smagorinsky_noaverage (:436-583)
+//   LES = 1  constant coefficient  sgs_Smagorinsky (:609-662)
+// with filter_xzy (:140-187) and compute_Sij (:672-730).
 //
-// The reference builds work arrays, then fixes their ghost entries in place
-// with apply_Neumann_bc_x (boundary_conditions.f90:326-344) and
-// apply_periodic_bc_z (:732-757).  Here those fixes become index maps applied
-// when an entry is read: an x index is clamped into the interior and a z index
-// on a ghost or duplicated plane is redirected to the plane it is copied from.
+// The dynamic models build work arrays, then fix their ghost entries in place
+// with apply_Neumann_bc_x (LES = 2, boundary_conditions.f90:326-344) or
+// apply_periodic_bc_x (LES = 3, :295-314) and apply_periodic_bc_z (:732-757).
+// Here those fixes become index maps applied when an entry is read: an x index
+// is clamped into the interior (Neumann) or redirected to its periodic source,
+// and a z index on a ghost or duplicated plane is redirected to the plane it
+// is copied from.
+//   xmap_c periodic (x centres):               0 <- nxg-3, nxg-2 <- 1, nxg-1 <- 2
+//   xmap_f periodic (x faces, U):              0 <- nx-2,  nx-1 <- 1
 //   zmap_c (cell-centred in z, periodic id 1/2/4): 0 <- nzg-3, nzg-2 <- 1, nzg-1 <- 2
 //   zmap_f (z faces, id 3):                      0 <- nz-2,  nz-1 <- 1
 //
@@ -18,7 +24,9 @@
 //   k_dsm_filter   per interior cell: the filtered products, velocities and
 //                  |S|S_ij, the filtered strain, L_ij, M_ij, then L:M and M:M
 //   k_dsm_column   z averages per (x, y) column, nu_t = max(-<L:M>/<M:M>/2*|S|, 0)
-//                  and the y ghost rows (:371-412)
+//                  and the y ghost rows (:371-412); LES = 3: the ratio per
+//                  cell, periodic x and z copies, clipping, y ghost rows (:545-581)
+//   k_smag         LES = 1: (Cs Delta_j f_j)^2 |S| on the interior cells
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -50,6 +58,17 @@ __device__ __forceinline__ int zmap_c(const Dims &d, int k) {
     return k == 0 ? d.nzg - 3 : (k == d.nzg - 2 ? 1 : (k == d.nzg - 1 ? 2 : k));
 }
 __device__ __forceinline__ int zmap_f(const Dims &d, int k) { return k == 0 ? d.nz - 2 : (k == d.nz - 1 ? 1 : k); }
+// x fix of the cell-centred work arrays (extent nxg) and of the filtered U (extent nx)
+template <bool PX>
+__device__ __forceinline__ int xmap_c(const Dims &d, int i) {
+    if constexpr (PX) return i == 0 ? d.nxg - 3 : (i == d.nxg - 2 ? 1 : (i == d.nxg - 1 ? 2 : i));
+    return clampi(i, 1, d.nxg - 2);
+}
+template <bool PX>
+__device__ __forceinline__ int xmap_f(const Dims &d, int i) {
+    if constexpr (PX) return i == 0 ? d.nx - 2 : (i == d.nx - 1 ? 1 : i);
+    return clampi(i, 1, d.nx - 2);
+}
 
 // compute_Sij at interior cell (i, j, k): diagonal terms at the centre, the
 // off-diagonal terms averaged from the four surrounding edges; returns |S|.
@@ -111,7 +130,7 @@ __device__ double filt_face(const Dims &d, const T *__restrict__ F, int ic, int 
     return acc;
 }
 
-template <class T>
+template <bool PX, class T>
 __global__ __launch_bounds__(256) void k_dsm_cells(Dims d, Metrics m, const T *__restrict__ U, const T *__restrict__ V,
                                                    const T *__restrict__ W, double *__restrict__ ws, long long nf) {
     const int i = blockIdx.x * 64 + (threadIdx.x & 63);
@@ -119,7 +138,7 @@ __global__ __launch_bounds__(256) void k_dsm_cells(Dims d, Metrics m, const T *_
     const int k = blockIdx.z;
     if (i >= d.nxg || j >= d.nyg) return;
     const long long p = fidx(d, i, j, k);
-    const int ic = clampi(i, 1, d.nxg - 2);
+    const int ic = xmap_c<PX>(d, i);
     ws[D_CU * nf + p] = 0.5 * ((double)U[fidx(d, ic - 1, j, k)] + (double)U[fidx(d, ic, j, k)]);
     ws[D_CV * nf + p] = (j >= 1 && j <= d.nyg - 2) ? 0.5 * ((double)V[p - d.sx] + (double)V[p]) : 0.0;
     const int kc = zmap_c(d, k);
@@ -133,7 +152,7 @@ __global__ __launch_bounds__(256) void k_dsm_cells(Dims d, Metrics m, const T *_
     }
 }
 
-template <class T>
+template <bool PX, class T>
 __global__ __launch_bounds__(256) void k_dsm_vfilter(Dims d, const T *__restrict__ U, const T *__restrict__ V,
                                                      const T *__restrict__ W, double *__restrict__ ws, long long nf) {
     const int i = blockIdx.x * 64 + (threadIdx.x & 63);
@@ -141,11 +160,12 @@ __global__ __launch_bounds__(256) void k_dsm_vfilter(Dims d, const T *__restrict
     const int k = blockIdx.z;
     if (i >= d.nxg || j >= d.nyg) return;
     const long long p = fidx(d, i, j, k);
-    if (i < d.nx) ws[D_UF * nf + p] = filt_face(d, U, clampi(i, 1, d.nx - 2), j, zmap_c(d, k), d.nyg - 1);
-    if (j < d.ny) ws[D_VF * nf + p] = filt_face(d, V, clampi(i, 1, d.nxg - 2), j, zmap_c(d, k), d.ny - 1);
-    if (k < d.nz) ws[D_WF * nf + p] = filt_face(d, W, clampi(i, 1, d.nxg - 2), j, zmap_f(d, k), d.nyg - 1);
+    if (i < d.nx) ws[D_UF * nf + p] = filt_face(d, U, xmap_f<PX>(d, i), j, zmap_c(d, k), d.nyg - 1);
+    if (j < d.ny) ws[D_VF * nf + p] = filt_face(d, V, xmap_c<PX>(d, i), j, zmap_c(d, k), d.ny - 1);
+    if (k < d.nz) ws[D_WF * nf + p] = filt_face(d, W, xmap_c<PX>(d, i), j, zmap_f(d, k), d.nyg - 1);
 }
 
+template <bool PX>
 __global__ __launch_bounds__(256) void k_dsm_filter(Dims d, Metrics m, double *__restrict__ ws, long long nf) {
     const int i = 1 + blockIdx.x * 64 + (threadIdx.x & 63);
     const int j = 1 + blockIdx.y * 4 + (threadIdx.x >> 6);
@@ -161,7 +181,7 @@ __global__ __launch_bounds__(256) void k_dsm_filter(Dims d, Metrics m, double *_
             const double f = fil(a, c);
             const int in = i + a, kn = k + c;
             const long long q1 = fidx(d, in, j, kn);
-            const long long q2 = fidx(d, clampi(in, 1, d.nxg - 2), j, zmap_c(d, kn));
+            const long long q2 = fidx(d, xmap_c<PX>(d, in), j, zmap_c(d, kn));
             hu = hu + yfilt(cu, q1, sx, j, jlo, jhi) * f;
             hv = hv + yfilt(cv, q1, sx, j, jlo, jhi) * f;
             hw = hw + yfilt(cw, q1, sx, j, jlo, jhi) * f;
@@ -243,6 +263,41 @@ __global__ __launch_bounds__(256) void k_dsm_column(Dims d, const double *__rest
         nu_t[fidx(d, i, j, k)] = (T)(sgn * fmax(-0.5 * avg * S[fidx(d, i, js, k)], 0.0));
 }
 
+// LES = 3: nu_t(i, j, k) = -0.5 (L:M)/(M:M) |S| per interior cell, then
+// (subgrid.f90:571-581) the periodic x and z copies, clipping, and the y
+// ghost rows from rows 1 and nyg-2; one thread per (i, j), all k
+template <class T>
+__global__ __launch_bounds__(256) void k_dsm_cellwise(Dims d, const double *__restrict__ ws, long long nf,
+                                                      int dirichlet, T *__restrict__ nu_t) {
+    const int i = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int j = blockIdx.y * 4 + (threadIdx.x >> 6);
+    if (i >= d.nxg || j >= d.nyg) return;
+    const bool ghost_y = (j == 0 || j == d.nyg - 1);
+    const int js = j == 0 ? 1 : (j == d.nyg - 1 ? d.nyg - 2 : j);
+    const double sgn = (ghost_y && dirichlet == 1) ? -1.0 : 1.0;
+    const int is = xmap_c<true>(d, i);
+    const double *LM = ws + D_LM * nf, *MM = ws + D_MM * nf, *S = ws + D_S * nf;
+    for (int k = 0; k < d.nzg; ++k) {
+        const long long q = fidx(d, is, js, zmap_c(d, k));
+        nu_t[fidx(d, i, j, k)] = (T)(sgn * fmax(-0.5 * (LM[q] / MM[q]) * S[q], 0.0));
+    }
+}
+
+// LES = 1: nu_t = coef[j] |S| on the interior cells, coef[j] = (Cs Delta_j
+// f_j)^2 from the host; the ghost entries are left as they are
+template <class T>
+__global__ __launch_bounds__(256) void k_smag(Dims d, Metrics m, const T *__restrict__ U, const T *__restrict__ V,
+                                              const T *__restrict__ W, const double *__restrict__ coef,
+                                              T *__restrict__ nu_t) {
+    const int i = 1 + blockIdx.x * 64 + (threadIdx.x & 63);
+    const int j = 1 + blockIdx.y * 4 + (threadIdx.x >> 6);
+    const int k = 1 + blockIdx.z;
+    if (i > d.nxg - 2 || j > d.nyg - 2) return;
+    double s[6];
+    const double mag = strain(d, m, U, V, W, i, j, k, s);
+    nu_t[fidx(d, i, j, k)] = (T)(coef[j] * mag);
+}
+
 void check() {
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) {
@@ -251,7 +306,7 @@ void check() {
     }
 }
 
-template <class T>
+template <bool PX, class T>
 void dsm_t(Handle &h, const Fld3<T> &f, T *nu_t) {
     const Dims &d = h.d;
     const long long nf = d.sxy * d.nzg;
@@ -259,35 +314,57 @@ void dsm_t(Handle &h, const Fld3<T> &f, T *nu_t) {
     const dim3 gfull((d.nxg + 63) / 64, (d.nyg + 3) / 4, d.nzg);
     {
         ProfScope ps(h, KID_DSM_CELLS);
-        hipLaunchKernelGGL(k_dsm_cells<T>, gfull, block, 0, h.stream, d, h.m, f.u, f.v, f.w, h.dsm, nf);
+        hipLaunchKernelGGL((k_dsm_cells<PX, T>), gfull, block, 0, h.stream, d, h.m, f.u, f.v, f.w, h.dsm, nf);
         check();
     }
     {
         ProfScope ps(h, KID_DSM_VFILTER);
-        hipLaunchKernelGGL(k_dsm_vfilter<T>, gfull, block, 0, h.stream, d, f.u, f.v, f.w, h.dsm, nf);
+        hipLaunchKernelGGL((k_dsm_vfilter<PX, T>), gfull, block, 0, h.stream, d, f.u, f.v, f.w, h.dsm, nf);
         check();
     }
     {
         ProfScope ps(h, KID_DSM_FILTER);
         const dim3 g((d.nxg - 2 + 63) / 64, (d.nyg - 2 + 3) / 4, d.nzg - 2);
-        hipLaunchKernelGGL(k_dsm_filter, g, block, 0, h.stream, d, h.m, h.dsm, nf);
+        hipLaunchKernelGGL(k_dsm_filter<PX>, g, block, 0, h.stream, d, h.m, h.dsm, nf);
         check();
     }
     {
         ProfScope ps(h, KID_DSM_COLUMN);
         const dim3 g((d.nxg + 63) / 64, (d.nyg + 3) / 4);
-        hipLaunchKernelGGL(k_dsm_column<T>, g, block, 0, h.stream, d, h.dsm, nf, h.nut_bc, nu_t);
+        if constexpr (PX)
+            hipLaunchKernelGGL(k_dsm_cellwise<T>, g, block, 0, h.stream, d, h.dsm, nf, h.nut_bc, nu_t);
+        else
+            hipLaunchKernelGGL(k_dsm_column<T>, g, block, 0, h.stream, d, h.dsm, nf, h.nut_bc, nu_t);
         check();
     }
+}
+
+template <class T>
+void smag_t(Handle &h, const Fld3<T> &f, T *nu_t) {
+    const Dims &d = h.d;
+    ProfScope ps(h, KID_DSM_CELLS);
+    const dim3 g((d.nxg - 2 + 63) / 64, (d.nyg - 2 + 3) / 4, d.nzg - 2);
+    hipLaunchKernelGGL(k_smag<T>, g, dim3(256), 0, h.stream, d, h.m, f.u, f.v, f.w, (const double *)h.smag_c, nu_t);
+    check();
+}
+
+template <class T>
+void eddy_t(Handle &h, const Fld3<T> &f, T *nu_t) {
+    if (h.les == 1)
+        smag_t(h, f, nu_t);
+    else if (h.les == 3)
+        dsm_t<true>(h, f, nu_t);
+    else
+        dsm_t<false>(h, f, nu_t);
 }
 
 }  // namespace
 
 void launch_dsm(Handle &h, const Field3 &f) {
     if (h.precision == BL_PREC_MIXED)
-        dsm_t(h, as<float>(f), reinterpret_cast<float *>(h.nu_t));
+        eddy_t(h, as<float>(f), reinterpret_cast<float *>(h.nu_t));
     else
-        dsm_t(h, as<double>(f), h.nu_t);
+        eddy_t(h, as<double>(f), h.nu_t);
 }
 
 }  // namespace blgpu

@@ -1,6 +1,6 @@
 // Internal state of a blgpu handle and the kernel launch interface.
 //
-// Device layout: every 3-D array (U, V, W, P, phi, F1, G3, nu_t) uses one padded
+// Device layout: every 3-D array (U, V, W, P, phi, F1, nu_t) uses one padded
 // x-fastest layout idx = i + sx*(j + sy*k) with 0-based (i, j, k),
 // sx = nxg rounded up to 16 doubles (128-byte rows), sy = nyg, nzg planes.
 // A staggered field simply leaves the unused tail of its shorter dimension
@@ -151,10 +151,10 @@ struct Handle {
     Field3 buf[3];  // rotating state buffers
     int iold;       // index of the step-start buffer (Uo)
     int icur;       // index of the current (most recent) buffer
-    Field3 F1;      // first-stage RHS (RK3); FP64 RK3: G2 = Uo + dt a21 F1 instead
-    Field3 G3;      // FP64 RK3: G3 = Uo + dt a31 F1 (else null)
+    Field3 F1;      // first-stage RHS (RK3); FP64 RK3: G = Uo + dt a21 F1 instead (a21 = a31)
     double *P, *phi, *nu_t;  // float in mixed precision
-    double *dsm;  // dynamic Smagorinsky work arrays (LES = 2 and 4), kDsmArrays field-sized slots
+    double *dsm;  // dynamic Smagorinsky work arrays (LES = 2, 3 and 4), kDsmArrays field-sized slots
+    double *smag_c;  // LES = 1: (Cs Delta_j f_j)^2 per y row (nyg)
     // learned closure (LES = 4): BF16 weights, FP32 biases, input range
     unsigned short *cl_w;
     float *cl_b, *cl_lo, *cl_hi;
@@ -247,6 +247,19 @@ template <class T>
 void launch_rhs_only(Handle &h, const Fld3<T> &cur, Fld3<T> &rhs);
 template <class T>
 double max_divergence(Handle &h, const Fld3<T> &f);
+// the max-divergence reduction on fields with their own row (x1) and plane
+// (x2) strides and reciprocal face spacings rdx[i] = 1/(x[i]-x[i-1]) (i >= 1)
+struct DivArgs {
+    int nxg, nyg, nzg;
+    long long u1, u2, v1, v2, w1, w2;
+    const double *rdx, *rdy, *rdz;
+};
+// partials the reduction writes (maxdiv_partials of the same extents) into
+// part, then the maximum into out[0]; both device memory, stream-ordered
+template <class T>
+void launch_maxdiv(hipStream_t stream, const DivArgs &a, const T *U, const T *V, const T *W, double *part,
+                   double *out);
+long long maxdiv_partials(int nxg, int nyg, int nzg);
 template <class T>
 void wall_stats(Handle &h, const Fld3<T> &f, double *cf_host);
 template <class T>
@@ -256,7 +269,8 @@ double dt_convective(Handle &h, const Fld3<T> &f);
 // ---- mixed precision (bl_mixed.hip) ----
 void build_mixed(Handle &h);
 void launch_poisson_mixed(Handle &h, float *phi, const Fld3<float> &f);
-// dynamic Smagorinsky eddy viscosity of f into h.nu_t (bl_dsm.hip)
+// eddy viscosity of f into h.nu_t (bl_dsm.hip): LES = 1 constant-coefficient
+// Smagorinsky, 2 and 4 dynamic Smagorinsky z-averaged, 3 per cell
 constexpr int kDsmArrays = 15;
 void launch_dsm(Handle &h, const Field3 &f);
 // learned closure over the DSM values in h.nu_t (bl_closure.hip)

@@ -32,12 +32,14 @@ __device__ __forceinline__ T sq(T a) {
 // ------------------------------------------------------------------------
 // Fused RHS + Runge-Kutta stage update, marching in z.
 // MODE 0: new = old + c0*F              (first stage, and RK2 second stage;
-//         FP64 RK3 stages 2 and 3 with old = G2 or G3, below)
+//         FP64 RK3 stages 2 and 3 with old = G, below)
 // MODE 1: new = old + dt*(c1*F1 + c0*F) (RK3 stages 2 and 3, FP32 fields)
 // MODE 2: write F only (operator test)
-// FOUT (first stage of RK3): 1 writes F1 = F; 2 writes G2 = old + g2*F and
-// G3 = old + g3*F instead (g_s = dt a_s1), so that stages 2 and 3 read one
-// field per component (G_s) instead of two (old and F1).  A block owns a
+// FOUT (first stage of RK3): 1 writes F1 = F; 2 writes G = old + g2*F
+// instead (g2 = dt a21), so that stages 2 and 3 read one field per component
+// instead of two (old and F1): a21 = a31 = 1/4 (initialization.f90:526,528)
+// makes Uo + dt a21 F1 and Uo + dt a31 F1 the same field, and nothing
+// writes G between stages 2 and 3.  A block owns a
 // 64 (x) x 4 RPT (y) column of cells and a chunk of z planes; U, V, W of
 // planes k-1, k, k+1 live in an LDS ring with a one-cell halo in x and y, so
 // every velocity value is read from HBM once per chunk (plus halo) instead of
@@ -65,8 +67,7 @@ __global__ __launch_bounds__(256) void k_rk_march(
     const T *__restrict__ Uo, const T *__restrict__ Vo, const T *__restrict__ Wo,
     const T *__restrict__ F1u, const T *__restrict__ F1v, const T *__restrict__ F1w,
     T *__restrict__ Un, T *__restrict__ Vn, T *__restrict__ Wn, T *__restrict__ Fou,
-    T *__restrict__ Fov, T *__restrict__ Fow, T c0, T c1, T dt,
-    T *__restrict__ Gou, T *__restrict__ Gov, T *__restrict__ Gow, T g2, T g3) {
+    T *__restrict__ Fov, T *__restrict__ Fow, T c0, T c1, T dt, T g2) {
     constexpr int RPT = MarchTile<T>::RPT, kTY = MarchTile<T>::TY, kPlane = MarchTile<T>::Plane;
     constexpr int kTX = MarchTile<T>::TX, kHX = MarchTile<T>::HX;
     __shared__ T sU[3][kPlane], sV[3][kPlane], sW[3][kPlane];
@@ -129,7 +130,7 @@ __global__ __launch_bounds__(256) void k_rk_march(
     // a first stage starts from the field it differentiates: the old values
     // are then the LDS centre values and are not read again
     const bool old_is_cur = MODE == 0 && Uo == U && Vo == V && Wo == W;
-    // MODE 0 with a separate old field (G2 / G3, or the RK2 second stage):
+    // MODE 0 with a separate old field (G, or the RK2 second stage):
     // its three values per cell are loaded one plane ahead of their use
     // (the registers fit without lowering occupancy; MODE 1's six do not)
     constexpr bool kAheadOld = MODE == 0 && FOUT == 0;
@@ -223,10 +224,7 @@ __global__ __launch_bounds__(256) void k_rk_march(
                 } else {
                     const T base = old_is_cur ? sU[sc][e] : uo[r][0];
                     if (FOUT == 1) Fou[p] = rr;
-                    if (FOUT == 2) {
-                        Fou[p] = base + g2 * rr;
-                        Gou[p] = base + g3 * rr;
-                    }
+                    if (FOUT == 2) Fou[p] = base + g2 * rr;
                     Un[p] = (MODE == 0) ? base + c0 * rr
                                         : uo[r][0] + dt * (c1 * fo[r][0] + c0 * rr);
                 }
@@ -268,10 +266,7 @@ __global__ __launch_bounds__(256) void k_rk_march(
                 } else {
                     const T base = old_is_cur ? sV[sc][e] : uo[r][1];
                     if (FOUT == 1) Fov[p] = rr;
-                    if (FOUT == 2) {
-                        Fov[p] = base + g2 * rr;
-                        Gov[p] = base + g3 * rr;
-                    }
+                    if (FOUT == 2) Fov[p] = base + g2 * rr;
                     Vn[p] = (MODE == 0) ? base + c0 * rr
                                         : uo[r][1] + dt * (c1 * fo[r][1] + c0 * rr);
                 }
@@ -315,10 +310,7 @@ __global__ __launch_bounds__(256) void k_rk_march(
                 } else {
                     const T base = old_is_cur ? sW[sc][e] : uo[r][2];
                     if (FOUT == 1) Fow[p] = rr;
-                    if (FOUT == 2) {
-                        Fow[p] = base + g2 * rr;
-                        Gow[p] = base + g3 * rr;
-                    }
+                    if (FOUT == 2) Fow[p] = base + g2 * rr;
                     Wn[p] = (MODE == 0) ? base + c0 * rr
                                         : uo[r][2] + dt * (c1 * fo[r][2] + c0 * rr);
                 }
@@ -1510,19 +1502,22 @@ __device__ __forceinline__ double block_max_256(double v, double *sh) {
     return fmax(fmax(sh[0], sh[1]), fmax(sh[2], sh[3]));
 }
 
+// max |div u| partials (check_divergence projection.f90:391-413); each field
+// has its own row and plane strides (the padded solver layout, or the
+// reference's unpadded extents for bl_max_divergence_noalloc)
 template <class T>
-__global__ __launch_bounds__(256) void k_maxdiv(Dims d, Metrics m, const T *U, const T *V, const T *W,
-                                                double *out) {
+__global__ __launch_bounds__(256) void k_maxdiv(DivArgs a, const T *U, const T *V, const T *W, double *out) {
     __shared__ double sh[4];
     const int i = blockIdx.x * kWave + threadIdx.x;
     const int j = blockIdx.y * 4 + threadIdx.y + 1;
     double v = 0.0;
-    if (i >= 1 && i <= d.nxg - 3 && j <= d.nyg - 2)
-        for (int k = 1 + blockIdx.z; k <= d.nzg - 2; k += gridDim.z) {
-            const long long p = fidx(d, i, j, k);
-            v = fmax(v, fabs(((double)U[p] - (double)U[p - 1]) * m.rdx[i] +
-                             ((double)V[p] - (double)V[p - d.sx]) * m.rdy[j] +
-                             ((double)W[p] - (double)W[p - d.sxy]) * m.rdz[k]));
+    if (i >= 1 && i <= a.nxg - 3 && j <= a.nyg - 2)
+        for (int k = 1 + blockIdx.z; k <= a.nzg - 2; k += gridDim.z) {
+            const long long pu = i + a.u1 * j + a.u2 * k, pv = i + a.v1 * j + a.v2 * k;
+            const long long pw = i + a.w1 * j + a.w2 * k;
+            v = fmax(v, fabs(((double)U[pu] - (double)U[pu - 1]) * a.rdx[i] +
+                             ((double)V[pv] - (double)V[pv - a.v1]) * a.rdy[j] +
+                             ((double)W[pw] - (double)W[pw - a.w2]) * a.rdz[k]));
         }
     v = block_max_256(v, sh);
     const long long bid = blockIdx.x + (long long)gridDim.x * (blockIdx.y + (long long)gridDim.y * blockIdx.z);
@@ -1778,7 +1773,7 @@ inline int march_chunk(const Dims &d) {
 
 template <int MODE, int FOUT, class T>
 void march(Handle &h, const Fld3<T> &cur, const Fld3<T> *old, const Fld3<T> *f1, Fld3<T> *nxt, Fld3<T> *fo,
-           double c0, double c1, double dt, Fld3<T> *go = nullptr, double g2 = 0.0, double g3 = 0.0) {
+           double c0, double c1, double dt, double g2 = 0.0) {
     const Dims &d = h.d;
     const int chunk = march_chunk(d);
     const dim3 grid = march_grid<T>(d, chunk), block(256);
@@ -1786,18 +1781,16 @@ void march(Handle &h, const Fld3<T> &cur, const Fld3<T> *old, const Fld3<T> *f1,
     const T *f[3] = {f1 ? f1->u : nullptr, f1 ? f1->v : nullptr, f1 ? f1->w : nullptr};
     T *n[3] = {nxt ? nxt->u : nullptr, nxt ? nxt->v : nullptr, nxt ? nxt->w : nullptr};
     T *r[3] = {fo ? fo->u : nullptr, fo ? fo->v : nullptr, fo ? fo->w : nullptr};
-    T *g[3] = {go ? go->u : nullptr, go ? go->v : nullptr, go ? go->w : nullptr};
     const MetricsT<T> &m = metrics<T>(h);
     if (h.les != 0)
         hipLaunchKernelGGL((k_rk_march<MODE, FOUT, true, T>), grid, block, 0, h.stream, d, m, (T)h.nu,
                            (T)h.dPdx, (T)h.dPdz, chunk, cur.u, cur.v, cur.w, reinterpret_cast<const T *>(h.nu_t),
                            o[0], o[1], o[2], f[0], f[1], f[2], n[0], n[1], n[2], r[0], r[1], r[2], (T)c0, (T)c1,
-                           (T)dt, g[0], g[1], g[2], (T)g2, (T)g3);
+                           (T)dt, (T)g2);
     else
         hipLaunchKernelGGL((k_rk_march<MODE, FOUT, false, T>), grid, block, 0, h.stream, d, m, (T)h.nu,
                            (T)h.dPdx, (T)h.dPdz, chunk, cur.u, cur.v, cur.w, nullptr, o[0], o[1], o[2], f[0],
-                           f[1], f[2], n[0], n[1], n[2], r[0], r[1], r[2], (T)c0, (T)c1, (T)dt, g[0], g[1], g[2],
-                           (T)g2, (T)g3);
+                           f[1], f[2], n[0], n[1], n[2], r[0], r[1], r[2], (T)c0, (T)c1, (T)dt, (T)g2);
     check_launch();
 }
 
@@ -1806,16 +1799,14 @@ void launch_rk_stage(Handle &h, const Fld3<T> &cur, const Fld3<T> &old, Fld3<T> 
     const bool rk3 = h.scheme == 3;
     Fld3<T> F1 = as<T>(h.F1);
     ProfScope ps(h, KID_RK_STAGE);
-    // FP64 RK3 carries G_s = Uo + dt a_s1 F1 (s = 2, 3) from the first stage
-    // (in the F1 buffers and G3); FP32 fields carry Uo and F1, since storing
-    // G_s in FP32 would round a velocity-sized value once more per stage
-    const bool gform = rk3 && sizeof(T) == 8 && h.G3.u != nullptr;
-    Fld3<T> G3 = as<T>(h.G3);
+    // FP64 RK3 carries G = Uo + dt a21 F1 (= Uo + dt a31 F1) from the first
+    // stage in the F1 buffers; FP32 fields carry Uo and F1, since storing G in
+    // FP32 would round a velocity-sized value once more per stage
+    const bool gform = rk3 && sizeof(T) == 8;
     if (stage == 1) {
         const double c0 = dt * (rk3 ? h.rk_coef[1][1] : h.rk2_coef[1][1]);
         if (gform)
-            march<0, 2>(h, cur, &old, (const Fld3<T> *)nullptr, &nxt, &F1, c0, 0.0, dt, &G3,
-                        dt * h.rk_coef[2][1], dt * h.rk_coef[3][1]);
+            march<0, 2>(h, cur, &old, (const Fld3<T> *)nullptr, &nxt, &F1, c0, 0.0, dt, dt * h.rk_coef[2][1]);
         else if (rk3)
             march<0, 1>(h, cur, &old, (const Fld3<T> *)nullptr, &nxt, &F1, c0, 0.0, dt);
         else
@@ -1825,9 +1816,8 @@ void launch_rk_stage(Handle &h, const Fld3<T> &cur, const Fld3<T> &old, Fld3<T> 
         march<0, 0>(h, cur, &old, (const Fld3<T> *)nullptr, &nxt, (Fld3<T> *)nullptr, dt * h.rk2_coef[2][2],
                     0.0, dt);
     } else if (gform) {
-        // RK3 stages 2, 3: new = G_s + dt a_ss F (a_32 = 0 so F2 is never needed)
-        const Fld3<T> &gs = stage == 2 ? F1 : G3;
-        march<0, 0>(h, cur, &gs, (const Fld3<T> *)nullptr, &nxt, (Fld3<T> *)nullptr, dt * h.rk_coef[stage][stage],
+        // RK3 stages 2, 3: new = G + dt a_ss F (a_32 = 0 so F2 is never needed)
+        march<0, 0>(h, cur, &F1, (const Fld3<T> *)nullptr, &nxt, (Fld3<T> *)nullptr, dt * h.rk_coef[stage][stage],
                     0.0, dt);
     } else {
         // RK3 stages 2, 3: a_32 = 0 so F2 is never needed
@@ -1903,12 +1893,16 @@ void launch_poisson(Handle &h, double *phi, const Field3 *div) {
     zdft_launch(h, KID_XFORM_Z_FWD, true, h.T1, h.S);
     {
     ProfScope ps(h, KID_THOMAS);
-    // the (0,0) column of S in place (the partitioned kernel leaves it alone)
-    if (!launch_thomas_seg<double>(h, h.S, N, (long long)plane, solve00_args(h, h.S, N, 1.0, nullptr, 0))) {
+    // the partitioned kernel leaves the (0,0) column of S to its block 0,
+    // which solves it in place; the table kernel (no length limit) solves it
+    // with the other systems
+    if (h.th_p != 0) {
+        launch_thomas_seg<double>(h, h.S, N, (long long)plane, solve00_args(h, h.S, N, 1.0, nullptr, 0));
+    } else {
         hipLaunchKernelGGL(k_thomas<8>, dim3((N + 255) / 256 * M), dim3(256), 0, h.stream, d, h.lu_dl,
                            h.lu_dp, h.lu_du, h.S);
+        check_launch();
     }
-    check_launch();
     }
     // inverse z: S -> T1
     zdft_launch(h, KID_XFORM_Z_INV, false, h.S, h.T1);
@@ -1979,15 +1973,45 @@ void launch_pressure(Handle &h, const T *phi, T *P, double dt) {
     check_launch();
 }
 
+long long maxdiv_partials(int nxg, int nyg, int nzg) {
+    Dims d{};
+    d.nxg = nxg;
+    d.nyg = nyg;
+    d.nzg = nzg;
+    const dim3 g = reduce_grid(d);
+    return (long long)g.x * g.y * g.z;
+}
+
+template <class T>
+void launch_maxdiv(hipStream_t stream, const DivArgs &a, const T *U, const T *V, const T *W, double *part,
+                   double *out) {
+    Dims d{};
+    d.nxg = a.nxg;
+    d.nyg = a.nyg;
+    d.nzg = a.nzg;
+    const dim3 g = reduce_grid(d);
+    const long long nb = (long long)g.x * g.y * g.z;
+    hipLaunchKernelGGL(k_maxdiv<T>, g, dim3(kWave, 4), 0, stream, a, U, V, W, part);
+    check_launch();
+    hipLaunchKernelGGL(k_reduce_max, dim3(1), dim3(1024), 0, stream, part, nb, 1, out);
+    check_launch();
+}
+template void launch_maxdiv<double>(hipStream_t, const DivArgs &, const double *, const double *, const double *,
+                                    double *, double *);
+
 template <class T>
 double max_divergence(Handle &h, const Fld3<T> &f) {
     const Dims &d = h.d;
-    const dim3 g = reduce_grid(d);
-    const long long nb = (long long)g.x * g.y * g.z;
-    hipLaunchKernelGGL(k_maxdiv<T>, g, dim3(kWave, 4), 0, h.stream, d, h.m, f.u, f.v, f.w, h.red);
-    check_launch();
-    hipLaunchKernelGGL(k_reduce_max, dim3(1), dim3(1024), 0, h.stream, h.red, nb, 1, h.scal);
-    check_launch();
+    DivArgs a{};
+    a.nxg = d.nxg;
+    a.nyg = d.nyg;
+    a.nzg = d.nzg;
+    a.u1 = a.v1 = a.w1 = d.sx;
+    a.u2 = a.v2 = a.w2 = d.sxy;
+    a.rdx = h.m.rdx;
+    a.rdy = h.m.rdy;
+    a.rdz = h.m.rdz;
+    launch_maxdiv(h.stream, a, f.u, f.v, f.w, h.red, h.scal);
     double v = 0.0;
     BL_HIP(hipMemcpyAsync(&v, h.scal, sizeof(double), hipMemcpyDeviceToHost, h.stream));
     BL_HIP(hipStreamSynchronize(h.stream));

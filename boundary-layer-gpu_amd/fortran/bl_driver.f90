@@ -191,7 +191,7 @@ contains
     ! conditions (time_integration.f90:145-465)
     if (itime_step /= 2 .and. itime_step /= 3) stop 'Error: only RKscheme = 2 or 3 is built'
     if (iwall_model /= 0) stop 'Error: only WM = 0 is built'
-    if (les_model /= 0 .and. les_model /= 2) stop 'Error: only LES = 0 or 2 is built'
+    if (les_model < 0 .or. les_model > 3) stop 'Error: LES must be 0, 1, 2 or 3'
     if (iprecision /= 0 .and. iprecision /= 1) stop 'Error: precision must be 0 (FP64) or 1 (mixed)'
     if (inflow_flag == 3 .or. inflow_flag == 5) stop 'Error: turbulent (Lund) inflow is not built'
   end subroutine read_parameters

@@ -42,8 +42,12 @@ enum { BL_PREC_FP64 = 0, BL_PREC_MIXED = 1 };
  * LU / twiddle precompute (initialization.f90:313-320, 427-469): the library
  * derives metrics, transform matrices and LU factors itself from the grids.
  *  x(nx), y(ny), z(nz): face coordinates (x_global, y_global, z_global).
- *  scheme: RKscheme (2 or 3), les: LES flag (0 DNS, 2 dynamic Smagorinsky,
- *  4 learned closure with dynamic Smagorinsky fallback),
+ *  scheme: RKscheme (2 or 3), les: LES flag as compute_eddy_viscosity
+ *  dispatches it (subgrid.f90:32-45): 0 DNS, 1 constant-coefficient
+ *  Smagorinsky with van Driest damping (sgs_Smagorinsky :609-662), 2 dynamic
+ *  Smagorinsky averaged in z (:261-414), 3 dynamic Smagorinsky per cell
+ *  (:436-583); and 4, this library's learned closure with dynamic Smagorinsky
+ *  fallback,
  *  cfl: .turbb CFL (negative = fixed dt = -cfl).
  *  Boundary tables as built by compute_blasius_solution_for_bc
  *  (boundary_conditions.f90:1452-1797): U_inlet(nyg) V_inlet(ny) W_inlet(nyg)
@@ -52,8 +56,10 @@ enum { BL_PREC_FP64 = 0, BL_PREC_MIXED = 1 };
  *  tmode(m_modes) already multiplied by beta/omega, and q*_re/q*_im laid out
  *  (j, n, m) with j fastest, already interpolated to yg (u, w) and y (v).
  *  nut_bc: .turbb nutBC (params.f90:156), the y ghost rows of nu_t after the
- *  dynamic Smagorinsky model: 1 = antisymmetric (apply_Dirichlet_bc_y),
- *  otherwise symmetric (apply_Neumann_bc_y), subgrid.f90:406-411.
+ *  dynamic Smagorinsky models: 1 = antisymmetric (apply_Dirichlet_bc_y),
+ *  otherwise symmetric (apply_Neumann_bc_y), subgrid.f90:406-411; LES = 1
+ *  applies its van Driest damping unless nut_bc = 0 and leaves the ghost
+ *  entries of nu_t untouched (subgrid.f90:635-646).
  */
 typedef struct bl_config {
     int nx, ny, nz;
@@ -103,8 +109,8 @@ int bl_run(bl_handle *h, int nsteps);
 int bl_synchronize(bl_handle *h);
 
 /* compute_eddy_viscosity (subgrid.f90:23-47) on the current fields:
- * recomputes nu_t on the device (LES = 2: dynamic Smagorinsky,
- * sgs_dynamic_Smagorinsky subgrid.f90:261-414; LES = 0: nu_t = 0).  bl_step
+ * recomputes nu_t on the device for the handle's LES model (1, 2, 3, 4 as in
+ * bl_config; LES = 0: nu_t = 0).  bl_step
  * does this itself at the start of every RK stage, as the reference does
  * (time_integration.f90:341-346, 383-388, 425-430). */
 int bl_eddy_viscosity(bl_handle *h);

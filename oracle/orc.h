@@ -22,8 +22,10 @@ typedef struct orc_state orc_state;
 
 /* Build the state from face grids x(nx), y(ny), z(nz) (initialization.f90:
  * 174-227, 254-263, 328-344, 394-425, 479-484, 503-530).  scheme = 2 or 3
- * (RKscheme), les = LES flag (0 DNS, 1 Smagorinsky, 2 DSM z-averaged).
- * cfl is the .turbb CFL (negative = fixed dt). */
+ * (RKscheme), les = LES flag (0 DNS, 1 constant-coefficient Smagorinsky,
+ * 2 dynamic Smagorinsky z-averaged, 3 dynamic Smagorinsky per cell;
+ * subgrid.f90:32-45).  cfl is the .turbb CFL (negative = fixed dt).
+ * Returns NULL for any other les. */
 orc_state *orc_create(int nx, int ny, int nz, const double *x, const double *y, const double *z,
                       double nu, double dPdx, double dPdz, int scheme, int les, double cfl);
 void orc_destroy(orc_state *s);

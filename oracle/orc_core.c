@@ -22,7 +22,9 @@
 #include "orc_fft.h"
 #include "orc_internal.h"
 
-void orc_dsm(orc_state *s); /* orc_dsm.c */
+void orc_dsm(orc_state *s);          /* orc_dsm.c */
+void orc_dsm_noavg(orc_state *s);    /* orc_dsm.c */
+void orc_smagorinsky(orc_state *s);  /* orc_dsm.c */
 
 static double *dalloc(size_t n) { return (double *)calloc(n ? n : 1, sizeof(double)); }
 static double sq(double a) { return a * a; }
@@ -42,6 +44,8 @@ void orc_centers(int n, const double *x, double *xm, double *xg) {
 
 orc_state *orc_create(int nx, int ny, int nz, const double *x, const double *y, const double *z,
                       double nu, double dPdx, double dPdz, int scheme, int les, double cfl) {
+    /* LES_model 0..3 (subgrid.f90:32-45); anything else has no model */
+    if (les < 0 || les > 3) return NULL;
     orc_state *s = (orc_state *)calloc(1, sizeof(*s));
     s->nx = nx;
     s->ny = ny;
@@ -1072,8 +1076,12 @@ void orc_monitor(const orc_state *s, double *out) {
 
 void orc_eddy_viscosity(orc_state *s) {
     /* subgrid.f90:32-45 */
-    if (s->les == 2) {
+    if (s->les == 1) {
+        orc_smagorinsky(s);
+    } else if (s->les == 2) {
         orc_dsm(s);
+    } else if (s->les == 3) {
+        orc_dsm_noavg(s);
     } else {
         memset(s->nu_t, 0, sizeof(double) * nP(s));
         memset(s->avg_nu_t, 0, sizeof(double) * (size_t)s->nxg * s->nyg);

@@ -1,11 +1,14 @@
 /*
  * CPU-ORACLE TEST INFRASTRUCTURE — not part of the product.
  *
- * Dynamic Smagorinsky eddy viscosity, z-averaged (LES = 2): FP64
- * restatement of sgs_dynamic_Smagorinsky (subgrid.f90:261-414) with
- * filter_xzy (:140-187), compute_Sij (:672-730), and the single-rank
- * behaviour of the helpers it calls: apply_Neumann_bc_x
- * (boundary_conditions.f90:326-344), apply_periodic_bc_z (:732-757),
+ * Smagorinsky-type eddy viscosities: FP64 restatements of
+ * sgs_dynamic_Smagorinsky (LES = 2, z-averaged, subgrid.f90:261-414),
+ * sgs_dynamic_Smagorinsky_noaverage (LES = 3, per cell, :436-583) and
+ * sgs_Smagorinsky (LES = 1, constant coefficient with van Driest damping,
+ * :609-662), with filter_xzy (:140-187), compute_Sij (:672-730), and the
+ * single-rank behaviour of the helpers they call: apply_Neumann_bc_x
+ * (boundary_conditions.f90:326-344), apply_periodic_bc_x (:295-314),
+ * apply_periodic_bc_z (:732-757),
  * update_ghost_interior_planes (:1185-1354, a no-op on one rank),
  * apply_Neumann_bc_y / apply_Dirichlet_bc_y (:1072-1087, :1020-1035), and the
  * face-to-centre interpolations (interpolation.f90:24-114).
@@ -18,7 +21,10 @@
  * Pinned by tests/test_oracle_dsm.py: uniform-strain and linear fields give
  * nu_t = 0 (L_ij = 0), a brute-force numpy restatement on tiny grids written
  * independently from the formulas, the filter's moments (it reproduces
- * linear fields in x and z exactly and has unit sum), and nu_t >= 0.
+ * linear fields in x and z exactly and has unit sum), and nu_t >= 0; LES = 3
+ * against LES = 2 on fields uniform in z and x-periodic (where the z average
+ * and the x fixes change nothing), and LES = 1 against the closed form
+ * (Cs Delta f)^2 |S| of uniform-shear fields.
  */
 #include <math.h>
 #include <stdlib.h>
@@ -87,6 +93,32 @@ static void neumann_x(double *F, int n1, int n2, int n3, int last) {
         }
 }
 
+/* apply_periodic_bc_x: id 1 (x faces, extent nx): F(1)=F(nx-1), F(nx)=F(2);
+ * otherwise (x centres, extent nxg): F(1)=F(nxg-2), F(nxg-1)=F(2),
+ * F(nxg)=F(3) -- the last interior centre is overwritten too */
+static void periodic_x(double *F, int n1, int n2, int n3, int faces) {
+    for (int k = 0; k < n3; k++)
+        for (int j = 0; j < n2; j++) {
+            double *row = F + (size_t)n1 * ((size_t)j + (size_t)n2 * k);
+            if (faces) {
+                row[0] = row[n1 - 2];
+                row[n1 - 1] = row[1];
+            } else {
+                row[0] = row[n1 - 3];
+                row[n1 - 2] = row[1];
+                row[n1 - 1] = row[2];
+            }
+        }
+}
+
+/* the x fix of the dynamic models: Neumann (LES = 2) or periodic (LES = 3) */
+static void fix_x(int per, double *F, int n1, int n2, int n3, int faces) {
+    if (per)
+        periodic_x(F, n1, n2, n3, faces);
+    else
+        neumann_x(F, n1, n2, n3, n1);
+}
+
 /* apply_periodic_bc_z, single rank; id 1/2/4 (z centres) or 3 (z faces) */
 static void periodic_z(orc_state *s, double *F, int n1, int n2, int id) {
     size_t pl = (size_t)n1 * n2;
@@ -151,7 +183,12 @@ static void compute_sij(orc_state *s, const double *U, const double *V, const do
 #undef W_
 }
 
-void orc_dsm(orc_state *s) {
+/* LES = 2 (noavg = 0): sgs_dynamic_Smagorinsky; LES = 3 (noavg = 1):
+ * sgs_dynamic_Smagorinsky_noaverage, which differs in its x fixes
+ * (apply_periodic_bc_x where LES = 2 has apply_Neumann_bc_x), in the ratio
+ * <L:M>/<M:M> taken per cell instead of over z, in the periodic x and z
+ * fixes of nu_t before clipping, and in avg_nu_t = 0 */
+static void dsm_common(orc_state *s, int noavg) {
     const int nx = s->nx, ny = s->ny, nz = s->nz, nxg = s->nxg, nyg = s->nyg, nzg = s->nzg;
     const size_t nP = (size_t)nxg * nyg * nzg;
     const double *U = s->U, *V = s->V, *W = s->W;
@@ -185,7 +222,7 @@ void orc_dsm(orc_state *s) {
         for (int j = 1; j <= nyg; j++)
             for (int i = 1; i <= nxg; i++)
                 A3(term_2, i, j, k) = (k <= nz) ? 0.5 * (W[IW(i, j, k - 1)] + W[IW(i, j, k)]) : 0.0;
-    neumann_x(term, nxg, nyg, nzg, nxg);
+    fix_x(noavg, term, nxg, nyg, nzg, 0);
     periodic_z(s, term_2, nxg, nyg, 4);
 
     /* products at cell centres (Sij as placeholder), copied to ten_buf */
@@ -201,7 +238,7 @@ void orc_dsm(orc_state *s) {
                 A3(ten[5], i, j, k) = b * c;
             }
     for (int c = 0; c < 6; c++) {
-        neumann_x(ten[c], nxg, nyg, nzg, nxg);
+        fix_x(noavg, ten[c], nxg, nyg, nzg, 0);
         periodic_z(s, ten[c], nxg, nyg, 4);
         /* hat(ui uj): filter_xzy(ten_buf(1:nxg,2:nyg-1,1:nzg), Lij(2:nxg-1,2:nyg-1,2:nzg-1)) */
         filter_xzy(ten[c], nxg, nyg, 1, nxg, 2, nyg - 2, 1, nzg, Lij[c], nxg, nyg, 2, 2, 2);
@@ -221,11 +258,11 @@ void orc_dsm(orc_state *s) {
     filter_xzy(U, nx, nyg, 1, nx, 1, nyg, 1, nzg, Uf, nx, nyg, 2, 1, 2);
     filter_xzy(V, nxg, ny, 1, nxg, 1, ny, 1, nzg, Vf, nxg, ny, 2, 1, 2);
     filter_xzy(W, nxg, nyg, 1, nxg, 1, nyg, 1, nz, Wf, nxg, nyg, 2, 1, 2);
-    neumann_x(Uf, nx, nyg, nzg, nx);
+    fix_x(noavg, Uf, nx, nyg, nzg, 1);
     periodic_z(s, Uf, nx, nyg, 1);
-    neumann_x(Vf, nxg, ny, nzg, nxg);
+    fix_x(noavg, Vf, nxg, ny, nzg, 0);
     periodic_z(s, Vf, nxg, ny, 2);
-    neumann_x(Wf, nxg, nyg, nz, nxg);
+    fix_x(noavg, Wf, nxg, nyg, nz, 0);
     periodic_z(s, Wf, nxg, nyg, 3);
 
     compute_sij(s, Uf, Vf, Wf, Sij, S);
@@ -242,7 +279,7 @@ void orc_dsm(orc_state *s) {
         for (int k = 2; k <= nzg - 1; k++)
             for (int j = 2; j <= nyg - 1; j++)
                 for (int i = 2; i <= nxg - 1; i++) A3(term, i, j, k) = A3(S, i, j, k) * A3(Sij[c], i, j, k);
-        neumann_x(term, nxg, nyg, nzg, nxg);
+        fix_x(noavg, term, nxg, nyg, nzg, 0);
         periodic_z(s, term, nxg, nyg, 4);
         filter_xzy(term, nxg, nyg, 1, nxg, 2, nyg - 2, 1, nzg, Sij[c], nxg, nyg, 2, 2, 2);
     }
@@ -261,6 +298,23 @@ void orc_dsm(orc_state *s) {
                                         : 2.0 * A3(Mij[c], i, j, k) * A3(Lij[c], i, j, k);
                     A3(nu_t, i, j, k) = A3(nu_t, i, j, k) + ml;
                 }
+    if (noavg) {
+        /* subgrid.f90:545-581 */
+        for (int k = 2; k <= nzg - 1; k++)
+            for (int j = 2; j <= nyg - 1; j++)
+                for (int i = 2; i <= nxg - 1; i++) {
+                    double m1 = A3(Mij[0], i, j, k), m2 = A3(Mij[1], i, j, k), m3 = A3(Mij[2], i, j, k);
+                    double m4 = A3(Mij[3], i, j, k), m5 = A3(Mij[4], i, j, k), m6 = A3(Mij[5], i, j, k);
+                    A3(nu_t, i, j, k) = A3(nu_t, i, j, k) /
+                                        (m1 * m1 + m2 * m2 + m3 * m3 + 2.0 * (m4 * m4 + m5 * m5 + m6 * m6));
+                }
+        for (int k = 2; k <= nzg - 1; k++)
+            for (int j = 2; j <= nyg - 1; j++)
+                for (int i = 2; i <= nxg - 1; i++) A3(nu_t, i, j, k) = -0.5 * A3(nu_t, i, j, k) * A3(S, i, j, k);
+        periodic_x(nu_t, nxg, nyg, nzg, 0);
+        periodic_z(s, nu_t, nxg, nyg, 4);
+        memset(avg, 0, sizeof(double) * (size_t)nxg * nyg);
+    } else {
     const double nz3 = (double)(nzg - 3);
 #define AVG(i, j) avg[(size_t)((i)-1) + (size_t)nxg * (size_t)((j)-1)]
     for (int i = 2; i <= nxg - 1; i++)
@@ -286,6 +340,7 @@ void orc_dsm(orc_state *s) {
         for (int j = 2; j <= nyg - 1; j++)
             for (int k = 2; k <= nzg - 1; k++) A3(nu_t, i, j, k) = -0.5 * AVG(i, j) * A3(S, i, j, k);
 #undef AVG
+    }
     /* clipping (Max(nu_t, 0): a NaN ratio becomes 0), then the y ghosts */
     for (size_t q = 0; q < nP; q++) nu_t[q] = fmax(nu_t[q], 0.0);
     for (int k = 1; k <= nzg; k++)
@@ -298,5 +353,51 @@ void orc_dsm(orc_state *s) {
                 A3(nu_t, i, nyg, k) = A3(nu_t, i, nyg - 1, k);
             }
         }
+    free(work);
+}
+
+void orc_dsm(orc_state *s) { dsm_common(s, 0); }
+
+void orc_dsm_noavg(orc_state *s) { dsm_common(s, 1); }
+
+/* sgs_Smagorinsky (subgrid.f90:609-662): nu_t = (Cs Delta f)^2 |S| on the
+ * interior cells, Cs = 0.11, Delta = (dx dz dy_j)^(1/3) with dx, dz the
+ * smallest spacings, van Driest f = 1 - exp(-y_wall/25/(nu/utau)) with
+ * utau = dPdx^0.5 and y_wall the distance to the nearer wall (j < nyg/2:
+ * bottom), f = 1 unless nutBC = 1.  The ghost entries of nu_t are not
+ * written (they keep their previous values); the wall-parallel average
+ * avg_nu_t(j) is clipped and gets the y ghost condition. */
+void orc_smagorinsky(orc_state *s) {
+    const int nxg = s->nxg, nyg = s->nyg, nzg = s->nzg;
+    double *Sij[6], *S;
+    const size_t nP = (size_t)nxg * nyg * nzg;
+    double *work = (double *)calloc(nP * 7, sizeof(double));
+    for (int c = 0; c < 6; c++) Sij[c] = work + (size_t)c * nP;
+    S = work + 6 * nP;
+    compute_sij(s, s->U, s->V, s->W, Sij, S);
+    const double Cs = 0.11;
+    const double utau_t = pow(s->dPdx, 0.5);
+    double *nu_t = s->nu_t, *avg = s->avg_nu_t;
+    for (int j = 2; j <= nyg - 1; j++) {
+        double f;
+        if (j < nyg / 2)
+            f = 1.0 - exp(-(YG(j) - Y(1)) / 25.0 / (s->nu / utau_t));
+        else
+            f = 1.0 - exp(-(Y(s->ny) - YG(j)) / 25.0 / (s->nu / utau_t));
+        if (s->nut_dirichlet == 0) f = 1.0;
+        const double Delta = pow(s->dx * s->dz * (Y(j) - Y(j - 1)), 1.0 / 3.0);
+        const double c = pow(Cs * Delta * f, 2.0);
+        for (int k = 2; k <= nzg - 1; k++)
+            for (int i = 2; i <= nxg - 1; i++) A3(nu_t, i, j, k) = c * A3(S, i, j, k);
+    }
+    /* avg_nu_t(1, j, 1) in the first column of the (nxg, nyg) array */
+    for (int j = 2; j <= nyg - 1; j++) {
+        double a = 0.0;
+        for (int k = 2; k <= nzg - 1; k++)
+            for (int i = 2; i <= nxg - 1; i++) a += A3(nu_t, i, j, k);
+        avg[(size_t)nxg * (j - 1)] = fmax(a / (double)((nxg - 2) * (nzg - 2)), 0.0);
+    }
+    avg[0] = s->nut_dirichlet == 1 ? -avg[(size_t)nxg] : avg[(size_t)nxg];
+    avg[(size_t)nxg * (nyg - 1)] = s->nut_dirichlet == 1 ? -avg[(size_t)nxg * (nyg - 2)] : avg[(size_t)nxg * (nyg - 2)];
     free(work);
 }

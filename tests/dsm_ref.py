@@ -1,5 +1,8 @@
-"""Vectorised numpy statement of the z-averaged dynamic Smagorinsky model
-(subgrid.f90:261-414) in index-map form (test helper).
+"""Vectorised numpy statement of the dynamic Smagorinsky models in index-map
+form (test helper): z-averaged (LES = 2, subgrid.f90:261-414) and per cell
+(LES = 3, sgs_dynamic_Smagorinsky_noaverage, :436-583, whose x fixes are
+apply_periodic_bc_x, boundary_conditions.f90:295-314: centres 0 <- nxg-3,
+nxg-2 <- 1, nxg-1 <- 2, faces 0 <- nx-2, nx-1 <- 1).
 
 The reference fills the ghost entries of its work arrays in place
 (apply_Neumann_bc_x, apply_periodic_bc_z) between steps.  This statement
@@ -80,13 +83,23 @@ def strain(x, y, z, xg, yg, zg, U, V, W):
     return s, mag
 
 
-def dsm(x, y, z, U, V, W, nut_bc=0):
+def xmap_periodic(n1, faces):
+    m = np.arange(n1)
+    if faces:
+        m[0], m[n1 - 1] = n1 - 2, 1
+    else:
+        m[0], m[n1 - 2], m[n1 - 1] = n1 - 3, 1, 2
+    return m
+
+
+def dsm(x, y, z, U, V, W, nut_bc=0, noavg=False):
     nx, ny, nz = x.size, y.size, z.size
     nxg, nyg, nzg = nx + 1, ny + 1, nz + 1
     _, xg = orc.centers(x)
     _, yg = orc.centers(y)
     _, zg = orc.centers(z)
-    ic = np.clip(np.arange(nxg), 1, nxg - 2)
+    ic = xmap_periodic(nxg, False) if noavg else np.clip(np.arange(nxg), 1, nxg - 2)
+    iface = xmap_periodic(nx, True) if noavg else np.clip(np.arange(nx), 1, nx - 2)
     zc = zmap_c(nzg)
     Jall = np.arange(nyg)
     # cell-centred velocities (interpolation.f90 + the fixes of subgrid.f90:282-284)
@@ -106,7 +119,7 @@ def dsm(x, y, z, U, V, W, nut_bc=0):
     s, S = strain(x, y, z, xg, yg, zg, U, V, W)
     hs = [filt((S * s[c])[remap]) for c in range(6)]
     # test-filtered velocities with their fixes
-    Uf = xzfilter(yfilter(U, 0, nyg - 1))[np.ix_(np.clip(np.arange(nx), 1, nx - 2), np.arange(nyg), zc)]
+    Uf = xzfilter(yfilter(U, 0, nyg - 1))[np.ix_(iface, np.arange(nyg), zc)]
     Vf = xzfilter(yfilter(V, 0, ny - 1))[np.ix_(ic, np.arange(ny), zc)]
     Wf = xzfilter(yfilter(W, 0, nyg - 1))[np.ix_(ic, Jall, zmap_f(nz))]
     sh, Sh = strain(x, y, z, xg, yg, zg, Uf, Vf, Wf)
@@ -115,10 +128,14 @@ def dsm(x, y, z, U, V, W, nut_bc=0):
     MM = M[0] ** 2 + M[1] ** 2 + M[2] ** 2 + 2 * (M[3] ** 2 + M[4] ** 2 + M[5] ** 2)
     ks = slice(1, nzg - 2)
     with np.errstate(invalid="ignore", divide="ignore"):
-        avg = LM[:, :, ks].mean(axis=2) / MM[:, :, ks].mean(axis=2)
         nut = np.zeros((nxg, nyg, nzg))
-        core = -0.5 * avg[:, :, None] * S
-        nut[1:nxg - 1, 1:nyg - 1, 1:nzg - 1] = core[1:nxg - 1, 1:nyg - 1, 1:nzg - 1]
+        if noavg:
+            core = -0.5 * LM / MM * S
+            nut[:, 1:nyg - 1] = core[np.ix_(ic, np.arange(1, nyg - 1), zc)]
+        else:
+            avg = LM[:, :, ks].mean(axis=2) / MM[:, :, ks].mean(axis=2)
+            core = -0.5 * avg[:, :, None] * S
+            nut[1:nxg - 1, 1:nyg - 1, 1:nzg - 1] = core[1:nxg - 1, 1:nyg - 1, 1:nzg - 1]
     nut = np.fmax(nut, 0.0)
     sg = -1.0 if nut_bc == 1 else 1.0
     nut[:, 0] = sg * nut[:, 1]

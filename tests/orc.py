@@ -129,6 +129,8 @@ class Oracle:
         self.nx, self.ny, self.nz = self.x.size, self.y.size, self.z.size
         self.h = lib().orc_create(self.nx, self.ny, self.nz, _p(self.x), _p(self.y), _p(self.z), nu, dPdx,
                                   dPdz, scheme, les, cfl)
+        if not self.h:
+            raise ValueError(f"oracle: unsupported configuration (les={les})")
         self.shapes = [(self.nx, self.ny + 1, self.nz + 1), (self.nx + 1, self.ny, self.nz + 1),
                        (self.nx + 1, self.ny + 1, self.nz), (self.nx + 1, self.ny + 1, self.nz + 1),
                        (self.nx + 1, self.ny + 1, self.nz + 1)]

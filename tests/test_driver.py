@@ -175,7 +175,7 @@ def test_driver_requires_input_file(tmp_path):
 @pytest.mark.parametrize("key,val,msg", [
     ("RKscheme", "1", "only RKscheme = 2 or 3"),
     ("WM", "1", "only WM = 0"),
-    ("LES", "1", "only LES = 0 or 2"),
+    ("LES", "5", "LES must be 0, 1, 2 or 3"),
     ("inflow_flag", "3", "Lund"),
     ("precision", "2", "precision must be 0"),
 ])

@@ -1,4 +1,4 @@
-"""LES path on the device (LES = 2): dynamic Smagorinsky eddy viscosity
+"""LES path on the device: dynamic Smagorinsky eddy viscosity (LES = 2)
 (subgrid.f90:261-414) and the variable-viscosity RHS (equations.f90:97-132,
 231-270, 367-404), compared with the FP64 oracle through the C ABI.
 
@@ -118,3 +118,59 @@ def test_les_step_parity(les_case, nut_bc):
     assert rel_l2(cf_g, cf_o) < 1e-10
     mo, mg = o.monitor(), s.monitor()
     np.testing.assert_allclose(mg, mo, rtol=1e-10, atol=1e-14)
+
+
+# ---- LES = 1 (constant-coefficient Smagorinsky, subgrid.f90:609-662) and
+# ---- LES = 3 (dynamic Smagorinsky per cell, subgrid.f90:436-583)
+OTHER_MODELS = {
+    "smag_tanh": dict(nx=24, ny=16, nz=10, noise=0.05, seed=21, scheme=3, les=1, dPdx=4e-4),
+    "noavg_tanh": dict(nx=24, ny=16, nz=10, noise=0.05, seed=22, scheme=3, les=3),
+    "noavg_sinh_rk2": dict(nx=26, ny=14, nz=9, grid="sinh", alpha=2.5, noise=0.05, seed=23, scheme=2, les=3),
+}
+
+
+@pytest.mark.parametrize("name", sorted(OTHER_MODELS))
+@pytest.mark.parametrize("nut_bc", [0, 1])
+def test_other_models_nu_t_parity(name, nut_bc):
+    say("model case", name, OTHER_MODELS[name], "nut_bc", nut_bc)
+    c = cases.laminar_small(**OTHER_MODELS[name])
+    o = c.oracle()
+    o.set_nut_bc(nut_bc)
+    s, _ = solver_for(c, o, nut_bc=nut_bc)
+    # LES = 1 leaves the ghost entries alone: both start from the same values
+    seed = np.random.default_rng(5).random(o.shapes[4])
+    o.set(4, seed)
+    s.upload(4, seed)
+    ref = o.eddy_viscosity()
+    s.eddy_viscosity()
+    got = s.download(4)
+    scale = np.abs(ref[1:-1, 1:-1, 1:-1]).max()
+    assert scale > 0
+    err = np.abs(got - ref).max() / scale
+    say("nu_t max", scale, "err", err)
+    assert err < 1e-11
+
+
+@pytest.mark.parametrize("name", sorted(OTHER_MODELS))
+def test_other_models_step_parity(name):
+    say("model case", name, OTHER_MODELS[name])
+    c = cases.laminar_small(**OTHER_MODELS[name])
+    o = c.oracle()
+    o.set_nut_bc(1)
+    s, _ = solver_for(c, o, nut_bc=1)
+    for n in range(1, 11):
+        o.compute_dt()
+        s.compute_dt()
+        o.step()
+        s.step()
+        if n in (1, 10):
+            tol = 1e-12 if n == 1 else 1e-10
+            U = o.fields()[0]
+            for fname, a, b in zip("UVW", s.fields(), o.fields()):
+                e = np.linalg.norm(a - b) / np.linalg.norm(U)
+                say("step", n, fname, e)
+                assert e < tol, (n, fname, e)
+            e = np.abs(s.download(4) - o.get(4)).max() / max(np.abs(o.get(4)).max(), 1e-300)
+            say("step", n, "nu_t", e)
+            assert e < 1e-9
+    assert rel_l2(s.wall_stats(), o.wall_stats()) < 1e-10

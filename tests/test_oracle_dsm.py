@@ -10,6 +10,14 @@ subgrid.f90:261-414).
   loop-by-loop C restatement on random fields, for both nutBC choices;
 - nu_t >= 0 on the interior rows, ghost rows follow nutBC, and the x-ghost
   columns and z-ghost planes stay 0 as on one rank.
+
+The per-cell model (LES = 3, subgrid.f90:436-583) has the same closed form
+away from the x ends, agrees with the index-map statement with its periodic
+x maps, and its x and z ghosts are periodic copies.  The constant-coefficient
+model (LES = 1, :609-662) gives (Cs Delta_j f_j)^2 |b| for the uniform shear
+u = b y, with Delta_j^3 = dx dz dy_j and the van Driest factor f of the
+distance to the nearer wall in units nu / sqrt(dPdx), and leaves the ghost
+entries of nu_t as they were.
 """
 import numpy as np
 import pytest
@@ -22,8 +30,8 @@ def uniform_grid(nx, ny, nz, hx, hy, hz):
     return 1.0 + hx * np.arange(nx), hy * np.arange(ny), hz * np.arange(nz)
 
 
-def oracle_dsm(x, y, z, U, V, W, nut_bc=0):
-    o = orc.Oracle(x, y, z, 1e-5, les=2)
+def oracle_dsm(x, y, z, U, V, W, nut_bc=0, les=2):
+    o = orc.Oracle(x, y, z, 1e-5, les=les)
     o.set_nut_bc(nut_bc)
     o.set_fields(U, V, W)
     nut = o.eddy_viscosity()
@@ -109,3 +117,77 @@ def test_dsm_sign_and_ghosts(nut_bc):
     # one rank: the x-ghost columns and z-ghost planes are never filled
     assert not nut[0].any() and not nut[-1].any()
     assert not nut[:, :, 0].any() and not nut[:, :, -1].any()
+
+
+# ---------------------------------------------------------------- LES = 3 ----
+@pytest.mark.parametrize("hx,hy,a", [(0.01, 0.02, 1.0), (0.03, 0.01, -2.0)])
+def test_dsm_noavg_plane_strain_closed_form(hx, hy, a):
+    x, y, z = uniform_grid(14, 12, 7, hx, hy, 0.01)
+    nut = oracle_dsm(x, y, z, *plane_strain(x, y, z, a), les=3)
+    nxg, nyg = x.size + 1, y.size + 1
+    core = nut[2:nxg - 3, 2:nyg - 2, 1:-1]
+    expect = max(a * (hy ** 2 - hx ** 2) / 36.0, 0.0)
+    np.testing.assert_allclose(core, expect, rtol=1e-9, atol=1e-12 * abs(a) * hy ** 2)
+
+
+@pytest.mark.parametrize("seed,nut_bc", [(0, 0), (1, 1), (5, 0)])
+def test_dsm_noavg_matches_index_map_statement(seed, nut_bc):
+    x, y, z, U, V, W = random_case(seed)
+    a = oracle_dsm(x, y, z, U, V, W, nut_bc, les=3)
+    b = dsm_numpy(x, y, z, U, V, W, nut_bc, noavg=True)
+    assert np.abs(b).max() > 0
+    np.testing.assert_allclose(a, b, rtol=0, atol=1e-12 * np.abs(b).max())
+
+
+def test_dsm_noavg_periodic_ghosts():
+    x, y, z, U, V, W = random_case(6)
+    nut = oracle_dsm(x, y, z, U, V, W, 0, les=3)
+    n1, n3 = nut.shape[0], nut.shape[2]
+    np.testing.assert_array_equal(nut[0], nut[n1 - 3])
+    np.testing.assert_array_equal(nut[n1 - 2], nut[1])
+    np.testing.assert_array_equal(nut[n1 - 1], nut[2])
+    np.testing.assert_array_equal(nut[:, :, 0], nut[:, :, n3 - 3])
+    np.testing.assert_array_equal(nut[:, :, n3 - 1], nut[:, :, 2])
+    assert (nut >= 0).all() and (nut[1:-1, 1:-1, 1:-1] > 0).any()
+
+
+# ---------------------------------------------------------------- LES = 1 ----
+@pytest.mark.parametrize("nut_bc", [0, 1])
+def test_smagorinsky_uniform_shear(nut_bc):
+    nu, dPdx, b = 1e-5, 4e-4, 3.0
+    rng = np.random.default_rng(2)
+    x = 1.0 + 0.02 * np.arange(12)
+    y = np.cumsum(np.r_[0, 0.002 + 0.004 * rng.random(11)])
+    z = 0.013 * np.arange(8)
+    _, yg = orc.centers(y)
+    U = np.zeros((x.size, y.size + 1, z.size + 1), order="F")
+    U[:] = b * yg[None, :, None]
+    V = np.zeros((x.size + 1, y.size, z.size + 1), order="F")
+    W = np.zeros((x.size + 1, y.size + 1, z.size), order="F")
+    o = orc.Oracle(x, y, z, nu, dPdx, 0.0, 3, 1)
+    o.set_nut_bc(nut_bc)
+    o.set_fields(U, V, W)
+    sentinel = np.full(o.shapes[4], 7.5, order="F")
+    o.set(4, sentinel)
+    nut = o.eddy_viscosity()
+    _, xg = orc.centers(x)
+    _, zg = orc.centers(z)
+    dx, dz = np.diff(xg).min(), np.diff(zg).min()
+    nyg = y.size + 1
+    for j in range(1, nyg - 1):  # 0-based interior rows
+        delta = (dx * dz * (y[j] - y[j - 1])) ** (1.0 / 3.0)
+        # distance to the bottom wall for the lower rows (j1 < nyg/2 in 1-based
+        # rows), to the top wall otherwise, in viscous units nu / utau
+        dist = yg[j] - y[0] if (j + 1) < nyg // 2 else y[-1] - yg[j]
+        f = 1.0 - np.exp(-dist / 25.0 / (nu / np.sqrt(dPdx))) if nut_bc == 1 else 1.0
+        np.testing.assert_allclose(nut[1:-1, j, 1:-1], (0.11 * delta * f) ** 2 * abs(b), rtol=1e-12)
+    # ghost entries keep what they held
+    for g in (nut[0], nut[-1], nut[:, 0], nut[:, -1], nut[:, :, 0], nut[:, :, -1]):
+        assert np.all(g == 7.5)
+
+
+def test_unknown_les_model_is_rejected():
+    x, y, z = uniform_grid(8, 6, 6, 0.01, 0.01, 0.01)
+    with pytest.raises(ValueError):
+        orc.Oracle(x, y, z, 1e-5, les=4)
+
