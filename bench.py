@@ -18,8 +18,8 @@ events on the solver's stream; it gives `kernels` and picks the class with
 the largest share of GPU time.  In the timed steps only that
 class is bracketed (the events of every class would add ~0.3 ms to a step),
 and `roofline` is its average launch time there.  `cpu_baseline` is the
-FP64 C oracle timed on this host for one RK3 step of a z-slab of the same
-workload.
+FP64 C oracle timed on this host at the full size (all cores this process may
+use), with a one-core estimate from a z-slab.
 `mixed` is the same workload in mixed precision (FP32 fields, split-BF16x3
 MFMA transforms, FP64 Thomas / mass flux / statistics): its ns/cell/step, the
 MFMA rate of its x-transform GEMM, and the relative L2 difference of u, v, w,
@@ -103,25 +103,50 @@ def kernel_units(name, nx, ny, nz):
     return "byte", None
 
 
-def cpu_baseline(nz_sample):
+def cpu_baseline(steps, nz_slab):
+    """The FP64 C oracle (kind "port") on this host: the full 814x257x193
+    workload, one warm-up and `steps` timed RK3 steps on every core this
+    process may run on; then a z-slab (814x257xnz_slab) of the same workload,
+    one step on all of those cores and one on a single core, which scales the
+    full-size time to a one-core estimate."""
     from tests import orc
 
-    nx, ny = 814, 257
-    x, y, z, tables, (U, V, W) = workload(nx, ny, nz_sample)
-    o = orc.Oracle(x, y, z, NU, 0.0, 0.0, 3, 0, CFL)
-    o.set_bc_tables(tables)
-    o.set_fields(U, V, W)
-    o.compute_dt()
-    o.step()  # warm-up (page-in, thread pool)
-    t0 = time.perf_counter()
-    o.compute_dt()
-    o.step()
-    el = time.perf_counter() - t0
-    cells = nx * ny * nz_sample
-    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    return dict(value=el / cells * 1e9, unit="ns/cell/step", cores=cores, kind="port",
-                sample=f"FP64 C oracle, 1 timed RK3 step (after 1 warm-up) of an {nx}x{ny}x{nz_sample} "
-                       f"z-slab of the same workload, OpenMP on {cores} threads")
+    cores = len(os.sched_getaffinity(0))
+
+    def oracle(nz):
+        x, y, z, tables, (U, V, W) = workload(814, 257, nz)
+        o = orc.Oracle(x, y, z, NU, 0.0, 0.0, 3, 0, CFL)
+        o.set_bc_tables(tables)
+        o.set_fields(U, V, W)
+        return o, 814 * 257 * nz
+
+    def timed(o, n):
+        t0 = time.perf_counter()
+        for _ in range(n):
+            o.compute_dt()
+            o.step()
+        return (time.perf_counter() - t0) / n
+
+    orc.set_threads(cores)
+    o, cells = oracle(193)
+    timed(o, 1)  # warm-up (page-in, thread pool)
+    full = timed(o, steps) / cells * 1e9
+    o.close()
+    del o
+    o, cells_s = oracle(nz_slab)
+    timed(o, 1)
+    slab_n = timed(o, 1) / cells_s * 1e9
+    orc.set_threads(1)
+    slab_1 = timed(o, 1) / cells_s * 1e9
+    orc.set_threads(cores)
+    o.close()
+    return dict(value=full, unit="ns/cell/step", cores=cores, kind="port",
+                sample=f"FP64 C oracle (OpenMP), 814x257x193, {steps} timed RK3 steps after 1 warm-up, "
+                       f"{cores} threads",
+                one_core_estimate=slab_1 * full / slab_n,
+                one_core_basis=f"one RK3 step of an 814x257x{nz_slab} z-slab on 1 thread ({slab_1:.1f} "
+                               f"ns/cell/step) and on {cores} ({slab_n:.1f}), scaled by the full-size "
+                               f"{cores}-thread time")
 
 
 def run_leg(blgpu, precision, grid, args, keep_fields=False):
@@ -352,7 +377,9 @@ def main():
     ap.add_argument("--nx", type=int, default=814)
     ap.add_argument("--ny", type=int, default=257)
     ap.add_argument("--nz", type=int, default=193)
-    ap.add_argument("--cpu-nz", type=int, default=17, help="z extent of the CPU-baseline sample (0 = skip)")
+    ap.add_argument("--cpu-steps", type=int, default=3,
+                    help="timed oracle steps at full size for the CPU baseline (0 = skip the baseline)")
+    ap.add_argument("--cpu-nz", type=int, default=17, help="z extent of the one-core CPU-baseline slab")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision leg")
     ap.add_argument("--no-les", action="store_true", help="skip the learned-closure LES leg")
     ap.add_argument("--calib-copy", type=int, default=0, metavar="MIB",
@@ -431,9 +458,9 @@ def main():
         les_out = les_leg(blgpu, args, lambda m: print(m, file=sys.stderr, flush=True))
 
     cpu = None
-    if args.cpu_nz > 0:
-        print(f"bench: CPU baseline sample nz={args.cpu_nz}", file=sys.stderr, flush=True)
-        cpu = cpu_baseline(args.cpu_nz)
+    if args.cpu_steps > 0:
+        print(f"bench: CPU baseline, {args.cpu_steps} full-size oracle steps", file=sys.stderr, flush=True)
+        cpu = cpu_baseline(args.cpu_steps, args.cpu_nz)
 
     out = {
         "metric": "ns/cell/step, 814x257x193 RK3 DNS (FP64 vs mixed); LES Cf error at equal cost",

@@ -46,6 +46,13 @@ SIGNATURES = {
     "bl_max_divergence": (_I, [_P, _DP]),
     "bl_wall_stats": (_I, [_P, _P]),
     "bl_monitor": (_I, [_P, _P]),
+    "bl_statistics": (_I, [_P, _P, _P, _P]),
+    "bl_fftz_create": (_I, [_I, ctypes.POINTER(_P)]),
+    "bl_fftz_destroy": (None, [_P]),
+    "bl_fft_z": (_I, [_P, _I, _P, _P]),
+    "bl_ifft_z": (_I, [_P, _I, _P, _P]),
+    "bl_snapshot_begin": (_I, [_P]),
+    "bl_snapshot_end": (_I, [_P] + [ctypes.POINTER(_DP)] * 5),
     "bl_rhs": (_I, [_P, _P, _P, _P]),
     "bl_set_old": (_I, [_P, _P, _P, _P]),
     "bl_apply_bc": (_I, [_P, _I, _D]),
@@ -268,6 +275,37 @@ class Solver:
         _check(lib().bl_monitor(self.h, out.ctypes.data), "bl_monitor")
         return out
 
+    STAT_NAMES = ("Umean", "Vmean", "Wmean", "U2mean", "V2mean", "W2mean", "UVmean", "Pmean", "P2mean")
+
+    def statistics(self):
+        """compute_statistics' z means (bl_statistics): a dict of the nine
+        (nx, ny) arrays, nu_t_mean (nxg, nyg) and Cf (nx)."""
+        nx, ny = self.nx, self.ny
+        means = np.zeros(9 * nx * ny)
+        ntm = np.zeros((nx + 1, ny + 1), order="F")
+        cf = np.zeros(nx)
+        _check(lib().bl_statistics(self.h, means.ctypes.data, ntm.ctypes.data, cf.ctypes.data), "bl_statistics")
+        m = means.reshape((9, ny, nx))
+        out = {k: np.asfortranarray(m[q].T) for q, k in enumerate(self.STAT_NAMES)}
+        out["nu_t_mean"] = ntm
+        out["Cf"] = cf
+        return out
+
+    def snapshot_begin(self):
+        _check(lib().bl_snapshot_begin(self.h), "bl_snapshot_begin")
+
+    def snapshot_end(self):
+        """U, V, W, P, nu_t of the last snapshot_begin (copies of the
+        library's pinned buffer)."""
+        ptrs = [_DP() for _ in range(5)]
+        _check(lib().bl_snapshot_end(self.h, *[ctypes.byref(q) for q in ptrs]), "bl_snapshot_end")
+        out = []
+        for q, c in zip(ptrs, (0, 1, 2, 3, 4)):
+            shape = self.shapes[c]
+            a = np.ctypeslib.as_array(q, shape=(int(np.prod(shape)),))
+            out.append(np.array(a.reshape(shape, order="F"), order="F", copy=True))
+        return tuple(out)
+
     def rhs(self):
         r = [np.zeros(self.shapes[c], order="F") for c in range(3)]
         _check(lib().bl_rhs(self.h, *[a.ctypes.data for a in r]), "bl_rhs")
@@ -313,6 +351,47 @@ class Solver:
         assert r.shape == self.shapes[FIELD_P]
         _check(lib().bl_poisson(self.h, r.ctypes.data), "bl_poisson")
         return r
+
+
+class FFTZ:
+    """The fftz module's plan (bl_fftz_*): real DFT along the second axis of
+    (n1, N) arrays, r2c unnormalised and c2r divided by N."""
+
+    def __init__(self, N):
+        self.N = int(N)
+        p = _P()
+        _check(lib().bl_fftz_create(self.N, ctypes.byref(p)), "bl_fftz_create")
+        self.p = p
+
+    def close(self):
+        if getattr(self, "p", None):
+            lib().bl_fftz_destroy(self.p)
+            self.p = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def forward(self, U):
+        U = np.asfortranarray(U, dtype=np.float64)
+        n1, N = U.shape
+        assert N == self.N
+        out = np.zeros((2, n1, N // 2 + 1), order="F")
+        _check(lib().bl_fft_z(self.p, n1, U.ctypes.data, out.ctypes.data), "bl_fft_z")
+        return out[0] + 1j * out[1]
+
+    def inverse(self, Uh):
+        Uh = np.asarray(Uh, dtype=np.complex128)
+        n1, K = Uh.shape
+        assert K == self.N // 2 + 1
+        buf = np.zeros((2, n1, K), order="F")
+        buf[0] = Uh.real
+        buf[1] = Uh.imag
+        out = np.zeros((n1, self.N), order="F")
+        _check(lib().bl_ifft_z(self.p, n1, buf.ctypes.data, out.ctypes.data), "bl_ifft_z")
+        return out
 
 
 def _f32(a):

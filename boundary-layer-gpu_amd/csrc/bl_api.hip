@@ -35,6 +35,25 @@ constexpr double kPiD = 3.141592653589793238462643383279502884;  // = 4*atan(1) 
 
 int round_up(int v, int m) { return (v + m - 1) / m * m; }
 
+}  // namespace
+
+// cos/sin(2 pi k m / M) in long double with the exact integer reduction of
+// k m mod M, rounded once; symmetric in (k, m), lda covers padded row blocks
+void z_dft_tables(int M, int &lda, std::vector<double> &zc, std::vector<double> &zs) {
+    lda = round_up(M + 1, 64);
+    zc.assign((size_t)lda * lda, 0.0);
+    zs.assign((size_t)lda * lda, 0.0);
+    for (int m = 0; m < M; ++m)
+        for (int k = 0; k < M; ++k) {
+            const long long r = ((long long)k * m) % M;
+            const long double ang = 2.0L * kPi * (long double)r / (long double)M;
+            zc[(size_t)m * lda + k] = (double)cosl(ang);
+            zs[(size_t)m * lda + k] = (double)sinl(ang);
+        }
+}
+
+namespace {
+
 // initialization.f90:181-207
 void centers(int n, const double *x, std::vector<double> &xg) {
     std::vector<double> xm(n - 1);
@@ -511,15 +530,8 @@ void build(Handle &h, const bl_config *c) {
     }
     // z cos/sin tables, symmetric in (k, m): lda covers the padded row blocks
     {
-        h.lda_zt = round_up(M + 1, 64);
-        std::vector<double> zc((size_t)h.lda_zt * h.lda_zt, 0.0), zs((size_t)h.lda_zt * h.lda_zt, 0.0);
-        for (int m = 0; m < M; ++m)
-            for (int k = 0; k < M; ++k) {
-                const long long r = ((long long)k * m) % M;
-                const long double ang = 2.0L * kPi * (long double)r / (long double)M;
-                zc[(size_t)m * h.lda_zt + k] = (double)cosl(ang);
-                zs[(size_t)m * h.lda_zt + k] = (double)sinl(ang);
-            }
+        std::vector<double> zc, zs;
+        z_dft_tables(M, h.lda_zt, zc, zs);
         h.zc = dmalloc(zc.size());
         h.zs = dmalloc(zs.size());
         upload_vec(h.zc, zc, h.stream);
@@ -598,6 +610,7 @@ void build(Handle &h, const bl_config *c) {
 void prof_free(Handle &h);
 
 void free_all(Handle &h) {
+    snapshot_free(h);
     auto f = [](void *p) {
         if (p) (void)hipFree(p);
     };
@@ -611,19 +624,20 @@ void free_all(Handle &h) {
                     (void *)h.dct_wN, (void *)h.dct_rot, (void *)h.dct_mf, h.mx2, (void *)h.zc, (void *)h.zs, (void *)h.lu_dl, (void *)h.lu_dp, (void *)h.lu_du,
                     (void *)h.lu00, (void *)h.th_co,
                     (void *)h.scal, (void *)h.red, (void *)h.dsm, (void *)h.smag_c, h.mem_grid, h.mem_gridf, (void *)h.mx_tab,
-                    (void *)h.p00, (void *)h.cl_w, (void *)h.cl_b, (void *)h.cl_cnt})
+                    (void *)h.p00, (void *)h.inc1, (void *)h.stats, (void *)h.cl_w, (void *)h.cl_b, (void *)h.cl_cnt})
         f(p);
     if (h.stream) (void)hipStreamSynchronize(h.stream);
     prof_free(h);
     if (h.stream) (void)hipStreamDestroy(h.stream);
 }
 
-// one projection sub-step on the current buffer (projection.f90:21-87)
-void projection_on(Handle &h, Field3 &f, int rk_step, double dt) {
+// one projection sub-step on the current buffer (projection.f90:21-87);
+// inc: the mixed path's first-stage source (see IncSrc)
+void projection_on(Handle &h, Field3 &f, int rk_step, double dt, const IncSrc *inc = nullptr) {
     if (h.precision == BL_PREC_MIXED) {
         Fld3<float> ff = as<float>(f);
         float *phi = reinterpret_cast<float *>(h.phi);
-        launch_poisson_mixed(h, phi, ff);
+        launch_poisson_mixed(h, phi, ff, inc);
         launch_project(h, ff, (const float *)phi);
         if (rk_step == 1) launch_pressure(h, (const float *)phi, reinterpret_cast<float *>(h.P), dt);
         return;
@@ -657,13 +671,26 @@ void do_step_t(Handle &h) {
         Fld3<T> next = as<T>(h.buf[nxt]);
         launch_rk_stage(h, as<T>(h.buf[cur]), old, next, s, dt);
         const double ts = to + (nst == 3 ? h.rk_t[s] : h.rk2_t[s]) * dt;
-        launch_bc(h, next, old, s, dt, ts);
-        projection_on(h, h.buf[nxt], s, dt);
+        // FP32 fields that a previous step projected: the first stage's
+        // pressure comes from the divergence of the stage increment, not of
+        // U* (see IncSrc); fields set from outside may carry a divergence of
+        // their own, which the first stage projects as the reference does
+        const bool inc = sizeof(T) == 4 && s == 1 && h.inc_ok;
+        launch_bc(h, next, old, s, dt, ts, inc ? h.inc1 : nullptr, to);
+        if (inc) {
+            const Fld3<float> f1 = as<float>(h.F1), vn = as<float>(h.buf[X]), vs = as<float>(h.buf[nxt]);
+            const IncSrc src{f1.u, f1.v, f1.w, vs.v, vn.v, h.inc1,
+                             dt * (nst == 3 ? h.rk_coef[1][1] : h.rk2_coef[1][1])};
+            projection_on(h, h.buf[nxt], s, dt, &src);
+        } else {
+            projection_on(h, h.buf[nxt], s, dt);
+        }
         launch_bc(h, next, old, s, dt, ts);
         h.t = ts;
         cur = nxt;
     }
     h.icur = h.iold = cur;
+    h.inc_ok = true;
 }
 
 void do_step(Handle &h) {
@@ -750,6 +777,8 @@ int guarded(F &&fn) {
         return BL_EINVAL;
     } catch (const DeviceError &) {
         return BL_EDEVICE;
+    } catch (const StateError &) {
+        return BL_ESTATE;
     } catch (const std::bad_alloc &) {
         if (last_error().empty()) last_error() = "out of memory";
         return BL_ENOMEM;
@@ -822,6 +851,7 @@ int bl_upload(bl_handle *hh, int field, const double *host) {
         int n1, n2, n3;
         field_extent(h.d, field, n1, n2, n3);
         copy_to_device(h, field_dev(h, field), host, n1, n2, n3);
+        if (field <= BL_FIELD_W) h.inc_ok = false;
     });
 }
 
@@ -926,6 +956,53 @@ int bl_wall_stats(bl_handle *hh, double *cf) {
     });
 }
 
+int bl_statistics(bl_handle *hh, double *means, double *nu_t_mean, double *cf) {
+    return guarded([&] {
+        Handle &h = H(hh);
+        if (!means) {
+            last_error() = "bl_statistics: null means array";
+            throw ArgError();
+        }
+        const Dims &d = h.d;
+        const long long n2 = (long long)d.nx * d.ny;
+        if (!h.stats) BL_HIP(hipMalloc(&h.stats, sizeof(double) * stats_size(d)));
+        launch_statistics(h, h.buf[h.icur], h.stats);
+        BL_HIP(hipMemcpyAsync(means, h.stats, sizeof(double) * 9 * n2, hipMemcpyDeviceToHost, h.stream));
+        if (nu_t_mean)
+            BL_HIP(hipMemcpyAsync(nu_t_mean, h.stats + 9 * n2, sizeof(double) * (size_t)d.nxg * d.nyg,
+                                  hipMemcpyDeviceToHost, h.stream));
+        BL_HIP(hipStreamSynchronize(h.stream));
+        if (cf) {
+            if (h.precision == BL_PREC_MIXED)
+                wall_stats(h, as<float>(h.buf[h.icur]), cf);
+            else
+                wall_stats(h, as<double>(h.buf[h.icur]), cf);
+        }
+    });
+}
+
+int bl_snapshot_begin(bl_handle *hh) {
+    return guarded([&] { snapshot_begin(H(hh)); });
+}
+
+int bl_snapshot_end(bl_handle *hh, const double **U, const double **V, const double **W, const double **P,
+                    const double **nu_t) {
+    return guarded([&] {
+        Handle &h = H(hh);
+        if (!U || !V || !W || !P || !nu_t) {
+            last_error() = "bl_snapshot_end: null output pointer";
+            throw ArgError();
+        }
+        const double *ptr[5];
+        snapshot_end(h, ptr);
+        *U = ptr[0];
+        *V = ptr[1];
+        *W = ptr[2];
+        *P = ptr[3];
+        *nu_t = ptr[4];
+    });
+}
+
 int bl_monitor(bl_handle *hh, double *out) {
     return guarded([&] {
         Handle &h = H(hh);
@@ -979,6 +1056,7 @@ int bl_apply_bc(bl_handle *hh, int rk_step, double dt) {
     return guarded([&] {
         Handle &h = H(hh);
         if (rk_step < 1 || rk_step > 3) throw ArgError();
+        h.inc_ok = false;
         if (h.precision == BL_PREC_MIXED)
             bc_on<float>(h, h.buf[h.icur], h.buf[h.iold], rk_step, dt, h.t);
         else
@@ -990,6 +1068,7 @@ int bl_apply_bc(bl_handle *hh, int rk_step, double dt) {
 int bl_projection(bl_handle *hh, int rk_step, double dt) {
     return guarded([&] {
         Handle &h = H(hh);
+        h.inc_ok = false;
         projection_on(h, h.buf[h.icur], rk_step, dt);
         BL_HIP(hipStreamSynchronize(h.stream));
     });

@@ -657,6 +657,7 @@ void closure_samples(Handle &fine, Handle &coarse, Mlp &m, int fx, int fy, int f
     BL_HIP(hipStreamSynchronize(m.stream));
     Coarsen cz{f, c, fx, fy, fz, ddy};
     const Field3 &F = fine.buf[fine.icur], &C = coarse.buf[coarse.icur];
+    coarse.inc_ok = false;
     const dim3 blk(256);
     hipLaunchKernelGGL(k_coarsen_u, dim3((c.nx + 63) / 64, (c.nyg + 3) / 4, c.nzg), blk, 0, fine.stream, cz, F.u, C.u);
     check();

@@ -11,6 +11,7 @@
 
 #include <cstddef>
 #include <string>
+#include <vector>
 
 namespace blgpu {
 
@@ -193,6 +194,19 @@ struct Handle {
     const unsigned short *dxf_hi, *dxf_lo, *dxi_hi, *dxi_lo, *zf_hi, *zf_lo, *zi_hi, *zi_lo;
     double *p00;                    // FP64 plane partials of the divergence
     double *c00;                    // phi00(j) / (2 N M)
+    double *inc1 = nullptr;         // first-stage FP64 boundary increments: inflow and outflow U
+                                    // planes [k][j] (nyg nzg each), then the mass-flux shift
+    bool inc_ok = false;            // the current fields are the output of a step (projected: their
+                                    // divergence is rounding only), not uploaded or set otherwise
+    double *stats = nullptr;        // device statistics: 9 (nx, ny) means, then nu_t_mean (nxg, nyg)
+    // asynchronous snapshots: device staging and pinned host copy of U, V, W,
+    // P, nu_t in the reference's unpadded FP64 layout (offsets snap_off), the
+    // copy stream and its events
+    double *snap_dev = nullptr, *snap_host = nullptr;
+    size_t snap_off[5] = {0, 0, 0, 0, 0};
+    hipStream_t copy_stream = nullptr;
+    hipEvent_t snap_packed = nullptr, snap_copied = nullptr;
+    bool snap_pending = false;
     int np00;                       // partials per y row
     ProfState *prof;                // null unless profiling is enabled
     unsigned long long prof_mask = ~0ull;  // kernel classes the profile brackets (bit = class id)
@@ -214,8 +228,12 @@ template <class T>
 const MetricsT<T> &metrics(const Handle &h);
 template <class T>
 void launch_rk_stage(Handle &h, const Fld3<T> &cur, const Fld3<T> &old, Fld3<T> &nxt, int stage, double dt);
+// inc1 != null: also write the FP64 increments of the inflow and outflow U
+// planes over the step-start field (t_old its time) and the mass-flux shift
+// into inc1 (layout of Handle::inc1)
 template <class T>
-void launch_bc(Handle &h, Fld3<T> &f, const Fld3<T> &old, int rk_step, double dt, double t);
+void launch_bc(Handle &h, Fld3<T> &f, const Fld3<T> &old, int rk_step, double dt, double t,
+               double *inc1 = nullptr, double t_old = 0.0);
 // div != null: the rhs is the divergence of *div (computed inside the x transform)
 void launch_poisson(Handle &h, double *phi_inout, const Field3 *div = nullptr);
 template <class T>
@@ -266,9 +284,40 @@ template <class T>
 void monitor(Handle &h, const Fld3<T> &f, double *out_host);
 template <class T>
 double dt_convective(Handle &h, const Fld3<T> &f);
+// the z DFT tables cos/sin(2 pi k m / M), [m][k] with leading dimension lda
+void z_dft_tables(int M, int &lda, std::vector<double> &zc, std::vector<double> &zs);
+// the real z DFT of C columns (plane stride `plane`) on stream s: forward to
+// the packed half spectrum (rows 0, 2k-1, 2k), or its unnormalised inverse
+void launch_zdft(hipStream_t s, int M, int C, long long plane, int lda, const double *zc, const double *zs,
+                 bool forward, const double *in, double *out);
+// ---- statistics and snapshots (bl_stats.hip) ----
+long long stats_size(const Dims &d);
+// compute_statistics' z means of f (and the handle's P, nu_t) into out
+// (device, stats_size doubles): Umean, Vmean, Wmean, U2mean, V2mean, W2mean,
+// UVmean, Pmean, P2mean as (nx, ny), then nu_t_mean (nxg, nyg)
+void launch_statistics(Handle &h, const Field3 &f, double *out);
+// pack the current U, V, W, P, nu_t on the solver's stream and copy them to
+// pinned host memory on the copy stream
+void snapshot_begin(Handle &h);
+// wait for that copy; ptr[0..4] point into the pinned copy
+void snapshot_end(Handle &h, const double **ptr);
+void snapshot_free(Handle &h);
 // ---- mixed precision (bl_mixed.hip) ----
 void build_mixed(Handle &h);
-void launch_poisson_mixed(Handle &h, float *phi, const Fld3<float> &f);
+// The first stage's pressure source in the mixed path: the divergence of the
+// stage's increment U* - Un, taken as c0 F1 on the faces the stage update
+// moves and as the FP64 boundary increments (Handle::inc1) or the difference
+// of the stored values on the boundary faces, instead of div U*.  In exact
+// arithmetic the two agree (div Un is zero after the previous projection);
+// in FP32 storage div Un is rounding noise of order eps |u| / dx, which the
+// projection would divide by dt a11 into the recorded pressure.
+struct IncSrc {
+    const float *fu, *fv, *fw;     // F1 of the stage
+    const float *vs, *vn;          // V after the stage's boundary update, and at the step start
+    const double *inc1;            // Handle::inc1
+    double c0;                     // dt a11
+};
+void launch_poisson_mixed(Handle &h, float *phi, const Fld3<float> &f, const IncSrc *inc = nullptr);
 // eddy viscosity of f into h.nu_t (bl_dsm.hip): LES = 1 constant-coefficient
 // Smagorinsky, 2 and 4 dynamic Smagorinsky z-averaged, 3 per cell
 constexpr int kDsmArrays = 15;
@@ -294,4 +343,5 @@ struct bl_handle : blgpu::Handle {};
 namespace blgpu {
 struct DeviceError {};
 struct ArgError {};
+struct StateError {};  // the call is not valid in the handle's current state
 }  // namespace blgpu

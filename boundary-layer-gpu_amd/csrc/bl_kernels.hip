@@ -191,14 +191,32 @@ __global__ __launch_bounds__(256) void k_rk_march(
                     const T up = sU[sc][e], ue = sU[sc][e + 1], uw = sU[sc][e - 1];
                     const T un = sU[sc][e + kHX], us = sU[sc][e - kHX];
                     const T ut = sU[sp][e], ub = sU[sm][e];
-                    rr = -(sq(0.5 * (up + ue)) - sq(0.5 * (uw + up))) * m.rdxg[i];
-                    rr = rr - ((m.wy0[j] * up + m.wy1[j] * un) * 0.5 * (sV[sc][e] + sV[sc][e + 1]) -
-                               (m.wy0[j - 1] * us + m.wy1[j - 1] * up) * 0.5 *
-                                   (sV[sc][e - kHX] + sV[sc][e + 1 - kHX])) *
-                                  m.rdy[j];
-                    rr = rr - (0.5 * (up + ut) * 0.5 * (sW[sc][e] + sW[sc][e + 1]) -
-                               0.5 * (ub + up) * 0.5 * (sW[sm][e] + sW[sm][e + 1])) *
-                                  rdzk;
+                    if constexpr (sizeof(T) == 8) {
+                        rr = -(sq(0.5 * (up + ue)) - sq(0.5 * (uw + up))) * m.rdxg[i];
+                        rr = rr - ((m.wy0[j] * up + m.wy1[j] * un) * 0.5 * (sV[sc][e] + sV[sc][e + 1]) -
+                                   (m.wy0[j - 1] * us + m.wy1[j - 1] * up) * 0.5 *
+                                       (sV[sc][e - kHX] + sV[sc][e + 1 - kHX])) *
+                                      m.rdy[j];
+                        rr = rr - (0.5 * (up + ut) * 0.5 * (sW[sc][e] + sW[sc][e + 1]) -
+                                   0.5 * (ub + up) * 0.5 * (sW[sm][e] + sW[sm][e + 1])) *
+                                      rdzk;
+                    } else {
+                        // FP32: the same fluxes with each difference of two
+                        // products of u (~U_inf) regrouped as a product of
+                        // differences, a b - c d = a (b - d) + d (a - c), so
+                        // that the O(1) parts cancel exactly (neighbouring u
+                        // differ by far less than a factor 2) instead of
+                        // leaving eps |u|^2 / dx of rounding in the flux
+                        // difference
+                        rr = -(T)0.25 * (ue - uw) * (ue + uw + 2 * up) * m.rdxg[i];
+                        const T an = up + m.wy1[j] * (un - up);
+                        const T da = (up - us) + m.wy1[j] * (un - up) - m.wy1[j - 1] * (up - us);
+                        const T bn = 0.5 * (sV[sc][e] + sV[sc][e + 1]), bs = 0.5 * (sV[sc][e - kHX] + sV[sc][e + 1 - kHX]);
+                        rr = rr - (an * (bn - bs) + bs * da) * m.rdy[j];
+                        const T ct = 0.5 * (up + ut);
+                        const T dtp = 0.5 * (sW[sc][e] + sW[sc][e + 1]), dbt = 0.5 * (sW[sm][e] + sW[sm][e + 1]);
+                        rr = rr - (ct * (dtp - dbt) + dbt * (T)0.5 * (ut - ub)) * rdzk;
+                    }
                     if constexpr (!LES) {
                         rr = rr + dPdx +
                              nu * ((m.rdx[i + 1] * (ue - up) - m.rdx[i] * (up - uw)) * m.rdxg[i] +
@@ -236,10 +254,20 @@ __global__ __launch_bounds__(256) void k_rk_march(
                     const T vp = sV[sc][e], ve = sV[sc][e + 1], vw = sV[sc][e - 1];
                     const T vn = sV[sc][e + kHX], vs = sV[sc][e - kHX];
                     const T vt = sV[sp][e], vb = sV[sm][e];
-                    rr = -(sq(0.5 * (vp + vn)) - sq(0.5 * (vs + vp))) * m.rdyg[j];
-                    rr = rr - ((m.wy0[j] * sU[sc][e] + m.wy1[j] * sU[sc][e + kHX]) * 0.5 * (vp + ve) -
-                               (m.wy0[j] * sU[sc][e - 1] + m.wy1[j] * sU[sc][e - 1 + kHX]) * 0.5 * (vw + vp)) *
-                                  m.rdx[i];
+                    if constexpr (sizeof(T) == 8) {
+                        rr = -(sq(0.5 * (vp + vn)) - sq(0.5 * (vs + vp))) * m.rdyg[j];
+                        rr = rr - ((m.wy0[j] * sU[sc][e] + m.wy1[j] * sU[sc][e + kHX]) * 0.5 * (vp + ve) -
+                                   (m.wy0[j] * sU[sc][e - 1] + m.wy1[j] * sU[sc][e - 1 + kHX]) * 0.5 * (vw + vp)) *
+                                      m.rdx[i];
+                    } else {
+                        // FP32: products of differences, as for u above
+                        rr = -(T)0.25 * (vn - vs) * (vn + vs + 2 * vp) * m.rdyg[j];
+                        const T ue0 = sU[sc][e], uw0 = sU[sc][e - 1];
+                        const T ae = ue0 + m.wy1[j] * (sU[sc][e + kHX] - ue0);
+                        const T da = (ue0 - uw0) + m.wy1[j] * ((sU[sc][e + kHX] - ue0) - (sU[sc][e - 1 + kHX] - uw0));
+                        const T bw = 0.5 * (vw + vp);
+                        rr = rr - (ae * (T)0.5 * (ve - vw) + bw * da) * m.rdx[i];
+                    }
                     rr = rr - (0.5 * (vp + vt) * (m.wy0[j] * sW[sc][e] + m.wy1[j] * sW[sc][e + kHX]) -
                                0.5 * (vb + vp) * (m.wy0[j] * sW[sm][e] + m.wy1[j] * sW[sm][e + kHX])) *
                                   rdzk;
@@ -278,10 +306,19 @@ __global__ __launch_bounds__(256) void k_rk_march(
                     const T wp = sW[sc][e], we = sW[sc][e + 1], ww = sW[sc][e - 1];
                     const T wn = sW[sc][e + kHX], ws = sW[sc][e - kHX];
                     const T wt = sW[sp][e], wb = sW[sm][e];
-                    rr = -(sq(0.5 * (wp + wt)) - sq(0.5 * (wb + wp))) * rdzgk;
-                    rr = rr - (0.5 * (sU[sc][e] + sU[sp][e]) * 0.5 * (wp + we) -
-                               0.5 * (sU[sc][e - 1] + sU[sp][e - 1]) * 0.5 * (ww + wp)) *
-                                  m.rdx[i];
+                    if constexpr (sizeof(T) == 8) {
+                        rr = -(sq(0.5 * (wp + wt)) - sq(0.5 * (wb + wp))) * rdzgk;
+                        rr = rr - (0.5 * (sU[sc][e] + sU[sp][e]) * 0.5 * (wp + we) -
+                                   0.5 * (sU[sc][e - 1] + sU[sp][e - 1]) * 0.5 * (ww + wp)) *
+                                      m.rdx[i];
+                    } else {
+                        // FP32: products of differences, as for u above
+                        rr = -(T)0.25 * (wt - wb) * (wt + wb + 2 * wp) * rdzgk;
+                        const T at = 0.5 * (sU[sc][e] + sU[sp][e]);
+                        const T da = 0.5 * ((sU[sc][e] - sU[sc][e - 1]) + (sU[sp][e] - sU[sp][e - 1]));
+                        const T bw = 0.5 * (ww + wp);
+                        rr = rr - (at * (T)0.5 * (we - ww) + bw * da) * m.rdx[i];
+                    }
                     rr = rr - (0.5 * (sV[sc][e] + sV[sp][e]) * (m.wy0[j] * wp + m.wy1[j] * wn) -
                                0.5 * (sV[sc][e - kHX] + sV[sp][e - kHX]) * (m.wy0[j - 1] * ws + m.wy1[j - 1] * wp)) *
                                   m.rdy[j];
@@ -337,10 +374,34 @@ __device__ __forceinline__ double block_sum_256(double v, double *sh) {
     return sh[0];
 }
 
+// temporal inflow modes Re(q e^{i(beta z - omega t)}) of row j, plane k at
+// time t (boundary_conditions.f90:153-170)
+__device__ __forceinline__ void inflow_modes(const Dims &d, const Metrics &m, const BCTables &b, int j, int k,
+                                             double t, double &ut, double &vt, double &wt) {
+    ut = vt = wt = 0.0;
+    const double zc = m.zg[k];
+    const double zf = (k < d.nz) ? m.z[k] : 0.0;
+    for (int n = 0; n < b.n_modes; ++n)
+        for (int mm = 0; mm < b.m_modes; ++mm) {
+            const double pc = b.zmode[n] * zc - b.tmode[mm] * t;
+            const double pf = b.zmode[n] * zf - b.tmode[mm] * t;
+            const long long qi = (long long)n + (long long)b.n_modes * mm;
+            const long long qu = j + (long long)d.nyg * qi;
+            const double cc = cos(pc), sc = sin(pc);
+            ut = ut + (b.qu_re[qu] * cc - b.qu_im[qu] * sc);
+            if (j < d.ny) {
+                const long long qv = j + (long long)d.ny * qi;
+                vt = vt + (b.qv_re[qv] * cc - b.qv_im[qv] * sc);
+            }
+            const double cf = cos(pf), sf = sin(pf);
+            wt = wt + (b.qw_re[qu] * cf - b.qw_im[qu] * sf);
+        }
+}
+
 template <class T>
 __global__ __launch_bounds__(256) void k_bc_inout(Dims d, Metrics m, BCTables b, T *U, T *V, T *W,
                                                   const T *Uo, const T *Vo, const T *Wo, double coef,
-                                                  double t, double *qpart) {
+                                                  double t, double *qpart, double *inc1, double t_old) {
     __shared__ double sh[256];
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
     const int k = blockIdx.y;
@@ -349,37 +410,32 @@ __global__ __launch_bounds__(256) void k_bc_inout(Dims d, Metrics m, BCTables b,
     double q = 0.0;
     if (j < d.nyg) {
     const int nx = d.nx, nxg = d.nxg;
-    // temporal modes Re(q e^{i(beta z - omega t)})
     double ut = 0.0, vt = 0.0, wt = 0.0;
-    if (b.n_modes > 0) {
-        const double zc = m.zg[k];
-        const double zf = (k < d.nz) ? m.z[k] : 0.0;
-        for (int n = 0; n < b.n_modes; ++n)
-            for (int mm = 0; mm < b.m_modes; ++mm) {
-                const double pc = b.zmode[n] * zc - b.tmode[mm] * t;
-                const double pf = b.zmode[n] * zf - b.tmode[mm] * t;
-                const long long qi = (long long)n + (long long)b.n_modes * mm;
-                const long long qu = j + (long long)d.nyg * qi;
-                const double cc = cos(pc), sc = sin(pc);
-                ut = ut + (b.qu_re[qu] * cc - b.qu_im[qu] * sc);
-                if (j < d.ny) {
-                    const long long qv = j + (long long)d.ny * qi;
-                    vt = vt + (b.qv_re[qv] * cc - b.qv_im[qv] * sc);
-                }
-                const double cf = cos(pf), sf = sin(pf);
-                wt = wt + (b.qw_re[qu] * cf - b.qw_im[qu] * sf);
-            }
-    }
+    if (b.n_modes > 0) inflow_modes(d, m, b, j, k, t, ut, vt, wt);
     const long long p0 = fidx(d, 0, j, k);
     // U (all j < nyg, k < nzg)
     const double uin = (double)(T)(b.U_inlet[j] + ut);  // the value stored
+    const long long pl = j + (long long)d.nyg * k;
+    if (inc1) {
+        // FP64 increment of the inflow face over the step-start value: that
+        // value is taken unrounded when it is the stored boundary value of
+        // t_old, as it is after any step
+        double uo = 0.0, vo = 0.0, wo = 0.0;
+        if (b.n_modes > 0) inflow_modes(d, m, b, j, k, t_old, uo, vo, wo);
+        const double bo = b.U_inlet[j] + uo;
+        const double so = (double)Uo[p0];
+        inc1[pl] = (b.U_inlet[j] + ut) - ((double)(T)bo == so ? bo : so);
+    }
     U[p0] = (T)uin;
     {
         const long long pa = fidx(d, nx - 2, j, k);
         const double o1 = Uo[pa], o0 = Uo[pa - 1];
-        const double v = (double)(T)(o1 - coef * (o1 - o0) / (m.x[nx - 2] - m.x[nx - 3]));
+        const double du = coef * (o1 - o0) / (m.x[nx - 2] - m.x[nx - 3]);
+        const double v = (double)(T)(o1 - du);
         U[pa] = (T)v;
         U[pa + 1] = (T)v;
+        // FP64 increment of the outflow face before the mass-flux shift
+        if (inc1) inc1[(long long)d.nyg * d.nzg + pl] = -du;
         if (j >= 1 && j <= d.nyg - 2 && k >= 1 && k <= d.nzg - 3) q = (uin - v) * (m.y[j] - m.y[j - 1]);
     }
     if (k < d.nz) {
@@ -458,12 +514,13 @@ __global__ __launch_bounds__(256) void k_bc_topwall(Dims d, Metrics m, BCTables 
 // outflow planes.  One thread per (j,k).
 template <class T>
 __global__ __launch_bounds__(256) void k_bc_massflux(Dims d, double length_y, const double *qpart,
-                                                     int nq, T *U) {
+                                                     int nq, T *U, double *du_out) {
     __shared__ double sh[256];
     double acc = 0.0;
     for (int t = threadIdx.x; t < nq; t += 256) acc += qpart[t];
     const double Q = block_sum_256(acc, sh);
     const double du = Q / length_y / (double)(d.nzg - 3);
+    if (du_out && blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) du_out[0] = du;
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
     const int k = blockIdx.y;
     if (j >= d.nyg) return;
@@ -1590,6 +1647,9 @@ __global__ __launch_bounds__(256) void k_wallstats(Dims d, Metrics m, double nu,
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= d.nx) return;
     const double nz2 = (double)(d.nzg - 2);
+    // the x neighbour of the last station is column nxg of nu_t_mean, which
+    // compute_statistics never fills (statistics.f90:77 runs ii = 1..nx): 0
+    const bool last = i == d.nx - 1;
     double u1 = 0.0, u2 = 0.0, uv = 0.0, n1a = 0.0, n2a = 0.0, n1b = 0.0, n2b = 0.0;
     for (int k = 1; k <= d.nzg - 2; ++k) {
         const long long p = fidx(d, i, 0, k);
@@ -1599,8 +1659,10 @@ __global__ __launch_bounds__(256) void k_wallstats(Dims d, Metrics m, double nu,
         uv += (m.wy0[0] * a + m.wy1[0] * b) * (0.5 * ((double)V[p] + (double)V[p + 1]));
         n1a += (double)nu_t[p];
         n2a += (double)nu_t[p + d.sx];
-        n1b += (double)nu_t[p + 1];
-        n2b += (double)nu_t[p + 1 + d.sx];
+        if (!last) {
+            n1b += (double)nu_t[p + 1];
+            n2b += (double)nu_t[p + 1 + d.sx];
+        }
     }
     const double dudy = (u2 / nz2 - u1 / nz2) / (m.yg[1] - m.yg[0]);
     const double tau = 0.25 * (n1a / nz2 + n2a / nz2 + n1b / nz2 + n2b / nz2) * dudy;
@@ -1667,20 +1729,23 @@ void check_launch() {
 constexpr int kDctLines = 2;
 constexpr int kZRows = 16, kZPre = 4;
 
+// the real z DFT (M points) of C columns with plane stride `plane`
+void zdft_raw(hipStream_t s, int M, int C, long long plane, int lda, const double *zc, const double *zs,
+              bool forward, const double *in, double *out) {
+    const int rows = M / 2 + 1;
+    const dim3 grid(xcd_blocks((rows + kZRows - 1) / kZRows * ((C + 255) / 256)));
+    if (forward)
+        hipLaunchKernelGGL((k_zdft_fwd<kZRows, kZPre>), grid, dim3(256), 0, s, M, C, plane, lda, zc, zs, in, out);
+    else
+        hipLaunchKernelGGL((k_zdft_inv<kZRows, kZPre>), grid, dim3(256), 0, s, M, C, plane, lda, zc, zs, in, out);
+    check_launch();
+}
+
 void zdft_launch(Handle &h, int kid, bool forward, const double *in, double *out) {
     ProfScope ps(h, kid);
     const Dims &d = h.d;
     const int C = d.N * d.NM;
-    const long long plane = (long long)C;
-    const int rows = d.M / 2 + 1;
-    const dim3 grid(xcd_blocks((rows + kZRows - 1) / kZRows * ((C + 255) / 256)));
-    if (forward)
-        hipLaunchKernelGGL((k_zdft_fwd<kZRows, kZPre>), grid, dim3(256), 0, h.stream, d.M, C, plane, h.lda_zt,
-                           h.zc, h.zs, in, out);
-    else
-        hipLaunchKernelGGL((k_zdft_inv<kZRows, kZPre>), grid, dim3(256), 0, h.stream, d.M, C, plane, h.lda_zt,
-                           h.zc, h.zs, in, out);
-    check_launch();
+    zdft_raw(h.stream, d.M, C, (long long)C, h.lda_zt, h.zc, h.zs, forward, in, out);
 }
 
 void dct_launch(Handle &h, int kid, const DctArgs &a, bool forward, const double *in, double *out,
@@ -1727,6 +1792,11 @@ void dct_launch(Handle &h, int kid, const DctArgs &a, bool forward, const double
 }
 
 }  // namespace
+
+void launch_zdft(hipStream_t s, int M, int C, long long plane, int lda, const double *zc, const double *zs,
+                 bool forward, const double *in, double *out) {
+    zdft_raw(s, M, C, plane, lda, zc, zs, forward, in, out);
+}
 
 const char *kernel_name(int id) {
     static const char *names[KID_COUNT] = {
@@ -1807,7 +1877,8 @@ void launch_rk_stage(Handle &h, const Fld3<T> &cur, const Fld3<T> &old, Fld3<T> 
         const double c0 = dt * (rk3 ? h.rk_coef[1][1] : h.rk2_coef[1][1]);
         if (gform)
             march<0, 2>(h, cur, &old, (const Fld3<T> *)nullptr, &nxt, &F1, c0, 0.0, dt, dt * h.rk_coef[2][1]);
-        else if (rk3)
+        else if (rk3 || sizeof(T) == 4)
+            // FP32 RK2 keeps F1 too: it is the first stage's pressure source
             march<0, 1>(h, cur, &old, (const Fld3<T> *)nullptr, &nxt, &F1, c0, 0.0, dt);
         else
             march<0, 0>(h, cur, &old, (const Fld3<T> *)nullptr, &nxt, (Fld3<T> *)nullptr, c0, 0.0, dt);
@@ -1833,7 +1904,8 @@ void launch_rhs_only(Handle &h, const Fld3<T> &cur, Fld3<T> &rhs) {
 }
 
 template <class T>
-void launch_bc(Handle &h, Fld3<T> &f, const Fld3<T> &old, int rk_step, double dt, double t) {
+void launch_bc(Handle &h, Fld3<T> &f, const Fld3<T> &old, int rk_step, double dt, double t, double *inc1,
+               double t_old) {
     const Dims &d = h.d;
     const double coef = h.rk_t[rk_step] * dt * h.bc_uc;
     prof_begin(h, KID_BC_INOUT);
@@ -1841,7 +1913,7 @@ void launch_bc(Handle &h, Fld3<T> &f, const Fld3<T> &old, int rk_step, double dt
     const int nq_in = g_in.x * g_in.y, nq_tw = g_tw.x * g_tw.y;
     double *qpart = h.red;
     hipLaunchKernelGGL(k_bc_inout<T>, g_in, dim3(256), 0, h.stream, d, h.m, h.bc, f.u, f.v, f.w, old.u, old.v,
-                       old.w, coef, t, qpart);
+                       old.w, coef, t, qpart, inc1, t_old);
     check_launch();
     prof_end(h, KID_BC_INOUT);
     prof_begin(h, KID_BC_PERIODIC);
@@ -1854,7 +1926,8 @@ void launch_bc(Handle &h, Fld3<T> &f, const Fld3<T> &old, int rk_step, double dt
     check_launch();
     prof_end(h, KID_BC_TOPWALL);
     prof_begin(h, KID_BC_MASSFLUX);
-    hipLaunchKernelGGL(k_bc_massflux<T>, g_in, dim3(256), 0, h.stream, d, h.length_y, qpart, nq_in + nq_tw, f.u);
+    hipLaunchKernelGGL(k_bc_massflux<T>, g_in, dim3(256), 0, h.stream, d, h.length_y, qpart, nq_in + nq_tw, f.u,
+                       inc1 ? inc1 + 2 * (long long)d.nyg * d.nzg : nullptr);
     check_launch();
     prof_end(h, KID_BC_MASSFLUX);
 }
@@ -2070,7 +2143,7 @@ void monitor(Handle &h, const Fld3<T> &f, double *out) {
 #define BL_INSTANTIATE(T)                                                                                   \
     template void launch_rk_stage<T>(Handle &, const Fld3<T> &, const Fld3<T> &, Fld3<T> &, int, double); \
     template void launch_rhs_only<T>(Handle &, const Fld3<T> &, Fld3<T> &);                                \
-    template void launch_bc<T>(Handle &, Fld3<T> &, const Fld3<T> &, int, double, double);                 \
+    template void launch_bc<T>(Handle &, Fld3<T> &, const Fld3<T> &, int, double, double, double *, double); \
     template void launch_project<T>(Handle &, Fld3<T> &, const T *);                                      \
     template bool launch_thomas_seg<T>(Handle &, T *, int, long long, const Solve00 &);                   \
     template void launch_pressure<T>(Handle &, const T *, T *, double);                                   \

@@ -381,13 +381,33 @@ __device__ __forceinline__ f32x4 mfma_split(const u16x8 *A, bf16x8 bh, bf16x8 bl
     return __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh, acc, 0, 0, 0);
 }
 
+// divergence of the first stage's increment U* - Un at cell (i, j, k), p its
+// index (the IncSrc form of the pseudo-pressure rhs): c0 F1 on the faces the
+// stage update moves, the FP64 increments on the inflow and outflow U faces
+// (plus the mass-flux shift on the outflow), the difference of the stored
+// values on the wall and top V faces; the W face below plane 1 is the
+// periodic image of plane nz-2
+__device__ __forceinline__ double inc_div(const Dims &d, const Metrics &mt, const IncSrc &s, long long p, int i,
+                                          int j, int k) {
+    const long long pl = j + (long long)d.nyg * k, npl = (long long)d.nyg * d.nzg;
+    const double ur = (i == d.nx - 2) ? s.inc1[npl + pl] + s.inc1[2 * npl] : s.c0 * (double)s.fu[p];
+    const double ul = (i == 1) ? s.inc1[pl] : s.c0 * (double)s.fu[p - 1];
+    const double vt = (j == d.ny - 1) ? (double)s.vs[p] - (double)s.vn[p] : s.c0 * (double)s.fv[p];
+    const long long pb = p - d.sx;
+    const double vb = (j == 1) ? (double)s.vs[pb] - (double)s.vn[pb] : s.c0 * (double)s.fv[pb];
+    const long long pw = (k == 1) ? p + d.sxy * (long long)(d.nz - 3) : p - d.sxy;
+    const double wt = s.c0 * (double)s.fw[p], wb = s.c0 * (double)s.fw[pw];
+    return (ur - ul) * mt.rdx[i] + (vt - vb) * mt.rdy[j] + (wt - wb) * mt.rdz[k];
+}
+
 // forward: the pseudo-pressure rhs (projection.f90:38-47) of kXL lines
-// (cells i = 1..N of row j, plane m; line c = j-1 + NM (m-1)) formed in FP64,
-// its FP64 line sums for the (0,0) mode (part[(j-1) M + m-1]), then the
-// transform into T1[c NP + kx]
+// (cells i = 1..N of row j, plane m; line c = j-1 + NM (m-1)) formed in FP64
+// (from the increment when use_inc), its FP64 line sums for the (0,0) mode
+// (part[(j-1) M + m-1]), then the transform into T1[c NP + kx]
 __global__ __launch_bounds__(256) void k_xfwd_mx(Dims d, Metrics mt, XArgs a, const float *__restrict__ U,
                                                  const float *__restrict__ V, const float *__restrict__ W,
-                                                 float *__restrict__ out, double *__restrict__ part) {
+                                                 float *__restrict__ out, double *__restrict__ part, IncSrc inc,
+                                                 int use_inc) {
     extern __shared__ float smx[];
     const int N = a.N, N1 = a.N1, N2 = a.N2, K1 = a.K1h + 1;
     // twiddle indices n2 k1 stay below N (n2 < N2, k1 <= N1/2): no reduction
@@ -408,8 +428,10 @@ __global__ __launch_bounds__(256) void k_xfwd_mx(Dims d, Metrics mt, XArgs a, co
         const double ry = mt.rdy[c1 + 1], rz = mt.rdz[c2 + 1];
         for (int n = tid; n < N; n += 256) {
             const long long p = base + n;
-            const double dv = ((double)U[p] - (double)U[p - 1]) * mt.rdx[n + 1] +
-                              ((double)V[p] - (double)V[p - d.sx]) * ry + ((double)W[p] - (double)W[p - d.sxy]) * rz;
+            const double dv = use_inc ? inc_div(d, mt, inc, p, n + 1, c1 + 1, c2 + 1)
+                                      : ((double)U[p] - (double)U[p - 1]) * mt.rdx[n + 1] +
+                                            ((double)V[p] - (double)V[p - d.sx]) * ry +
+                                            ((double)W[p] - (double)W[p - d.sxy]) * rz;
             ps[l] += dv;
             v[l * N + ((n & 1) ? N - 1 - (n >> 1) : (n >> 1))] = (float)dv;
         }
@@ -622,7 +644,8 @@ __global__ __launch_bounds__(256) void k_xinv_mx(Dims d, XArgs a, const float *_
 // ------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_div_mixed(Dims d, Metrics m, const float *__restrict__ U,
                                                    const float *__restrict__ V, const float *__restrict__ W,
-                                                   int NP, float *__restrict__ div, double *__restrict__ part) {
+                                                   int NP, float *__restrict__ div, double *__restrict__ part,
+                                                   IncSrc inc, int use_inc) {
     const int ii = blockIdx.x * 64 + (threadIdx.x & 63);
     const int jj = blockIdx.y * 4 + (threadIdx.x >> 6);
     const int mm = blockIdx.z;
@@ -630,8 +653,9 @@ __global__ __launch_bounds__(256) void k_div_mixed(Dims d, Metrics m, const floa
     if (ii < d.N && jj < d.NM) {
         const int i = ii + 1, j = jj + 1, k = mm + 1;
         const long long p = fidx(d, i, j, k);
-        v = ((double)U[p] - (double)U[p - 1]) * m.rdx[i] + ((double)V[p] - (double)V[p - d.sx]) * m.rdy[j] +
-            ((double)W[p] - (double)W[p - d.sxy]) * m.rdz[k];
+        v = use_inc ? inc_div(d, m, inc, p, i, j, k)
+                    : ((double)U[p] - (double)U[p - 1]) * m.rdx[i] + ((double)V[p] - (double)V[p - d.sx]) * m.rdy[j] +
+                          ((double)W[p] - (double)W[p - d.sxy]) * m.rdz[k];
         div[((long long)mm * d.NM + jj) * NP + ii] = (float)v;
     }
     for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o);
@@ -911,11 +935,18 @@ void build_mixed(Handle &h) {
     h.np00 = h.mx_two ? M : ((N + 63) / 64) * M;
     BL_HIP(hipMalloc(&h.p00, sizeof(double) * ((size_t)h.np00 * d.NM + d.NM)));
     h.c00 = h.p00 + (size_t)h.np00 * d.NM;
+    // first-stage boundary increments (written by the boundary update of
+    // every first stage before they are read)
+    const size_t ninc = 2 * (size_t)d.nyg * d.nzg + 1;
+    BL_HIP(hipMalloc(&h.inc1, sizeof(double) * ninc));
+    BL_HIP(hipMemsetAsync(h.inc1, 0, sizeof(double) * ninc, h.stream));
     BL_HIP(hipStreamSynchronize(h.stream));
 }
 
-void launch_poisson_mixed(Handle &h, float *phi, const Fld3<float> &f) {
+void launch_poisson_mixed(Handle &h, float *phi, const Fld3<float> &f, const IncSrc *inc) {
     const Dims &d = h.d;
+    const IncSrc src = inc ? *inc : IncSrc{};
+    const int use_inc = inc ? 1 : 0;
     const int N = d.N, M = d.M, NM = d.NM, NP = h.NP;
     float *S = reinterpret_cast<float *>(h.S);
     float *T1 = reinterpret_cast<float *>(h.T1);
@@ -943,14 +974,15 @@ void launch_poisson_mixed(Handle &h, float *phi, const Fld3<float> &f) {
         // divergence, FP64 line sums and forward x transform in one kernel
         ProfScope ps(h, KID_XFORM_X_FWD);
         const size_t lds = sizeof(float) * (size_t)kXL * N + sizeof(float2) * (size_t)kXL * K1 * N2;
-        hipLaunchKernelGGL(k_xfwd_mx, xgrid, dim3(256), lds, h.stream, d, h.m, xa, f.u, f.v, f.w, T1, h.p00);
+        hipLaunchKernelGGL(k_xfwd_mx, xgrid, dim3(256), lds, h.stream, d, h.m, xa, f.u, f.v, f.w, T1, h.p00, src,
+                           use_inc);
         check();
     }
     // rhs: divergence lines into S (as scratch), FP64 plane partials
     if (!h.mx_two) {
         ProfScope ps(h, KID_DIVERGENCE);
         hipLaunchKernelGGL(k_div_mixed, dim3((N + 63) / 64, (NM + 3) / 4, M), dim3(256), 0, h.stream, d, h.m,
-                           f.u, f.v, f.w, NP, S, h.p00);
+                           f.u, f.v, f.w, NP, S, h.p00, src, use_inc);
         check();
     }
     // forward x: T1[(m*NM + j)*NP + kx] = sum_n Dxf[kx][n] div[(m*NM + j)*NP + n]

@@ -5,10 +5,12 @@
 ! Kept from the reference, on the host: the .turbb reader (params.f90,          !
 ! miscel.f90), init_flow and the snapshot reader (input_output.f90:130-487),    !
 ! Blasius / temporal-mode boundary tables (boundary_conditions.f90:1452-1797),  !
-! statistics and their text file (statistics.f90:26-216,                        !
+! the inlet thickness and the statistics text file (statistics.f90:177-201,     !
 ! input_output.f90:685-746), the monitor block (monitor.f90:20-173) and the     !
-! snapshot writer (input_output.f90:496-680).  Single rank only, as the        !
-! reference's GPU path (initialization.f90:311).                               !
+! snapshot writer (input_output.f90:496-680).  The z means of the statistics   !
+! come from the device (bl_statistics); snapshot fields are copied to the host  !
+! behind the next step (bl_snapshot_begin / bl_snapshot_end).  Single rank     !
+! only, as the reference's GPU path (initialization.f90:311).                  !
 !------------------------------------------------------------------------------!
 module bl_params
   use, intrinsic :: iso_fortran_env, only : int32, real64
@@ -396,46 +398,22 @@ module bl_output
   implicit none
 contains
 
-  ! compute_statistics (statistics.f90:44-201) on the downloaded fields; Cf
-  ! comes from the device (bl_wall_stats)
-  subroutine compute_statistics()
-    integer :: ii, jj, k, j, jref
-    real(real64) :: wf, tu, tv, theta, Umean99, w0, w1
+  ! compute_statistics (statistics.f90:44-201): the z means and Cf from the
+  ! device (bl_statistics), the inlet thickness on the host
+  subroutine compute_statistics(h)
+    use, intrinsic :: iso_c_binding, only : c_ptr, c_null_ptr
+    use blgpu_api
+    type(c_ptr), intent(in) :: h
+    integer :: j, jref
+    real(real64) :: theta, Umean99, w0, w1
     real(real64) :: temp(ny)
-    Umean = 0d0; Vmean = 0d0; Wmean = 0d0; U2mean = 0d0; V2mean = 0d0; W2mean = 0d0
-    UVmean = 0d0; Pmean = 0d0; P2mean = 0d0
-    do ii = 1, nx
-       do jj = 1, ny
-          do k = 2, nzg - 1
-             ! term_1 = interpolate_y(U, centres->faces), term_2 = interpolate_x(V)
-             tu = wy0(jj) * U(ii, jj, k) + wy1(jj) * U(ii, jj + 1, k)
-             tv = 0.5d0 * (V(ii, jj, k) + V(ii + 1, jj, k))
-             Umean(ii, jj) = Umean(ii, jj) + tu
-             Vmean(ii, jj) = Vmean(ii, jj) + tv
-             U2mean(ii, jj) = U2mean(ii, jj) + tu**2
-             V2mean(ii, jj) = V2mean(ii, jj) + tv**2
-             UVmean(ii, jj) = UVmean(ii, jj) + tu * tv
-             Pmean(ii, jj) = Pmean(ii, jj) + P(ii, jj, k)
-             P2mean(ii, jj) = P2mean(ii, jj) + P(ii, jj, k)**2
-          end do
-          ! term = interpolate_y(interpolate_x(W)), summed over k = 1..nz-2
-          do k = 1, nz - 2
-             wf = wy0(jj) * 0.5d0 * (W(ii, jj, k) + W(ii + 1, jj, k)) + &
-                  wy1(jj) * 0.5d0 * (W(ii, jj + 1, k) + W(ii + 1, jj + 1, k))
-             Wmean(ii, jj) = Wmean(ii, jj) + wf
-             W2mean(ii, jj) = W2mean(ii, jj) + wf**2
-          end do
-       end do
-    end do
-    Umean = Umean / real(nzg - 2, real64)
-    Vmean = Vmean / real(nzg - 2, real64)
-    Wmean = Wmean / real(nz - 2, real64)
-    U2mean = U2mean / real(nzg - 2, real64)
-    V2mean = V2mean / real(nzg - 2, real64)
-    W2mean = W2mean / real(nz - 2, real64)
-    UVmean = UVmean / real(nzg - 2, real64)
-    Pmean = Pmean / real(nzg - 2, real64)
-    P2mean = P2mean / real(nzg - 2, real64)
+    real(real64), allocatable :: means(:,:,:)
+    allocate(means(nx, ny, 9))
+    call bl_check(bl_statistics(h, means, c_null_ptr, Cf), 'bl_statistics')
+    Umean = means(:, :, 1); Vmean = means(:, :, 2); Wmean = means(:, :, 3)
+    U2mean = means(:, :, 4); V2mean = means(:, :, 5); W2mean = means(:, :, 6)
+    UVmean = means(:, :, 7); Pmean = means(:, :, 8); P2mean = means(:, :, 9)
+    deallocate(means)
     ! momentum thickness and delta99 at the inlet (statistics.f90:177-201)
     theta = 0d0
     temp = Umean(1, :) * (1d0 - Umean(1, :))
@@ -504,18 +482,23 @@ contains
   end subroutine output_statistics
 
   ! output_data, single rank (input_output.f90:496-666): the z extents are
-  ! one plane short for U, V, W and nu_t, P is written in full
-  subroutine output_data()
+  ! one plane short for U, V, W and nu_t, P is written in full; the fields
+  ! of step istep_s at time t_s
+  subroutine output_data(Us, Vs, Ws, Ps, NTs, istep_s, t_s)
+    real(real64), intent(in) :: Us(nx, nyg, nzg), Vs(nxg, ny, nzg), Ws(nxg, nyg, nz)
+    real(real64), intent(in) :: Ps(nxg, nyg, nzg), NTs(nxg, nyg, nzg)
+    integer, intent(in) :: istep_s
+    real(real64), intent(in) :: t_s
     character(len=512) :: fname, lnk
     character(len=8) :: ext
     integer :: iu, ist, ics
     logical :: ex
-    write(ext, '(I0.8)') istep + nstep_init
+    write(ext, '(I0.8)') istep_s + nstep_init
     fname = trim(adjustl(fileout)) // '.' // trim(adjustl(ext))
     write(*,*) 'writting ', trim(adjustl(fname))
     open(newunit=iu, file=fname, access='stream', form='unformatted', action='write', status='replace')
-    write(iu) t, nu
-    write(iu) -73, istep + nstep_init
+    write(iu) t_s, nu
+    write(iu) -73, istep_s + nstep_init
     write(iu) shape(x), x
     write(iu) shape(y), y
     write(iu) shape(z), z
@@ -523,15 +506,15 @@ contains
     write(iu) shape(ym), ym
     write(iu) shape(zm), zm
     write(iu) nx, nyg, nzg
-    write(iu) U(:,:,1:nzg-1)
+    write(iu) Us(:,:,1:nzg-1)
     write(iu) nxg, ny, nzg
-    write(iu) V(:,:,1:nzg-1)
+    write(iu) Vs(:,:,1:nzg-1)
     write(iu) nxg, nyg, nz
-    write(iu) W(:,:,1:nz-1)
+    write(iu) Ws(:,:,1:nz-1)
     write(iu) nxg, nyg, nzg
-    write(iu) nu_t(:,:,1:nzg-1)
+    write(iu) NTs(:,:,1:nzg-1)
     write(iu) nxg, nyg, nzg
-    write(iu) P
+    write(iu) Ps
     close(iu)
     ! restart symlink (input_output.f90:651-664); like the reference's
     ! `call system`, a failing command is not an error
@@ -540,6 +523,25 @@ contains
     if (ex) call execute_command_line('rm ' // trim(lnk), exitstat=ist, cmdstat=ics)
     call execute_command_line('ln -s ' // trim(adjustl(fname)) // ' ' // trim(lnk), exitstat=ist, cmdstat=ics)
   end subroutine output_data
+
+  ! the fields of the snapshot begun at step istep_s: wait for their copy
+  ! (it ran under the kernels queued since) and write them
+  subroutine write_snapshot(h, istep_s, t_s)
+    use, intrinsic :: iso_c_binding, only : c_ptr, c_f_pointer
+    use blgpu_api
+    type(c_ptr), intent(in) :: h
+    integer, intent(in) :: istep_s
+    real(real64), intent(in) :: t_s
+    type(c_ptr) :: pu, pv, pw, pp, pn
+    real(real64), pointer :: Us(:,:,:), Vs(:,:,:), Ws(:,:,:), Ps(:,:,:), NTs(:,:,:)
+    call bl_check(bl_snapshot_end(h, pu, pv, pw, pp, pn), 'bl_snapshot_end')
+    call c_f_pointer(pu, Us, [nx, nyg, nzg])
+    call c_f_pointer(pv, Vs, [nxg, ny, nzg])
+    call c_f_pointer(pw, Ws, [nxg, nyg, nz])
+    call c_f_pointer(pp, Ps, [nxg, nyg, nzg])
+    call c_f_pointer(pn, NTs, [nxg, nyg, nzg])
+    call output_data(Us, Vs, Ws, Ps, NTs, istep_s, t_s)
+  end subroutine write_snapshot
 
 end module bl_output
 
@@ -554,9 +556,10 @@ program bl_driver
   type(bl_config), target :: cfg
   type(c_ptr) :: h
   real(c_double) :: mon(8), div, t_dev
-  real(real64) :: time1, time2
+  real(real64) :: time1, time2, snap_t
   integer(int64) :: c0, c1, crate
-  integer :: i
+  integer :: i, snap_step
+  logical :: snap_pending
 
   write(*,*) '----------------------------------------------------------------------'
   write(*,*) '  boundary layer, MI355X per-step path (blgpu) with the reference driver'
@@ -607,23 +610,23 @@ program bl_driver
   call system_clock(c0, crate)
   time1 = real(c0, real64) / real(crate, real64)
 
-  ! time loop (main.f90:84-113)
+  ! time loop (main.f90:84-113).  Nothing is downloaded per step: the
+  ! statistics are reduced on the device, and a snapshot's copy to the host
+  ! runs under the next step's kernels (written once that step is queued).
+  snap_pending = .false.
+  snap_step = 0
+  snap_t = 0d0
   do istep = 1, nsteps
      call bl_check(bl_compute_dt(h, dt), 'bl_compute_dt')
      call bl_check(bl_step(h), 'bl_step')
-     if (istep == 1 .or. mod(istep, nstats) == 0 .or. mod(istep, nmonitor) == 0 .or. &
-         mod(istep, nsave) == 0) then
-        call bl_check(bl_download(h, BL_FIELD_U, U), 'bl_download U')
-        call bl_check(bl_download(h, BL_FIELD_V, V), 'bl_download V')
-        call bl_check(bl_download(h, BL_FIELD_W, W), 'bl_download W')
-        call bl_check(bl_download(h, BL_FIELD_P, P), 'bl_download P')
-        call bl_check(bl_download(h, BL_FIELD_NUT, nu_t), 'bl_download nu_t')
-        call bl_check(bl_get_time(h, t_dev), 'bl_get_time')
-        t = t_dev
+     if (snap_pending) then
+        call write_snapshot(h, snap_step, snap_t)
+        snap_pending = .false.
      end if
+     call bl_check(bl_get_time(h, t_dev), 'bl_get_time')
+     t = t_dev
      if (mod(istep, nstats) == 0 .or. istep == 1) then
-        call bl_check(bl_wall_stats(h, Cf), 'bl_wall_stats')
-        call compute_statistics()
+        call compute_statistics(h)
         call output_statistics()
      end if
      if (mod(istep, nmonitor) == 0 .or. istep == 1) then
@@ -660,8 +663,14 @@ program bl_driver
         write(*,*) '------------------------------------------------------'
         time1 = time2
      end if
-     if (mod(istep, nsave) == 0) call output_data()
+     if (mod(istep, nsave) == 0) then
+        call bl_check(bl_snapshot_begin(h), 'bl_snapshot_begin')
+        snap_pending = .true.
+        snap_step = istep
+        snap_t = t
+     end if
   end do
+  if (snap_pending) call write_snapshot(h, snap_step, snap_t)
 
   call bl_destroy(h)
   i = 0

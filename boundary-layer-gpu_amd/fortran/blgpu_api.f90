@@ -108,6 +108,27 @@ module blgpu_api
        real(c_double), intent(out) :: out(8)
      end function bl_monitor
 
+     ! compute_statistics' z means on the device (statistics.f90:44-80)
+     integer(c_int) function bl_statistics(h, means, nu_t_mean, cf) bind(C, name='bl_statistics')
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value          :: h
+       real(c_double), intent(out) :: means(*)
+       type(c_ptr), value          :: nu_t_mean
+       real(c_double), intent(out) :: cf(*)
+     end function bl_statistics
+
+     ! snapshot fields copied to pinned host memory behind the time loop
+     integer(c_int) function bl_snapshot_begin(h) bind(C, name='bl_snapshot_begin')
+       import :: c_int, c_ptr
+       type(c_ptr), value :: h
+     end function bl_snapshot_begin
+
+     integer(c_int) function bl_snapshot_end(h, pu, pv, pw, pp, pn) bind(C, name='bl_snapshot_end')
+       import :: c_int, c_ptr
+       type(c_ptr), value        :: h
+       type(c_ptr), intent(out)  :: pu, pv, pw, pp, pn
+     end function bl_snapshot_end
+
      integer(c_int) function bl_blasius_tables(nx, ny, x, y, nu, n, eta, f, df, U_inlet, V_inlet, &
           W_inlet, U_top, V_top, W_top) bind(C, name='bl_blasius_tables')
        import :: c_int, c_double

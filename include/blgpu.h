@@ -122,6 +122,26 @@ int bl_wall_stats(bl_handle *h, double *cf);
 /* output_monitor reductions (monitor.f90:29-52): out[8] = sum U, V, W,
  * nu_t over the interiors, then max U, V, W, nu_t. */
 int bl_monitor(bl_handle *h, double *out);
+/* The z means of compute_statistics (statistics.f90:44-80, 125-135) on the
+ * device, FP64 accumulation: means[9*nx*ny] receives Umean, Vmean, Wmean,
+ * U2mean, V2mean, W2mean, UVmean, Pmean, P2mean, each (nx, ny) x fastest,
+ * from term_1 = interpolate_y(U), term_2 = interpolate_x(V) over z planes
+ * 2..nzg-1 and term = interpolate_y(interpolate_x(W)) over planes 1..nz-2;
+ * nu_t_mean[nxg*nyg] (may be NULL) the nu_t mean, with column nxg and row
+ * nyg 0 as compute_statistics leaves them; cf[nx] (may be NULL) as
+ * bl_wall_stats.  Replaces the field download of statistics.f90:58. */
+int bl_statistics(bl_handle *h, double *means, double *nu_t_mean, double *cf);
+/* Snapshot fields for output_data (input_output.f90:496-680) without
+ * stalling the time loop: bl_snapshot_begin queues a copy of the current U,
+ * V, W, P, nu_t (FP64, unpadded, the extents above) into library-owned
+ * pinned host memory on a second stream and returns at once; later steps may
+ * be queued meanwhile.  bl_snapshot_end waits for that copy and returns
+ * pointers into it, valid until the next bl_snapshot_begin or bl_destroy;
+ * BL_ESTATE if no snapshot was begun.  Replaces the `!$acc update self` of
+ * the fields before output_data (main.f90:100). */
+int bl_snapshot_begin(bl_handle *h);
+int bl_snapshot_end(bl_handle *h, const double **U, const double **V, const double **W, const double **P,
+                    const double **nu_t);
 
 /* ---- single operators, for parity checks against the reference ---- */
 
@@ -141,6 +161,24 @@ int bl_projection(bl_handle *h, int rk_step, double dt);
  * result in a host array of the P extent; cells i=2..nxg-2, j=2..nyg-1,
  * k=2..nzg-2 are read and written. */
 int bl_poisson(bl_handle *h, double *rhs_inout);
+
+/* ---- the fftz module (fftz.f90:32-127) ----
+ * Real DFT along z of xz planes with the solver's z-DFT kernels.  The
+ * reference plans it at start-up (initialize_fftz, initialization.f90:720)
+ * and never executes it on the time step.
+ *  bl_fftz_create     initialize_fftz(N): N points in z
+ *  bl_fft_z           compute_fft_z: U(n1, N) -> U_hat(n1, N/2+1), complex
+ *                     as interleaved (re, im) doubles, unnormalised (r2c)
+ *  bl_ifft_z          compute_ifft_z: U_hat -> U, divided by N (c2r; the
+ *                     imaginary parts of the zero and, for even N, Nyquist
+ *                     terms are ignored)
+ *  bl_fftz_destroy    finalize_fftz
+ * Host arrays, x (the n1 index) fastest, as the reference's. */
+typedef struct bl_fftz bl_fftz;
+int bl_fftz_create(int N, bl_fftz **out);
+int bl_fft_z(bl_fftz *p, int n1, const double *U, double *U_hat);
+int bl_ifft_z(bl_fftz *p, int n1, const double *U_hat, double *U);
+void bl_fftz_destroy(bl_fftz *p);
 
 /* ---- per-kernel timing (no counterpart in the reference, which prints a
  * system_clock sub-step profile, time_integration.f90:283-298) ---- */

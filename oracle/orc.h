@@ -78,6 +78,12 @@ void orc_poisson(orc_state *s, double *rhs);
 double orc_max_divergence(const orc_state *s);
 /* Cf (statistics.f90:137-160) into cf(nx) */
 void orc_wall_stats(const orc_state *s, double *cf);
+/* the z-averaged statistics of compute_statistics (statistics.f90:44-80,
+ * 125-135) on the current U, V, W, P, nu_t: means = Umean, Vmean, Wmean,
+ * U2mean, V2mean, W2mean, UVmean, Pmean, P2mean, each (nx, ny), x fastest;
+ * nu_t_mean (nxg, nyg), whose column nxg and row nyg compute_statistics never
+ * fills (they stay 0) */
+void orc_statistics(const orc_state *s, double *means, double *nu_t_mean);
 /* monitor sums/maxima (monitor.f90:29-52): out[8] = sumU,sumV,sumW,sumNut,
  * maxU,maxV,maxW,maxNut */
 void orc_monitor(const orc_state *s, double *out);
@@ -112,6 +118,9 @@ void orc_init_flow(int nx, int ny, int nz, double Lx, double Ly, double Lz, doub
                    double *x, double *y, double *z, double *U, double *V, double *W);
 /* grid helpers: xg from x (initialization.f90:181-207) */
 void orc_centers(int n, const double *x, double *xm, double *xg);
+/* OpenMP threads of the oracle's parallel loops (the CPU-baseline timing) */
+void orc_set_threads(int n);
+int orc_get_threads(void);
 
 #ifdef __cplusplus
 }

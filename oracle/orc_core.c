@@ -1004,7 +1004,10 @@ void orc_wall_stats(const orc_state *s, double *cf) {
     const double *U = s->U, *V = s->V, *nu_t = s->nu_t;
     double nz2 = (double)(nzg - 2);
     double *dUdy = dalloc(nx), *ntm1 = dalloc(s->nxg), *ntm2 = dalloc(s->nxg);
-    for (int ii = 1; ii <= s->nxg; ii++) {
+    /* nu_t_mean(ii, 1:2) for ii = 1..nx (statistics.f90:77, 135); column nxg
+     * is never filled and keeps its initial 0 (initialization.f90:501), so the
+     * SGS term of the last station takes 0 for its ii+1 neighbours */
+    for (int ii = 1; ii <= nx; ii++) {
         double a1 = 0.0, a2 = 0.0;
         for (int k = 2; k <= nzg - 1; k++) {
             a1 += nu_t[IP(ii, 1, k)];
@@ -1034,6 +1037,53 @@ void orc_wall_stats(const orc_state *s, double *cf) {
     free(dUdy);
     free(ntm1);
     free(ntm2);
+}
+
+/* compute_statistics (statistics.f90:44-80, 125-135), single rank:
+ * term_1 = interpolate_y(U, 2), term_2 = interpolate_x(V, 1) and
+ * term = interpolate_y(interpolate_x(W, 1), 2) (interpolation.f90:24-85),
+ * summed over z planes 2..nzg-1 (W: 1..nz-2) and divided by nzg-2 (nz-2) */
+void orc_statistics(const orc_state *s, double *means, double *nu_t_mean) {
+    int nx = s->nx, ny = s->ny, nz = s->nz, nxg = s->nxg, nyg = s->nyg, nzg = s->nzg;
+    const double *U = s->U, *V = s->V, *W = s->W, *P = s->P, *nu_t = s->nu_t;
+    size_t n2 = (size_t)nx * ny;
+    memset(nu_t_mean, 0, sizeof(double) * (size_t)nxg * nyg);
+    for (int jj = 1; jj <= ny; jj++)
+        for (int ii = 1; ii <= nx; ii++) {
+            double su = 0, sv = 0, sw = 0, su2 = 0, sv2 = 0, sw2 = 0, suv = 0, sp = 0, sp2 = 0, sn = 0;
+            for (int k = 2; k <= nzg - 1; k++) {
+                double t1 = WY0(jj) * U[IU(ii, jj, k)] + WY1(jj) * U[IU(ii, jj + 1, k)];
+                double t2 = 0.5 * (V[IV(ii, jj, k)] + V[IV(ii + 1, jj, k)]);
+                double p = P[IP(ii, jj, k)];
+                su += t1;
+                sv += t2;
+                su2 += pow(t1, 2.0);
+                sv2 += pow(t2, 2.0);
+                suv += t1 * t2;
+                sp += p;
+                sp2 += pow(p, 2.0);
+                sn += nu_t[IP(ii, jj, k)];
+            }
+            for (int k = 1; k <= nz - 2; k++) {
+                double a = 0.5 * (W[IW(ii, jj, k)] + W[IW(ii + 1, jj, k)]);
+                double b = 0.5 * (W[IW(ii, jj + 1, k)] + W[IW(ii + 1, jj + 1, k)]);
+                double t = WY0(jj) * a + WY1(jj) * b;
+                sw += t;
+                sw2 += pow(t, 2.0);
+            }
+            size_t q = (size_t)(ii - 1) + (size_t)nx * (jj - 1);
+            double dz = (double)(nzg - 2), dw = (double)(nz - 2);
+            means[0 * n2 + q] = su / dz;
+            means[1 * n2 + q] = sv / dz;
+            means[2 * n2 + q] = sw / dw;
+            means[3 * n2 + q] = su2 / dz;
+            means[4 * n2 + q] = sv2 / dz;
+            means[5 * n2 + q] = sw2 / dw;
+            means[6 * n2 + q] = suv / dz;
+            means[7 * n2 + q] = sp / dz;
+            means[8 * n2 + q] = sp2 / dz;
+            nu_t_mean[(size_t)(ii - 1) + (size_t)nxg * (jj - 1)] = sn / dz;
+        }
 }
 
 /* output_monitor reductions (monitor.f90:29-52) */
@@ -1092,3 +1142,6 @@ void orc_set_nut_bc(orc_state *s, int dirichlet) { s->nut_dirichlet = dirichlet;
 
 /* accessors for orc_dsm.c */
 double *orc__nu_t(orc_state *s) { return s->nu_t; }
+
+void orc_set_threads(int n) { omp_set_num_threads(n > 0 ? n : 1); }
+int orc_get_threads(void) { return omp_get_max_threads(); }

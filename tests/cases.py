@@ -142,3 +142,27 @@ def laminar_small(**kw):
     args = dict(nx=24, ny=16, nz=8, Lx=0.15, Ly=0.015, Lz=0.006, nu=4.898844e-6, alpha=2.0)
     args.update(kw)
     return Case(**args)
+
+
+def oracle_p_sensitivity(case, fields, t, seed=3):
+    """Relative L2 change (interior, mean removed) of the P that one oracle
+    step records when the state it starts from is perturbed by one unit in
+    the last place.  Two FP64 evaluations of the same step that differ only
+    in rounding (a different transform algorithm, summation order) cannot be
+    expected to agree on P more closely than this."""
+
+    def one(F):
+        o = case.oracle()
+        o.set_fields(*F)
+        o.t = t
+        o.compute_dt()
+        o.step()
+        P = o.get(3)[1:-1, 1:-1, 1:-2].copy()
+        o.close()
+        return P - P.mean()
+
+    rng = np.random.default_rng(seed)
+    eps = np.finfo(np.float64).eps
+    P1 = one(fields)
+    P2 = one([a * (1.0 + eps * rng.choice([-1.0, 1.0], a.shape)) for a in fields])
+    return float(np.linalg.norm(P2 - P1) / np.linalg.norm(P1))

@@ -10,7 +10,8 @@ import os
 import numpy as np
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-LIB_PATH = os.path.join(ROOT, "oracle", "liborc.so")
+# ORC_LIB selects another build of the oracle (the sanitizer build of oracle/Makefile)
+LIB_PATH = os.environ.get("ORC_LIB") or os.path.join(ROOT, "oracle", "liborc.so")
 
 _lib = None
 
@@ -56,6 +57,9 @@ def lib():
             "orc_init_flow": (None, [_I, _I, _I, _D, _D, _D, _D, _D, _I] + [_P] * 9),
             "orc_centers": (None, [_I, _P, _P, _P]),
             "orc_fft_once": (None, [_I, _I, _P, _P]),
+            "orc_statistics": (None, [_P, _P, _P]),
+            "orc_set_threads": (None, [_I]),
+            "orc_get_threads": (_I, []),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -71,6 +75,14 @@ def _p(a):
 
 def f64(a):
     return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def set_threads(n):
+    lib().orc_set_threads(int(n))
+
+
+def get_threads():
+    return lib().orc_get_threads()
 
 
 def centers(x):
@@ -225,6 +237,18 @@ class Oracle:
         cf = np.zeros(self.nx)
         lib().orc_wall_stats(self.h, _p(cf))
         return cf
+
+    def statistics(self):
+        """compute_statistics' z means: dict of the nine (nx, ny) arrays and
+        nu_t_mean (nxg, nyg)"""
+        means = np.zeros(9 * self.nx * self.ny)
+        ntm = np.zeros((self.nx + 1, self.ny + 1), order="F")
+        lib().orc_statistics(self.h, _p(means), ntm.ctypes.data)
+        names = ("Umean", "Vmean", "Wmean", "U2mean", "V2mean", "W2mean", "UVmean", "Pmean", "P2mean")
+        m = means.reshape((9, self.ny, self.nx))
+        out = {k: np.asfortranarray(m[q].T) for q, k in enumerate(names)}
+        out["nu_t_mean"] = ntm
+        return out
 
     def monitor(self):
         out = np.zeros(8)

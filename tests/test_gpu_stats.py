@@ -1,0 +1,99 @@
+"""Device statistics (bl_statistics) and asynchronous snapshots
+(bl_snapshot_begin / bl_snapshot_end) against the oracle and against plain
+downloads.
+
+- FP64 DNS and LES (dynamic Smagorinsky, nu_t nonzero) after a few RK3
+  steps: every mean of compute_statistics (statistics.f90:44-80, 125-135),
+  nu_t_mean and Cf against the oracle after the same steps, relative 1e-12
+  (FP64 sums in a different order).
+- Mixed precision: the device means of the FP32 fields against the oracle's
+  means of the same fields, downloaded (the statistics accumulate in FP64:
+  1e-12).
+- Snapshot: begin, three more steps queued, end: the returned fields equal
+  the fields downloaded before those steps, bit for bit, in both precisions;
+  ending without a begin is a state error.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from . import cases
+from .blgpu_import import blgpu, solver_for
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def rel(a, b):
+    nb = np.linalg.norm(b)
+    return np.linalg.norm(a - b) / (nb if nb > 0 else 1.0)
+
+
+@pytest.mark.parametrize("les", [0, 2])
+def test_statistics_match_oracle(les):
+    c = cases.laminar_small(nx=40, ny=21, nz=11, noise=0.02, seed=6, les=les, cfl=-2e-4)
+    o = c.oracle()
+    s, _ = solver_for(c, o)
+    for _ in range(4):
+        o.compute_dt()
+        o.step()
+        s.compute_dt()
+        s.step()
+    if les:
+        o.eddy_viscosity()
+        s.eddy_viscosity()
+    so, sg = o.statistics(), s.statistics()
+    for k in blgpu.Solver.STAT_NAMES + ("nu_t_mean",):
+        e = rel(sg[k], so[k])
+        say("stats les", les, k, e)
+        assert e < 1e-12, k
+    assert np.all(sg["nu_t_mean"][-1, :] == 0) and np.all(sg["nu_t_mean"][:, -1] == 0)
+    e = rel(sg["Cf"], o.wall_stats())
+    say("stats Cf", e)
+    assert e < 1e-12
+    s.close()
+
+
+def test_statistics_mixed_fields():
+    c = cases.laminar_small(nx=40, ny=21, nz=11, noise=0.02, seed=7, cfl=-2e-4)
+    o = c.oracle()
+    s, _ = solver_for(c, o, precision=blgpu.PREC_MIXED)
+    for _ in range(4):
+        s.compute_dt()
+        s.step()
+    o.set_fields(*s.fields())
+    o.set(3, s.download(blgpu.FIELD_P))
+    o.set(4, s.download(blgpu.FIELD_NUT))
+    so, sg = o.statistics(), s.statistics()
+    for k in blgpu.Solver.STAT_NAMES:
+        e = rel(sg[k], so[k])
+        say("mixed stats", k, e)
+        assert e < 1e-12, k
+    s.close()
+
+
+@pytest.mark.parametrize("prec", [blgpu.PREC_FP64, blgpu.PREC_MIXED])
+def test_snapshot_overlaps_steps(prec):
+    c = cases.laminar_small(nx=30, ny=17, nz=9, noise=0.02, seed=8, les=2, cfl=-2e-4)
+    s, _ = solver_for(c, precision=prec)
+    with pytest.raises(blgpu.BLError):
+        s.snapshot_end()
+    for rnd in range(2):
+        for _ in range(2):
+            s.compute_dt()
+            s.step()
+        ref = s.fields() + (s.download(blgpu.FIELD_P), s.download(blgpu.FIELD_NUT))
+        s.snapshot_begin()
+        for _ in range(3):
+            s.step()
+        snap = s.snapshot_end()
+        same = [np.array_equal(a, b) for a, b in zip(snap, ref)]
+        say("snapshot round", rnd, "prec", prec, "equal U V W P nu_t", same)
+        assert all(same)
+        assert not np.array_equal(s.fields()[0], ref[0])
+    s.close()

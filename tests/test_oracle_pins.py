@@ -134,7 +134,10 @@ def test_cf_with_eddy_viscosity():
     u_w + b yg (so its interpolation to the wall is u_w and dU/dy = b),
     V = v_w on the wall row, nu_t = NT0 + C1 xg + C3 zg uniform in y; the
     face value averages x neighbours and the z mean runs over planes
-    k = 2..nzg-1 (statistics.f90:44-52, 137-160)"""
+    k = 2..nzg-1 (statistics.f90:44-52, 137-160).  At the last station the
+    x neighbour is column nxg of nu_t_mean, which compute_statistics never
+    fills (statistics.f90:77 runs ii = 1..nx) and so holds its initial 0
+    (initialization.f90:501): half the face value there."""
     u_w, b, v_w = 0.02, 40.0, 0.003
     x, y, z, xg, yg, zg = _grid(stretched=True)
     f = _zero_fields(x, y, z, xg, yg, zg)
@@ -147,7 +150,9 @@ def test_cf_with_eddy_viscosity():
     o.set(4, nut)
     cf = o.wall_stats()
     zbar = zg[1:-1].mean()
-    exact = 2 * (-u_w * v_w + (NU + NT0 + C1 * x + C3 * zbar) * b)
+    nut_face = NT0 + C1 * x + C3 * zbar
+    nut_face[-1] = 0.5 * (NT0 + C1 * xg[-2] + C3 * zbar)
+    exact = 2 * (-u_w * v_w + (NU + nut_face) * b)
     assert np.allclose(cf, exact, rtol=1e-11, atol=0)
 
 
