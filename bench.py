@@ -95,8 +95,7 @@ def kernel_units(name, nx, ny, nz):
         "xform_z_fwd": 2.0,
         "xform_z_inv": 2.0,
         "thomas": 2.0,  # read and write the spectrum
-        "project": 7.0,
-        "pressure": 2.0,
+        "project": 7.0,  # plus the P record (1 pass) in the first stage's launch
     }
     if name in per_cell_passes:
         return "byte", per_cell_passes[name] * 8.0 * cells

@@ -638,13 +638,11 @@ void projection_on(Handle &h, Field3 &f, int rk_step, double dt, const IncSrc *i
         Fld3<float> ff = as<float>(f);
         float *phi = reinterpret_cast<float *>(h.phi);
         launch_poisson_mixed(h, phi, ff, inc);
-        launch_project(h, ff, (const float *)phi);
-        if (rk_step == 1) launch_pressure(h, (const float *)phi, reinterpret_cast<float *>(h.P), dt);
+        launch_project(h, ff, (const float *)phi, rk_step == 1 ? reinterpret_cast<float *>(h.P) : nullptr, dt);
         return;
     }
     launch_poisson(h, h.phi, &f);
-    launch_project(h, f, (const double *)h.phi);
-    if (rk_step == 1) launch_pressure(h, (const double *)h.phi, h.P, dt);
+    launch_project(h, f, (const double *)h.phi, rk_step == 1 ? h.P : nullptr, dt);
 }
 
 template <class T>

@@ -120,7 +120,6 @@ enum KernelId {
     KID_XFORM_Z_INV,
     KID_XFORM_X_INV,
     KID_PROJECT,
-    KID_PRESSURE,
     KID_REDUCE,
     KID_DSM_CELLS,
     KID_DSM_VFILTER,
@@ -236,8 +235,10 @@ void launch_bc(Handle &h, Fld3<T> &f, const Fld3<T> &old, int rk_step, double dt
                double *inc1 = nullptr, double t_old = 0.0);
 // div != null: the rhs is the divergence of *div (computed inside the x transform)
 void launch_poisson(Handle &h, double *phi_inout, const Field3 *div = nullptr);
+// u -= grad phi (project_velocity, projection.f90:355-386); with P set, also
+// P = phi / (dt a11) on every cell (projection.f90:75-85)
 template <class T>
-void launch_project(Handle &h, Fld3<T> &f, const T *phi);
+void launch_project(Handle &h, Fld3<T> &f, const T *phi, T *P = nullptr, double dt = 0.0);
 // The pinned (0,0) system (NM <= kMax00 rows) solved in place in x (row
 // stride ldx) and scaled; with part set, its right-hand side is first formed
 // from the FP64 plane sums part[j][0..npart) (mixed path).
@@ -259,8 +260,6 @@ void launch_solve00(Handle &h, const Solve00 &z);
 // zeroed (z.part set: z writes its own output).
 template <class T>
 bool launch_thomas_seg(Handle &h, T *S, int ldj, long long ldq, const Solve00 &z);
-template <class T>
-void launch_pressure(Handle &h, const T *phi, T *P, double dt);
 template <class T>
 void launch_rhs_only(Handle &h, const Fld3<T> &cur, Fld3<T> &rhs);
 template <class T>

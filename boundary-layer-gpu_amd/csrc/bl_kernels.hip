@@ -6,8 +6,8 @@
 //   k_divergence    projection.f90:34-49
 //   k_dct_*, k_zdft_*  the DCT-II/DFT stages of solve_poisson_gpu projection.f90:110-202
 //   k_thomas        projection.f90:137-150
-//   k_project       projection.f90:355-386 with apply_periodic_z_pressure :293-303
-//   k_pressure      projection.f90:75-85
+//   k_project       projection.f90:355-386 with apply_periodic_z_pressure :293-303,
+//                   and the P record of the first sub-step :75-85
 //   reductions      projection.f90:391-413, statistics.f90:137-160, monitor.f90:29-52,
 //                   time_integration.f90:546-558
 #include <hip/hip_runtime.h>
@@ -191,32 +191,14 @@ __global__ __launch_bounds__(256) void k_rk_march(
                     const T up = sU[sc][e], ue = sU[sc][e + 1], uw = sU[sc][e - 1];
                     const T un = sU[sc][e + kHX], us = sU[sc][e - kHX];
                     const T ut = sU[sp][e], ub = sU[sm][e];
-                    if constexpr (sizeof(T) == 8) {
-                        rr = -(sq(0.5 * (up + ue)) - sq(0.5 * (uw + up))) * m.rdxg[i];
-                        rr = rr - ((m.wy0[j] * up + m.wy1[j] * un) * 0.5 * (sV[sc][e] + sV[sc][e + 1]) -
-                                   (m.wy0[j - 1] * us + m.wy1[j - 1] * up) * 0.5 *
-                                       (sV[sc][e - kHX] + sV[sc][e + 1 - kHX])) *
-                                      m.rdy[j];
-                        rr = rr - (0.5 * (up + ut) * 0.5 * (sW[sc][e] + sW[sc][e + 1]) -
-                                   0.5 * (ub + up) * 0.5 * (sW[sm][e] + sW[sm][e + 1])) *
-                                      rdzk;
-                    } else {
-                        // FP32: the same fluxes with each difference of two
-                        // products of u (~U_inf) regrouped as a product of
-                        // differences, a b - c d = a (b - d) + d (a - c), so
-                        // that the O(1) parts cancel exactly (neighbouring u
-                        // differ by far less than a factor 2) instead of
-                        // leaving eps |u|^2 / dx of rounding in the flux
-                        // difference
-                        rr = -(T)0.25 * (ue - uw) * (ue + uw + 2 * up) * m.rdxg[i];
-                        const T an = up + m.wy1[j] * (un - up);
-                        const T da = (up - us) + m.wy1[j] * (un - up) - m.wy1[j - 1] * (up - us);
-                        const T bn = 0.5 * (sV[sc][e] + sV[sc][e + 1]), bs = 0.5 * (sV[sc][e - kHX] + sV[sc][e + 1 - kHX]);
-                        rr = rr - (an * (bn - bs) + bs * da) * m.rdy[j];
-                        const T ct = 0.5 * (up + ut);
-                        const T dtp = 0.5 * (sW[sc][e] + sW[sc][e + 1]), dbt = 0.5 * (sW[sm][e] + sW[sm][e + 1]);
-                        rr = rr - (ct * (dtp - dbt) + dbt * (T)0.5 * (ut - ub)) * rdzk;
-                    }
+                    rr = -(sq(0.5 * (up + ue)) - sq(0.5 * (uw + up))) * m.rdxg[i];
+                    rr = rr - ((m.wy0[j] * up + m.wy1[j] * un) * 0.5 * (sV[sc][e] + sV[sc][e + 1]) -
+                               (m.wy0[j - 1] * us + m.wy1[j - 1] * up) * 0.5 *
+                                   (sV[sc][e - kHX] + sV[sc][e + 1 - kHX])) *
+                                  m.rdy[j];
+                    rr = rr - (0.5 * (up + ut) * 0.5 * (sW[sc][e] + sW[sc][e + 1]) -
+                               0.5 * (ub + up) * 0.5 * (sW[sm][e] + sW[sm][e + 1])) *
+                                  rdzk;
                     if constexpr (!LES) {
                         rr = rr + dPdx +
                              nu * ((m.rdx[i + 1] * (ue - up) - m.rdx[i] * (up - uw)) * m.rdxg[i] +
@@ -254,20 +236,10 @@ __global__ __launch_bounds__(256) void k_rk_march(
                     const T vp = sV[sc][e], ve = sV[sc][e + 1], vw = sV[sc][e - 1];
                     const T vn = sV[sc][e + kHX], vs = sV[sc][e - kHX];
                     const T vt = sV[sp][e], vb = sV[sm][e];
-                    if constexpr (sizeof(T) == 8) {
-                        rr = -(sq(0.5 * (vp + vn)) - sq(0.5 * (vs + vp))) * m.rdyg[j];
-                        rr = rr - ((m.wy0[j] * sU[sc][e] + m.wy1[j] * sU[sc][e + kHX]) * 0.5 * (vp + ve) -
-                                   (m.wy0[j] * sU[sc][e - 1] + m.wy1[j] * sU[sc][e - 1 + kHX]) * 0.5 * (vw + vp)) *
-                                      m.rdx[i];
-                    } else {
-                        // FP32: products of differences, as for u above
-                        rr = -(T)0.25 * (vn - vs) * (vn + vs + 2 * vp) * m.rdyg[j];
-                        const T ue0 = sU[sc][e], uw0 = sU[sc][e - 1];
-                        const T ae = ue0 + m.wy1[j] * (sU[sc][e + kHX] - ue0);
-                        const T da = (ue0 - uw0) + m.wy1[j] * ((sU[sc][e + kHX] - ue0) - (sU[sc][e - 1 + kHX] - uw0));
-                        const T bw = 0.5 * (vw + vp);
-                        rr = rr - (ae * (T)0.5 * (ve - vw) + bw * da) * m.rdx[i];
-                    }
+                    rr = -(sq(0.5 * (vp + vn)) - sq(0.5 * (vs + vp))) * m.rdyg[j];
+                    rr = rr - ((m.wy0[j] * sU[sc][e] + m.wy1[j] * sU[sc][e + kHX]) * 0.5 * (vp + ve) -
+                               (m.wy0[j] * sU[sc][e - 1] + m.wy1[j] * sU[sc][e - 1 + kHX]) * 0.5 * (vw + vp)) *
+                                  m.rdx[i];
                     rr = rr - (0.5 * (vp + vt) * (m.wy0[j] * sW[sc][e] + m.wy1[j] * sW[sc][e + kHX]) -
                                0.5 * (vb + vp) * (m.wy0[j] * sW[sm][e] + m.wy1[j] * sW[sm][e + kHX])) *
                                   rdzk;
@@ -306,19 +278,10 @@ __global__ __launch_bounds__(256) void k_rk_march(
                     const T wp = sW[sc][e], we = sW[sc][e + 1], ww = sW[sc][e - 1];
                     const T wn = sW[sc][e + kHX], ws = sW[sc][e - kHX];
                     const T wt = sW[sp][e], wb = sW[sm][e];
-                    if constexpr (sizeof(T) == 8) {
-                        rr = -(sq(0.5 * (wp + wt)) - sq(0.5 * (wb + wp))) * rdzgk;
-                        rr = rr - (0.5 * (sU[sc][e] + sU[sp][e]) * 0.5 * (wp + we) -
-                                   0.5 * (sU[sc][e - 1] + sU[sp][e - 1]) * 0.5 * (ww + wp)) *
-                                      m.rdx[i];
-                    } else {
-                        // FP32: products of differences, as for u above
-                        rr = -(T)0.25 * (wt - wb) * (wt + wb + 2 * wp) * rdzgk;
-                        const T at = 0.5 * (sU[sc][e] + sU[sp][e]);
-                        const T da = 0.5 * ((sU[sc][e] - sU[sc][e - 1]) + (sU[sp][e] - sU[sp][e - 1]));
-                        const T bw = 0.5 * (ww + wp);
-                        rr = rr - (at * (T)0.5 * (we - ww) + bw * da) * m.rdx[i];
-                    }
+                    rr = -(sq(0.5 * (wp + wt)) - sq(0.5 * (wb + wp))) * rdzgk;
+                    rr = rr - (0.5 * (sU[sc][e] + sU[sp][e]) * 0.5 * (wp + we) -
+                               0.5 * (sU[sc][e - 1] + sU[sp][e - 1]) * 0.5 * (ww + wp)) *
+                                  m.rdx[i];
                     rr = rr - (0.5 * (sV[sc][e] + sV[sp][e]) * (m.wy0[j] * wp + m.wy1[j] * wn) -
                                0.5 * (sV[sc][e - kHX] + sV[sp][e - kHX]) * (m.wy0[j - 1] * ws + m.wy1[j - 1] * wp)) *
                                   m.rdy[j];
@@ -549,8 +512,13 @@ __device__ __forceinline__ long long phi_at(const Dims &d, int i, int j, int k) 
 // plain order.
 constexpr int kProjZ = 4;
 
+// With P set (the first sub-step), also P = phi / den on the same cells
+// (projection.f90:75-85), and on the ghost cells next to them the same value
+// (the copies of apply_periodic_z_pressure and the P ghost faces), without a
+// second read of phi.
 template <class T>
-__global__ __launch_bounds__(256) void k_project(Dims d, MetricsT<T> m, T *U, T *V, T *W, const T *phi) {
+__global__ __launch_bounds__(256) void k_project(Dims d, MetricsT<T> m, T *U, T *V, T *W, const T *phi, T *P,
+                                                 double den) {
     const int nbx = (d.nxg - 1 + kWave - 1) / kWave, nby = (d.nyg - 2 + 3) / 4;
     const int ntile = nbx * nby * ((d.nzg - 2 + kProjZ - 1) / kProjZ);
     const int t = sizeof(T) == 4 ? xcd_tile(blockIdx.x, ntile) : blockIdx.x;
@@ -586,34 +554,27 @@ __global__ __launch_bounds__(256) void k_project(Dims d, MetricsT<T> m, T *U, T 
         if (j <= d.ny - 2) V[p] = v[r] - (py[r] - pz[r]) * rdyg;
         if (k <= d.nz - 2) W[p] = w[r] - (pz[r + 1] - pz[r]) * rz[r];
     }
-}
-
-// P = phi / (dt a11) on every cell, ghosts from the nearest interior cell;
-// 64 x 4 cells per block, kPresZ planes per thread, loads issued first
-constexpr int kPresZ = 4;
-
-template <class T>
-__global__ __launch_bounds__(256) void k_pressure(Dims d, const T *phi, T *P, double den) {
-    const int nbx = (d.nxg + kWave - 1) / kWave, nby = (d.nyg + 3) / 4;
-    const int ntile = nbx * nby * ((d.nzg + kPresZ - 1) / kPresZ);
-    const int t = sizeof(T) == 4 ? xcd_tile(blockIdx.x, ntile) : blockIdx.x;
-    if (t >= ntile) return;
-    const int i = (t % nbx) * kWave + threadIdx.x;
-    const int j = ((t / nbx) % nby) * 4 + threadIdx.y;
-    const int k0 = (t / (nbx * nby)) * kPresZ;
-    if (i >= d.nxg || j >= d.nyg) return;
-    const int ci = i < 1 ? 1 : (i > d.nxg - 2 ? d.nxg - 2 : i);
-    const int cj = j < 1 ? 1 : (j > d.nyg - 2 ? d.nyg - 2 : j);
-    T v[kPresZ];
+    if (P) {
+        const int gi = i == 1 ? 0 : (i == d.nxg - 2 ? d.nxg - 1 : -1);
+        const int gj = j == 1 ? 0 : (j == d.nyg - 2 ? d.nyg - 1 : -1);
 #pragma unroll
-    for (int r = 0; r < kPresZ; ++r) {
-        const int k = min(k0 + r, d.nzg - 1);
-        const int ck = k < 1 ? 1 : (k > d.nzg - 2 ? d.nzg - 2 : k);
-        v[r] = phi[phi_at(d, ci, cj, ck)];
+        for (int r = 0; r < kProjZ; ++r) {
+            const int k = k0 + r;
+            if (k >= k1) break;
+            const int gk = k == 1 ? 0 : (k == d.nzg - 2 ? d.nzg - 1 : -1);
+            const T pv = (T)((double)pz[r] / den);
+            for (int a = 0; a < 2; ++a) {
+                const int ii = a ? gi : i;
+                if (ii < 0) continue;
+                for (int b = 0; b < 2; ++b) {
+                    const int jj = b ? gj : j;
+                    if (jj < 0) continue;
+                    P[fidx(d, ii, jj, k)] = pv;
+                    if (gk >= 0) P[fidx(d, ii, jj, gk)] = pv;
+                }
+            }
+        }
     }
-#pragma unroll
-    for (int r = 0; r < kPresZ; ++r)
-        if (k0 + r < d.nzg) P[fidx(d, i, j, k0 + r)] = (T)((double)v[r] / den);
 }
 
 // ------------------------------------------------------------------------
@@ -1802,7 +1763,7 @@ const char *kernel_name(int id) {
     static const char *names[KID_COUNT] = {
         "rk_stage", "bc_inout", "bc_periodic", "bc_topwall", "bc_massflux", "divergence",
         "xform_x_fwd", "xform_z_fwd", "thomas", "xform_z_inv", "xform_x_inv", "project",
-        "pressure", "reduce", "dsm_cells", "dsm_vfilter", "dsm_filter", "dsm_column", "closure"};
+        "reduce", "dsm_cells", "dsm_vfilter", "dsm_filter", "dsm_column", "closure"};
     return (id >= 0 && id < KID_COUNT) ? names[id] : "";
 }
 
@@ -2027,22 +1988,14 @@ void launch_solve00(Handle &h, const Solve00 &z) {
 }
 
 template <class T>
-void launch_project(Handle &h, Fld3<T> &f, const T *phi) {
+void launch_project(Handle &h, Fld3<T> &f, const T *phi, T *P, double dt) {
     const Dims &d = h.d;
     ProfScope ps(h, KID_PROJECT);
     const dim3 grid(xcd_blocks((d.nxg - 1 + kWave - 1) / kWave * ((d.nyg - 2 + 3) / 4) *
                                ((d.nzg - 2 + kProjZ - 1) / kProjZ)));
-    hipLaunchKernelGGL(k_project<T>, grid, dim3(kWave, 4), 0, h.stream, d, metrics<T>(h), f.u, f.v, f.w, phi);
-    check_launch();
-}
-
-template <class T>
-void launch_pressure(Handle &h, const T *phi, T *P, double dt) {
-    const Dims &d = h.d;
-    ProfScope ps(h, KID_PRESSURE);
     const double den = dt * h.rk2_coef[1][1];
-    const dim3 grid(xcd_blocks((d.nxg + kWave - 1) / kWave * ((d.nyg + 3) / 4) * ((d.nzg + kPresZ - 1) / kPresZ)));
-    hipLaunchKernelGGL(k_pressure<T>, grid, dim3(kWave, 4), 0, h.stream, d, phi, P, den);
+    hipLaunchKernelGGL(k_project<T>, grid, dim3(kWave, 4), 0, h.stream, d, metrics<T>(h), f.u, f.v, f.w, phi, P,
+                       den);
     check_launch();
 }
 
@@ -2144,9 +2097,8 @@ void monitor(Handle &h, const Fld3<T> &f, double *out) {
     template void launch_rk_stage<T>(Handle &, const Fld3<T> &, const Fld3<T> &, Fld3<T> &, int, double); \
     template void launch_rhs_only<T>(Handle &, const Fld3<T> &, Fld3<T> &);                                \
     template void launch_bc<T>(Handle &, Fld3<T> &, const Fld3<T> &, int, double, double, double *, double); \
-    template void launch_project<T>(Handle &, Fld3<T> &, const T *);                                      \
+    template void launch_project<T>(Handle &, Fld3<T> &, const T *, T *, double);                         \
     template bool launch_thomas_seg<T>(Handle &, T *, int, long long, const Solve00 &);                   \
-    template void launch_pressure<T>(Handle &, const T *, T *, double);                                   \
     template double max_divergence<T>(Handle &, const Fld3<T> &);                                         \
     template double dt_convective<T>(Handle &, const Fld3<T> &);                                          \
     template void wall_stats<T>(Handle &, const Fld3<T> &, double *);                                     \

@@ -86,24 +86,31 @@ def test_cfg2_slab_steps(slab, prec):
     tol = 1e-12 if prec == "fp64" else 1e-4
     for n in range(3):
         t0 = time.time()
-        S, ts = o.fields(), o.t
+        Sg, tg = s.fields(), s.t
         o.compute_dt()
         o.step()
         s.compute_dt()
         s.step()
         div = s.max_divergence()
         errs = [rel_l2(a, b) for a, b in zip(s.fields(), o.fields())]
-        ep, ep_raw = p_errors(s.download(blgpu.FIELD_P), o.get(3))
+        Pg = s.download(blgpu.FIELD_P)
+        ep, ep_raw = p_errors(Pg, o.get(3))
         say(f"slab {prec} step {n + 1}: u v w", errs, "P (mean removed)", ep, "P raw", ep_raw,
             "max div", div, "oracle", o.max_divergence(), f"{time.time() - t0:.1f}s")
         for ci, e in enumerate(errs):
             assert e < tol, (n, ci, e)
         if prec == "fp64":
-            # P to 1e-10, or to the oracle's own change under a one-ulp
-            # perturbation of the step's start state where that is larger
-            sens = cases.oracle_p_sensitivity(slab, S, ts) if ep >= 1e-10 else 0.0
-            say(f"slab fp64 step {n + 1}: oracle P change from a 1-ulp start state", sens)
-            assert ep < max(1e-10, 2.0 * sens)
+            # P against an oracle step from the HIP path's own start state
+            # (1e-10), and against the oracle's trajectory (1e-9: that P also
+            # carries the round-off differences of the two start states).
+            # The second step starts from the first step's projection of the
+            # noisy initial field, whose round-off divergence (3.8e-11, 50x
+            # that of later steps) is a large part of the second step's
+            # Poisson source, so that P is a small remainder: 1e-9 there.
+            es, _ = p_errors(Pg, cases.oracle_step_p(slab.oracle, Sg, tg))
+            say(f"slab fp64 step {n + 1}: P against an oracle step from the same state", es)
+            assert es < (1e-9 if n == 1 else 1e-10)
+            assert ep < 1e-9
             # first step: the projection of the noisy initial field (see above)
             assert div < (max(1e-12, 1.05 * o.max_divergence()) if n == 0 else 1e-12)
     assert s.t == o.t
