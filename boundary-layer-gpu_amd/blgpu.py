@@ -47,6 +47,7 @@ SIGNATURES = {
     "bl_wall_stats": (_I, [_P, _P]),
     "bl_monitor": (_I, [_P, _P]),
     "bl_statistics": (_I, [_P, _P, _P, _P]),
+    "bl_max_divergence_noalloc": (_I, [_P, _P, _P, _I, _I, _I, _P, _P, _P, _P, _P]),
     "bl_fftz_create": (_I, [_I, ctypes.POINTER(_P)]),
     "bl_fftz_destroy": (None, [_P]),
     "bl_fft_z": (_I, [_P, _I, _P, _P]),

@@ -162,6 +162,20 @@ int bl_projection(bl_handle *h, int rk_step, double dt);
  * k=2..nzg-2 are read and written. */
 int bl_poisson(bl_handle *h, double *rhs_inout);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops): the maximum
+ * divergence of check_divergence (projection.f90:391-413) of caller-owned
+ * device arrays U(nx,nyg,nzg), V(nxg,ny,nzg), W(nxg,nyg,nz) in the layout
+ * above, with the face coordinates x(nx), y(ny), z(nz) also on the device,
+ * written to the device double div_out on `stream` (a hipStream_t; NULL is
+ * the null stream), asynchronously.  No allocation: the reduction partials
+ * and reciprocal spacings live in a file-scope device array of 1 MiB per
+ * device; a grid that needs more (about 1.3e5 reduction blocks plus
+ * nx + ny + nz) is rejected with BL_EINVAL.  Every call on a device shares
+ * that array: calls that overlap on one device race. */
+int bl_max_divergence_noalloc(const double *U, const double *V, const double *W, int nx, int ny, int nz,
+                              const double *x, const double *y, const double *z, double *div_out, void *stream);
+
 /* ---- the fftz module (fftz.f90:32-127) ----
  * Real DFT along z of xz planes with the solver's z-DFT kernels.  The
  * reference plans it at start-up (initialize_fftz, initialization.f90:720)
