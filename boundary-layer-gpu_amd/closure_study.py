@@ -8,8 +8,12 @@ LES = 4 (learned, dynamic Smagorinsky fallback) and LES = 2 run side by side
 from the held-out filtered field in the mixed-precision form, and their Cf
 and mean U(y) are compared with the DNS continued to the same time.
 
-    python closure_study.py [--les 256 97 64] [--snapshots 40] [--every 50]
-                            [--les-steps 1000] [--out result.json]
+    python closure_study.py [--les 257 97 64] [--snapshots 40] [--every 50]
+                            [--les-steps 5000] [--out result.json]
+
+The default LES grid is 257x97x64 and the DNS 1025x193x250: the DNS x
+transform has 1023 = 31 x 33 points, where 256 LES points would give the
+prime 1019, which the x transform does not take.
 
 Prints one JSON object (and writes it to --out).
 """
@@ -38,7 +42,7 @@ NU = 4.898844e-6
 
 def main(argv=None):
     ap = argparse.ArgumentParser()
-    ap.add_argument("--les", type=int, nargs=3, default=[256, 97, 64], metavar=("NX", "NY", "NZ"))
+    ap.add_argument("--les", type=int, nargs=3, default=[257, 97, 64], metavar=("NX", "NY", "NZ"))
     ap.add_argument("--box", type=float, nargs=3, default=[400.0, 40.0, 50.0], metavar=("LX", "LY", "LZ"),
                     help="box in units of theta_in")
     ap.add_argument("--noise", type=float, default=0.01)
@@ -48,7 +52,7 @@ def main(argv=None):
     ap.add_argument("--every", type=int, default=50, help="DNS steps between snapshots")
     ap.add_argument("--train-steps", type=int, default=2000)
     ap.add_argument("--batch", type=int, default=65536)
-    ap.add_argument("--les-steps", type=int, default=1000)
+    ap.add_argument("--les-steps", type=int, default=5000)
     ap.add_argument("--nsigma", type=float, default=3.0)
     ap.add_argument("--out", default="")
     a = ap.parse_args(argv)
@@ -124,6 +128,8 @@ def main(argv=None):
         fine.compute_dt()
         fine.step()
         n_dns += 1
+        if n_dns % 1000 == 0:
+            log(f"study: DNS continued {n_dns} steps, t = {fine.t:.5f} of {t_end:.5f}")
     cf_dns = fine.wall_stats()[:: fac[0]]
     T.collect(fine, judge, cl, fac, d99, snapshots=1, every=0, smin=1e-3 / THETA)
     prof = lambda U: U[1:-1, 1:-1, 1:-2].mean(axis=(0, 2))

@@ -39,7 +39,7 @@ int round_up(int v, int m) { return (v + m - 1) / m * m; }
 
 // cos/sin(2 pi k m / M) in long double with the exact integer reduction of
 // k m mod M, rounded once; symmetric in (k, m), lda covers padded row blocks
-void z_dft_tables(int M, int &lda, std::vector<double> &zc, std::vector<double> &zs) {
+void z_dft_tables(int M, int &lda, std::vector<double> &zc, std::vector<double> &zs, std::vector<double> &zf) {
     lda = round_up(M + 1, 64);
     zc.assign((size_t)lda * lda, 0.0);
     zs.assign((size_t)lda * lda, 0.0);
@@ -50,6 +50,20 @@ void z_dft_tables(int M, int &lda, std::vector<double> &zc, std::vector<double> 
             zc[(size_t)m * lda + k] = (double)cosl(ang);
             zs[(size_t)m * lda + k] = (double)sinl(ang);
         }
+    zf.clear();
+    const int Kc = M / 2 + 1;
+    if (Kc > 96) return;
+    const int RT = (Kc + 15) / 16, KS = (Kc + 3) / 4;
+    zf.assign(2 * (size_t)RT * KS * 64, 0.0);
+    for (int t = 0; t < RT; ++t)
+        for (int ks = 0; ks < KS; ++ks)
+            for (int l = 0; l < 64; ++l) {
+                // the same entries as row 1 of zc, zs
+                const int j = (int)(((long long)(16 * t + l % 16) * (4 * ks + l / 16)) % M);
+                const size_t e = 2 * (((size_t)t * KS + ks) * 64 + l);
+                zf[e] = zc[(size_t)lda + j];
+                zf[e + 1] = zs[(size_t)lda + j];
+            }
 }
 
 namespace {
@@ -469,6 +483,16 @@ void build(Handle &h, const bl_config *c) {
         // 8 rt + q + 4 (r / 2).
         const int k1n = n1 / 2 + 1;
         h.dct_mfma = n1 <= 32 && k1n <= 16 && n2 <= 32;
+        if (!h.dct_mfma && h.precision == BL_PREC_FP64) {
+            // the VALU kernels keep two lines and their spectra in 64 KB of LDS
+            const size_t lds = sizeof(double2) * (2 * (size_t)N + 2 * (size_t)k1n * n2 + n1 + n2);
+            if (lds > 65536) {
+                last_error() = "bl_create: the x transform of nx - 2 = " + std::to_string(N) + " points (" +
+                               std::to_string(n1) + " x " + std::to_string(n2) +
+                               ") does not fit the LDS-resident DCT; choose nx - 2 with a divisor near its square root";
+                throw ArgError();
+            }
+        }
         if (h.dct_mfma) {
             auto frag = [](int tiles, int ks_n, auto f) {
                 std::vector<double> t((size_t)tiles * ks_n * 64, 0.0);
@@ -530,12 +554,16 @@ void build(Handle &h, const bl_config *c) {
     }
     // z cos/sin tables, symmetric in (k, m): lda covers the padded row blocks
     {
-        std::vector<double> zc, zs;
-        z_dft_tables(M, h.lda_zt, zc, zs);
+        std::vector<double> zc, zs, zf;
+        z_dft_tables(M, h.lda_zt, zc, zs, zf);
         h.zc = dmalloc(zc.size());
         h.zs = dmalloc(zs.size());
         upload_vec(h.zc, zc, h.stream);
         upload_vec(h.zs, zs, h.stream);
+        if (!zf.empty()) {
+            h.zf = dmalloc(zf.size());
+            upload_vec(h.zf, zf, h.stream);
+        }
     }
     // ---- Thomas factors on the half spectrum (initialization.f90:328-344, 427-469)
     const int NM = d.NM, Kz = d.Kz;
@@ -621,7 +649,7 @@ void free_all(Handle &h) {
     }
     for (void *p : {(void *)h.F1.u, (void *)h.F1.v, (void *)h.F1.w, (void *)h.P, (void *)h.phi,
                     (void *)h.nu_t, (void *)h.T1, (void *)h.S, (void *)h.dct_w1, (void *)h.dct_w2,
-                    (void *)h.dct_wN, (void *)h.dct_rot, (void *)h.dct_mf, h.mx2, (void *)h.zc, (void *)h.zs, (void *)h.lu_dl, (void *)h.lu_dp, (void *)h.lu_du,
+                    (void *)h.dct_wN, (void *)h.dct_rot, (void *)h.dct_mf, h.mx2, (void *)h.zc, (void *)h.zs, (void *)h.zf, (void *)h.lu_dl, (void *)h.lu_dp, (void *)h.lu_du,
                     (void *)h.lu00, (void *)h.th_co,
                     (void *)h.scal, (void *)h.red, (void *)h.dsm, (void *)h.smag_c, h.mem_grid, h.mem_gridf, (void *)h.mx_tab,
                     (void *)h.p00, (void *)h.inc1, (void *)h.stats, (void *)h.cl_w, (void *)h.cl_b, (void *)h.cl_cnt})

@@ -15,7 +15,7 @@
 struct bl_fftz {
     int N = 0;
     int lda = 0;
-    double *zc = nullptr, *zs = nullptr;
+    double *zc = nullptr, *zs = nullptr, *zf = nullptr;
     hipStream_t stream = nullptr;
 };
 
@@ -100,6 +100,16 @@ struct DevBuf {
     }
 };
 
+ZTables tables(const bl_fftz &f) {
+    ZTables z;
+    z.M = f.N;
+    z.lda = f.lda;
+    z.zc = f.zc;
+    z.zs = f.zs;
+    z.zf = f.zf;
+    return z;
+}
+
 void run(bl_fftz &f, int n1, const double *in, double *out, bool forward) {
     if (n1 < 1) {
         last_error() = "fftz: n1 must be positive";
@@ -111,7 +121,7 @@ void run(bl_fftz &f, int n1, const double *in, double *out, bool forward) {
     const dim3 gk((n1 + 255) / 256, K);
     if (forward) {
         BL_HIP(hipMemcpyAsync(a.p, in, nreal * sizeof(double), hipMemcpyHostToDevice, f.stream));
-        launch_zdft(f.stream, N, n1, n1, f.lda, f.zc, f.zs, true, a.p, b.p);
+        launch_zdft(f.stream, tables(f), n1, n1, true, a.p, b.p);
         hipLaunchKernelGGL(k_packed_to_complex, gk, dim3(256), 0, f.stream, N, n1, b.p, c.p);
         BL_HIP(hipGetLastError());
         BL_HIP(hipMemcpyAsync(out, c.p, ncplx * sizeof(double), hipMemcpyDeviceToHost, f.stream));
@@ -119,7 +129,7 @@ void run(bl_fftz &f, int n1, const double *in, double *out, bool forward) {
         BL_HIP(hipMemcpyAsync(c.p, in, ncplx * sizeof(double), hipMemcpyHostToDevice, f.stream));
         hipLaunchKernelGGL(k_complex_to_packed, gk, dim3(256), 0, f.stream, N, n1, c.p, b.p);
         BL_HIP(hipGetLastError());
-        launch_zdft(f.stream, N, n1, n1, f.lda, f.zc, f.zs, false, b.p, a.p);
+        launch_zdft(f.stream, tables(f), n1, n1, false, b.p, a.p);
         hipLaunchKernelGGL(k_scale, dim3((unsigned)((nreal + 255) / 256)), dim3(256), 0, f.stream, (long long)nreal,
                            1.0 / (double)N, a.p);
         BL_HIP(hipGetLastError());
@@ -145,13 +155,17 @@ int bl_fftz_create(int N, bl_fftz **out) {
             throw ArgError();
         }
         f->N = N;
-        std::vector<double> zc, zs;
-        z_dft_tables(N, f->lda, zc, zs);
+        std::vector<double> zc, zs, zf;
+        z_dft_tables(N, f->lda, zc, zs, zf);
         BL_HIP(hipStreamCreateWithFlags(&f->stream, hipStreamNonBlocking));
         BL_HIP(hipMalloc(&f->zc, zc.size() * sizeof(double)));
         BL_HIP(hipMalloc(&f->zs, zs.size() * sizeof(double)));
         BL_HIP(hipMemcpy(f->zc, zc.data(), zc.size() * sizeof(double), hipMemcpyHostToDevice));
         BL_HIP(hipMemcpy(f->zs, zs.data(), zs.size() * sizeof(double), hipMemcpyHostToDevice));
+        if (!zf.empty()) {
+            BL_HIP(hipMalloc(&f->zf, zf.size() * sizeof(double)));
+            BL_HIP(hipMemcpy(f->zf, zf.data(), zf.size() * sizeof(double), hipMemcpyHostToDevice));
+        }
     });
     if (rc != BL_OK) {
         bl_fftz_destroy(f);
@@ -166,6 +180,7 @@ void bl_fftz_destroy(bl_fftz *f) {
     if (f->stream) (void)hipStreamSynchronize(f->stream);
     if (f->zc) (void)hipFree(f->zc);
     if (f->zs) (void)hipFree(f->zs);
+    if (f->zf) (void)hipFree(f->zf);
     if (f->stream) (void)hipStreamDestroy(f->stream);
     delete f;
 }

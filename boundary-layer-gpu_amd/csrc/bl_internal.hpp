@@ -166,6 +166,7 @@ struct Handle {
     int dct_n1;                      // x-DCT factorisation N = dct_n1 * (N / dct_n1)
     int lda_zt;                      // leading dimension of the z cos/sin tables
     double *zc, *zs;                 // z tables cos/sin(2 pi k m / M), [m][k]
+    double *zf = nullptr;            // their matrix-core fragments (M/2 + 1 <= 96)
     double *dct_w1, *dct_w2, *dct_wN, *dct_rot;  // x-DCT twiddle tables (complex)
     bool dct_mfma = false;                       // x-DCT on the FP64 matrix cores (shape fits)
     double *dct_mf = nullptr;                    // their operand tables: fA, fC, iC, iA
@@ -283,12 +284,21 @@ template <class T>
 void monitor(Handle &h, const Fld3<T> &f, double *out_host);
 template <class T>
 double dt_convective(Handle &h, const Fld3<T> &f);
-// the z DFT tables cos/sin(2 pi k m / M), [m][k] with leading dimension lda
-void z_dft_tables(int M, int &lda, std::vector<double> &zc, std::vector<double> &zs);
+// the z DFT tables cos/sin(2 pi k m / M), [m][k] with leading dimension lda,
+// and for M/2 + 1 <= 96 the matrix-core operand fragments zf: for row tile
+// t < (M/2 + 16) / 16, k step ks < (M/2 + 4) / 4 and lane l, the pair
+// (cos, sin)(2 pi r kk / M) with r = 16 t + l % 16, kk = 4 ks + l / 16, at
+// zf[2 ((t KS + ks) 64 + l)] (empty otherwise)
+void z_dft_tables(int M, int &lda, std::vector<double> &zc, std::vector<double> &zs, std::vector<double> &zf);
+// device z tables of one length M
+struct ZTables {
+    int M = 0, lda = 0;
+    const double *zc = nullptr, *zs = nullptr, *zf = nullptr;
+};
 // the real z DFT of C columns (plane stride `plane`) on stream s: forward to
 // the packed half spectrum (rows 0, 2k-1, 2k), or its unnormalised inverse
-void launch_zdft(hipStream_t s, int M, int C, long long plane, int lda, const double *zc, const double *zs,
-                 bool forward, const double *in, double *out);
+void launch_zdft(hipStream_t s, const ZTables &z, int C, long long plane, bool forward, const double *in,
+                 double *out);
 // ---- statistics and snapshots (bl_stats.hip) ----
 long long stats_size(const Dims &d);
 // compute_statistics' z means of f (and the handle's P, nu_t) into out

@@ -940,6 +940,126 @@ __device__ __forceinline__ dbl4 mfma64(double a, double b, dbl4 c) {
     return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
 }
 
+// ------------------------------------------------------------------------
+// The folded real z DFT above on the FP64 matrix cores, for M/2 + 1 <= 96
+// rows.  Both directions are two real products D = A B over the folded
+// index kk = 0..M/2 with A[row][kk] = cos or sin(2 pi row kk / M), the entry
+// (row kk) mod M of a single M-point table:
+//   forward  B = e[kk], o[kk] (the pairs T[kk] +- T[M-kk]); rows k give
+//            Re X_k and -Im X_k;
+//   inverse  B = R[k], I[k] (the Hermitian-weighted Re and Im rows); rows m
+//            give c[m], s[m], and g[m] = c - s, g[M-m] = c + s.
+// The products run in kk order, so the sums are those of the VALU kernels
+// bit for bit.  A block holds every A fragment (cos, sin) in LDS in lane
+// order (RT x KS x 64 pairs, 144 KB at M = 191; the table zf of
+// z_dft_tables) and its waves loop over
+// 16-column tiles, each owning every row tile of both products for its
+// columns; the raw inputs of the next kZmPF k steps (into the next tile at
+// the end of one) are loaded ahead.
+// ------------------------------------------------------------------------
+constexpr int kZmRT = 6, kZmKS = 24, kZmPF = 4, kZmW = 8;
+
+// RT = (M/2 + 1 + 15) / 16 row tiles
+template <bool FWD, int RT>
+__global__ __launch_bounds__(64 * kZmW) void k_zdft_mf(int M, int C, long long plane,
+                                                       const double2 *__restrict__ zf,
+                                                       const double *__restrict__ in,
+                                                       double *__restrict__ out) {
+    __shared__ double2 sA[RT * kZmKS * 64];
+    const int hm = M / 2, Kc = hm + 1, KS = (Kc + 3) / 4;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int r16 = lane & 15, q4 = lane >> 4;
+    // fragment (t, ks), lane l: A[16 t + l % 16][4 ks + l / 16] as (cos, sin),
+    // built on the host in this order (zf)
+    for (int e = threadIdx.x; e < RT * KS * 64; e += 64 * kZmW) sA[e] = zf[e];
+    __syncthreads();
+    const int ntile = (C + 15) / 16, stride = gridDim.x * kZmW;
+    int tile = blockIdx.x * kZmW + wave;
+    if (tile >= ntile) return;
+    // k steps past M/2 multiply zeros
+    const int KSr = (KS + kZmPF - 1) / kZmPF * kZmPF;
+    // raw inputs of folded index kk = 4 ks + q4 of tile tl (rows clamped to
+    // M/2, columns to C - 1); the loads are unconditional and masked only
+    // where they are used, so that those issued ahead stay in flight
+    auto raw = [&](int tl, int ks, double &a, double &b) {
+        const int kk = min(4 * ks + q4, hm);
+        const bool single = kk == 0 || 2 * kk == M;
+        int ra, rb;
+        if (FWD) {
+            ra = kk;
+            rb = single ? kk : M - kk;
+        } else {
+            ra = kk == 0 ? 0 : (2 * kk == M ? M - 1 : 2 * kk - 1);
+            rb = single ? ra : 2 * kk;
+        }
+        const double *src = in + min(16 * tl + r16, C - 1);
+        a = src[(long long)ra * plane];
+        b = src[(long long)rb * plane];
+    };
+    double pa[kZmPF], pb[kZmPF];
+#pragma unroll
+    for (int u = 0; u < kZmPF; ++u) raw(tile, u, pa[u], pb[u]);
+
+    for (; tile < ntile; tile += stride) {
+        dbl4 ac[RT], as[RT];
+#pragma unroll
+        for (int t = 0; t < RT; ++t) {
+            ac[t] = dbl4{0.0, 0.0, 0.0, 0.0};
+            as[t] = dbl4{0.0, 0.0, 0.0, 0.0};
+        }
+        for (int ks0 = 0; ks0 < KSr; ks0 += kZmPF) {
+#pragma unroll
+            for (int u = 0; u < kZmPF; ++u) {
+                const int ks = ks0 + u;
+                const int kk = 4 * ks + q4;
+                const bool live = kk <= hm, single = kk == 0 || 2 * kk == M;
+                const double a = live ? pa[u] : 0.0, b = live && !single ? pb[u] : 0.0;
+                // ahead: k step ks + kZmPF of this tile, or of the next one
+                const int kn = ks + kZmPF;
+                const bool here = kn < KSr;
+                raw(here ? tile : min(tile + stride, ntile - 1), here ? kn : kn - KSr, pa[u], pb[u]);
+                double bc, bs;
+                if (FWD) {
+                    bc = a + b;                  // b = 0 for the single elements
+                    bs = single ? 0.0 : a - b;
+                } else {
+                    bc = single ? a : 2.0 * a;   // R: weight 1 for k = 0 and the Nyquist row
+                    bs = 2.0 * b;                // I: b = 0 there
+                }
+                // A fragments of this step (clamped to the table: steps past
+                // M/2 multiply zero operands)
+                const double2 *fa = sA + min(ks, KS - 1) * 64 + lane;
+#pragma unroll
+                for (int t = 0; t < RT; ++t) {
+                    const double2 w = fa[t * KS * 64];
+                    ac[t] = mfma64(w.x, bc, ac[t]);
+                    as[t] = mfma64(w.y, bs, as[t]);
+                }
+            }
+        }
+        const int col = 16 * tile + r16;
+        if (col < C) {
+            double *dst = out + col;
+#pragma unroll
+            for (int t = 0; t < RT; ++t) {
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int row = 16 * t + q4 + 4 * r;
+                    if (row > hm) continue;
+                    const double c = ac[t][r], s = as[t][r];
+                    if (FWD) {
+                        dst[(long long)(row == 0 ? 0 : 2 * row - 1) * plane] = c;
+                        if (row >= 1 && 2 * row <= M - 1) dst[(long long)(2 * row) * plane] = -s;
+                    } else {
+                        dst[(long long)row * plane] = c - s;
+                        if (row >= 1 && M - row != row) dst[(long long)(M - row) * plane] = c + s;
+                    }
+                }
+            }
+        }
+    }
+}
+
 template <bool DIV>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k_dct_fwd_m(DctArgs a, Dims d, Metrics mt, const double *__restrict__ in,
                                                    const double *__restrict__ in2, const double *__restrict__ in3,
@@ -1690,10 +1810,43 @@ void check_launch() {
 constexpr int kDctLines = 2;
 constexpr int kZRows = 16, kZPre = 4;
 
+// the matrix-core z DFT with RT row tiles (RT = rt, one instantiation per
+// count); persistent blocks, one per compute unit
+template <int RT>
+void zdft_mf_launch(hipStream_t s, int rt, const ZTables &z, int C, long long plane, bool forward,
+                    const double *in, double *out) {
+    if constexpr (RT > 1) {
+        if (rt < RT) {
+            zdft_mf_launch<RT - 1>(s, rt, z, C, plane, forward, in, out);
+            return;
+        }
+    }
+    const double2 *zf = reinterpret_cast<const double2 *>(z.zf);
+    int dev = 0, ncu = 0;
+    BL_HIP(hipGetDevice(&dev));
+    BL_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+    const int tiles = (C + 15) / 16;
+    const dim3 grid((unsigned)std::max(1, std::min(ncu, (tiles + kZmW - 1) / kZmW)));
+    if (forward)
+        hipLaunchKernelGGL((k_zdft_mf<true, RT>), grid, dim3(64 * kZmW), 0, s, z.M, C, plane, zf, in, out);
+    else
+        hipLaunchKernelGGL((k_zdft_mf<false, RT>), grid, dim3(64 * kZmW), 0, s, z.M, C, plane, zf, in, out);
+    check_launch();
+}
+
 // the real z DFT (M points) of C columns with plane stride `plane`
-void zdft_raw(hipStream_t s, int M, int C, long long plane, int lda, const double *zc, const double *zs,
-              bool forward, const double *in, double *out) {
-    const int rows = M / 2 + 1;
+void zdft_raw(hipStream_t s, const ZTables &z, int C, long long plane, bool forward, const double *in,
+              double *out) {
+    const int M = z.M, lda = z.lda, rows = M / 2 + 1;
+    const double *zc = z.zc, *zs = z.zs;
+    if (rows <= 16 * kZmRT) {
+        if (!z.zf) {
+            last_error() = "z transform: matrix-core tables missing";
+            throw DeviceError();
+        }
+        zdft_mf_launch<kZmRT>(s, (rows + 15) / 16, z, C, plane, forward, in, out);
+        return;
+    }
     const dim3 grid(xcd_blocks((rows + kZRows - 1) / kZRows * ((C + 255) / 256)));
     if (forward)
         hipLaunchKernelGGL((k_zdft_fwd<kZRows, kZPre>), grid, dim3(256), 0, s, M, C, plane, lda, zc, zs, in, out);
@@ -1706,7 +1859,13 @@ void zdft_launch(Handle &h, int kid, bool forward, const double *in, double *out
     ProfScope ps(h, kid);
     const Dims &d = h.d;
     const int C = d.N * d.NM;
-    zdft_raw(h.stream, d.M, C, (long long)C, h.lda_zt, h.zc, h.zs, forward, in, out);
+    ZTables z;
+    z.M = d.M;
+    z.lda = h.lda_zt;
+    z.zc = h.zc;
+    z.zs = h.zs;
+    z.zf = h.zf;
+    zdft_raw(h.stream, z, C, (long long)C, forward, in, out);
 }
 
 void dct_launch(Handle &h, int kid, const DctArgs &a, bool forward, const double *in, double *out,
@@ -1754,9 +1913,9 @@ void dct_launch(Handle &h, int kid, const DctArgs &a, bool forward, const double
 
 }  // namespace
 
-void launch_zdft(hipStream_t s, int M, int C, long long plane, int lda, const double *zc, const double *zs,
-                 bool forward, const double *in, double *out) {
-    zdft_raw(s, M, C, plane, lda, zc, zs, forward, in, out);
+void launch_zdft(hipStream_t s, const ZTables &z, int C, long long plane, bool forward, const double *in,
+                 double *out) {
+    zdft_raw(s, z, C, plane, forward, in, out);
 }
 
 const char *kernel_name(int id) {
