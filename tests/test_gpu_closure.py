@@ -315,11 +315,16 @@ def test_les_closure_nu_t_mixed():
     c.close()
 
 
-def test_trained_network_gains_a_priori():
+def test_trained_network_on_filtered_dns():
     """Train on filtered snapshots of a small noisy-Blasius DNS, judge on a
-    held-out snapshot: the trained network's SGS-stress error is below the
-    untrained network's and below no model; the dynamic Smagorinsky and
-    constant-coefficient errors are reported beside it."""
+    held-out snapshot.  This laminar flow has little an eddy viscosity can
+    explain (at the training configuration's size the least-squares floor is
+    1 - 4e-8 of no model; here 0.992), so the check is that training lowers
+    the loss and the held-out error of the untrained network, that the
+    trained network is no worse than no model by more than 1e-5, and that it
+    takes at least 30 % of the room between no model and the floor; the
+    dynamic Smagorinsky and constant-coefficient errors are reported beside
+    it.  Learning where there is something to learn is test_network_learns_a_known_closure."""
     import importlib.util
     import os
 
@@ -356,5 +361,64 @@ def test_trained_network_gains_a_priori():
         say("a priori", k, after[k], "(untrained" if k == "learned" else "", before[k] if k == "learned" else "")
     assert hist[-10:].mean() < hist[:10].mean()
     assert after["learned"]["rel_error"] < before["learned"]["rel_error"]
-    assert after["learned"]["rel_error"] < after["no_model"]["rel_error"]
+    assert after["learned"]["rel_error"] < after["no_model"]["rel_error"] + 1e-5
+    # this small, noisier case leaves the floor 0.8 % below no model; the
+    # network takes at least 30 % of that room (58 % measured)
+    room = after["no_model"]["rel_error"] - after["least_squares_floor"]["rel_error"]
+    gain = after["no_model"]["rel_error"] - after["learned"]["rel_error"]
+    say("share of the eddy-viscosity floor captured", gain / room)
+    assert room > 0 and gain >= 0.3 * room
+    c.close()
+
+
+def test_network_learns_a_known_closure():
+    """Rows whose subgrid stress is an eddy viscosity by construction:
+    tau = -2 nu S + 1 % noise with nu = c(x) Delta^2|S| and a smooth c of the
+    features.  The least-squares floor is then near 0, and training on the
+    device must take the held-out stress error from the no-model 1.0 to
+    below 0.3 with an off-diagonal correlation above 0.9 (the machinery
+    learns when there is something to learn; the laminar filtered-DNS case
+    above has nothing)."""
+    import importlib.util
+    import os
+
+    spec = importlib.util.spec_from_file_location(
+        "closure_train", os.path.join(os.path.dirname(blgpu.__file__), "closure_train.py"))
+    T = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(T)
+    rng = np.random.default_rng(21)
+    n, nh = 60000, 10000
+    g = rng.standard_normal((n, 9))
+    g /= np.linalg.norm(g, axis=1, keepdims=True)
+    yd = rng.uniform(0.0, 1.5, n)
+    X = np.concatenate([g, yd[:, None]], axis=1)
+    cstar = 0.02 + 0.01 * np.tanh(2.0 * g[:, 0] * g[:, 4] + yd - 0.7)
+    scale = rng.uniform(0.5, 2.0, n)
+    S = rng.standard_normal((n, 6))
+    S[:, :3] -= S[:, :3].mean(axis=1, keepdims=True)
+    nu = cstar * scale
+    tau = -2.0 * nu[:, None] * S
+    tau += 0.01 * np.abs(tau).mean() * rng.standard_normal(tau.shape)
+    rows = np.zeros((n, blgpu.MLP_ROW), dtype=np.float32)
+    rows[:, :10] = X
+    Sd = S.copy()
+    sts = (Sd[:, :3] ** 2).sum(1) + 2.0 * (Sd[:, 3:] ** 2).sum(1)
+    td = T._dev(tau)
+    tds = (td[:, :3] * Sd[:, :3]).sum(1) + 2.0 * (td[:, 3:] * Sd[:, 3:]).sum(1)
+    rows[:, T.ROW_T] = -tds / (2.0 * sts) / scale  # least-squares target in units of Delta^2|S|
+    rows[:, T.ROW_W] = scale ** 2 * sts  # the stress-error weight of the filtered-DNS rows
+    rows[:, T.ROW_SCALE] = scale
+    rows[:, T.ROW_S:T.ROW_S + 6] = S
+    rows[:, T.ROW_TAU:T.ROW_TAU + 6] = tau
+    c = blgpu.Closure(capacity=n, batch_max=4096)
+    c.add_rows(rows[:n - nh])
+    c.params = M.init_params(3).astype(np.float32)
+    hist = c.train(2000, 4096, lr=1e-3, seed=9)
+    ap = T.apriori(c, rows[n - nh:])
+    for k in ap:
+        say("known closure a priori", k, ap[k])
+    say("loss", hist[:10].mean(), "->", hist[-10:].mean())
+    assert ap["least_squares_floor"]["rel_error"] < 0.05
+    assert ap["learned"]["rel_error"] < 0.3 * ap["no_model"]["rel_error"]
+    assert ap["learned"]["corr_offdiag"] > 0.9
     c.close()
