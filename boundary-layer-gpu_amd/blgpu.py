@@ -29,6 +29,13 @@ class BLConfig(ctypes.Structure):
                 ("qw_im", _DP), ("nut_bc", _I)]
 
 
+class BLBoundary(ctypes.Structure):
+    """bl_boundary: the general dispatcher's flags (boundary_conditions.f90:22-121)."""
+    _fields_ = [("inflow_flag", _I), ("top_flag", _I), ("amplitude", _D), ("seed", ctypes.c_ulonglong),
+                ("vbs_max", _D), ("x_bs", _D), ("sigma_bs", _D), ("phi_bs", _D), ("i_rescale", _I),
+                ("beta_hartree", _D)]
+
+
 # every entry point declared in include/blgpu.h with its ctypes signature
 SIGNATURES = {
     "bl_create": (_I, [ctypes.POINTER(BLConfig), ctypes.POINTER(_P)]),
@@ -37,6 +44,7 @@ SIGNATURES = {
     "bl_upload": (_I, [_P, _I, _P]),
     "bl_download": (_I, [_P, _I, _P]),
     "bl_set_time": (_I, [_P, _D]),
+    "bl_set_boundary": (_I, [_P, _P]),
     "bl_get_time": (_I, [_P, _DP]),
     "bl_compute_dt": (_I, [_P, _DP]),
     "bl_step": (_I, [_P]),
@@ -243,6 +251,14 @@ class Solver:
     @t.setter
     def t(self, v):
         _check(lib().bl_set_time(self.h, v), "bl_set_time")
+
+    def set_boundary(self, inflow_flag=1, top_flag=0, amplitude=0.0, seed=0, vbs_max=0.0, x_bs=0.0, sigma_bs=1.0,
+                     phi_bs=0.0, i_rescale=0, beta_hartree=0.0):
+        """bl_set_boundary: inflow_flag 1/2/4, top_flag 0-3 and their parameters
+        (.turbb Amplitude; Vmax, x0, sigma, phi, Lund_ix; betaH)."""
+        b = BLBoundary(inflow_flag, top_flag, amplitude, seed, vbs_max, x_bs, sigma_bs, phi_bs, i_rescale,
+                       beta_hartree)
+        _check(lib().bl_set_boundary(self.h, ctypes.byref(b)), "bl_set_boundary")
 
     def compute_dt(self):
         v = _D()

@@ -317,6 +317,9 @@ void build(Handle &h, const bl_config *c) {
     for (int j = 1; j <= d.nyg - 2; ++j) h.length_y = h.length_y + (g.y[j] - g.y[j - 1]);
     h.bc_uc = c->U_top[d.nx - 1];
     h.dxmin = g.dxmin;
+    h.hx = g.x;
+    h.hxg = g.xg;
+    h.hyg = g.yg;
     h.dymin = g.dymin;
     h.dzmin = g.dzmin;
 
@@ -649,7 +652,7 @@ void free_all(Handle &h) {
     }
     for (void *p : {(void *)h.F1.u, (void *)h.F1.v, (void *)h.F1.w, (void *)h.P, (void *)h.phi,
                     (void *)h.nu_t, (void *)h.T1, (void *)h.S, (void *)h.dct_w1, (void *)h.dct_w2,
-                    (void *)h.dct_wN, (void *)h.dct_rot, (void *)h.dct_mf, h.mx2, (void *)h.zc, (void *)h.zs, (void *)h.zf, (void *)h.lu_dl, (void *)h.lu_dp, (void *)h.lu_du,
+                    (void *)h.dct_wN, (void *)h.dct_rot, (void *)h.dct_mf, h.mx2, (void *)h.zc, (void *)h.zs, (void *)h.zf, (void *)h.bc_extra, (void *)h.lu_dl, (void *)h.lu_dp, (void *)h.lu_du,
                     (void *)h.lu00, (void *)h.th_co,
                     (void *)h.scal, (void *)h.red, (void *)h.dsm, (void *)h.smag_c, h.mem_grid, h.mem_gridf, (void *)h.mx_tab,
                     (void *)h.p00, (void *)h.inc1, (void *)h.stats, (void *)h.cl_w, (void *)h.cl_b, (void *)h.cl_cnt})
@@ -689,6 +692,7 @@ void do_step_t(Handle &h) {
     const int nst = h.scheme == 3 ? 3 : 2;
     const int order3[3] = {Y, Z, Y};
     int cur = X;
+    h.bc.istep = h.bc_step;  // the inflow_flag 4 draw of this step
     for (int s = 1; s <= nst; ++s) {
         const int nxt = order3[s - 1];
         // eddy viscosity of the stage's input fields (time_integration.f90:341-346)
@@ -717,6 +721,7 @@ void do_step_t(Handle &h) {
     }
     h.icur = h.iold = cur;
     h.inc_ok = true;
+    ++h.bc_step;
 }
 
 void do_step(Handle &h) {
@@ -891,6 +896,64 @@ int bl_download(bl_handle *hh, int field, double *host) {
         int n1, n2, n3;
         field_extent(h.d, field, n1, n2, n3);
         copy_to_host(h, host, field_dev(h, field), n1, n2, n3);
+    });
+}
+
+int bl_set_boundary(bl_handle *hh, const bl_boundary *b) {
+    return guarded([&] {
+        Handle &h = H(hh);
+        const Dims &d = h.d;
+        auto bad = [](const char *m) {
+            last_error() = m;
+            throw ArgError();
+        };
+        if (!b) bad("bl_set_boundary: null flags");
+        if (b->inflow_flag != 1 && b->inflow_flag != 2 && b->inflow_flag != 4)
+            bad("bl_set_boundary: inflow_flag must be 1, 2 or 4 (3 and 5, Lund rescaling, are not built)");
+        if (b->top_flag < 0 || b->top_flag > 3) bad("bl_set_boundary: top_flag must be 0, 1, 2 or 3");
+        if (b->inflow_flag == 4 && !(std::isfinite(b->amplitude) && b->amplitude >= 0.0))
+            bad("bl_set_boundary: amplitude must be finite and >= 0");
+        if ((b->top_flag == 1 || b->top_flag == 2) &&
+            !(std::isfinite(b->vbs_max) && std::isfinite(b->x_bs) && b->sigma_bs > 0.0 && std::isfinite(b->sigma_bs) &&
+              std::isfinite(b->phi_bs)))
+            bad("bl_set_boundary: blowing/suction needs finite vbs_max, x_bs, phi_bs and sigma_bs > 0");
+        if ((b->top_flag == 1 || b->top_flag == 2) && (b->i_rescale < 0 || b->i_rescale > d.nxg))
+            bad("bl_set_boundary: i_rescale must be in [0, nx + 1]");
+        if (b->top_flag == 3 && !(std::isfinite(b->beta_hartree) && b->beta_hartree < 2.0))
+            bad("bl_set_boundary: beta_hartree must be finite and below 2");
+        // host tables: V(i, ny) of blowing/suction (boundary_conditions.f90:
+        // 657-667) at xg(i); Falkner-Skan U(i, nyg) at xg(i) (:994-1000)
+        std::vector<double> ex((size_t)d.nxg + d.nx, 0.0);
+        if (b->top_flag == 1 || b->top_flag == 2) {
+            for (int i = 0; i < d.nxg; ++i) {
+                const double r = (b->x_bs - h.hxg[i]) / b->sigma_bs;
+                const double e = b->top_flag == 1 ? std::exp(0.5 - r * r) : std::exp(b->phi_bs - r * r);
+                ex[i] = std::sqrt(2.0) * b->vbs_max * (b->x_bs - h.hxg[i]) / b->sigma_bs * e +
+                        (b->top_flag == 1 ? b->phi_bs : 0.0);
+            }
+        }
+        double uc_fs = 0.0;
+        if (b->top_flag == 3) {
+            const double xf = -1.0, ex_m = b->beta_hartree / (2.0 - b->beta_hartree);
+            const double c = 1.0 / std::pow(h.hx[0] - xf, ex_m);
+            for (int i = 0; i < d.nx; ++i) ex[(size_t)d.nxg + i] = c * std::pow(h.hxg[i] - xf, ex_m);
+            uc_fs = ex[(size_t)d.nxg + d.nx - 1];
+        }
+        BL_HIP(hipStreamSynchronize(h.stream));
+        if (!h.bc_extra) h.bc_extra = dmalloc(ex.size());
+        BL_HIP(hipMemcpy(h.bc_extra, ex.data(), ex.size() * sizeof(double), hipMemcpyHostToDevice));
+        h.bc.inflow_flag = b->inflow_flag;
+        h.bc.top_flag = b->top_flag;
+        h.bc.amp = b->amplitude;
+        h.bc.seed = b->seed;
+        h.bc.vtop_bs = h.bc_extra;
+        h.bc.utop_fs = h.bc_extra + d.nxg;
+        h.bc.dy_dx = (h.hyg[d.nyg - 1] - h.hyg[d.nyg - 2]) / h.dxmin;
+        h.bc.i_rescale = b->i_rescale;
+        h.bc_step = 0;
+        h.fs_uc_pending = b->top_flag == 3;
+        h.fs_uc = uc_fs;
+        h.inc_ok = false;
     });
 }
 
@@ -1083,6 +1146,7 @@ int bl_apply_bc(bl_handle *hh, int rk_step, double dt) {
         Handle &h = H(hh);
         if (rk_step < 1 || rk_step > 3) throw ArgError();
         h.inc_ok = false;
+        h.bc.istep = h.bc_step;
         if (h.precision == BL_PREC_MIXED)
             bc_on<float>(h, h.buf[h.icur], h.buf[h.iold], rk_step, dt, h.t);
         else

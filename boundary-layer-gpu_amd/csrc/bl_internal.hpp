@@ -92,6 +92,15 @@ struct BCTables {
     const double *U_inlet, *V_inlet, *W_inlet, *U_top, *V_top, *W_top;
     int n_modes, m_modes;
     const double *zmode, *tmode, *qu_re, *qu_im, *qv_re, *qv_im, *qw_re, *qw_im;
+    // the general dispatcher's flags (bl_set_boundary); defaults: the GPU set
+    int inflow_flag = 1, top_flag = 0;
+    double amp = 0.0;               // inflow_flag 4 amplitude
+    unsigned long long seed = 0;    // inflow_flag 4 stream
+    long long istep = 0;            // inflow_flag 4: the step whose values are drawn
+    const double *vtop_bs = nullptr;  // top_flag 1, 2: V(i, ny) of blowing/suction (nxg)
+    const double *utop_fs = nullptr;  // top_flag 3: Falkner-Skan U(i, nyg) (nx)
+    double dy_dx = 0.0;             // top_flag 1, 2: (yg(nyg) - yg(nyg-1)) / dxmin
+    int i_rescale = 0;              // top_flag 1, 2: columns kept at U/V/W_top
 };
 
 template <class T>
@@ -140,6 +149,12 @@ struct Handle {
     double t, dt;
     double length_y;
     double bc_uc;      // U_top(nx): outflow convection speed
+    // the general dispatcher (bl_set_boundary)
+    long long bc_step = 0;          // steps since the flags were set (inflow_flag 4 stream)
+    double *bc_extra = nullptr;     // device V(i, ny) blowing/suction (nxg) + Falkner-Skan U(i, nyg) (nx)
+    bool fs_uc_pending = false;     // top_flag 3: U_top(nx) changes after the next BC call
+    double fs_uc = 0.0;
+    std::vector<double> hx, hxg, hyg;  // host copies of x, xg, yg
     double dxmin, dymin, dzmin;  // smallest centre spacings (initialization.f90:220-222)
     double rk_t[4];
     double rk_coef[4][4];

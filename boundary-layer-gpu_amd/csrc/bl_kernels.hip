@@ -361,6 +361,39 @@ __device__ __forceinline__ void inflow_modes(const Dims &d, const Metrics &m, co
         }
 }
 
+// inflow_flag 4: uniform [0, 1) value of (step, component, j, k) from a
+// counter hash (splitmix64 finaliser) of the stream's seed
+__device__ __forceinline__ unsigned long long bc_mix64(unsigned long long z) {
+    z += 0x9E3779B97F4A7C15ULL;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+    return z ^ (z >> 31);
+}
+__device__ __forceinline__ double inflow_rand(unsigned long long seed, long long step, int comp, int j, int k) {
+    const unsigned long long key = (((unsigned long long)step * 4ULL + (unsigned long long)comp) << 40) ^
+                                   ((unsigned long long)k << 20) ^ (unsigned long long)j;
+    return (double)(bc_mix64(seed ^ bc_mix64(key)) >> 11) * 0x1.0p-53;
+}
+
+// the inflow values of row j, plane k (FP64, before storage rounding):
+// Blasius + temporal modes at time t (boundary_conditions.f90:183-190), or
+// Blasius + the perturbations of step `step` (inflow_flag 4, :423-445)
+__device__ __forceinline__ void inflow_values(const Dims &d, const Metrics &m, const BCTables &b, int j, int k,
+                                              double t, long long step, double &u, double &v, double &w) {
+    if (b.inflow_flag == 4) {
+        const double a = b.amp * (1.0 - b.U_inlet[j]);
+        u = b.U_inlet[j] + a * (inflow_rand(b.seed, step, 0, j, k) - 0.5);
+        w = b.W_inlet[j] + a * (inflow_rand(b.seed, step, 2, j, k) - 0.5);
+        v = (j < d.ny) ? b.V_inlet[j] + a * (inflow_rand(b.seed, step, 1, j, k) - 0.5) : 0.0;
+        return;
+    }
+    double ut = 0.0, vt = 0.0, wt = 0.0;
+    if (b.n_modes > 0) inflow_modes(d, m, b, j, k, t, ut, vt, wt);
+    u = b.U_inlet[j] + ut;
+    w = b.W_inlet[j] + wt;
+    v = (j < d.ny) ? b.V_inlet[j] + vt : 0.0;
+}
+
 template <class T>
 __global__ __launch_bounds__(256) void k_bc_inout(Dims d, Metrics m, BCTables b, T *U, T *V, T *W,
                                                   const T *Uo, const T *Vo, const T *Wo, double coef,
@@ -373,21 +406,20 @@ __global__ __launch_bounds__(256) void k_bc_inout(Dims d, Metrics m, BCTables b,
     double q = 0.0;
     if (j < d.nyg) {
     const int nx = d.nx, nxg = d.nxg;
-    double ut = 0.0, vt = 0.0, wt = 0.0;
-    if (b.n_modes > 0) inflow_modes(d, m, b, j, k, t, ut, vt, wt);
+    double ui, vi, wi;
+    inflow_values(d, m, b, j, k, t, b.istep, ui, vi, wi);
     const long long p0 = fidx(d, 0, j, k);
     // U (all j < nyg, k < nzg)
-    const double uin = (double)(T)(b.U_inlet[j] + ut);  // the value stored
+    const double uin = (double)(T)ui;  // the value stored
     const long long pl = j + (long long)d.nyg * k;
     if (inc1) {
         // FP64 increment of the inflow face over the step-start value: that
         // value is taken unrounded when it is the stored boundary value of
-        // t_old, as it is after any step
-        double uo = 0.0, vo = 0.0, wo = 0.0;
-        if (b.n_modes > 0) inflow_modes(d, m, b, j, k, t_old, uo, vo, wo);
-        const double bo = b.U_inlet[j] + uo;
+        // t_old (the previous step), as it is after any step
+        double uo, vo, wo;
+        inflow_values(d, m, b, j, k, t_old, b.istep - 1, uo, vo, wo);
         const double so = (double)Uo[p0];
-        inc1[pl] = (b.U_inlet[j] + ut) - ((double)(T)bo == so ? bo : so);
+        inc1[pl] = ui - ((double)(T)uo == so ? uo : so);
     }
     U[p0] = (T)uin;
     {
@@ -402,7 +434,7 @@ __global__ __launch_bounds__(256) void k_bc_inout(Dims d, Metrics m, BCTables b,
         if (j >= 1 && j <= d.nyg - 2 && k >= 1 && k <= d.nzg - 3) q = (uin - v) * (m.y[j] - m.y[j - 1]);
     }
     if (k < d.nz) {
-        W[p0] = (T)(b.W_inlet[j] + wt);
+        W[p0] = (T)wi;
         const long long pa = fidx(d, nxg - 2, j, k);
         const double o1 = Wo[pa], o0 = Wo[pa - 1];
         const double v = o1 - coef * (o1 - o0) / (m.xg[nxg - 2] - m.xg[nxg - 3]);
@@ -410,7 +442,7 @@ __global__ __launch_bounds__(256) void k_bc_inout(Dims d, Metrics m, BCTables b,
         W[pa + 1] = (T)v;
     }
     if (j < d.ny) {
-        V[p0] = (T)(b.V_inlet[j] + vt);
+        V[p0] = (T)vi;
         const long long pa = fidx(d, nxg - 2, j, k);
         const double o1 = Vo[pa], o0 = Vo[pa - 1];
         const double v = o1 - coef * (o1 - o0) / (m.xg[nxg - 2] - m.xg[nxg - 3]);
@@ -453,23 +485,49 @@ __global__ __launch_bounds__(256) void k_bc_topwall(Dims d, Metrics m, BCTables 
     const int k = blockIdx.y;
     // mass flux through the y boundaries (boundary_conditions.f90:251-253) from
     // the values written here: (V(i,1,k) - V(i,ny,k))*(x(i)-x(i-1))
+    // V(i, ny) as stored at the top: V_top (top_flag 0), blowing/suction
+    // (1, 2: :648-673; columns below i_rescale keep it, because the
+    // reference's V_top copy of zero_wz_top, :279, writes V(i, nyg), one row
+    // past the top face, which the wall condition then overwrites), or 0
+    // (Falkner-Skan, :1001)
+    const int ii = min(i, d.nxg - 1);
+    const double vtop = b.top_flag == 0 ? (double)(T)b.V_top[ii]
+                                        : (b.top_flag == 3 ? 0.0 : (double)(T)b.vtop_bs[ii]);
     double q = 0.0;
-    if (i >= 1 && i <= d.nxg - 3 && k >= 1 && k <= d.nzg - 3) q = (0.0 - (double)(T)b.V_top[i]) * (m.x[i] - m.x[i - 1]);
+    if (i >= 1 && i <= d.nxg - 3 && k >= 1 && k <= d.nzg - 3) q = (0.0 - vtop) * (m.x[i] - m.x[i - 1]);
     q = block_sum_256(q, sh);
     if (threadIdx.x == 0) qpart[blockIdx.x + gridDim.x * blockIdx.y] = q;
     if (i >= d.nxg) return;
     const long long sx = d.sx;
     const long long p0 = fidx(d, i, 0, k);
-    if (i < d.nx) {
-        U[p0 + sx * (d.nyg - 1)] = (T)b.U_top[i];
-        U[p0] = -U[p0 + sx];
+    const long long ut_ = p0 + sx * (d.nyg - 1), vt_ = p0 + sx * (d.ny - 1), wt_ = p0 + sx * (d.nyg - 1);
+    if (b.top_flag == 0) {
+        // Dirichlet (:220-228, :947-964)
+        if (i < d.nx) U[ut_] = (T)b.U_top[i];
+        V[vt_] = (T)b.V_top[i];
+        if (k < d.nz) W[wt_] = (T)b.W_top[i];
+    } else if (b.top_flag == 3) {
+        // Falkner-Skan (:979-1007)
+        if (i < d.nx) U[ut_] = (T)b.utop_fs[i];
+        V[vt_] = (T)0;
+        if (k < d.nz) W[wt_] = W[wt_ - sx];
+    } else {
+        // blowing/suction, then zero_wz_top (:264-283)
+        V[vt_] = (T)b.vtop_bs[i];
+        if (i < d.nx) {
+            const double vp = (double)(T)b.vtop_bs[i + 1], vm = (double)(T)b.vtop_bs[i];
+            U[ut_] = (T)(b.dy_dx * (vp - vm) + (double)U[ut_ - sx]);
+        }
+        if (k < d.nz) W[wt_] = W[wt_ - sx];
+        if (i < b.i_rescale) {
+            if (i < d.nx) U[ut_] = (T)b.U_top[i];
+            if (k < d.nz) W[wt_] = (T)b.W_top[i];
+        }
     }
-    V[p0 + sx * (d.ny - 1)] = (T)b.V_top[i];
+    // walls (:231-238)
+    if (i < d.nx) U[p0] = -U[p0 + sx];
     V[p0] = (T)0;
-    if (k < d.nz) {
-        W[p0 + sx * (d.nyg - 1)] = (T)b.W_top[i];
-        W[p0] = -W[p0 + sx];
-    }
+    if (k < d.nz) W[p0] = -W[p0 + sx];
 }
 
 // global mass conservation (:241-260): every block re-sums the flux partials
@@ -2050,6 +2108,12 @@ void launch_bc(Handle &h, Fld3<T> &f, const Fld3<T> &old, int rk_step, double dt
                        inc1 ? inc1 + 2 * (long long)d.nyg * d.nzg : nullptr);
     check_launch();
     prof_end(h, KID_BC_MASSFLUX);
+    // Falkner-Skan top: U_top is the Falkner-Skan profile from the first call
+    // on, and with it the outflow's convective velocity U_top(nx)
+    if (h.fs_uc_pending) {
+        h.bc_uc = h.fs_uc;
+        h.fs_uc_pending = false;
+    }
 }
 
 void launch_poisson(Handle &h, double *phi, const Field3 *div) {

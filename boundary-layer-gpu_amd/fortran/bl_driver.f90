@@ -108,6 +108,11 @@ module bl_state
   ! 1 = mixed precision (FP32 fields, split-BF16x3 transforms, FP64 Thomas)
   integer :: iprecision = 0
   integer :: iwall_model = 0, inflow_flag = 1, top_flag = 0, random_init = 1
+  ! boundary-condition parameters (params.f90:122-125, 144, 162); beta_hartree
+  ! (Falkner-Skan) is 0 in the reference (input_output.f90:36) and an optional
+  ! key of this build
+  real(real64) :: vbs_max = 0d0, x_bs = 0d0, sigma_bs = 1d0, phi_bs = 0d0, amplitude = 0d0, beta_hartree = 0d0
+  integer :: i_rescale = 0
   integer :: nstep_init = 0, nstep_init_input = -45, istep = 0
   real(real64) :: nu = 0d0, cfl = -1d-4, dPdx = 0d0, dPdz = 0d0, t = 0d0, dt = 0d0
   real(real64) :: boxsize(4) = 0d0
@@ -169,6 +174,13 @@ contains
     call get_dbl_arr('boxsize', boxsize, f)
     call get_int('inflow_flag', inflow_flag, f)
     call get_int('top_flag', top_flag, f)
+    call get_dbl('Vmax', vbs_max, f)
+    call get_dbl('x0', x_bs, f)
+    call get_dbl('sigma', sigma_bs, f)
+    call get_dbl('phi', phi_bs, f)
+    call get_int('Lund_ix', i_rescale, f)
+    call get_dbl('Amplitude', amplitude, f)
+    call get_dbl('beta_hartree', beta_hartree, f)
     call get_str('inflow_file', file_inflow, f)
     if (.not. f) stop ' ERROR: you must specify a BL profile in inflow_file '
     call get_dbl('dPdx', dPdx, f)
@@ -196,6 +208,8 @@ contains
     if (les_model < 0 .or. les_model > 3) stop 'Error: LES must be 0, 1, 2 or 3'
     if (iprecision /= 0 .and. iprecision /= 1) stop 'Error: precision must be 0 (FP64) or 1 (mixed)'
     if (inflow_flag == 3 .or. inflow_flag == 5) stop 'Error: turbulent (Lund) inflow is not built'
+    if (inflow_flag /= 1 .and. inflow_flag /= 2 .and. inflow_flag /= 4) stop 'Error: inflow_flag must be 1, 2 or 4'
+    if (top_flag < 0 .or. top_flag > 3) stop 'Error: top_flag must be 0, 1, 2 or 3'
   end subroutine read_parameters
 
   subroutine allocate_all()
@@ -554,6 +568,7 @@ program bl_driver
   use blgpu_api
   implicit none
   type(bl_config), target :: cfg
+  type(bl_boundary) :: bnd
   type(c_ptr) :: h
   real(c_double) :: mon(8), div, t_dev
   real(real64) :: time1, time2, snap_t
@@ -600,6 +615,20 @@ program bl_driver
      cfg%qw_re = c_loc(qw_re); cfg%qw_im = c_loc(qw_im)
   end if
   call bl_check(bl_create(cfg, h), 'bl_create')
+  ! the general dispatcher's inflow and top flags (boundary_conditions.f90:22-121)
+  if (inflow_flag /= 1 .or. top_flag /= 0) then
+     bnd%inflow_flag = inflow_flag
+     bnd%top_flag = top_flag
+     bnd%amplitude = amplitude
+     bnd%seed = 0_c_long_long
+     bnd%vbs_max = vbs_max
+     bnd%x_bs = x_bs
+     bnd%sigma_bs = sigma_bs
+     bnd%phi_bs = phi_bs
+     bnd%i_rescale = i_rescale
+     bnd%beta_hartree = beta_hartree
+     call bl_check(bl_set_boundary(h, bnd), 'bl_set_boundary')
+  end if
   call bl_check(bl_upload(h, BL_FIELD_U, U), 'bl_upload U')
   call bl_check(bl_upload(h, BL_FIELD_V, V), 'bl_upload V')
   call bl_check(bl_upload(h, BL_FIELD_W, W), 'bl_upload W')

@@ -27,6 +27,16 @@ module blgpu_api
      integer(c_int) :: nut_bc
   end type bl_config
 
+  ! bl_boundary: the general dispatcher's flags (boundary_conditions.f90:22-121)
+  type, bind(C) :: bl_boundary
+     integer(c_int)       :: inflow_flag, top_flag
+     real(c_double)       :: amplitude
+     integer(c_long_long) :: seed
+     real(c_double)       :: vbs_max, x_bs, sigma_bs, phi_bs
+     integer(c_int)       :: i_rescale
+     real(c_double)       :: beta_hartree
+  end type bl_boundary
+
   interface
      integer(c_int) function bl_create(cfg, h) bind(C, name='bl_create')
        import :: c_int, c_ptr, bl_config
@@ -62,6 +72,12 @@ module blgpu_api
        type(c_ptr), value    :: h
        real(c_double), value :: t
      end function bl_set_time
+
+     integer(c_int) function bl_set_boundary(h, b) bind(C, name='bl_set_boundary')
+       import :: c_int, c_ptr, bl_boundary
+       type(c_ptr), value            :: h
+       type(bl_boundary), intent(in) :: b
+     end function bl_set_boundary
 
      integer(c_int) function bl_get_time(h, t) bind(C, name='bl_get_time')
        import :: c_int, c_ptr, c_double

@@ -93,6 +93,40 @@ int bl_download(bl_handle *h, int field, double *host);
 int bl_set_time(bl_handle *h, double t);
 int bl_get_time(bl_handle *h, double *t);
 
+/* Boundary-condition flags of the general dispatcher apply_boundary_conditions
+ * (boundary_conditions.f90:22-121), read from the .turbb file in
+ * params.f90:122-162.  The default (bl_create) is the set the reference's
+ * GPU path hard-wires (boundary_conditions.f90:125-262): inflow_flag 1,
+ * top_flag 0, no-slip walls (iwall_model 0).
+ *   inflow_flag 1, 2: Blasius + temporal modes (:382-402; 2 reads the
+ *     profile from a file, which arrives here as the same U/V/W_inlet tables)
+ *   inflow_flag 4: Blasius + uniform random perturbations of amplitude
+ *     `amplitude` weighted by (1 - U_inlet(j)), drawn once per time step
+ *     (:412-451).  The random stream is a counter hash of (seed, step,
+ *     component, j, k), not gfortran's random_number: the values differ from
+ *     a reference run, their distribution and the once-per-step update do not.
+ *   top_flag 0: Dirichlet U_top, V_top, W_top (:947-964)
+ *   top_flag 1, 2: blowing/suction V at the top (Coleman 2018 / Abe 2017,
+ *     :648-673, parameters vbs_max, x_bs, sigma_bs, phi_bs = .turbb Vmax,
+ *     x0, sigma, phi), then zero spanwise vorticity and dW/dy = 0 at the top
+ *     with columns 1..i_rescale (.turbb Lund_ix) kept at U/V/W_top (:264-283)
+ *   top_flag 3: Falkner-Skan U_top(x) = C (x + 1)^(beta/(2 - beta)) matched
+ *     to 1 at x(1), V = 0, dW/dy = 0 (:979-1007); beta = beta_hartree.  As in
+ *     the reference, U_top takes the Falkner-Skan values at the first
+ *     boundary-condition call, so the outflow's convective velocity U_top(nx)
+ *     changes from the next call on.
+ * Inflow flags 3 and 5 (Lund rescaling) and wall models are not built and
+ * are rejected.  Resets the step counter of the inflow_flag 4 stream. */
+typedef struct bl_boundary {
+    int inflow_flag, top_flag;
+    double amplitude;
+    unsigned long long seed;
+    double vbs_max, x_bs, sigma_bs, phi_bs;
+    int i_rescale;
+    double beta_hartree;
+} bl_boundary;
+int bl_set_boundary(bl_handle *h, const bl_boundary *b);
+
 /* compute_dt (time_integration.f90:528-591): fixed dt for cfl < 0, else the
  * reference's max-based convective bound (a device reduction) and the
  * viscous bound.  The result is also stored as the handle's dt. */

@@ -58,6 +58,15 @@ void orc_apply_periodic_z(orc_state *s);
 double orc_compute_dt(orc_state *s);
 /* compute_time_step_RK2 / RK3 (time_integration.f90:145-300, 305-465) */
 void orc_step(orc_state *s);
+/* The general dispatcher apply_boundary_conditions (boundary_conditions.f90:
+ * 22-121) for inflow_flag 1, 2 (Blasius + temporal modes), 4 (Blasius +
+ * random perturbations once per step, :412-451) and top_flag 0 (Dirichlet,
+ * :947-964), 1, 2 (blowing/suction :648-673 with zero_wz_top :264-283), 3
+ * (Falkner-Skan :979-1007).  The random values are a counter hash of (seed,
+ * step, component, j, k) in place of gfortran's random_number.  Returns 0, or
+ * -1 for flags outside those.  Resets the step counter. */
+int orc_set_boundary(orc_state *s, int inflow_flag, int top_flag, double amplitude, unsigned long long seed,
+                     double vbs_max, double x_bs, double sigma_bs, double phi_bs, int i_rescale, double beta_hartree);
 
 /* compute_rhs_u/v/w (equations.f90:24-415) on the current U,V,W into
  * full-extent arrays ru(nx,nyg,nzg), rv(nxg,ny,nzg), rw(nxg,nyg,nz)

@@ -47,6 +47,8 @@ orc_state *orc_create(int nx, int ny, int nz, const double *x, const double *y, 
     /* LES_model 0..3 (subgrid.f90:32-45); anything else has no model */
     if (les < 0 || les > 3) return NULL;
     orc_state *s = (orc_state *)calloc(1, sizeof(*s));
+    s->inflow_flag = 1; /* the GPU path's set; orc_set_boundary changes it */
+    s->step_beginning = 1;
     s->nx = nx;
     s->ny = ny;
     s->nz = nz;
@@ -568,8 +570,42 @@ void orc_rhs(orc_state *s, double *ru, double *rv, double *rw) {
     rhs_w(s, s->U, s->V, s->W, rw);
 }
 
+int orc_set_boundary(orc_state *s, int inflow_flag, int top_flag, double amplitude, unsigned long long seed,
+                     double vbs_max, double x_bs, double sigma_bs, double phi_bs, int i_rescale, double beta_hartree) {
+    if (inflow_flag != 1 && inflow_flag != 2 && inflow_flag != 4) return -1;
+    if (top_flag < 0 || top_flag > 3) return -1;
+    s->inflow_flag = inflow_flag;
+    s->top_flag = top_flag;
+    s->amp = amplitude;
+    s->seed = seed;
+    s->vbs_max = vbs_max;
+    s->x_bs = x_bs;
+    s->sigma_bs = sigma_bs;
+    s->phi_bs = phi_bs;
+    s->i_rescale = i_rescale;
+    s->beta_hartree = beta_hartree;
+    s->bc_step = 0;
+    s->step_beginning = 1;
+    return 0;
+}
+
+/* the random_number of apply_inflow_bc_x_Blasius_random: a uniform [0, 1)
+ * value of (seed, step, component, j, k) from the splitmix64 finaliser */
+static unsigned long long mix64(unsigned long long z) {
+    z += 0x9E3779B97F4A7C15ULL;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+    return z ^ (z >> 31);
+}
+static double inflow_rand(const orc_state *s, int comp, int j0, int k0) {
+    unsigned long long key = (((unsigned long long)s->bc_step * 4ULL + (unsigned long long)comp) << 40) ^
+                             ((unsigned long long)k0 << 20) ^ (unsigned long long)j0;
+    return (double)(mix64(s->seed ^ mix64(key)) >> 11) * 0x1.0p-53;
+}
+
 /* ------------------------------------------------------------------ */
-/* apply_boundary_conditions_gpu (boundary_conditions.f90:127-262)     */
+/* apply_boundary_conditions_gpu (boundary_conditions.f90:127-262), and */
+/* the general dispatcher's inflow and top flags (:22-121)              */
 /* ------------------------------------------------------------------ */
 void orc_apply_bc(orc_state *s, int rk_step, double dt) {
     int nx = s->nx, ny = s->ny, nz = s->nz, nxg = s->nxg, nyg = s->nyg, nzg = s->nzg;
@@ -577,7 +613,7 @@ void orc_apply_bc(orc_state *s, int rk_step, double dt) {
     const double *Uo = s->Uo, *Vo = s->Vo, *Wo = s->Wo;
     const double t = s->t;
     /* temporal inflow modes, :137-180 */
-    if (s->n_modes > 0) {
+    if (s->inflow_flag != 4 && s->n_modes > 0) {
         int nm = s->n_modes, mm = s->m_modes;
         for (int j = 1; j <= nyg; j++)
             for (int k = 1; k <= nzg; k++) {
@@ -614,12 +650,34 @@ void orc_apply_bc(orc_state *s, int rk_step, double dt) {
             }
     }
     /* inflow, :183-190 */
-    for (int k = 1; k <= nzg; k++)
-        for (int j = 1; j <= nyg; j++) U[IU(1, j, k)] = s->U_inlet[j - 1] + s->Ut[(j - 1) + (size_t)nyg * (k - 1)];
-    for (int k = 1; k <= nz; k++)
-        for (int j = 1; j <= nyg; j++) W[IW(1, j, k)] = s->W_inlet[j - 1] + s->Wt[(j - 1) + (size_t)nyg * (k - 1)];
-    for (int k = 1; k <= nzg; k++)
-        for (int j = 1; j <= ny; j++) V[IV(1, j, k)] = s->V_inlet[j - 1] + s->Vt[(j - 1) + (size_t)ny * (k - 1)];
+    if (s->inflow_flag == 4) {
+        /* Blasius + random perturbations once per step, :423-447 */
+        if (s->step_beginning == 1) {
+            for (int j = 1; j <= nyg; j++) {
+                double amp_loc = s->amp * (1.0 - s->U_inlet[j - 1]);
+                for (int k = 1; k <= nzg; k++)
+                    U[IU(1, j, k)] = s->U_inlet[j - 1] + amp_loc * (inflow_rand(s, 0, j - 1, k - 1) - 0.5);
+            }
+            for (int j = 1; j <= nyg; j++) {
+                double amp_loc = s->amp * (1.0 - s->U_inlet[j - 1]);
+                for (int k = 1; k <= nz; k++)
+                    W[IW(1, j, k)] = s->W_inlet[j - 1] + amp_loc * (inflow_rand(s, 2, j - 1, k - 1) - 0.5);
+            }
+            for (int j = 1; j <= ny; j++) {
+                double amp_loc = s->amp * (1.0 - s->U_inlet[(j < nyg ? j : nyg) - 1]);
+                for (int k = 1; k <= nzg; k++)
+                    V[IV(1, j, k)] = s->V_inlet[j - 1] + amp_loc * (inflow_rand(s, 1, j - 1, k - 1) - 0.5);
+            }
+            s->step_beginning = 0;
+        }
+    } else {
+        for (int k = 1; k <= nzg; k++)
+            for (int j = 1; j <= nyg; j++) U[IU(1, j, k)] = s->U_inlet[j - 1] + s->Ut[(j - 1) + (size_t)nyg * (k - 1)];
+        for (int k = 1; k <= nz; k++)
+            for (int j = 1; j <= nyg; j++) W[IW(1, j, k)] = s->W_inlet[j - 1] + s->Wt[(j - 1) + (size_t)nyg * (k - 1)];
+        for (int k = 1; k <= nzg; k++)
+            for (int j = 1; j <= ny; j++) V[IV(1, j, k)] = s->V_inlet[j - 1] + s->Vt[(j - 1) + (size_t)ny * (k - 1)];
+    }
     /* outflow, :193-202 (rk_t of RK3 for every scheme) */
     double Uc = s->U_top[nx - 1];
     double c = s->rk_t[rk_step];
@@ -642,13 +700,54 @@ void orc_apply_bc(orc_state *s, int rk_step, double dt) {
     periodic_z_cc(s, U, nx, nyg);
     periodic_z_cc(s, V, nxg, ny);
     periodic_z_w(s, W);
-    /* top, :220-228 */
-    for (int k = 1; k <= nzg; k++)
-        for (int i = 1; i <= nx; i++) U[IU(i, nyg, k)] = s->U_top[i - 1];
-    for (int k = 1; k <= nzg; k++)
-        for (int i = 1; i <= nxg; i++) V[IV(i, ny, k)] = s->V_top[i - 1];
-    for (int k = 1; k <= nz; k++)
-        for (int i = 1; i <= nxg; i++) W[IW(i, nyg, k)] = s->W_top[i - 1];
+    /* top, :220-228 (top_flag 0, :947-964) */
+    if (s->top_flag == 1 || s->top_flag == 2) {
+        /* apply_Dirichlet_bc_y_top_BlowingSuction(V, top_flag), :648-673 */
+        for (int i = 1; i <= nxg; i++) {
+            double r = (s->x_bs - XG(i)) / s->sigma_bs;
+            double f = s->top_flag == 1
+                           ? sqrt(2.0) * s->vbs_max * (s->x_bs - XG(i)) / s->sigma_bs * exp(0.5 - r * r) + s->phi_bs
+                           : sqrt(2.0) * s->vbs_max * (s->x_bs - XG(i)) / s->sigma_bs * exp(s->phi_bs - r * r);
+            for (int k = 1; k <= nzg; k++) V[IV(i, ny, k)] = f;
+        }
+        /* zero_wz_top, :264-283 (dx = dxmin, initialization.f90:254) */
+        double dy = YG(nyg) - YG(nyg - 1);
+        for (int k = 1; k <= nzg; k++)
+            for (int i = 1; i <= nxg - 1; i++)
+                U[IU(i, nyg, k)] = (dy / s->dx) * (V[IV(i + 1, ny, k)] - V[IV(i, ny, k)]) + U[IU(i, nyg - 1, k)];
+        for (int k = 1; k <= nz; k++)
+            for (int i = 1; i <= nxg; i++) W[IW(i, nyg, k)] = W[IW(i, nyg - 1, k)];
+        /* columns 1..i_rescale: U and W back to U_top, W_top.  The V_top copy
+         * of :279 stores V(i, nyg, :), one row past V's top face; with V's
+         * column-major shape that is V(i, 1, k+1), which the wall condition
+         * below sets to 0 again, so it leaves no trace and is not restated */
+        for (int i = 1; i <= s->i_rescale; i++) {
+            if (i <= nx)
+                for (int k = 1; k <= nzg; k++) U[IU(i, nyg, k)] = s->U_top[i - 1];
+            if (i <= nxg)
+                for (int k = 1; k <= nz; k++) W[IW(i, nyg, k)] = s->W_top[i - 1];
+        }
+    } else if (s->top_flag == 3) {
+        /* apply_top_bc_y_Falkner_Skan, :979-1007 */
+        double x_falkner = -1.0, Uinf = 1.0, m = s->beta_hartree / (2.0 - s->beta_hartree);
+        double C_falkner = Uinf / pow(X(1) - x_falkner, m);
+        for (int i = 1; i <= nx; i++) {
+            double u = C_falkner * pow(XG(i) - x_falkner, m);
+            for (int k = 1; k <= nzg; k++) U[IU(i, nyg, k)] = u;
+            s->U_top[i - 1] = u;
+        }
+        for (int k = 1; k <= nzg; k++)
+            for (int i = 1; i <= nxg; i++) V[IV(i, ny, k)] = 0.0;
+        for (int k = 1; k <= nz; k++)
+            for (int i = 1; i <= nxg; i++) W[IW(i, nyg, k)] = W[IW(i, nyg - 1, k)];
+    } else {
+        for (int k = 1; k <= nzg; k++)
+            for (int i = 1; i <= nx; i++) U[IU(i, nyg, k)] = s->U_top[i - 1];
+        for (int k = 1; k <= nzg; k++)
+            for (int i = 1; i <= nxg; i++) V[IV(i, ny, k)] = s->V_top[i - 1];
+        for (int k = 1; k <= nz; k++)
+            for (int i = 1; i <= nxg; i++) W[IW(i, nyg, k)] = s->W_top[i - 1];
+    }
     /* wall, :231-238 */
     for (int k = 1; k <= nzg; k++)
         for (int i = 1; i <= nx; i++) U[IU(i, 1, k)] = -U[IU(i, 2, k)];
@@ -950,6 +1049,7 @@ void orc_step(orc_state *s) {
     int nx = s->nx, ny = s->ny, nz = s->nz, nxg = s->nxg, nyg = s->nyg, nzg = s->nzg;
     double dt = s->dt;
     double to = s->t;
+    s->step_beginning = 1; /* time_integration.f90:29 */
     memcpy(s->Uo, s->U, sizeof(double) * nU(s));
     memcpy(s->Vo, s->V, sizeof(double) * nV(s));
     memcpy(s->Wo, s->W, sizeof(double) * nW(s));
@@ -996,6 +1096,7 @@ void orc_step(orc_state *s) {
         orc_projection(s, rk_step, dt);
         orc_apply_bc(s, rk_step, dt);
     }
+    s->bc_step++;
 }
 
 /* Cf part of compute_statistics (statistics.f90:44-52, 61-80, 137-160) */

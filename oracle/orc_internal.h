@@ -30,6 +30,11 @@ struct orc_state {
     double *Fu[4], *Fv[4], *Fw[4];
     double *rhs_p;
     double t, dt;
+    /* the general dispatcher's flags (orc_set_boundary) */
+    int inflow_flag, top_flag, i_rescale, step_beginning;
+    double amp, vbs_max, x_bs, sigma_bs, phi_bs, beta_hartree;
+    unsigned long long seed;
+    long long bc_step;
 };
 
 /* ---- 1-based index helpers (Fortran allocation shapes, x fastest) ---- */

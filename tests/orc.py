@@ -39,6 +39,7 @@ def lib():
             "orc_apply_periodic_z": (None, [_P]),
             "orc_compute_dt": (_D, [_P]),
             "orc_step": (None, [_P]),
+            "orc_set_boundary": (_I, [_P, _I, _I, _D, ctypes.c_ulonglong, _D, _D, _D, _D, _I, _D]),
             "orc_rhs": (None, [_P, _P, _P, _P]),
             "orc_apply_bc": (None, [_P, _I, _D]),
             "orc_set_old": (None, [_P, _P, _P, _P]),
@@ -206,6 +207,13 @@ class Oracle:
 
     def step(self):
         lib().orc_step(self.h)
+
+    def set_boundary(self, inflow_flag=1, top_flag=0, amplitude=0.0, seed=0, vbs_max=0.0, x_bs=0.0, sigma_bs=1.0,
+                     phi_bs=0.0, i_rescale=0, beta_hartree=0.0):
+        rc = lib().orc_set_boundary(self.h, inflow_flag, top_flag, amplitude, seed, vbs_max, x_bs, sigma_bs, phi_bs,
+                                    i_rescale, beta_hartree)
+        if rc != 0:
+            raise ValueError("orc_set_boundary: unsupported flags")
 
     def rhs(self):
         ru = np.zeros(self.shapes[0], order="F")

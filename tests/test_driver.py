@@ -116,6 +116,35 @@ def test_driver_fresh_start_matches_oracle(tmp_path):
 
 @pytest.mark.gpu
 @needs_driver
+def test_driver_dispatcher_flags(tmp_path):
+    """The .turbb keys of the general dispatcher (params.f90:122-162):
+    top_flag = 1 with Vmax, x0, sigma, phi and Lund_ix, and inflow_flag = 4
+    with Amplitude, reach bl_set_boundary; the snapshot after 20 steps
+    matches the oracle with the same flags (seed 0)."""
+    d = str(tmp_path)
+    write_case(d, top_flag="1", Vmax="0.05", x0="1.1", sigma="0.02", phi="0.0", Lund_ix="3", inflow_flag="4",
+               Amplitude="0.02")
+    r = run_driver(d)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    case = Case(40, 20, 8, 0.2, 0.02, 0.01, 1e-5, alpha=3.0, grid="sinh", scheme=3, cfl=-2e-4)
+    o = case.oracle()
+    o.set_boundary(inflow_flag=4, amplitude=0.02, seed=0, top_flag=1, vbs_max=0.05, x_bs=1.1, sigma_bs=0.02,
+                   phi_bs=0.0, i_rescale=3)
+    for _ in range(20):
+        o.compute_dt()
+        o.step()
+    snap = read_snapshot(os.path.join(d, "data", "BL_lam.00000020"))
+    U, V, W = o.fields()
+    scale = np.abs(U).max()
+    for name, ref in (("U", U), ("V", V), ("W", W)):
+        err = np.abs(snap[name] - ref[:, :, :-1]).max() / scale
+        print("driver dispatcher flags", name, err, flush=True)
+        assert err < 1e-10, name
+    o.close()
+
+
+@pytest.mark.gpu
+@needs_driver
 def test_driver_mixed_precision(tmp_path):
     """The optional `precision = 1` key runs the mixed-precision path: the
     snapshot after 20 steps agrees with the FP64 oracle to FP32 accuracy."""
@@ -177,6 +206,7 @@ def test_driver_requires_input_file(tmp_path):
     ("WM", "1", "only WM = 0"),
     ("LES", "5", "LES must be 0, 1, 2 or 3"),
     ("inflow_flag", "3", "Lund"),
+    ("top_flag", "5", "top_flag"),
     ("precision", "2", "precision must be 0"),
 ])
 def test_driver_rejects_unbuilt_options(tmp_path, key, val, msg):

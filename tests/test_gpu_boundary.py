@@ -1,0 +1,131 @@
+"""The general dispatcher's boundary flags on the device (bl_set_boundary)
+against the oracle (boundary_conditions.f90:22-121): blowing/suction tops
+(top_flag 1, 2) with zero vorticity and i_rescale columns, the Falkner-Skan
+top (3, including the switch of the outflow's convective velocity after the
+first call), and the once-per-step random inflow (inflow_flag 4).
+
+- one boundary call on a noisy field: every face, relative 1e-14 (FP64);
+- three RK3 steps (BCs, projection and the random stream across stages and
+  steps): U, V, W relative L2 1e-12, and the absolute divergence 1e-12 (or
+  the oracle's own, where the perturbed inflow's round-off exceeds it);
+- mixed precision, random inflow with a blowing/suction top: U, V, W after
+  three steps within the mixed mode's 1e-4 relative L2 of the FP64 oracle
+  (measured 6e-6, 3.5e-5, 4.7e-5: the switch of the top V to the
+  blowing/suction profile makes the first projection remove a large top
+  divergence, and the FP32 projection's relative error scales with it; with
+  the default top the same case gives 6e-8, 7e-7, 6e-7);
+- rejected flags leave the handle usable.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from . import cases
+from .blgpu_import import blgpu, solver_for
+
+pytestmark = pytest.mark.gpu
+
+FLAGS = {
+    "coleman": dict(top_flag=1, vbs_max=0.2, x_bs=1.08, sigma_bs=0.02, phi_bs=0.001, i_rescale=4),
+    "abe": dict(top_flag=2, vbs_max=0.15, x_bs=1.06, sigma_bs=0.03, phi_bs=0.002, i_rescale=0),
+    "falkner_skan": dict(top_flag=3, beta_hartree=0.2),
+    "random": dict(inflow_flag=4, amplitude=0.05, seed=77),
+    "random_coleman": dict(inflow_flag=4, amplitude=0.03, seed=5, top_flag=1, vbs_max=0.1, x_bs=1.05,
+                           sigma_bs=0.02, phi_bs=0.0, i_rescale=2),
+}
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def rel(a, b):
+    nb = np.linalg.norm(b)
+    return np.linalg.norm(a - b) / (nb if nb > 0 else 1.0)
+
+
+def _case():
+    return cases.laminar_small(nx=34, ny=18, nz=10, noise=0.02, seed=9, cfl=-2e-4)
+
+
+@pytest.mark.parametrize("name", list(FLAGS))
+def test_boundary_call_matches_oracle(name):
+    c = _case()
+    o = c.oracle()
+    s, _ = solver_for(c, o)
+    U0, V0, W0 = o.fields()
+    o.set_old(U0, V0, W0)
+    s.set_old(U0, V0, W0)
+    o.set_boundary(**FLAGS[name])
+    s.set_boundary(**FLAGS[name])
+    o.apply_bc(1, 2e-4)
+    s.apply_bc(1, 2e-4)
+    for comp, (a, b) in zip("UVW", zip(s.fields(), o.fields())):
+        e = rel(a, b)
+        say("bc", name, comp, e)
+        assert e < 1e-14, comp
+    s.close()
+    o.close()
+
+
+@pytest.mark.parametrize("name", list(FLAGS))
+def test_steps_match_oracle(name):
+    c = _case()
+    o = c.oracle()
+    s, _ = solver_for(c, o)
+    o.set_boundary(**FLAGS[name])
+    s.set_boundary(**FLAGS[name])
+    for _ in range(3):
+        o.compute_dt()
+        o.step()
+        s.compute_dt()
+        s.step()
+    for comp, (a, b) in zip("UVW", zip(s.fields(), o.fields())):
+        e = rel(a, b)
+        say("steps", name, comp, e)
+        assert e < 1e-12, comp
+    div, odiv = s.max_divergence(), o.max_divergence()
+    say("steps", name, "max divergence", div, "oracle", odiv)
+    # the absolute 1e-12, or the oracle's own round-off where the perturbed
+    # inflow makes that larger
+    assert div < max(1e-12, 2.0 * odiv)
+    s.close()
+    o.close()
+
+
+def test_mixed_random_inflow_blowing_top():
+    name = "random_coleman"
+    c = _case()
+    o = c.oracle()
+    s, _ = solver_for(c, o, precision=blgpu.PREC_MIXED)
+    o.set_boundary(**FLAGS[name])
+    s.set_boundary(**FLAGS[name])
+    for _ in range(3):
+        o.compute_dt()
+        o.step()
+        s.compute_dt()
+        s.step()
+    for comp, (a, b) in zip("UVW", zip(s.fields(), o.fields())):
+        e = rel(a, b)
+        say("mixed steps", name, comp, e)
+        assert e < 1e-4, comp
+    s.close()
+    o.close()
+
+
+def test_rejected_flags_keep_the_handle():
+    c = _case()
+    s, o = solver_for(c)
+    for bad in (dict(inflow_flag=3), dict(inflow_flag=5), dict(top_flag=4),
+                dict(top_flag=1, sigma_bs=0.0), dict(inflow_flag=4, amplitude=-1.0)):
+        with pytest.raises(blgpu.BLError):
+            s.set_boundary(**bad)
+    s.compute_dt()
+    s.step()
+    o.compute_dt()
+    o.step()
+    assert rel(s.fields()[0], o.fields()[0]) < 1e-12
+    s.close()
+    o.close()
