@@ -654,7 +654,7 @@ void free_all(Handle &h) {
                     (void *)h.nu_t, (void *)h.T1, (void *)h.S, (void *)h.dct_w1, (void *)h.dct_w2,
                     (void *)h.dct_wN, (void *)h.dct_rot, (void *)h.dct_mf, h.mx2, (void *)h.zc, (void *)h.zs, (void *)h.zf, (void *)h.bc_extra, (void *)h.lu_dl, (void *)h.lu_dp, (void *)h.lu_du,
                     (void *)h.lu00, (void *)h.th_co,
-                    (void *)h.scal, (void *)h.red, (void *)h.dsm, (void *)h.smag_c, h.mem_grid, h.mem_gridf, (void *)h.mx_tab,
+                    (void *)h.scal, (void *)h.red, (void *)h.dsm, (void *)h.smag_c, h.mem_grid, h.mem_gridf, (void *)h.mx_tab, (void *)h.zmf,
                     (void *)h.p00, (void *)h.inc1, (void *)h.stats, (void *)h.cl_w, (void *)h.cl_b, (void *)h.cl_cnt})
         f(p);
     if (h.stream) (void)hipStreamSynchronize(h.stream);

@@ -206,6 +206,8 @@ struct Handle {
     const void *x2_fA = nullptr, *x2_fC = nullptr, *x2_iC = nullptr, *x2_iA = nullptr;
     const float *x2_tw = nullptr;   // wN[N], rot[N] as float2
     unsigned short *mx_tab;         // one allocation for the eight matrices below
+    unsigned short *zmf = nullptr;  // folded z transform fragments (k_zdft_mx), M/2 + 1 <= 96
+    int zmf_rt = 0;
     const unsigned short *dxf_hi, *dxf_lo, *dxi_hi, *dxi_lo, *zf_hi, *zf_lo, *zi_hi, *zi_lo;
     double *p00;                    // FP64 plane partials of the divergence
     double *c00;                    // phi00(j) / (2 N M)

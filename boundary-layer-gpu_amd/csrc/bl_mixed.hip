@@ -329,6 +329,158 @@ void gemm(Handle &h, int kid, const GemmArgs &g) {
 }
 
 // ------------------------------------------------------------------------
+// The real z DFT (M points, M/2 + 1 <= 96) folded by the symmetry m <-> M-m,
+// as the FP64 kernel k_zdft_mf does, with split-BF16x3 products accumulated
+// in FP32: D = A B over the folded index kk = 0..M/2, A[row][kk] = cos or
+// sin(2 pi row kk / M) pre-split on the host, B the folded pairs split on the
+// fly (forward: e, o = T[kk] +- T[M-kk]; inverse: the Hermitian-weighted Re
+// and Im rows R, I).  Half the products of the dense 191 x 191 GEMM.
+// v_mfma_f32_16x16x32_bf16: lane l holds A[l%16][8(l/16)+e], B[8(l/16)+e][l%16]
+// (e < 8) and D[4(l/16)+r][l%16] (r < 4).  A block holds every A fragment in
+// LDS (cos hi, cos lo, sin hi, sin lo per lane, RT x 3 k steps: 72 KB at
+// M = 191) and its waves loop over 16-column tiles; the raw inputs of the
+// next k step are loaded as each one's are consumed.
+// ------------------------------------------------------------------------
+constexpr int kZxRT = 6, kZxKS = 3, kZxW = 8;
+
+template <bool FWD, int RT>
+__global__ __launch_bounds__(64 * kZxW) void k_zdft_mx(int M, int C, long long plane, const u16x8 *__restrict__ zfr,
+                                                       const float *__restrict__ in, float *__restrict__ out) {
+    __shared__ u16x8 sA[RT * kZxKS * 64 * 4];
+    const int hm = M / 2, Kc = hm + 1, KS = (Kc + 31) / 32;
+    for (int e = threadIdx.x; e < RT * KS * 64 * 4; e += 64 * kZxW) sA[e] = zfr[e];
+    __syncthreads();
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, q = lane >> 4, c16 = lane & 15;
+    const int ntile = (C + 15) / 16, stride = gridDim.x * kZxW;
+    int tile = blockIdx.x * kZxW + wave;
+    if (tile >= ntile) return;
+    // raw inputs of k step s of tile tl: kk = 32 s + 8 q + e (rows clamped
+    // to M/2, columns to C - 1; masked where used), one step ahead: the
+    // steps run (tile, s) in order, the last step's successor is the next
+    // tile's first.  Offsets are 32-bit (M plane < 2^31, checked on the host).
+    float pa[8], pb[8];
+    auto load = [&](int tl, int s) {
+        const float *src = in + min(16 * tl + c16, C - 1);
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            const int kk = min(32 * s + 8 * q + e, hm);
+            const bool single = kk == 0 || 2 * kk == M;
+            int r0, r1;
+            if (FWD) {
+                r0 = kk;
+                r1 = single ? kk : M - kk;
+            } else {
+                r0 = kk == 0 ? 0 : (2 * kk == M ? M - 1 : 2 * kk - 1);
+                r1 = single ? r0 : 2 * kk;
+            }
+            pa[e] = src[r0 * (int)plane];
+            pb[e] = src[r1 * (int)plane];
+        }
+    };
+    load(tile, 0);
+    for (; tile < ntile; tile += stride) {
+        f32x4 ac[RT], as[RT];
+#pragma unroll
+        for (int t = 0; t < RT; ++t) {
+            ac[t] = f32x4{0.f, 0.f, 0.f, 0.f};
+            as[t] = f32x4{0.f, 0.f, 0.f, 0.f};
+        }
+        for (int s = 0; s < KS; ++s) {
+            u16x8 bch, bcl, bsh, bsl;
+#pragma unroll
+            for (int e = 0; e < 8; ++e) {
+                const int kk = 32 * s + 8 * q + e;
+                const bool live = kk <= hm, single = kk == 0 || 2 * kk == M;
+                const float a = live ? pa[e] : 0.f, b = live && !single ? pb[e] : 0.f;
+                float vc, vs;
+                if (FWD) {
+                    vc = a + b;
+                    vs = single ? 0.f : a - b;
+                } else {
+                    vc = single ? a : 2.f * a;
+                    vs = 2.f * b;
+                }
+                unsigned short h0, l0, h1, l1;
+                split2(vc, h0, l0);
+                split2(vs, h1, l1);
+                bch[e] = h0;
+                bcl[e] = l0;
+                bsh[e] = h1;
+                bsl[e] = l1;
+            }
+            const bool here = s + 1 < KS;
+            load(here ? tile : min(tile + stride, ntile - 1), here ? s + 1 : 0);
+            const bf16x8 ch = __builtin_bit_cast(bf16x8, bch), cl = __builtin_bit_cast(bf16x8, bcl);
+            const bf16x8 sh = __builtin_bit_cast(bf16x8, bsh), sl = __builtin_bit_cast(bf16x8, bsl);
+#pragma unroll
+            for (int t = 0; t < RT; ++t) {
+                const u16x8 *f = sA + ((t * KS + s) * 64 + lane) * 4;
+                const bf16x8 fch = __builtin_bit_cast(bf16x8, f[0]), fcl = __builtin_bit_cast(bf16x8, f[1]);
+                const bf16x8 fsh = __builtin_bit_cast(bf16x8, f[2]), fsl = __builtin_bit_cast(bf16x8, f[3]);
+                // small terms first, as in k_gemm_bf16x3
+                ac[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fcl, ch, ac[t], 0, 0, 0);
+                ac[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fch, cl, ac[t], 0, 0, 0);
+                ac[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fch, ch, ac[t], 0, 0, 0);
+                as[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fsl, sh, as[t], 0, 0, 0);
+                as[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fsh, sl, as[t], 0, 0, 0);
+                as[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fsh, sh, as[t], 0, 0, 0);
+            }
+        }
+        const int col = 16 * tile + c16;
+        if (col < C) {
+            float *dst = out + col;
+#pragma unroll
+            for (int t = 0; t < RT; ++t) {
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int row = 16 * t + 4 * q + r;
+                    if (row > hm) continue;
+                    const float c = ac[t][r], sv = as[t][r];
+                    if (FWD) {
+                        dst[(long long)(row == 0 ? 0 : 2 * row - 1) * plane] = c;
+                        if (row >= 1 && 2 * row <= M - 1) dst[(long long)(2 * row) * plane] = -sv;
+                    } else {
+                        dst[(long long)row * plane] = c - sv;
+                        if (row >= 1 && M - row != row) dst[(long long)(M - row) * plane] = c + sv;
+                    }
+                }
+            }
+        }
+    }
+}
+
+// the folded z DFT with RT = rt row tiles; persistent blocks, as many as fit
+template <int RT>
+void zdft_mx_launch(Handle &h, int kid, int rt, int M, int C, long long plane, bool forward, const float *in,
+                    float *out) {
+    if constexpr (RT > 1) {
+        if (rt < RT) {
+            zdft_mx_launch<RT - 1>(h, kid, rt, M, C, plane, forward, in, out);
+            return;
+        }
+    }
+    if ((long long)M * plane >= (1LL << 31)) {
+        last_error() = "z transform: M x plane exceeds 32-bit offsets";
+        throw DeviceError();
+    }
+    ProfScope ps(h, kid);
+    const int tiles = (C + 15) / 16;
+    int per_cu = 0;
+    if (forward)
+        BL_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_zdft_mx<true, RT>, 64 * kZxW, 0));
+    else
+        BL_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_zdft_mx<false, RT>, 64 * kZxW, 0));
+    const dim3 grid(
+        (unsigned)std::max(1, std::min(std::max(1, per_cu) * std::max(1, h.n_cu), (tiles + kZxW - 1) / kZxW)));
+    const u16x8 *zfr = reinterpret_cast<const u16x8 *>(h.zmf);
+    if (forward)
+        hipLaunchKernelGGL((k_zdft_mx<true, RT>), grid, dim3(64 * kZxW), 0, h.stream, M, C, plane, zfr, in, out);
+    else
+        hipLaunchKernelGGL((k_zdft_mx<false, RT>), grid, dim3(64 * kZxW), 0, h.stream, M, C, plane, zfr, in, out);
+    check();
+}
+
+// ------------------------------------------------------------------------
 // x transform as two short DFTs on the matrix cores (N = N1 N2, Makhoul's
 // reordering, k1 <= N1/2 only; the same factorisation as the FP64 kernels),
 // each DFT a small GEMM with split-BF16x3 products accumulated in FP32.
@@ -843,6 +995,30 @@ void build_mixed(Handle &h) {
         *dst[q] = h.mx_tab + off;
         off += t[q].size();
     }
+    // folded z transform fragments (k_zdft_mx) for M/2 + 1 <= 96: per (row
+    // tile t, k step s, lane l) the 8 values of A[16 t + l % 16][32 s + 8 (l / 16) + e]
+    // as cos hi, cos lo, sin hi, sin lo (zero past M/2)
+    if (M / 2 + 1 <= 16 * kZxRT) {
+        const int Kc = M / 2 + 1, RT = (Kc + 15) / 16, KS = (Kc + 31) / 32;
+        std::vector<uint16_t> zf((size_t)RT * KS * 64 * 32, 0);
+        for (int tt = 0; tt < RT; ++tt)
+            for (int ss = 0; ss < KS; ++ss)
+                for (int l = 0; l < 64; ++l)
+                    for (int e = 0; e < 8; ++e) {
+                        const int row = 16 * tt + l % 16, kk = 32 * ss + 8 * (l / 16) + e;
+                        if (kk >= Kc) continue;
+                        const size_t o = (((size_t)tt * KS + ss) * 64 + l) * 32 + e;
+                        const float vc = (float)cz(row, kk), vs = (float)sz(row, kk);
+                        const uint16_t ch = host_bf16(vc), sh = host_bf16(vs);
+                        zf[o] = ch;
+                        zf[o + 8] = host_bf16(vc - host_bf16_float(ch));
+                        zf[o + 16] = sh;
+                        zf[o + 24] = host_bf16(vs - host_bf16_float(sh));
+                    }
+        BL_HIP(hipMalloc(&h.zmf, zf.size() * sizeof(uint16_t)));
+        BL_HIP(hipMemcpyAsync(h.zmf, zf.data(), zf.size() * sizeof(uint16_t), hipMemcpyHostToDevice, h.stream));
+        h.zmf_rt = RT;
+    }
     // two-factor x transform (k_xfwd_mx / k_xinv_mx) when N = N1 N2 fits it
     {
         // N = N1 N2 with N1 the largest divisor <= sqrt(N), as for the FP64 tables
@@ -1015,7 +1191,10 @@ void launch_poisson_mixed(Handle &h, float *phi, const Fld3<float> &f, const Inc
     z.out = S;
     z.ldc = (long long)NM * NP;
     z.o_C1 = 1;
-    gemm<kWRz, kWCz, kBKz, false, false>(h, KID_XFORM_Z_FWD, z);
+    if (h.zmf)
+        zdft_mx_launch<kZxRT>(h, KID_XFORM_Z_FWD, h.zmf_rt, M, z.C, z.ldb, true, T1, S);
+    else
+        gemm<kWRz, kWCz, kBKz, false, false>(h, KID_XFORM_Z_FWD, z);
     // FP64 Thomas on the spectrum, FP64 (0,0) mode
     {
         ProfScope ps(h, KID_THOMAS);
@@ -1034,7 +1213,10 @@ void launch_poisson_mixed(Handle &h, float *phi, const Fld3<float> &f, const Inc
     z.Alo = h.zi_lo;
     z.B = S;
     z.out = T1;
-    gemm<kWRz, kWCz, kBKz, false, false>(h, KID_XFORM_Z_INV, z);
+    if (h.zmf)
+        zdft_mx_launch<kZxRT>(h, KID_XFORM_Z_INV, h.zmf_rt, M, z.C, z.ldb, false, S, T1);
+    else
+        gemm<kWRz, kWCz, kBKz, false, false>(h, KID_XFORM_Z_INV, z);
     // inverse x into the field layout, plus the (0,0) constant of each y row
     g.Ahi = h.dxi_hi;
     g.Alo = h.dxi_lo;
