@@ -33,7 +33,7 @@ class BLBoundary(ctypes.Structure):
     """bl_boundary: the general dispatcher's flags (boundary_conditions.f90:22-121)."""
     _fields_ = [("inflow_flag", _I), ("top_flag", _I), ("amplitude", _D), ("seed", ctypes.c_ulonglong),
                 ("vbs_max", _D), ("x_bs", _D), ("sigma_bs", _D), ("phi_bs", _D), ("i_rescale", _I),
-                ("beta_hartree", _D)]
+                ("beta_hartree", _D), ("delta_inlet", _D), ("t_resc", _D)]
 
 
 # every entry point declared in include/blgpu.h with its ctypes signature
@@ -253,11 +253,11 @@ class Solver:
         _check(lib().bl_set_time(self.h, v), "bl_set_time")
 
     def set_boundary(self, inflow_flag=1, top_flag=0, amplitude=0.0, seed=0, vbs_max=0.0, x_bs=0.0, sigma_bs=1.0,
-                     phi_bs=0.0, i_rescale=0, beta_hartree=0.0):
-        """bl_set_boundary: inflow_flag 1/2/4, top_flag 0-3 and their parameters
-        (.turbb Amplitude; Vmax, x0, sigma, phi, Lund_ix; betaH)."""
+                     phi_bs=0.0, i_rescale=0, beta_hartree=0.0, delta_inlet=0.0, t_resc=0.0):
+        """bl_set_boundary: inflow_flag 1-5, top_flag 0-3 and their parameters
+        (.turbb Amplitude; Vmax, x0, sigma, phi, Lund_ix, Lund_deltai, Lund_T)."""
         b = BLBoundary(inflow_flag, top_flag, amplitude, seed, vbs_max, x_bs, sigma_bs, phi_bs, i_rescale,
-                       beta_hartree)
+                       beta_hartree, delta_inlet, t_resc)
         _check(lib().bl_set_boundary(self.h, ctypes.byref(b)), "bl_set_boundary")
 
     def compute_dt(self):

@@ -652,7 +652,7 @@ void free_all(Handle &h) {
     }
     for (void *p : {(void *)h.F1.u, (void *)h.F1.v, (void *)h.F1.w, (void *)h.P, (void *)h.phi,
                     (void *)h.nu_t, (void *)h.T1, (void *)h.S, (void *)h.dct_w1, (void *)h.dct_w2,
-                    (void *)h.dct_wN, (void *)h.dct_rot, (void *)h.dct_mf, h.mx2, (void *)h.zc, (void *)h.zs, (void *)h.zf, (void *)h.bc_extra, (void *)h.lu_dl, (void *)h.lu_dp, (void *)h.lu_du,
+                    (void *)h.dct_wN, (void *)h.dct_rot, (void *)h.dct_mf, h.mx2, (void *)h.zc, (void *)h.zs, (void *)h.zf, (void *)h.bc_extra, (void *)h.lund_mem, (void *)h.lu_dl, (void *)h.lu_dp, (void *)h.lu_du,
                     (void *)h.lu00, (void *)h.th_co,
                     (void *)h.scal, (void *)h.red, (void *)h.dsm, (void *)h.smag_c, h.mem_grid, h.mem_gridf, (void *)h.mx_tab, (void *)h.zmf,
                     (void *)h.p00, (void *)h.inc1, (void *)h.stats, (void *)h.cl_w, (void *)h.cl_b, (void *)h.cl_cnt})
@@ -693,6 +693,9 @@ void do_step_t(Handle &h) {
     const int order3[3] = {Y, Z, Y};
     int cur = X;
     h.bc.istep = h.bc_step;  // the inflow_flag 4 draw of this step
+    // Lund rescaling: the step's inflow plane from the step-start field
+    if (h.bc.inflow_flag == 3 || h.bc.inflow_flag == 5) launch_lund<T>(h, old, dt);
+    h.lund_pending = false;
     for (int s = 1; s <= nst; ++s) {
         const int nxt = order3[s - 1];
         // eddy viscosity of the stage's input fields (time_integration.f90:341-346)
@@ -908,8 +911,13 @@ int bl_set_boundary(bl_handle *hh, const bl_boundary *b) {
             throw ArgError();
         };
         if (!b) bad("bl_set_boundary: null flags");
-        if (b->inflow_flag != 1 && b->inflow_flag != 2 && b->inflow_flag != 4)
-            bad("bl_set_boundary: inflow_flag must be 1, 2 or 4 (3 and 5, Lund rescaling, are not built)");
+        if (b->inflow_flag < 1 || b->inflow_flag > 5) bad("bl_set_boundary: inflow_flag must be 1 to 5");
+        const bool lund = b->inflow_flag == 3 || b->inflow_flag == 5;
+        if (lund && (b->i_rescale < 2 || b->i_rescale > d.nx - 1))
+            bad("bl_set_boundary: Lund rescaling needs i_rescale in [2, nx - 1]");
+        if (lund && !(b->delta_inlet > 0.0 && std::isfinite(b->delta_inlet) && b->t_resc > 0.0 &&
+                      std::isfinite(b->t_resc)))
+            bad("bl_set_boundary: Lund rescaling needs delta_inlet > 0 and t_resc > 0");
         if (b->top_flag < 0 || b->top_flag > 3) bad("bl_set_boundary: top_flag must be 0, 1, 2 or 3");
         if (b->inflow_flag == 4 && !(std::isfinite(b->amplitude) && b->amplitude >= 0.0))
             bad("bl_set_boundary: amplitude must be finite and >= 0");
@@ -954,6 +962,25 @@ int bl_set_boundary(bl_handle *hh, const bl_boundary *b) {
         h.fs_uc_pending = b->top_flag == 3;
         h.fs_uc = uc_fs;
         h.inc_ok = false;
+        if (lund) {
+            if (!h.lund_mem) {
+                h.lund_mem = dmalloc(lund_doubles(d));
+                BL_HIP(hipMemsetAsync(h.lund_mem, 0, lund_doubles(d) * sizeof(double), h.stream));
+                h.lund = lund_layout(d, h.lund_mem);
+            }
+            h.lund_ir = b->i_rescale - 1;
+            h.lund_t_resc = b->t_resc;
+            h.lund_delta_inlet = b->delta_inlet;
+            h.bc.lund_u = h.lund.pu;
+            h.bc.lund_v = h.lund.pv;
+            h.bc.lund_w = h.lund.pw;
+            if (h.precision == BL_PREC_MIXED)
+                lund_init_means<float>(h, as<float>(h.buf[h.icur]));
+            else
+                lund_init_means<double>(h, as<double>(h.buf[h.icur]));
+            BL_HIP(hipStreamSynchronize(h.stream));
+        }
+        h.lund_pending = lund;
     });
 }
 
@@ -1147,6 +1174,15 @@ int bl_apply_bc(bl_handle *hh, int rk_step, double dt) {
         if (rk_step < 1 || rk_step > 3) throw ArgError();
         h.inc_ok = false;
         h.bc.istep = h.bc_step;
+        if (h.lund_pending) {
+            // the first boundary call after set-up computes the rescaled plane
+            // from the old fields with the handle's dt (step_beginning)
+            if (h.precision == BL_PREC_MIXED)
+                launch_lund<float>(h, as<float>(h.buf[h.iold]), h.dt);
+            else
+                launch_lund<double>(h, as<double>(h.buf[h.iold]), h.dt);
+            h.lund_pending = false;
+        }
         if (h.precision == BL_PREC_MIXED)
             bc_on<float>(h, h.buf[h.icur], h.buf[h.iold], rk_step, dt, h.t);
         else

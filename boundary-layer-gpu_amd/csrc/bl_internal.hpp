@@ -88,6 +88,18 @@ struct MetricsT {
 };
 using Metrics = MetricsT<double>;
 
+// Lund rescaling state and work arrays (bl_lund.hip), FP64 on the device;
+// rows per grid: cell rows (nyg) and face rows (ny)
+struct LundArrays {
+    double *umr, *umi, *vmr, *vmi;      // z means of the step-start field (rescaling plane, inlet)
+    double *umrT, *umiT, *vmrT, *vmiT;  // their time averages (Umean_resc_T, ..., global.f90)
+    double *scal;                       // <UV> at the wall, gamma, U_inf
+    double *cgi, *cgo, *ugi, *ugo, *wg; // cell rows: interpolation weights, mean profiles, blend weight
+    double *cfi, *cfo, *vfi, *vfo, *wf; // face rows: the same for V
+    int *jgi, *jgo, *jfi, *jfo;         // interpolation rows (inner, outer) per target row
+    double *pu, *pv, *pw;               // the inflow plane: U(nyg, nzg), V(ny, nzg), W(nyg, nz)
+};
+
 struct BCTables {
     const double *U_inlet, *V_inlet, *W_inlet, *U_top, *V_top, *W_top;
     int n_modes, m_modes;
@@ -101,6 +113,7 @@ struct BCTables {
     const double *utop_fs = nullptr;  // top_flag 3: Falkner-Skan U(i, nyg) (nx)
     double dy_dx = 0.0;             // top_flag 1, 2: (yg(nyg) - yg(nyg-1)) / dxmin
     int i_rescale = 0;              // top_flag 1, 2: columns kept at U/V/W_top
+    const double *lund_u = nullptr, *lund_v = nullptr, *lund_w = nullptr;  // inflow_flag 3, 5: the step's plane
 };
 
 template <class T>
@@ -155,6 +168,12 @@ struct Handle {
     bool fs_uc_pending = false;     // top_flag 3: U_top(nx) changes after the next BC call
     double fs_uc = 0.0;
     std::vector<double> hx, hxg, hyg;  // host copies of x, xg, yg
+    // Lund rescaling (inflow_flag 3, 5)
+    double *lund_mem = nullptr;
+    LundArrays lund{};
+    int lund_ir = 0;                   // rescaling plane, 0-based
+    double lund_t_resc = 0.0, lund_delta_inlet = 0.0;
+    bool lund_pending = false;         // the next boundary call starts a step (step_beginning)
     double dxmin, dymin, dzmin;  // smallest centre spacings (initialization.f90:220-222)
     double rk_t[4];
     double rk_coef[4][4];
@@ -316,6 +335,15 @@ struct ZTables {
 // the packed half spectrum (rows 0, 2k-1, 2k), or its unnormalised inverse
 void launch_zdft(hipStream_t s, const ZTables &z, int C, long long plane, bool forward, const double *in,
                  double *out);
+// ---- Lund rescaling (bl_lund.hip) ----
+size_t lund_doubles(const Dims &d);
+LundArrays lund_layout(const Dims &d, double *base);
+// the step's inflow plane from the step-start field `old` (updates the time averages)
+template <class T>
+void launch_lund(Handle &h, const Fld3<T> &old, double dt);
+// the time averages start as the means of f
+template <class T>
+void lund_init_means(Handle &h, const Fld3<T> &f);
 // ---- statistics and snapshots (bl_stats.hip) ----
 long long stats_size(const Dims &d);
 // compute_statistics' z means of f (and the handle's P, nu_t) into out

@@ -407,7 +407,21 @@ __global__ __launch_bounds__(256) void k_bc_inout(Dims d, Metrics m, BCTables b,
     if (j < d.nyg) {
     const int nx = d.nx, nxg = d.nxg;
     double ui, vi, wi;
-    inflow_values(d, m, b, j, k, t, b.istep, ui, vi, wi);
+    const bool lund = b.inflow_flag == 3 || b.inflow_flag == 5;
+    if (lund) {
+        // the step's rescaled plane (:569-597); flag 5 adds it to the inlet
+        // means, with W's planes shifted by 16 (:512-559)
+        const bool f5 = b.inflow_flag == 5;
+        ui = (f5 ? b.U_inlet[j] : 0.0) + b.lund_u[j + (long long)d.nyg * k];
+        wi = 0.0;
+        if (k < d.nz) {
+            const int ks = f5 ? (k + 17) % d.nz : k;
+            wi = (f5 ? b.W_inlet[j] : 0.0) + b.lund_w[j + (long long)d.nyg * ks];
+        }
+        vi = (j < d.ny) ? (f5 ? b.V_inlet[j] : 0.0) + b.lund_v[j + (long long)d.ny * k] : 0.0;
+    } else {
+        inflow_values(d, m, b, j, k, t, b.istep, ui, vi, wi);
+    }
     const long long p0 = fidx(d, 0, j, k);
     // U (all j < nyg, k < nzg)
     const double uin = (double)(T)ui;  // the value stored
@@ -416,10 +430,11 @@ __global__ __launch_bounds__(256) void k_bc_inout(Dims d, Metrics m, BCTables b,
         // FP64 increment of the inflow face over the step-start value: that
         // value is taken unrounded when it is the stored boundary value of
         // t_old (the previous step), as it is after any step
-        double uo, vo, wo;
-        inflow_values(d, m, b, j, k, t_old, b.istep - 1, uo, vo, wo);
+        double uo = 0.0, vo, wo;
+        if (!lund) inflow_values(d, m, b, j, k, t_old, b.istep - 1, uo, vo, wo);
         const double so = (double)Uo[p0];
-        inc1[pl] = ui - ((double)(T)uo == so ? uo : so);
+        // (the previous step's rescaled plane is gone: the stored value)
+        inc1[pl] = ui - (!lund && (double)(T)uo == so ? uo : so);
     }
     U[p0] = (T)uin;
     {

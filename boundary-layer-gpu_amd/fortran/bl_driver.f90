@@ -207,7 +207,7 @@ contains
     if (iwall_model /= 0) stop 'Error: only WM = 0 is built'
     if (les_model < 0 .or. les_model > 3) stop 'Error: LES must be 0, 1, 2 or 3'
     if (iprecision /= 0 .and. iprecision /= 1) stop 'Error: precision must be 0 (FP64) or 1 (mixed)'
-    if (inflow_flag == 3 .or. inflow_flag == 5) stop 'Error: turbulent (Lund) inflow is not built'
+    if (inflow_flag == 3 .or. inflow_flag == 5) stop 'Error: Lund inflow (3, 5) is not wired into this driver'
     if (inflow_flag /= 1 .and. inflow_flag /= 2 .and. inflow_flag /= 4) stop 'Error: inflow_flag must be 1, 2 or 4'
     if (top_flag < 0 .or. top_flag > 3) stop 'Error: top_flag must be 0, 1, 2 or 3'
   end subroutine read_parameters
@@ -627,6 +627,8 @@ program bl_driver
      bnd%phi_bs = phi_bs
      bnd%i_rescale = i_rescale
      bnd%beta_hartree = beta_hartree
+     bnd%delta_inlet = 0d0
+     bnd%t_resc = 0d0
      call bl_check(bl_set_boundary(h, bnd), 'bl_set_boundary')
   end if
   call bl_check(bl_upload(h, BL_FIELD_U, U), 'bl_upload U')

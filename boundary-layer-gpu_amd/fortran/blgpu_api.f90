@@ -35,6 +35,7 @@ module blgpu_api
      real(c_double)       :: vbs_max, x_bs, sigma_bs, phi_bs
      integer(c_int)       :: i_rescale
      real(c_double)       :: beta_hartree
+     real(c_double)       :: delta_inlet, t_resc
   end type bl_boundary
 
   interface

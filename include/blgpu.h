@@ -115,8 +115,17 @@ int bl_get_time(bl_handle *h, double *t);
  *     the reference, U_top takes the Falkner-Skan values at the first
  *     boundary-condition call, so the outflow's convective velocity U_top(nx)
  *     changes from the next call on.
- * Inflow flags 3 and 5 (Lund rescaling) and wall models are not built and
- * are rejected.  Resets the step counter of the inflow_flag 4 stream. */
+ *   inflow_flag 3: Lund rescaled inflow (apply_inflow_bc_x_rescaling :569-597,
+ *     compute_rescaled_inflow rescaled_inlet_bc.f90:60-300): once per step,
+ *     from the step-start field, the plane i_rescale (.turbb Lund_ix,
+ *     1-based) rescaled in inner (u_tau) and outer (delta99 against
+ *     delta_inlet = Lund_deltai) coordinates, with the means averaged in time
+ *     over t_resc (Lund_T) from the field's means at this call
+ *     (initialization.f90:640-660); inflow_flag 5: its fluctuations only, on
+ *     the U/V/W_inlet means (:512-559).  The first boundary call after this
+ *     one (bl_apply_bc) or the next bl_step computes the plane.
+ * Wall models (WM > 0) are not built.  Resets the step counter of the
+ * inflow_flag 4 stream. */
 typedef struct bl_boundary {
     int inflow_flag, top_flag;
     double amplitude;
@@ -124,6 +133,7 @@ typedef struct bl_boundary {
     double vbs_max, x_bs, sigma_bs, phi_bs;
     int i_rescale;
     double beta_hartree;
+    double delta_inlet, t_resc;
 } bl_boundary;
 int bl_set_boundary(bl_handle *h, const bl_boundary *b);
 

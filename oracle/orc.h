@@ -66,7 +66,16 @@ void orc_step(orc_state *s);
  * step, component, j, k) in place of gfortran's random_number.  Returns 0, or
  * -1 for flags outside those.  Resets the step counter. */
 int orc_set_boundary(orc_state *s, int inflow_flag, int top_flag, double amplitude, unsigned long long seed,
-                     double vbs_max, double x_bs, double sigma_bs, double phi_bs, int i_rescale, double beta_hartree);
+                     double vbs_max, double x_bs, double sigma_bs, double phi_bs, int i_rescale, double beta_hartree,
+                     double delta_inlet, double t_resc);
+/* inflow_flag 3 (Lund rescaling, apply_inflow_bc_x_rescaling
+ * boundary_conditions.f90:569-597) and 5 (rescaled fluctuations on the
+ * U/V/W_inlet mean, :512-559) via compute_rescaled_inflow
+ * (rescaled_inlet_bc.f90:60-300), once per step from the step-start field;
+ * the time-averaged means start from the field at orc_set_boundary
+ * (initialization.f90:640-660).  Lund means (out): Umean_resc_T(nyg),
+ * Vmean_resc_T(ny), Umean_inlet_T(nyg), Vmean_inlet_T(ny). */
+void orc_lund_means(orc_state *s, double *umr, double *vmr, double *umi, double *vmi);
 
 /* compute_rhs_u/v/w (equations.f90:24-415) on the current U,V,W into
  * full-extent arrays ru(nx,nyg,nzg), rv(nxg,ny,nzg), rw(nxg,nyg,nz)

@@ -182,7 +182,8 @@ void orc_destroy(orc_state *s) {
                       s->V_inlet, s->W_inlet, s->U_top, s->V_top, s->W_top, s->zmode,
                       s->tmode, s->qu_re, s->qu_im, s->qv_re, s->qv_im, s->qw_re, s->qw_im,
                       s->Ut, s->Vt, s->Wt, s->U, s->V, s->W, s->P, s->Uo, s->Vo, s->Wo,
-                      s->nu_t, s->avg_nu_t, s->rhs_p};
+                      s->nu_t, s->avg_nu_t, s->rhs_p, s->Umean_resc_T, s->Vmean_resc_T, s->Umean_inlet_T,
+                      s->Vmean_inlet_T, s->Ut_in, s->Vt_in, s->Wt_in};
     for (size_t i = 0; i < sizeof(ptrs) / sizeof(ptrs[0]); i++) free(ptrs[i]);
     for (int st = 1; st <= 3; st++) {
         free(s->Fu[st]);
@@ -571,9 +572,13 @@ void orc_rhs(orc_state *s, double *ru, double *rv, double *rw) {
 }
 
 int orc_set_boundary(orc_state *s, int inflow_flag, int top_flag, double amplitude, unsigned long long seed,
-                     double vbs_max, double x_bs, double sigma_bs, double phi_bs, int i_rescale, double beta_hartree) {
-    if (inflow_flag != 1 && inflow_flag != 2 && inflow_flag != 4) return -1;
+                     double vbs_max, double x_bs, double sigma_bs, double phi_bs, int i_rescale, double beta_hartree,
+                     double delta_inlet, double t_resc) {
+    if (inflow_flag < 1 || inflow_flag > 5) return -1;
     if (top_flag < 0 || top_flag > 3) return -1;
+    if ((inflow_flag == 3 || inflow_flag == 5) &&
+        (i_rescale < 2 || i_rescale > s->nx - 1 || !(delta_inlet > 0.0) || !(t_resc > 0.0)))
+        return -1;
     s->inflow_flag = inflow_flag;
     s->top_flag = top_flag;
     s->amp = amplitude;
@@ -584,9 +589,216 @@ int orc_set_boundary(orc_state *s, int inflow_flag, int top_flag, double amplitu
     s->phi_bs = phi_bs;
     s->i_rescale = i_rescale;
     s->beta_hartree = beta_hartree;
+    s->delta_inlet = delta_inlet;
+    s->t_resc = t_resc;
     s->bc_step = 0;
     s->step_beginning = 1;
+    if (inflow_flag == 3 || inflow_flag == 5) {
+        int nyg = s->nyg, ny = s->ny, nzg = s->nzg;
+        if (!s->Umean_resc_T) {
+            s->Umean_resc_T = dalloc(nyg);
+            s->Vmean_resc_T = dalloc(ny);
+            s->Umean_inlet_T = dalloc(nyg);
+            s->Vmean_inlet_T = dalloc(ny);
+            s->Ut_in = dalloc((size_t)nyg * nzg);
+            s->Vt_in = dalloc((size_t)ny * nzg);
+            s->Wt_in = dalloc((size_t)nyg * s->nz);
+        }
+        /* first-time means from the field, initialization.f90:640-660 */
+        const double *U = s->U, *V = s->V;
+        for (int j = 1; j <= nyg; j++) {
+            double a = 0.0, b = 0.0;
+            for (int k = 2; k <= nzg - 1; k++) {
+                a += U[IU(i_rescale, j, k)];
+                b += U[IU(1, j, k)];
+            }
+            s->Umean_resc_T[j - 1] = a / (double)(nzg - 2);
+            s->Umean_inlet_T[j - 1] = b / (double)(nzg - 2);
+        }
+        for (int j = 1; j <= ny; j++) {
+            double a = 0.0, b = 0.0;
+            for (int k = 2; k <= nzg - 1; k++) {
+                a += V[IV(i_rescale, j, k)];
+                b += V[IV(1, j, k)];
+            }
+            s->Vmean_resc_T[j - 1] = a / (double)(nzg - 2);
+            s->Vmean_inlet_T[j - 1] = b / (double)(nzg - 2);
+        }
+    }
     return 0;
+}
+
+void orc_lund_means(orc_state *s, double *umr, double *vmr, double *umi, double *vmi) {
+    if (!s->Umean_resc_T) return;
+    memcpy(umr, s->Umean_resc_T, sizeof(double) * s->nyg);
+    memcpy(vmr, s->Vmean_resc_T, sizeof(double) * s->ny);
+    memcpy(umi, s->Umean_inlet_T, sizeof(double) * s->nyg);
+    memcpy(vmi, s->Vmean_inlet_T, sizeof(double) * s->ny);
+}
+
+/* interp_vel (rescaled_inlet_bc.f90:305-345) of `cols` columns of a profile
+ * f(n, cols) given at y_ to the points y_i (both n long), rows 2..n; row 1
+ * (the ghost, left unset there and overwritten by the wall condition) is 0 */
+static void interp_vel(int n, int cols, const double *f, const double *y_, const double *y_i, double *out) {
+    for (int c = 0; c < cols; c++) out[(size_t)n * c] = 0.0;
+    for (int i = 2; i <= n; i++) {
+        int j = 0;
+        for (int k = 1; k <= n; k++)
+            if (y_[k - 1] > y_i[i - 1]) {
+                j = k;
+                break;
+            }
+        for (int c = 0; c < cols; c++) {
+            double *o = out + (size_t)n * c;
+            const double *g = f + (size_t)n * c;
+            if (j > 1) {
+                double coef = (y_[j - 1] - y_i[i - 1]) / (y_[j - 1] - y_[j - 2]);
+                o[i - 1] = coef * g[j - 2] + (1.0 - coef) * g[j - 1];
+            } else {
+                o[i - 1] = g[n - 1]; /* default: the top value (j = 1 stops the reference) */
+            }
+        }
+    }
+}
+
+/* the first j (1-based) with m(j) >= level, nyg/2 if below 2, and the
+ * interpolated height (rescaled_inlet_bc.f90:144-175) */
+static double height99(int nyg, const double *m, const double *yg, double level) {
+    int jref = 0;
+    for (int j = 1; j <= nyg; j++)
+        if (m[j - 1] >= level) {
+            jref = j;
+            break;
+        }
+    if (jref < 2) jref = nyg / 2;
+    double w1 = (level - m[jref - 2]) / (m[jref - 1] - m[jref - 2]);
+    double w0 = 1.0 - w1;
+    double d = w1 * yg[jref - 1] + w0 * yg[jref - 2];
+    return d < 0 ? fabs(d) : d;
+}
+
+static double theta_of(int nyg, const double *m, const double *yg, double uinf) {
+    double th = 0.0;
+    for (int j = 2; j <= nyg; j++) {
+        double a = m[j - 1] / uinf * (1.0 - m[j - 1] / uinf), b = m[j - 2] / uinf * (1.0 - m[j - 2] / uinf);
+        th = th + 0.5 * (a + b) * (yg[j - 1] - yg[j - 2]);
+    }
+    return th < 0 ? fabs(th) : th;
+}
+
+/* compute_rescaled_inflow (rescaled_inlet_bc.f90:60-300) from the
+ * step-start field into Ut_in(nyg,nzg), Vt_in(ny,nzg), Wt_in(nyg,nz) */
+static void lund_inflow(orc_state *s, int iflag_fluc) {
+    int ny = s->ny, nz = s->nz, nyg = s->nyg, nzg = s->nzg, ir = s->i_rescale;
+    const double *U = s->Uo, *V = s->Vo, *W = s->Wo;
+    double nu = s->nu, dt = s->dt, tr = s->t_resc;
+    double *umr = dalloc(nyg), *vmr = dalloc(ny), *umi = dalloc(nyg), *vmi = dalloc(ny);
+    for (int j = 1; j <= nyg; j++) {
+        double a = 0.0, b = 0.0;
+        for (int k = 2; k <= nzg - 1; k++) {
+            a += U[IU(ir, j, k)];
+            b += U[IU(1, j, k)];
+        }
+        umr[j - 1] = a / (double)(nzg - 2);
+        umi[j - 1] = b / (double)(nzg - 2);
+    }
+    for (int j = 1; j <= ny; j++) {
+        double a = 0.0, b = 0.0;
+        for (int k = 2; k <= nzg - 1; k++) {
+            a += V[IV(ir, j, k)];
+            b += V[IV(1, j, k)];
+        }
+        vmr[j - 1] = a / (double)(nzg - 2);
+        vmi[j - 1] = b / (double)(nzg - 2);
+    }
+    double Uinf = umr[nyg - 1];
+    for (int j = 0; j < nyg; j++) {
+        s->Umean_resc_T[j] = dt / tr * umr[j] + (1.0 - dt / tr) * s->Umean_resc_T[j];
+        s->Umean_inlet_T[j] = dt / tr * umi[j] + (1.0 - dt / tr) * s->Umean_inlet_T[j];
+    }
+    for (int j = 0; j < ny; j++) {
+        s->Vmean_resc_T[j] = dt / tr * vmr[j] + (1.0 - dt / tr) * s->Vmean_resc_T[j];
+        s->Vmean_inlet_T[j] = dt / tr * vmi[j] + (1.0 - dt / tr) * s->Vmean_inlet_T[j];
+    }
+    /* fluctuations of the rescaling plane (W has no mean removed) */
+    double *uf = dalloc((size_t)nyg * nzg), *vf = dalloc((size_t)ny * nzg), *wf = dalloc((size_t)nyg * nz);
+    for (int k = 1; k <= nzg; k++)
+        for (int j = 1; j <= nyg; j++) uf[(j - 1) + (size_t)nyg * (k - 1)] = U[IU(ir, j, k)] - s->Umean_resc_T[j - 1];
+    for (int k = 1; k <= nzg; k++)
+        for (int j = 1; j <= ny; j++) vf[(j - 1) + (size_t)ny * (k - 1)] = V[IV(ir, j, k)] - s->Vmean_resc_T[j - 1];
+    for (int k = 1; k <= nz; k++)
+        for (int j = 1; j <= nyg; j++) wf[(j - 1) + (size_t)nyg * (k - 1)] = W[IW(ir, j, k)];
+    double delta_resc = height99(nyg, s->Umean_resc_T, s->yg, 0.99 * Uinf);
+    double theta_resc = theta_of(nyg, s->Umean_resc_T, s->yg, Uinf);
+    double theta_inlet = theta_of(nyg, s->Umean_inlet_T, s->yg, Uinf);
+    double uvw = 0.0;
+    for (int k = 2; k <= nzg - 1; k++) uvw += 0.5 * (U[IU(ir, 1, k)] + U[IU(ir, 2, k)]) * V[IV(ir, 1, k)];
+    uvw = uvw / (double)(nzg - 2);
+    double dU = (s->Umean_resc_T[1] - s->Umean_resc_T[0]) / (YG(2) - YG(1));
+    double utau_resc = pow(fabs(nu * dU - uvw), 0.5);
+    double utau_inlet = utau_resc * pow(theta_resc / theta_inlet, 1.0 / 8.0);
+    double gamma = utau_inlet / utau_resc;
+    double *ygpi = dalloc(nyg), *ygpr = dalloc(nyg), *etgi = dalloc(nyg), *etgr = dalloc(nyg), *weig = dalloc(nyg);
+    double *ypi = dalloc(ny), *ypr = dalloc(ny), *eti = dalloc(ny), *etr = dalloc(ny), *wei = dalloc(ny);
+    double alpha = 4.0, b = 0.2;
+    for (int j = 0; j < nyg; j++) {
+        ygpi[j] = utau_inlet * s->yg[j] / nu;
+        ygpr[j] = utau_resc * s->yg[j] / nu;
+        etgi[j] = s->yg[j] / s->delta_inlet;
+        etgr[j] = s->yg[j] / delta_resc;
+        weig[j] = 0.5 * (1.0 + tanh(alpha * (etgi[j] - b) / ((1.0 - 2.0 * b) * etgi[j] + b)) / tanh(alpha));
+    }
+    for (int j = 0; j < ny; j++) {
+        ypi[j] = utau_inlet * s->y[j] / nu;
+        ypr[j] = utau_resc * s->y[j] / nu;
+        eti[j] = s->y[j] / s->delta_inlet;
+        etr[j] = s->y[j] / delta_resc;
+        wei[j] = 0.5 * (1.0 + tanh(alpha * (eti[j] - b) / ((1.0 - 2.0 * b) * eti[j] + b)) / tanh(alpha));
+    }
+    double *umi_in = dalloc(nyg), *umi_out = dalloc(nyg), *vmi_in = dalloc(ny), *vmi_out = dalloc(ny);
+    interp_vel(nyg, 1, s->Umean_resc_T, ygpr, ygpi, umi_in);
+    interp_vel(nyg, 1, s->Umean_resc_T, etgr, etgi, umi_out);
+    for (int j = 0; j < nyg; j++) {
+        umi_in[j] = gamma * umi_in[j];
+        umi_out[j] = gamma * umi_out[j] + (1.0 - gamma) * Uinf;
+    }
+    interp_vel(ny, 1, s->Vmean_resc_T, ypr, ypi, vmi_in);
+    interp_vel(ny, 1, s->Vmean_resc_T, etr, eti, vmi_out);
+    double *ufi = dalloc((size_t)nyg * nzg), *ufo = dalloc((size_t)nyg * nzg);
+    double *vfi = dalloc((size_t)ny * nzg), *vfo = dalloc((size_t)ny * nzg);
+    double *wfi = dalloc((size_t)nyg * nz), *wfo = dalloc((size_t)nyg * nz);
+    interp_vel(nyg, nzg, uf, ygpr, ygpi, ufi);
+    interp_vel(nyg, nzg, uf, etgr, etgi, ufo);
+    interp_vel(ny, nzg, vf, ypr, ypi, vfi);
+    interp_vel(ny, nzg, vf, etr, eti, vfo);
+    interp_vel(nyg, nz, wf, ygpr, ygpi, wfi);
+    interp_vel(nyg, nz, wf, etgr, etgi, wfo);
+    if (iflag_fluc == 1) {
+        for (int j = 0; j < nyg; j++) umi_in[j] = umi_out[j] = 0.0;
+        for (int j = 0; j < ny; j++) vmi_in[j] = vmi_out[j] = 0.0;
+    }
+    for (int k = 0; k < nzg; k++)
+        for (int j = 0; j < nyg; j++) {
+            size_t q = j + (size_t)nyg * k;
+            s->Ut_in[q] = (umi_in[j] + gamma * ufi[q]) * (1.0 - weig[j]) + (umi_out[j] + gamma * ufo[q]) * weig[j];
+        }
+    for (int k = 0; k < nz; k++)
+        for (int j = 0; j < nyg; j++) {
+            size_t q = j + (size_t)nyg * k;
+            s->Wt_in[q] = (gamma * wfi[q]) * (1.0 - weig[j]) + (gamma * wfo[q]) * weig[j];
+        }
+    for (int k = 0; k < nzg; k++)
+        for (int j = 0; j < ny; j++) {
+            size_t q = j + (size_t)ny * k;
+            s->Vt_in[q] = (vmi_in[j] + gamma * vfi[q]) * (1.0 - wei[j]) + (vmi_out[j] + gamma * vfo[q]) * wei[j];
+        }
+    /* row 1 (the wall ghost, unset by interp_vel and overwritten by the wall
+     * condition) is 0 */
+    for (int k = 0; k < nzg; k++) s->Ut_in[(size_t)nyg * k] = s->Vt_in[(size_t)ny * k] = 0.0;
+    for (int k = 0; k < nz; k++) s->Wt_in[(size_t)nyg * k] = 0.0;
+    double *fr[] = {umr, vmr, umi, vmi, uf, vf, wf, ygpi, ygpr, etgi, etgr, weig, ypi, ypr, eti, etr, wei,
+                    umi_in, umi_out, vmi_in, vmi_out, ufi, ufo, vfi, vfo, wfi, wfo};
+    for (size_t q = 0; q < sizeof(fr) / sizeof(fr[0]); q++) free(fr[q]);
 }
 
 /* the random_number of apply_inflow_bc_x_Blasius_random: a uniform [0, 1)
@@ -650,7 +862,27 @@ void orc_apply_bc(orc_state *s, int rk_step, double dt) {
             }
     }
     /* inflow, :183-190 */
-    if (s->inflow_flag == 4) {
+    if (s->inflow_flag == 3 || s->inflow_flag == 5) {
+        /* Lund rescaling once per step from the step-start field, :569-597
+         * (flag 3) and :512-559 (flag 5, on the inlet means, W shifted by 16
+         * planes in z) */
+        if (s->step_beginning == 1) {
+            lund_inflow(s, s->inflow_flag == 5 ? 1 : 0);
+            int fl = s->inflow_flag == 5;
+            for (int k = 1; k <= nzg; k++)
+                for (int j = 1; j <= nyg; j++)
+                    U[IU(1, j, k)] = (fl ? s->U_inlet[j - 1] : 0.0) + s->Ut_in[(j - 1) + (size_t)nyg * (k - 1)];
+            for (int k = 1; k <= nz; k++) {
+                int ks = fl ? (k + 16) % nz + 1 : k;
+                for (int j = 1; j <= nyg; j++)
+                    W[IW(1, j, k)] = (fl ? s->W_inlet[j - 1] : 0.0) + s->Wt_in[(j - 1) + (size_t)nyg * (ks - 1)];
+            }
+            for (int k = 1; k <= nzg; k++)
+                for (int j = 1; j <= ny; j++)
+                    V[IV(1, j, k)] = (fl ? s->V_inlet[j - 1] : 0.0) + s->Vt_in[(j - 1) + (size_t)ny * (k - 1)];
+            s->step_beginning = 0;
+        }
+    } else if (s->inflow_flag == 4) {
         /* Blasius + random perturbations once per step, :423-447 */
         if (s->step_beginning == 1) {
             for (int j = 1; j <= nyg; j++) {

@@ -35,6 +35,11 @@ struct orc_state {
     double amp, vbs_max, x_bs, sigma_bs, phi_bs, beta_hartree;
     unsigned long long seed;
     long long bc_step;
+    /* Lund rescaling (inflow_flag 3, 5): parameters, the time-averaged mean
+     * profiles (global.f90) and the rescaled inflow plane of the step */
+    double delta_inlet, t_resc;
+    double *Umean_resc_T, *Vmean_resc_T, *Umean_inlet_T, *Vmean_inlet_T;
+    double *Ut_in, *Vt_in, *Wt_in;
 };
 
 /* ---- 1-based index helpers (Fortran allocation shapes, x fastest) ---- */

@@ -39,7 +39,8 @@ def lib():
             "orc_apply_periodic_z": (None, [_P]),
             "orc_compute_dt": (_D, [_P]),
             "orc_step": (None, [_P]),
-            "orc_set_boundary": (_I, [_P, _I, _I, _D, ctypes.c_ulonglong, _D, _D, _D, _D, _I, _D]),
+            "orc_set_boundary": (_I, [_P, _I, _I, _D, ctypes.c_ulonglong, _D, _D, _D, _D, _I, _D, _D, _D]),
+            "orc_lund_means": (None, [_P, _P, _P, _P, _P]),
             "orc_rhs": (None, [_P, _P, _P, _P]),
             "orc_apply_bc": (None, [_P, _I, _D]),
             "orc_set_old": (None, [_P, _P, _P, _P]),
@@ -209,11 +210,17 @@ class Oracle:
         lib().orc_step(self.h)
 
     def set_boundary(self, inflow_flag=1, top_flag=0, amplitude=0.0, seed=0, vbs_max=0.0, x_bs=0.0, sigma_bs=1.0,
-                     phi_bs=0.0, i_rescale=0, beta_hartree=0.0):
+                     phi_bs=0.0, i_rescale=0, beta_hartree=0.0, delta_inlet=0.0, t_resc=0.0):
         rc = lib().orc_set_boundary(self.h, inflow_flag, top_flag, amplitude, seed, vbs_max, x_bs, sigma_bs, phi_bs,
-                                    i_rescale, beta_hartree)
+                                    i_rescale, beta_hartree, delta_inlet, t_resc)
         if rc != 0:
             raise ValueError("orc_set_boundary: unsupported flags")
+
+    def lund_means(self):
+        nyg, ny = self.shapes[0][1], self.shapes[1][1]
+        a, b, c, d = np.zeros(nyg), np.zeros(ny), np.zeros(nyg), np.zeros(ny)
+        lib().orc_lund_means(self.h, _p(a), _p(b), _p(c), _p(d))
+        return a, b, c, d
 
     def rhs(self):
         ru = np.zeros(self.shapes[0], order="F")
