@@ -113,6 +113,8 @@ module bl_state
   ! key of this build
   real(real64) :: vbs_max = 0d0, x_bs = 0d0, sigma_bs = 1d0, phi_bs = 0d0, amplitude = 0d0, beta_hartree = 0d0
   integer :: i_rescale = 0
+  ! Lund rescaling (params.f90:144-146)
+  real(real64) :: delta_inlet = 0d0, t_resc = 0d0
   integer :: nstep_init = 0, nstep_init_input = -45, istep = 0
   real(real64) :: nu = 0d0, cfl = -1d-4, dPdx = 0d0, dPdz = 0d0, t = 0d0, dt = 0d0
   real(real64) :: boxsize(4) = 0d0
@@ -179,6 +181,8 @@ contains
     call get_dbl('sigma', sigma_bs, f)
     call get_dbl('phi', phi_bs, f)
     call get_int('Lund_ix', i_rescale, f)
+    call get_dbl('Lund_deltai', delta_inlet, f)
+    call get_dbl('Lund_T', t_resc, f)
     call get_dbl('Amplitude', amplitude, f)
     call get_dbl('beta_hartree', beta_hartree, f)
     call get_str('inflow_file', file_inflow, f)
@@ -207,8 +211,9 @@ contains
     if (iwall_model /= 0) stop 'Error: only WM = 0 is built'
     if (les_model < 0 .or. les_model > 3) stop 'Error: LES must be 0, 1, 2 or 3'
     if (iprecision /= 0 .and. iprecision /= 1) stop 'Error: precision must be 0 (FP64) or 1 (mixed)'
-    if (inflow_flag == 3 .or. inflow_flag == 5) stop 'Error: Lund inflow (3, 5) is not wired into this driver'
-    if (inflow_flag /= 1 .and. inflow_flag /= 2 .and. inflow_flag /= 4) stop 'Error: inflow_flag must be 1, 2 or 4'
+    if (inflow_flag < 1 .or. inflow_flag > 5) stop 'Error: inflow_flag must be 1 to 5'
+    if ((inflow_flag == 3 .or. inflow_flag == 5) .and. random_init == 1) &
+         stop 'Error: Lund inflow (3, 5) starts from a flow field file (init_rand = 0)'
     if (top_flag < 0 .or. top_flag > 3) stop 'Error: top_flag must be 0, 1, 2 or 3'
   end subroutine read_parameters
 
@@ -223,6 +228,32 @@ contains
     allocate(Cf(nx), Umean(nx, ny), Vmean(nx, ny), Wmean(nx, ny), U2mean(nx, ny), V2mean(nx, ny))
     allocate(W2mean(nx, ny), UVmean(nx, ny), Pmean(nx, ny), P2mean(nx, ny))
   end subroutine allocate_all
+
+  ! compute_turbulent_solution_for_bc (boundary_conditions.f90:1371-1425):
+  ! U_top = 1, V_top from the 1/7-power-law growth of delta, W_top = 0; for
+  ! inflow_flag 5 the inlet profile U_inlet(nyg), V_inlet(ny) from the
+  ! inflow_file stream (an int32 count ny, then the two arrays), W_inlet = 0
+  subroutine turbulent_tables()
+    integer :: iu, ny_in, i
+    real(real64) :: rex
+    U_top = 1d0
+    do i = 1, nxg
+       rex = 1d0 * xg(i) / nu
+       V_top(i) = 1d0 * (0.0160d0 * rex**(-1d0 / 7d0))
+    end do
+    W_top = 0d0
+    U_inlet = 0d0; V_inlet = 0d0; W_inlet = 0d0
+    n_modes = 0; m_modes = 0
+    if (inflow_flag == 5) then
+       write(*,*) 'Reading Turbulent profile for inlet from file'
+       open(newunit=iu, file=trim(file_inflow), form='unformatted', access='stream', action='read', status='old')
+       read(iu) ny_in
+       if (ny_in /= ny) stop 'Profile doesnt match dimensions'
+       read(iu) U_inlet
+       read(iu) V_inlet
+       close(iu)
+    end if
+  end subroutine turbulent_tables
 
   subroutine read_blasius_table()
     integer :: iu
@@ -581,7 +612,7 @@ program bl_driver
   write(*,*) '----------------------------------------------------------------------'
   call read_parameters()
   call allocate_all()
-  call read_blasius_table()
+  if (inflow_flag /= 3 .and. inflow_flag /= 5) call read_blasius_table()
   if (random_init == 1) then
      call init_flow_grid()
   else
@@ -593,9 +624,13 @@ program bl_driver
                    'bl_blasius_field')
   end if
   call periodic_z()
-  call bl_check(bl_blasius_tables(nx, ny, x, y, nu, n_source, eta_s, f_s, df_s, U_inlet, V_inlet, &
-                W_inlet, U_top, V_top, W_top), 'bl_blasius_tables')
-  call read_temporal_modes()
+  if (inflow_flag == 3 .or. inflow_flag == 5) then
+     call turbulent_tables()
+  else
+     call bl_check(bl_blasius_tables(nx, ny, x, y, nu, n_source, eta_s, f_s, df_s, U_inlet, V_inlet, &
+                   W_inlet, U_top, V_top, W_top), 'bl_blasius_tables')
+     call read_temporal_modes()
+  end if
 
   cfg%nx = nx; cfg%ny = ny; cfg%nz = nz
   cfg%x = c_loc(x); cfg%y = c_loc(y); cfg%z = c_loc(z)
@@ -615,7 +650,14 @@ program bl_driver
      cfg%qw_re = c_loc(qw_re); cfg%qw_im = c_loc(qw_im)
   end if
   call bl_check(bl_create(cfg, h), 'bl_create')
-  ! the general dispatcher's inflow and top flags (boundary_conditions.f90:22-121)
+  call bl_check(bl_upload(h, BL_FIELD_U, U), 'bl_upload U')
+  call bl_check(bl_upload(h, BL_FIELD_V, V), 'bl_upload V')
+  call bl_check(bl_upload(h, BL_FIELD_W, W), 'bl_upload W')
+  call bl_check(bl_upload(h, BL_FIELD_P, P), 'bl_upload P')
+  call bl_check(bl_upload(h, BL_FIELD_NUT, nu_t), 'bl_upload nu_t')
+  call bl_check(bl_set_time(h, t), 'bl_set_time')
+  ! the general dispatcher's inflow and top flags (boundary_conditions.f90:22-121),
+  ! after the fields: Lund rescaling starts its time averages from them
   if (inflow_flag /= 1 .or. top_flag /= 0) then
      bnd%inflow_flag = inflow_flag
      bnd%top_flag = top_flag
@@ -627,16 +669,10 @@ program bl_driver
      bnd%phi_bs = phi_bs
      bnd%i_rescale = i_rescale
      bnd%beta_hartree = beta_hartree
-     bnd%delta_inlet = 0d0
-     bnd%t_resc = 0d0
+     bnd%delta_inlet = delta_inlet
+     bnd%t_resc = t_resc
      call bl_check(bl_set_boundary(h, bnd), 'bl_set_boundary')
   end if
-  call bl_check(bl_upload(h, BL_FIELD_U, U), 'bl_upload U')
-  call bl_check(bl_upload(h, BL_FIELD_V, V), 'bl_upload V')
-  call bl_check(bl_upload(h, BL_FIELD_W, W), 'bl_upload W')
-  call bl_check(bl_upload(h, BL_FIELD_P, P), 'bl_upload P')
-  call bl_check(bl_upload(h, BL_FIELD_NUT, nu_t), 'bl_upload nu_t')
-  call bl_check(bl_set_time(h, t), 'bl_set_time')
   write(*,*) 'grid ', nx, ny, nz, ' RKscheme ', itime_step, ' LES ', les_model
   call system_clock(c0, crate)
   time1 = real(c0, real64) / real(crate, real64)

@@ -145,6 +145,54 @@ def test_driver_dispatcher_flags(tmp_path):
 
 @pytest.mark.gpu
 @needs_driver
+def test_driver_lund_fluctuation_inflow(tmp_path):
+    """inflow_flag = 5 from a restart file: the turbulent tables of
+    compute_turbulent_solution_for_bc (boundary_conditions.f90:1371-1425: the
+    inlet profile read from the inflow_file stream, U_top = 1, the 1/7-law
+    V_top) and the Lund keys (Lund_ix, Lund_deltai, Lund_T) reach the
+    library; the snapshot after 6 steps matches the oracle with the same
+    tables and flags."""
+    d = str(tmp_path)
+    case = Case(40, 20, 8, 0.2, 0.02, 0.01, 1e-5, alpha=3.0, grid="sinh", noise=0.03, seed=8, scheme=3,
+                cfl=-2e-4)
+    t0 = 0.05
+    write_initial(os.path.join(d, "start.bin"), t0, case.nu, 10, case.x, case.y, case.z, case.U0, case.V0, case.W0)
+    nyg, ny = case.y.size + 1, case.y.size
+    uin = case.tables["U_inlet"] * 1.01
+    vin = case.tables["V_inlet"] * 0.9
+    with open(os.path.join(d, "profile.bin"), "wb") as fh:
+        fh.write(np.array([ny], "<i4").tobytes())
+        fh.write(np.asarray(uin, "<f8").tobytes())
+        fh.write(np.asarray(vin, "<f8").tobytes())
+    write_case(d, init_rand="0", filein="start.bin", init_step="-45", nsteps="6", nsave="6", nstats="6",
+               nmonitor="6", inflow_flag="5", inflow_file="profile.bin", Lund_ix="20", Lund_deltai="0.004",
+               Lund_T="0.4")
+    r = run_driver(d)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    o = orc.Oracle(case.x, case.y, case.z, case.nu, 0.0, 0.0, 3, 0, -2e-4)
+    _, xg = orc.centers(case.x)
+    tab = dict(U_inlet=uin, V_inlet=vin, W_inlet=np.zeros(nyg), U_top=np.ones(case.x.size),
+               V_top=0.0160 * (xg / case.nu) ** (-1.0 / 7.0), W_top=np.zeros(xg.size))
+    o.set_bc_tables(tab)
+    o.set_fields(case.U0, case.V0, case.W0)
+    o.periodic()
+    o.t = t0
+    o.set_boundary(inflow_flag=5, i_rescale=20, delta_inlet=0.004, t_resc=0.4)
+    for _ in range(6):
+        o.compute_dt()
+        o.step()
+    snap = read_snapshot(os.path.join(d, "data", "BL_lam.00000016"))
+    U, V, W = o.fields()
+    scale = np.abs(U).max()
+    for name, ref in (("U", U), ("V", V), ("W", W)):
+        err = np.abs(snap[name] - ref[:, :, :-1]).max() / scale
+        print("driver lund", name, err, flush=True)
+        assert err < 1e-10, name
+    o.close()
+
+
+@pytest.mark.gpu
+@needs_driver
 def test_driver_mixed_precision(tmp_path):
     """The optional `precision = 1` key runs the mixed-precision path: the
     snapshot after 20 steps agrees with the FP64 oracle to FP32 accuracy."""
@@ -205,7 +253,7 @@ def test_driver_requires_input_file(tmp_path):
     ("RKscheme", "1", "only RKscheme = 2 or 3"),
     ("WM", "1", "only WM = 0"),
     ("LES", "5", "LES must be 0, 1, 2 or 3"),
-    ("inflow_flag", "3", "Lund"),
+    ("inflow_flag", "3", "Lund inflow (3, 5) starts from a flow field file"),
     ("top_flag", "5", "top_flag"),
     ("precision", "2", "precision must be 0"),
 ])
