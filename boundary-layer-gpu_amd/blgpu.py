@@ -36,6 +36,12 @@ class BLBoundary(ctypes.Structure):
                 ("beta_hartree", _D), ("delta_inlet", _D), ("t_resc", _D)]
 
 
+class BLWallParams(ctypes.Structure):
+    """bl_wall_params: the wall model (.turbb WM, TauwModel, alphas)."""
+    _fields_ = [("iwall_model", _I), ("istress_model", _I), ("alpha_mean_x", _D), ("alpha_mean_y", _D),
+                ("alpha_mean_z", _D), ("alpha_std", _D), ("freq_mult", _D)]
+
+
 # every entry point declared in include/blgpu.h with its ctypes signature
 SIGNATURES = {
     "bl_create": (_I, [ctypes.POINTER(BLConfig), ctypes.POINTER(_P)]),
@@ -45,6 +51,9 @@ SIGNATURES = {
     "bl_download": (_I, [_P, _I, _P]),
     "bl_set_time": (_I, [_P, _D]),
     "bl_set_boundary": (_I, [_P, _P]),
+    "bl_set_wall_model": (_I, [_P, _P]),
+    "bl_compute_wall_model": (_I, [_P]),
+    "bl_get_wall_model": (_I, [_P, _P, _P, _P, _P]),
     "bl_get_time": (_I, [_P, _DP]),
     "bl_compute_dt": (_I, [_P, _DP]),
     "bl_step": (_I, [_P]),
@@ -259,6 +268,24 @@ class Solver:
         b = BLBoundary(inflow_flag, top_flag, amplitude, seed, vbs_max, x_bs, sigma_bs, phi_bs, i_rescale,
                        beta_hartree, delta_inlet, t_resc)
         _check(lib().bl_set_boundary(self.h, ctypes.byref(b)), "bl_set_boundary")
+
+    def set_wall_model(self, iwall_model=0, istress_model=0, alpha_mean_x=0.0, alpha_mean_y=0.0, alpha_mean_z=0.0,
+                       alpha_std=0.0, freq_mult=0.0):
+        """bl_set_wall_model: WM 0, 4, 5, 6, 8, 11, 15 and TauwModel 0/1
+        (compute_wall_model wallmodel.f90:25-144)."""
+        p = BLWallParams(iwall_model, istress_model, alpha_mean_x, alpha_mean_y, alpha_mean_z, alpha_std, freq_mult)
+        _check(lib().bl_set_wall_model(self.h, ctypes.byref(p)), "bl_set_wall_model")
+
+    def compute_wall_model(self):
+        _check(lib().bl_compute_wall_model(self.h), "bl_compute_wall_model")
+
+    def get_wall_model(self):
+        """alpha_x(nx), alpha_y(nxg), alpha_z(nxg), utau_model(nx)"""
+        nx = self.nx
+        a, b, c, d = np.zeros(nx), np.zeros(nx + 1), np.zeros(nx + 1), np.zeros(nx)
+        _check(lib().bl_get_wall_model(self.h, a.ctypes.data, b.ctypes.data, c.ctypes.data, d.ctypes.data),
+               "bl_get_wall_model")
+        return a, b, c, d
 
     def compute_dt(self):
         v = _D()

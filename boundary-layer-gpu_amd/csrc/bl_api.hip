@@ -320,6 +320,7 @@ void build(Handle &h, const bl_config *c) {
     h.hx = g.x;
     h.hxg = g.xg;
     h.hyg = g.yg;
+    h.hy = g.y;
     h.dymin = g.dymin;
     h.dzmin = g.dzmin;
 
@@ -652,7 +653,7 @@ void free_all(Handle &h) {
     }
     for (void *p : {(void *)h.F1.u, (void *)h.F1.v, (void *)h.F1.w, (void *)h.P, (void *)h.phi,
                     (void *)h.nu_t, (void *)h.T1, (void *)h.S, (void *)h.dct_w1, (void *)h.dct_w2,
-                    (void *)h.dct_wN, (void *)h.dct_rot, (void *)h.dct_mf, h.mx2, (void *)h.zc, (void *)h.zs, (void *)h.zf, (void *)h.bc_extra, (void *)h.lund_mem, (void *)h.lu_dl, (void *)h.lu_dp, (void *)h.lu_du,
+                    (void *)h.dct_wN, (void *)h.dct_rot, (void *)h.dct_mf, h.mx2, (void *)h.zc, (void *)h.zs, (void *)h.zf, (void *)h.bc_extra, (void *)h.lund_mem, (void *)h.wm_mem, (void *)h.lu_dl, (void *)h.lu_dp, (void *)h.lu_du,
                     (void *)h.lu00, (void *)h.th_co,
                     (void *)h.scal, (void *)h.red, (void *)h.dsm, (void *)h.smag_c, h.mem_grid, h.mem_gridf, (void *)h.mx_tab, (void *)h.zmf,
                     (void *)h.p00, (void *)h.inc1, (void *)h.stats, (void *)h.cl_w, (void *)h.cl_b, (void *)h.cl_cnt})
@@ -701,6 +702,10 @@ void do_step_t(Handle &h) {
         // eddy viscosity of the stage's input fields (time_integration.f90:341-346)
         if (h.les != 0) launch_dsm(h, h.buf[cur]);
         if (h.les == 4) launch_closure(h, h.buf[cur]);
+        // then the wall model (time_integration.f90:344): slip lengths of
+        // the stage, and the pseudo-Robin V's wall mean
+        if (h.wm_mem && (h.wm_model != 0 || (h.les != 0 && h.wm_istress == 1)))
+            launch_wall_model<T>(h, as<T>(h.buf[cur]), old, false);
         Fld3<T> next = as<T>(h.buf[nxt]);
         launch_rk_stage(h, as<T>(h.buf[cur]), old, next, s, dt);
         const double ts = to + (nst == 3 ? h.rk_t[s] : h.rk2_t[s]) * dt;
@@ -846,9 +851,9 @@ int bl_create(const bl_config *cfg, bl_handle **out) {
         return BL_EINVAL;
     }
     *out = nullptr;
+    // value-initialised: members without an initialiser start at zero
     bl_handle *h = new (std::nothrow) bl_handle();
     if (!h) return BL_ENOMEM;
-    std::memset(static_cast<Handle *>(h), 0, sizeof(Handle));
     int rc = guarded([&] {
         check_config(cfg);
         int ndev = 0;
@@ -1183,11 +1188,46 @@ int bl_apply_bc(bl_handle *hh, int rk_step, double dt) {
                 launch_lund<double>(h, as<double>(h.buf[h.iold]), h.dt);
             h.lund_pending = false;
         }
-        if (h.precision == BL_PREC_MIXED)
+        // the pseudo-Robin V's wall mean from the old fields and the current
+        // slip lengths, as apply_pseudo_Robin_bc_y_bottom forms it at every call
+        if (h.precision == BL_PREC_MIXED) {
+            launch_wall_model<float>(h, as<float>(h.buf[h.icur]), as<float>(h.buf[h.iold]), true);
             bc_on<float>(h, h.buf[h.icur], h.buf[h.iold], rk_step, dt, h.t);
-        else
+        } else {
+            launch_wall_model<double>(h, as<double>(h.buf[h.icur]), as<double>(h.buf[h.iold]), true);
             bc_on<double>(h, h.buf[h.icur], h.buf[h.iold], rk_step, dt, h.t);
+        }
         BL_HIP(hipStreamSynchronize(h.stream));
+    });
+}
+
+int bl_set_wall_model(bl_handle *hh, const bl_wall_params *p) {
+    return guarded([&] {
+        Handle &h = H(hh);
+        if (!p) {
+            last_error() = "bl_set_wall_model: null parameters";
+            throw ArgError();
+        }
+        const double am[3] = {p->alpha_mean_x, p->alpha_mean_y, p->alpha_mean_z};
+        wall_model_set(h, p->iwall_model, p->istress_model, am, p->alpha_std, p->freq_mult);
+    });
+}
+
+int bl_compute_wall_model(bl_handle *hh) {
+    return guarded([&] {
+        Handle &h = H(hh);
+        if (h.precision == BL_PREC_MIXED)
+            launch_wall_model<float>(h, as<float>(h.buf[h.icur]), as<float>(h.buf[h.iold]), false);
+        else
+            launch_wall_model<double>(h, as<double>(h.buf[h.icur]), as<double>(h.buf[h.iold]), false);
+        BL_HIP(hipStreamSynchronize(h.stream));
+    });
+}
+
+int bl_get_wall_model(bl_handle *hh, double *alpha_x, double *alpha_y, double *alpha_z, double *utau_model) {
+    return guarded([&] {
+        Handle &h = H(hh);
+        wall_model_get(h, alpha_x, alpha_y, alpha_z, utau_model);
     });
 }
 

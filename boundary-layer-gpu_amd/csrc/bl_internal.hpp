@@ -114,6 +114,11 @@ struct BCTables {
     double dy_dx = 0.0;             // top_flag 1, 2: (yg(nyg) - yg(nyg-1)) / dxmin
     int i_rescale = 0;              // top_flag 1, 2: columns kept at U/V/W_top
     const double *lund_u = nullptr, *lund_v = nullptr, *lund_w = nullptr;  // inflow_flag 3, 5: the step's plane
+    // the wall model's bottom-wall conditions (bl_set_wall_model): slip
+    // lengths alpha_x (nx), alpha_y, alpha_z (nxg) and the pseudo-Robin V's
+    // wall mean beta_y
+    int iwall = 0;
+    const double *wm_ax = nullptr, *wm_ay = nullptr, *wm_az = nullptr, *wm_beta = nullptr;
 };
 
 template <class T>
@@ -167,7 +172,11 @@ struct Handle {
     double *bc_extra = nullptr;     // device V(i, ny) blowing/suction (nxg) + Falkner-Skan U(i, nyg) (nx)
     bool fs_uc_pending = false;     // top_flag 3: U_top(nx) changes after the next BC call
     double fs_uc = 0.0;
-    std::vector<double> hx, hxg, hyg;  // host copies of x, xg, yg
+    std::vector<double> hx, hxg, hyg, hy;  // host copies of x, xg, yg, y
+    // wall model (bl_wall.hip): iwall_model, istress_model, alphas(5)
+    int wm_model = 0, wm_istress = 0;
+    double wm_am[3] = {0.0, 0.0, 0.0}, wm_std = 0.0, wm_freq = 0.0;
+    double *wm_mem = nullptr;
     // Lund rescaling (inflow_flag 3, 5)
     double *lund_mem = nullptr;
     LundArrays lund{};
@@ -344,6 +353,17 @@ void launch_lund(Handle &h, const Fld3<T> &old, double dt);
 // the time averages start as the means of f
 template <class T>
 void lund_init_means(Handle &h, const Fld3<T> &f);
+// ---- wall models (bl_wall.hip) ----
+// validate (ArgError with a message), allocate, and reset: slip lengths 0,
+// utau_model = utau_ref
+void wall_model_set(Handle &h, int model, int istress, const double am[3], double astd, double freq);
+// compute_wall_model on `cur` at h.t (old: the step-start field of the
+// pseudo-Robin V); beta_only: only the pseudo-Robin wall mean, slip lengths kept
+template <class T>
+void launch_wall_model(Handle &h, const Fld3<T> &cur, const Fld3<T> &old, bool beta_only);
+void wall_model_get(Handle &h, double *ax, double *ay, double *az, double *utau);
+// the models whose wall is Robin (U, W) and pseudo-Robin (V)
+bool wall_model_robin(int model);
 // ---- statistics and snapshots (bl_stats.hip) ----
 long long stats_size(const Dims &d);
 // compute_statistics' z means of f (and the handle's P, nu_t) into out

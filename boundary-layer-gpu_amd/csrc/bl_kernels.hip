@@ -494,7 +494,7 @@ __global__ __launch_bounds__(256) void k_bc_periodic(Dims d, T *U, T *V, T *W) {
 // top Dirichlet (:220-228) then wall (:231-238); one thread per (i,k)
 template <class T>
 __global__ __launch_bounds__(256) void k_bc_topwall(Dims d, Metrics m, BCTables b, T *U, T *V, T *W,
-                                                    double *qpart) {
+                                                    const T *__restrict__ Vo, double *qpart) {
     __shared__ double sh[256];
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int k = blockIdx.y;
@@ -508,8 +508,17 @@ __global__ __launch_bounds__(256) void k_bc_topwall(Dims d, Metrics m, BCTables 
     const int ii = min(i, d.nxg - 1);
     const double vtop = b.top_flag == 0 ? (double)(T)b.V_top[ii]
                                         : (b.top_flag == 3 ? 0.0 : (double)(T)b.vtop_bs[ii]);
+    // V(i, 1) as the wall condition below stores it: 0, or with a Robin wall
+    // model the pseudo-Robin alpha_y dVo/dy less its wall mean beta_y
+    // (apply_pseudo_Robin_bc_y_bottom :1152-1180, id 2)
+    const bool robin = b.iwall != 0 && b.iwall != 11 && b.iwall != 15;
+    double vwall = 0.0;
+    if (robin) {
+        const long long pw = fidx(d, ii, 0, k);
+        vwall = (double)(T)(b.wm_ay[ii] * ((double)Vo[pw + d.sx] - (double)Vo[pw]) / (m.y[1] - m.y[0]) - b.wm_beta[0]);
+    }
     double q = 0.0;
-    if (i >= 1 && i <= d.nxg - 3 && k >= 1 && k <= d.nzg - 3) q = (0.0 - vtop) * (m.x[i] - m.x[i - 1]);
+    if (i >= 1 && i <= d.nxg - 3 && k >= 1 && k <= d.nzg - 3) q = (vwall - vtop) * (m.x[i] - m.x[i - 1]);
     q = block_sum_256(q, sh);
     if (threadIdx.x == 0) qpart[blockIdx.x + gridDim.x * blockIdx.y] = q;
     if (i >= d.nxg) return;
@@ -539,10 +548,32 @@ __global__ __launch_bounds__(256) void k_bc_topwall(Dims d, Metrics m, BCTables 
             if (k < d.nz) W[wt_] = (T)b.W_top[i];
         }
     }
-    // walls (:231-238)
-    if (i < d.nx) U[p0] = -U[p0 + sx];
-    V[p0] = (T)0;
-    if (k < d.nz) W[p0] = -W[p0 + sx];
+    // walls: no-slip (:231-238), or the wall model's conditions (:73-108):
+    // Neumann U (iwall_model 11, 15: apply_Nozero_Neumann_bc_y :359-372)
+    // with V = W = 0, else Robin U, W (apply_Robin_bc_y_bottom :1107-1132,
+    // id 2) and the pseudo-Robin V
+    if (b.iwall == 0) {
+        if (i < d.nx) U[p0] = -U[p0 + sx];
+        V[p0] = (T)0;
+        if (k < d.nz) W[p0] = -W[p0 + sx];
+    } else {
+        const double dyg = m.yg[1] - m.yg[0];
+        if (!robin) {
+            if (i < d.nx) U[p0] = (T)((double)U[p0 + sx] - b.wm_ax[i] * dyg);
+            V[p0] = (T)0;
+            if (k < d.nz) W[p0] = -W[p0 + sx];
+        } else {
+            if (i < d.nx) {
+                const double r = 2.0 * b.wm_ax[i] / dyg;
+                U[p0] = (T)((r - 1.0) * (double)U[p0 + sx] / (r + 1.0));
+            }
+            V[p0] = (T)vwall;
+            if (k < d.nz) {
+                const double r = 2.0 * b.wm_az[i] / dyg;
+                W[p0] = (T)((r - 1.0) * (double)W[p0 + sx] / (r + 1.0));
+            }
+        }
+    }
 }
 
 // global mass conservation (:241-260): every block re-sums the flux partials
@@ -2115,7 +2146,8 @@ void launch_bc(Handle &h, Fld3<T> &f, const Fld3<T> &old, int rk_step, double dt
     check_launch();
     prof_end(h, KID_BC_PERIODIC);
     prof_begin(h, KID_BC_TOPWALL);
-    hipLaunchKernelGGL(k_bc_topwall<T>, g_tw, dim3(256), 0, h.stream, d, h.m, h.bc, f.u, f.v, f.w, qpart + nq_in);
+    hipLaunchKernelGGL(k_bc_topwall<T>, g_tw, dim3(256), 0, h.stream, d, h.m, h.bc, f.u, f.v, f.w, old.v,
+                       qpart + nq_in);
     check_launch();
     prof_end(h, KID_BC_TOPWALL);
     prof_begin(h, KID_BC_MASSFLUX);

@@ -106,21 +106,6 @@ __device__ double lund_theta(int n, const double *m, const double *yg, double ui
     return th < 0 ? fabs(th) : th;
 }
 
-// interp_vel's row for target t among y_[0..n-1]: the first k with
-// y_[k] > t (its lower neighbour k-1 and the weight of k-1), or -1 (the
-// top value)
-__device__ void lund_row(int n, const double *y_, double t, int &j, double &coef) {
-    j = -1;
-    for (int k = 0; k < n; ++k)
-        if (y_[k] > t) {
-            j = k;
-            break;
-        }
-    coef = 0.0;
-    if (j >= 1) coef = (y_[j] - t) / (y_[j] - y_[j - 1]);
-    if (j == 0) j = -1;  // below the first point: the reference stops; the top value here
-}
-
 __device__ __forceinline__ double lund_interp(const double *f, int n, int j, double coef) {
     return j >= 1 ? coef * f[j - 1] + (1.0 - coef) * f[j] : f[n - 1];
 }
@@ -164,7 +149,8 @@ __global__ __launch_bounds__(256) void k_lund_profile(Dims d, Metrics m, double 
         int ji, jo;
         double ci, co;
         {
-            // lund_row on a coordinate computed per point
+            // interp_vel's row (rescaled_inlet_bc.f90:305-345): the first k
+            // whose coordinate exceeds the target, its lower neighbour's weight
             ji = -1;
             for (int k = 0; k < nyg; ++k)
                 if (utau_r * m.yg[k] / nu > ti) {

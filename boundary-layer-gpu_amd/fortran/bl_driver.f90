@@ -115,6 +115,10 @@ module bl_state
   integer :: i_rescale = 0
   ! Lund rescaling (params.f90:144-146)
   real(real64) :: delta_inlet = 0d0, t_resc = 0d0
+  ! wall model (params.f90:154-155, 198-206): TauwModel, and alphas =
+  ! alpha_mean_x, alpha_mean_y, alpha_mean_z, alpha_std, freq_mult
+  integer :: istress_model = 0
+  real(real64) :: alphas(5) = 0d0
   integer :: nstep_init = 0, nstep_init_input = -45, istep = 0
   real(real64) :: nu = 0d0, cfl = -1d-4, dPdx = 0d0, dPdz = 0d0, t = 0d0, dt = 0d0
   real(real64) :: boxsize(4) = 0d0
@@ -193,6 +197,8 @@ contains
     call get_int('nutBC', nut_bc, f)
     call get_int('precision', iprecision, f)
     call get_int('WM', iwall_model, f)
+    call get_int('TauwModel', istress_model, f)
+    call get_dbl_arr('alphas', alphas, f)
     call get_int('init_step', nstep_init_input, f)
     call get_int('init_rand', random_init, f)
     call get_str('filein', filein, f1)
@@ -208,7 +214,7 @@ contains
     ! this build runs the RK path of the reference with the GPU boundary
     ! conditions (time_integration.f90:145-465)
     if (itime_step /= 2 .and. itime_step /= 3) stop 'Error: only RKscheme = 2 or 3 is built'
-    if (iwall_model /= 0) stop 'Error: only WM = 0 is built'
+    if (all(iwall_model /= [0, 4, 5, 6, 8, 11, 15])) stop 'Error: WM must be 0, 4, 5, 6, 8, 11 or 15'
     if (les_model < 0 .or. les_model > 3) stop 'Error: LES must be 0, 1, 2 or 3'
     if (iprecision /= 0 .and. iprecision /= 1) stop 'Error: precision must be 0 (FP64) or 1 (mixed)'
     if (inflow_flag < 1 .or. inflow_flag > 5) stop 'Error: inflow_flag must be 1 to 5'
@@ -600,6 +606,7 @@ program bl_driver
   implicit none
   type(bl_config), target :: cfg
   type(bl_boundary) :: bnd
+  type(bl_wall_params) :: wmp
   type(c_ptr) :: h
   real(c_double) :: mon(8), div, t_dev
   real(real64) :: time1, time2, snap_t
@@ -672,6 +679,18 @@ program bl_driver
      bnd%delta_inlet = delta_inlet
      bnd%t_resc = t_resc
      call bl_check(bl_set_boundary(h, bnd), 'bl_set_boundary')
+  end if
+  ! the wall model (compute_wall_model wallmodel.f90:25-144), computed on the
+  ! device at every RK stage
+  if (iwall_model /= 0 .or. istress_model /= 0) then
+     wmp%iwall_model = iwall_model
+     wmp%istress_model = istress_model
+     wmp%alpha_mean_x = alphas(1)
+     wmp%alpha_mean_y = alphas(2)
+     wmp%alpha_mean_z = alphas(3)
+     wmp%alpha_std = alphas(4)
+     wmp%freq_mult = alphas(5)
+     call bl_check(bl_set_wall_model(h, wmp), 'bl_set_wall_model')
   end if
   write(*,*) 'grid ', nx, ny, nz, ' RKscheme ', itime_step, ' LES ', les_model
   call system_clock(c0, crate)

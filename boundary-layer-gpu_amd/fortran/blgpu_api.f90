@@ -38,6 +38,12 @@ module blgpu_api
      real(c_double)       :: delta_inlet, t_resc
   end type bl_boundary
 
+  ! bl_wall_params: the wall model (compute_wall_model wallmodel.f90:25-144)
+  type, bind(C) :: bl_wall_params
+     integer(c_int) :: iwall_model, istress_model
+     real(c_double) :: alpha_mean_x, alpha_mean_y, alpha_mean_z, alpha_std, freq_mult
+  end type bl_wall_params
+
   interface
      integer(c_int) function bl_create(cfg, h) bind(C, name='bl_create')
        import :: c_int, c_ptr, bl_config
@@ -79,6 +85,12 @@ module blgpu_api
        type(c_ptr), value            :: h
        type(bl_boundary), intent(in) :: b
      end function bl_set_boundary
+
+     integer(c_int) function bl_set_wall_model(h, p) bind(C, name='bl_set_wall_model')
+       import :: c_int, c_ptr, bl_wall_params
+       type(c_ptr), value               :: h
+       type(bl_wall_params), intent(in) :: p
+     end function bl_set_wall_model
 
      integer(c_int) function bl_get_time(h, t) bind(C, name='bl_get_time')
        import :: c_int, c_ptr, c_double

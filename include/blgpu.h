@@ -124,7 +124,7 @@ int bl_get_time(bl_handle *h, double *t);
  *     (initialization.f90:640-660); inflow_flag 5: its fluctuations only, on
  *     the U/V/W_inlet means (:512-559).  The first boundary call after this
  *     one (bl_apply_bc) or the next bl_step computes the plane.
- * Wall models (WM > 0) are not built.  Resets the step counter of the
+ * Wall models: bl_set_wall_model.  Resets the step counter of the
  * inflow_flag 4 stream. */
 typedef struct bl_boundary {
     int inflow_flag, top_flag;
@@ -136,6 +136,50 @@ typedef struct bl_boundary {
     double delta_inlet, t_resc;
 } bl_boundary;
 int bl_set_boundary(bl_handle *h, const bl_boundary *b);
+
+/* Wall model (.turbb WM, TauwModel, alphas; params.f90:154-155, 198-206) and
+ * the bottom-wall conditions it feeds (apply_boundary_conditions
+ * boundary_conditions.f90:73-108).  compute_wall_model
+ * (wallmodel.f90:25-144) runs on the device at every RK stage after the eddy
+ * viscosity, as time_integration.f90:341-346 calls it when LES > 0 or
+ * WM > 0, and sets the slip lengths alpha_x(nx), alpha_y(nxg), alpha_z(nxg)
+ * (functions of x, as every model builds them):
+ *   iwall_model 0: no-slip (the default);
+ *   4: momentum balance at both walls with the wall value of avg_nu_t
+ *      (compute_alpha_momentum_wall_model :674-764; a channel model: its
+ *      averages include the top wall);
+ *   5: alpha_mean_[xyz] (1 + alpha_std sin(omega t)),
+ *      omega = 2 pi freq_mult/(y(2) - y(1)) dPdx^0.5 (compute_constant_alpha
+ *      :952-961);
+ *   6, 8: log-layer slip length with utau = dPdx^0.5, kappa 0.38, B 5.2,
+ *      with the wall value of avg_nu_t (6) or nu_t = kappa utau y (8)
+ *      (compute_log_layer_wall_model :987-1026); dPdx > 0;
+ *   11: Neumann dU/dy = (utau_model^2 + <tau_uv>_z)/nu, V = W = 0
+ *      (compute_alpha_u_momentum_wall_model :1444-1500);
+ *   15: Neumann dU/dy = 1/<(nu + nu_t)/tau_w>_z with tau_w from
+ *      Cf = 0.027 Re_x^(-1/7), V = W = 0 (compute_exact_Neumann_from_Cf
+ *      :2003-2066).
+ * With 4, 5, 6, 8 the wall is Robin for U and W (apply_Robin_bc_y_bottom
+ * :1107-1132) and pseudo-Robin for V from the step-start field with its wall
+ * mean removed (apply_pseudo_Robin_bc_y_bottom :1152-1180).  istress_model 1
+ * updates utau_model by Newton's method on the law of the wall at yg(3)
+ * (compute_law_of_the_wall_model :1951-1985); otherwise utau_model =
+ * utau_ref (initialization.f90:745).  avg_nu_t(1, :) of models 4 and 6 is the
+ * constant-coefficient Smagorinsky's wall-parallel mean (LES 1,
+ * subgrid.f90:648-660) and 0 for the other eddy viscosities, whose routines
+ * do not write it.  Models 1, 2, 3, 7, 9, 12, 13, 14 are not built and give
+ * BL_EINVAL.  Resets the slip lengths to 0 and utau_model to utau_ref. */
+typedef struct bl_wall_params {
+    int iwall_model, istress_model;
+    double alpha_mean_x, alpha_mean_y, alpha_mean_z, alpha_std, freq_mult;
+} bl_wall_params;
+int bl_set_wall_model(bl_handle *h, const bl_wall_params *p);
+/* compute_wall_model (wallmodel.f90:25-144) once on the current fields at
+ * the current time (the step does it itself at every stage). */
+int bl_compute_wall_model(bl_handle *h);
+/* The slip lengths alpha_x(nx), alpha_y(nxg), alpha_z(nxg) and utau_model(nx)
+ * (host arrays; any may be null). */
+int bl_get_wall_model(bl_handle *h, double *alpha_x, double *alpha_y, double *alpha_z, double *utau_model);
 
 /* compute_dt (time_integration.f90:528-591): fixed dt for cfl < 0, else the
  * reference's max-based convective bound (a device reduction) and the

@@ -27,6 +27,7 @@ void orc_dsm_noavg(orc_state *s);    /* orc_dsm.c */
 void orc_smagorinsky(orc_state *s);  /* orc_dsm.c */
 
 static double *dalloc(size_t n) { return (double *)calloc(n ? n : 1, sizeof(double)); }
+double *orc__dalloc(size_t n) { return dalloc(n); }
 static double sq(double a) { return a * a; }
 
 static size_t nU(const orc_state *s) { return (size_t)s->nx * s->nyg * s->nzg; }
@@ -183,7 +184,8 @@ void orc_destroy(orc_state *s) {
                       s->tmode, s->qu_re, s->qu_im, s->qv_re, s->qv_im, s->qw_re, s->qw_im,
                       s->Ut, s->Vt, s->Wt, s->U, s->V, s->W, s->P, s->Uo, s->Vo, s->Wo,
                       s->nu_t, s->avg_nu_t, s->rhs_p, s->Umean_resc_T, s->Vmean_resc_T, s->Umean_inlet_T,
-                      s->Vmean_inlet_T, s->Ut_in, s->Vt_in, s->Wt_in};
+                      s->Vmean_inlet_T, s->Ut_in, s->Vt_in, s->Wt_in, s->alpha_x, s->alpha_y,
+                      s->alpha_z, s->utau_model, s->utau_ref};
     for (size_t i = 0; i < sizeof(ptrs) / sizeof(ptrs[0]); i++) free(ptrs[i]);
     for (int st = 1; st <= 3; st++) {
         free(s->Fu[st]);
@@ -980,13 +982,8 @@ void orc_apply_bc(orc_state *s, int rk_step, double dt) {
         for (int k = 1; k <= nz; k++)
             for (int i = 1; i <= nxg; i++) W[IW(i, nyg, k)] = s->W_top[i - 1];
     }
-    /* wall, :231-238 */
-    for (int k = 1; k <= nzg; k++)
-        for (int i = 1; i <= nx; i++) U[IU(i, 1, k)] = -U[IU(i, 2, k)];
-    for (int k = 1; k <= nzg; k++)
-        for (int i = 1; i <= nxg; i++) V[IV(i, 1, k)] = 0.0;
-    for (int k = 1; k <= nz; k++)
-        for (int i = 1; i <= nxg; i++) W[IW(i, 1, k)] = -W[IW(i, 2, k)];
+    /* wall, :231-238 (no-slip), or the wall model's conditions, :73-108 */
+    orc__wall_bc(s);
     /* global mass conservation, :241-260 (single rank: kk = 2) */
     double Qx = 0.0, length_y = 0.0;
     int kk = 2;
@@ -1288,7 +1285,11 @@ void orc_step(orc_state *s) {
     int nst = s->scheme == 2 ? 2 : 3;
     for (int st = 1; st <= nst; st++) {
         int rk_step = st;
-        if (s->les > 0) orc_eddy_viscosity(s);
+        /* time_integration.f90:341-346: eddy viscosity, then the wall model */
+        if (s->les > 0 || s->iwall > 0) {
+            orc_eddy_viscosity(s);
+            orc_wall_model(s);
+        }
         rhs_u(s, s->U, s->V, s->W, s->Fu[st]);
         rhs_v(s, s->U, s->V, s->W, s->Fv[st]);
         rhs_w(s, s->U, s->V, s->W, s->Fw[st]);

@@ -40,7 +40,16 @@ struct orc_state {
     double delta_inlet, t_resc;
     double *Umean_resc_T, *Vmean_resc_T, *Umean_inlet_T, *Vmean_inlet_T;
     double *Ut_in, *Vt_in, *Wt_in;
+    /* wall model (orc_wall.c) */
+    int iwall, istress;
+    double alpha_mean_x, alpha_mean_y, alpha_mean_z, alpha_std, freq_mult, beta_y;
+    double *alpha_x, *alpha_y, *alpha_z, *utau_model, *utau_ref;
 };
+
+/* bottom-wall conditions of apply_boundary_conditions (orc_wall.c) */
+void orc__wall_bc(orc_state *s);
+/* calloc of n doubles (n = 0: one) */
+double *orc__dalloc(size_t n);
 
 /* ---- 1-based index helpers (Fortran allocation shapes, x fastest) ---- */
 #define X(i) s->x[(i)-1]

@@ -145,6 +145,35 @@ def test_driver_dispatcher_flags(tmp_path):
 
 @pytest.mark.gpu
 @needs_driver
+def test_driver_wall_model(tmp_path):
+    """The wall-model keys (params.f90:154-155, 198-206): WM = 5 with
+    alphas (alpha_mean_x, _y, _z, alpha_std, freq_mult) and dPdx reach
+    bl_set_wall_model; the Robin / pseudo-Robin wall runs every stage and the
+    snapshot after 20 steps matches the oracle with the same model."""
+    d = str(tmp_path)
+    write_case(d, WM="5", TauwModel="1", alphas="3e-4 2e-4 4e-4 0.3 0.2", dPdx="4e-4")
+    r = run_driver(d)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    case = Case(40, 20, 8, 0.2, 0.02, 0.01, 1e-5, alpha=3.0, grid="sinh", scheme=3, cfl=-2e-4, dPdx=4e-4)
+    o = case.oracle()
+    o.set_wall_model(5, istress_model=1, alpha_mean_x=3e-4, alpha_mean_y=2e-4, alpha_mean_z=4e-4, alpha_std=0.3,
+                     freq_mult=0.2)
+    for _ in range(20):
+        o.compute_dt()
+        o.step()
+    snap = read_snapshot(os.path.join(d, "data", "BL_lam.00000020"))
+    U, V, W = o.fields()
+    assert np.abs(V[:, 0, :]).max() > 0
+    scale = np.abs(U).max()
+    for name, ref in (("U", U), ("V", V), ("W", W)):
+        err = np.abs(snap[name] - ref[:, :, :-1]).max() / scale
+        print("driver wall model", name, err, flush=True)
+        assert err < 1e-10, name
+    o.close()
+
+
+@pytest.mark.gpu
+@needs_driver
 def test_driver_lund_fluctuation_inflow(tmp_path):
     """inflow_flag = 5 from a restart file: the turbulent tables of
     compute_turbulent_solution_for_bc (boundary_conditions.f90:1371-1425: the
@@ -251,7 +280,7 @@ def test_driver_requires_input_file(tmp_path):
 @needs_driver
 @pytest.mark.parametrize("key,val,msg", [
     ("RKscheme", "1", "only RKscheme = 2 or 3"),
-    ("WM", "1", "only WM = 0"),
+    ("WM", "1", "WM must be 0, 4, 5, 6, 8, 11 or 15"),
     ("LES", "5", "LES must be 0, 1, 2 or 3"),
     ("inflow_flag", "3", "Lund inflow (3, 5) starts from a flow field file"),
     ("top_flag", "5", "top_flag"),

@@ -1,0 +1,177 @@
+"""Wall models on the device (bl_set_wall_model; compute_wall_model
+wallmodel.f90:25-144 and the bottom-wall conditions boundary_conditions.f90:
+73-108) against the oracle:
+
+- the model on the same noisy field: alpha_x, alpha_y, alpha_z and
+  utau_model within 1e-13 relative (1e-11 for utau_model from Newton's
+  method, whose last step the device and host log() may take at slightly
+  different |f|);
+- one boundary call from set_old fields: U, V, W within 1e-14 relative L2;
+- three RK3 steps (model every stage, Robin / pseudo-Robin / Neumann wall,
+  mass correction with the wall flux): U, V, W within 1e-12 relative L2 and
+  the divergence at the oracle's level;
+- mixed precision with a Robin wall: within the mixed mode's 1e-4 of the
+  FP64 oracle past the first steps' switch of the wall V;
+- unsupported models are rejected and leave the handle usable.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from . import cases
+from .blgpu_import import blgpu, solver_for
+
+pytestmark = pytest.mark.gpu
+
+KAPPA, B = 0.38, 5.2
+
+# name: (case keywords, wall-model keywords, law-of-the-wall field)
+MODELS = {
+    "robin5": (dict(dPdx=4e-4), dict(iwall_model=5, alpha_mean_x=3e-4, alpha_mean_y=2e-4, alpha_mean_z=4e-4,
+                                     alpha_std=0.3, freq_mult=0.2), False),
+    "momentum4": (dict(dPdx=-0.1, noise=0.05), dict(iwall_model=4), False),
+    "log6": (dict(dPdx=2.5e-3), dict(iwall_model=6), False),
+    "log6_smag": (dict(dPdx=2.5e-3, les=1), dict(iwall_model=6), False),
+    "log8": (dict(dPdx=2.5e-3), dict(iwall_model=8), False),
+    "u_momentum11": (dict(), dict(iwall_model=11, istress_model=1), True),
+    "u_momentum11_dsm": (dict(les=2), dict(iwall_model=11, istress_model=1), True),
+    "neumann15": (dict(), dict(iwall_model=15), False),
+    "neumann15_smag": (dict(dPdx=1e-3, les=1), dict(iwall_model=15), False),
+}
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def rel(a, b):
+    nb = np.linalg.norm(b)
+    return np.linalg.norm(a - b) / (nb if nb > 0 else 1.0)
+
+
+def _setup(name, precision=None):
+    ckw, wkw, law = MODELS[name]
+    args = dict(nx=30, ny=18, nz=10, noise=0.02, seed=8, cfl=-1e-4)
+    args.update(ckw)
+    c = cases.laminar_small(**args)
+    o = c.oracle()
+    o.set_nut_bc(1)
+    if law:
+        # U row 3 at the log-law velocity of a chosen utau(x)
+        U, V, W = o.fields()
+        from . import orc
+        _, yg = orc.centers(c.y)
+        ut = 0.045 + 0.005 * np.sin(np.linspace(0, 3, c.x.size))
+        U[:, 2, :] = (ut / KAPPA * np.log(yg[2] * ut / c.nu) + B * ut)[:, None]
+        o.set_fields(U, V, W)
+    kw = dict(nut_bc=1)
+    if precision is not None:
+        kw["precision"] = precision
+    s, _ = solver_for(c, o, **kw)
+    o.set_wall_model(**wkw)
+    s.set_wall_model(**wkw)
+    return c, o, s
+
+
+@pytest.mark.parametrize("name", list(MODELS))
+def test_wall_model_matches_oracle(name):
+    c, o, s = _setup(name)
+    o.t = s.t = 0.37
+    if c.les:
+        o.eddy_viscosity()
+        s.eddy_viscosity()
+    o.wall_model()
+    s.compute_wall_model()
+    for what, a, b in zip(("alpha_x", "alpha_y", "alpha_z", "utau_model"), s.get_wall_model(), o.get_wall_model()):
+        e = np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)
+        say("model", name, what, e, "max", np.abs(b).max())
+        assert e < (1e-11 if what == "utau_model" else 1e-13), what
+    ax = o.get_wall_model()[0]
+    assert np.abs(ax).max() > 0
+    # one boundary call on the model's slip lengths
+    U0, V0, W0 = o.fields()
+    Vo = V0 * 0.98
+    o.set_old(U0, Vo, W0)
+    s.set_old(U0, Vo, W0)
+    o.apply_bc(1, 1e-4)
+    s.apply_bc(1, 1e-4)
+    for comp, (a, b) in zip("UVW", zip(s.fields(), o.fields())):
+        e = rel(a, b)
+        say("bc", name, comp, e)
+        assert e < 1e-14, comp
+    s.close()
+    o.close()
+
+
+@pytest.mark.parametrize("name", list(MODELS))
+def test_steps_with_wall_model(name):
+    c, o, s = _setup(name)
+    for _ in range(3):
+        o.compute_dt()
+        o.step()
+        s.compute_dt()
+        s.step()
+    for comp, (a, b) in zip("UVW", zip(s.fields(), o.fields())):
+        e = rel(a, b)
+        say("steps", name, comp, e)
+        assert e < 1e-12, comp
+    for what, a, b in zip(("alpha_x", "alpha_y", "alpha_z"), s.get_wall_model()[:3], o.get_wall_model()[:3]):
+        e = np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)
+        say("steps", name, what, e)
+        assert e < 1e-11, what
+    div, odiv = s.max_divergence(), o.max_divergence()
+    say("steps", name, "max divergence", div, "oracle", odiv)
+    assert div < max(1e-12, 2.0 * odiv)
+    s.close()
+    o.close()
+
+
+def test_mixed_robin_wall():
+    """Mixed precision with a Robin / pseudo-Robin wall (model 5), three
+    steps within the mixed mode's 1e-4 of the FP64 oracle, from a state two
+    FP64 steps past the start.  The first steps switch the wall V from 0 to
+    alpha_y dVo/dy, and the FP32 projection removes the divergence that
+    switch makes with its own relative error: from the start the same case
+    measures 2e-4, 7e-4, 5e-4 in V over steps 1-3 (7e-6 with alpha / 10,
+    1.8e-5 in step 1 alone with alpha = 0, the first projection of an
+    uploaded field); past the switch 7e-6, 1.4e-5, 3.9e-6."""
+    ckw, wkw, _ = MODELS["robin5"]
+    c = cases.laminar_small(nx=30, ny=18, nz=10, noise=0.02, seed=8, cfl=-1e-4, **ckw)
+    o = c.oracle()
+    o.set_nut_bc(1)
+    o.set_wall_model(**wkw)
+    for _ in range(2):
+        o.compute_dt()
+        o.step()
+    s, _ = solver_for(c, o, nut_bc=1, precision=blgpu.PREC_MIXED)
+    s.set_wall_model(**wkw)
+    for _ in range(3):
+        o.compute_dt()
+        o.step()
+        s.compute_dt()
+        s.step()
+    assert np.abs(o.fields()[1][:, 0, :]).max() > 0
+    for comp, (a, b) in zip("UVW", zip(s.fields(), o.fields())):
+        e = rel(a, b)
+        say("mixed steps robin5", comp, e)
+        assert e < 1e-4, comp
+    s.close()
+    o.close()
+
+
+def test_rejected_wall_models_keep_the_handle():
+    c = cases.laminar_small(nx=30, ny=18, nz=10, noise=0.02, seed=8, cfl=-1e-4)
+    s, o = solver_for(c)
+    for bad in (dict(iwall_model=1), dict(iwall_model=9), dict(iwall_model=13), dict(iwall_model=6),
+                dict(iwall_model=5, istress_model=2)):
+        with pytest.raises(blgpu.BLError):
+            s.set_wall_model(**bad)
+    s.compute_dt()
+    s.step()
+    o.compute_dt()
+    o.step()
+    assert rel(s.fields()[0], o.fields()[0]) < 1e-12
+    s.close()
+    o.close()
