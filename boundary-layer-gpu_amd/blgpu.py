@@ -39,7 +39,8 @@ class BLBoundary(ctypes.Structure):
 class BLWallParams(ctypes.Structure):
     """bl_wall_params: the wall model (.turbb WM, TauwModel, alphas)."""
     _fields_ = [("iwall_model", _I), ("istress_model", _I), ("alpha_mean_x", _D), ("alpha_mean_y", _D),
-                ("alpha_mean_z", _D), ("alpha_std", _D), ("freq_mult", _D)]
+                ("alpha_mean_z", _D), ("alpha_std", _D), ("freq_mult", _D), ("iwall_model_nut", _I),
+                ("frac_vis", _D)]
 
 
 # every entry point declared in include/blgpu.h with its ctypes signature
@@ -270,10 +271,11 @@ class Solver:
         _check(lib().bl_set_boundary(self.h, ctypes.byref(b)), "bl_set_boundary")
 
     def set_wall_model(self, iwall_model=0, istress_model=0, alpha_mean_x=0.0, alpha_mean_y=0.0, alpha_mean_z=0.0,
-                       alpha_std=0.0, freq_mult=0.0):
-        """bl_set_wall_model: WM 0, 4, 5, 6, 8, 11, 15 and TauwModel 0/1
-        (compute_wall_model wallmodel.f90:25-144)."""
-        p = BLWallParams(iwall_model, istress_model, alpha_mean_x, alpha_mean_y, alpha_mean_z, alpha_std, freq_mult)
+                       alpha_std=0.0, freq_mult=0.0, iwall_model_nut=0, frac_vis=0.0):
+        """bl_set_wall_model: WM 0, 4, 5, 6, 8, 9, 11, 12, 15, TauwModel 0/1,
+        WMnutflag 0/2 and WMnut (compute_wall_model wallmodel.f90:25-144)."""
+        p = BLWallParams(iwall_model, istress_model, alpha_mean_x, alpha_mean_y, alpha_mean_z, alpha_std, freq_mult,
+                         iwall_model_nut, frac_vis)
         _check(lib().bl_set_wall_model(self.h, ctypes.byref(p)), "bl_set_wall_model")
 
     def compute_wall_model(self):

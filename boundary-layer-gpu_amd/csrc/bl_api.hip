@@ -1209,7 +1209,8 @@ int bl_set_wall_model(bl_handle *hh, const bl_wall_params *p) {
             throw ArgError();
         }
         const double am[3] = {p->alpha_mean_x, p->alpha_mean_y, p->alpha_mean_z};
-        wall_model_set(h, p->iwall_model, p->istress_model, am, p->alpha_std, p->freq_mult);
+        wall_model_set(h, p->iwall_model, p->istress_model, am, p->alpha_std, p->freq_mult, p->iwall_model_nut,
+                       p->frac_vis);
     });
 }
 

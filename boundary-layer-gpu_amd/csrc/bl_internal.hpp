@@ -176,6 +176,8 @@ struct Handle {
     // wall model (bl_wall.hip): iwall_model, istress_model, alphas(5)
     int wm_model = 0, wm_istress = 0;
     double wm_am[3] = {0.0, 0.0, 0.0}, wm_std = 0.0, wm_freq = 0.0;
+    int wm_nut_flag = 0;         // iwall_model_nut (models 9, 12)
+    double wm_frac_vis = 0.0;    // frac_vis_wall_model (models 9, 12)
     double *wm_mem = nullptr;
     // Lund rescaling (inflow_flag 3, 5)
     double *lund_mem = nullptr;
@@ -356,7 +358,8 @@ void lund_init_means(Handle &h, const Fld3<T> &f);
 // ---- wall models (bl_wall.hip) ----
 // validate (ArgError with a message), allocate, and reset: slip lengths 0,
 // utau_model = utau_ref
-void wall_model_set(Handle &h, int model, int istress, const double am[3], double astd, double freq);
+void wall_model_set(Handle &h, int model, int istress, const double am[3], double astd, double freq, int nut_flag,
+                    double frac_vis);
 // compute_wall_model on `cur` at h.t (old: the step-start field of the
 // pseudo-Robin V); beta_only: only the pseudo-Robin wall mean, slip lengths kept
 template <class T>

@@ -34,10 +34,12 @@ namespace {
 void check() { BL_HIP(hipGetLastError()); }
 
 // per-column sums, rows of cols[q * nxg + i]
-enum WmQ { Q_U3 = 0, Q_TAU, Q_NUTW, Q_DVO, Q_SB, Q_ST, Q_DUV, Q_DU, Q_N2, Q_NN, kWmQ };
+enum WmQ { Q_U3 = 0, Q_TAU, Q_NUTW, Q_DVO, Q_SB, Q_ST, Q_DUV, Q_DU, Q_N2, Q_NN, Q_UW, Q_UR, Q_DUVB, Q_DUB, Q_H, kWmQ };
 
 struct WmArgs {
     int model, istress, les, nut_dirichlet;
+    int nut_flag;       // models 9, 12: iwall_model_nut (0 fixed, 2 dynamic viscous fraction)
+    double frac_vis;    // models 9, 12: frac_vis_wall_model
     int beta_only;      // only the pseudo-Robin wall mean (slip lengths kept)
     double nu, dPdx, Ly;
     double a5x, a5y, a5z;  // model 5 at the current t
@@ -101,6 +103,25 @@ __global__ __launch_bounds__(256) void k_wm_cols(Dims d, Metrics m, WmArgs a, co
                 const double ne = nt ? nut_edge(d, m, nt, i, k) : 0.0;
                 s[Q_TAU] += -2.0 * ne * suv(d, m, U, V, i, 0, k);
             }
+            if ((a.model == 9 || a.model == 12) && i <= nx - 1) {
+                // compute_alpha_v_from_Cf (:1532-1697) / compute_alpha_u_from_Cf
+                // (:1263-1424): U over the two wall rows (and rows 3:4 for the
+                // dynamic viscous fraction), tau_uv at the wall edge (model 9
+                // with dU/dy alone), <dU/dy dV/dy> (12; model 9: <U dVo/dy>
+                // of the step-start V) and <dU/dy>
+                const double u0 = (double)U[fidx(d, i, 0, k)], u1 = (double)U[fidx(d, i, 1, k)];
+                s[Q_UW] += u0 + u1;
+                if (a.nut_flag == 2) s[Q_UR] += (double)U[fidx(d, i, 2, k)] + (double)U[fidx(d, i, 3, k)];
+                const double ne = nt ? nut_edge(d, m, nt, i, k) : 0.0;
+                const double sv = a.model == 12 ? suv(d, m, U, V, i, 0, k) : 0.5 * ((u1 - u0) / (m.yg[1] - m.yg[0]));
+                s[Q_TAU] += -2.0 * ne * sv;
+                const T *VV = a.model == 12 ? V : Vo;
+                const double v1 = 0.5 * ((double)VV[fidx(d, i, 0, k)] + (double)VV[fidx(d, i + 1, 0, k)]);
+                const double v2 = 0.5 * ((double)VV[fidx(d, i, 1, k)] + (double)VV[fidx(d, i + 1, 1, k)]);
+                const double g = (u1 - u0) / (m.yg[1] - m.yg[0]);
+                s[Q_DUVB] += a.model == 12 ? g * (v2 - v1) / (m.y[1] - m.y[0]) : 0.5 * (u1 + u0) * (v2 - v1) / (m.y[1] - m.y[0]);
+                s[Q_DUB] += g;
+            }
             if (a.model == 15 && i <= nx - 1) {
                 const double ne = nt ? nut_edge(d, m, nt, i, k) : 0.0;
                 s[Q_NUTW] += (a.nu + ne) / a.tauw[i];
@@ -134,6 +155,10 @@ __global__ __launch_bounds__(256) void k_wm_cols(Dims d, Metrics m, WmArgs a, co
         const double r = wm_sum_256(s[q], sh);
         if (threadIdx.x == 0) a.cols[q * nxg + i] = r;
     }
+    // models 9, 12: U_inf = U(2, nyg, 2) (reference numbering) in the last
+    // column of the scratch row, which the per-column h (i < nx) leaves free
+    if ((a.model == 9 || a.model == 12) && i == nxg - 1 && threadIdx.x == 0)
+        a.cols[Q_H * nxg + i] = (double)U[fidx(d, 1, nyg - 1, 1)];
 }
 
 // f_law_of_wall, df_law_of_wall (Newton_solver.f90:98-118), kappa 0.38, B 5.2
@@ -149,6 +174,10 @@ __global__ __launch_bounds__(256) void k_wm_final(Dims d, Metrics m, WmArgs a) {
     const int nx = d.nx, nxg = d.nxg, ny = d.ny, nzg = d.nzg;
     const int tid = threadIdx.x;
     const double *cols = a.cols;
+    // model 9 reads beta_y of the last boundary call before this kernel
+    // replaces it
+    const double beta_prev = a.beta[0];
+    __syncthreads();
     if (!a.beta_only) {
         // utau_model (compute_wall_model :40-46)
         if (a.istress == 1) {
@@ -220,6 +249,46 @@ __global__ __launch_bounds__(256) void k_wm_final(Dims d, Metrics m, WmArgs a) {
             const double al = a.model == 6 ? (nb + a.nu) / (0.38 * a.utau_p) * a.logf : (m.y[1] - m.y[0]) * a.logf;
             for (int i = tid; i < nx; i += 256) a.ax[i] = al;
             for (int i = tid; i < nxg; i += 256) a.ay[i] = al, a.az[i] = al;
+        } else if (a.model == 9 || a.model == 12) {
+            // compute_alpha_v_from_Cf (:1532-1697), compute_alpha_u_from_Cf
+            // (:1263-1424): tau_w from Cf = 0.027 Re_x^(-1/7) with U_inf the
+            // field's U(2, nyg, 2); alpha_x from the viscous fraction; per
+            // column h = (nu <dU/dy> - tau_w - <tau_uv> [+ beta_y <U>_wall])
+            // / ([alpha_x] <...>), alpha_y the mean of h over the two
+            // columns beside each V face (model 12 clipped to 2e-3)
+            const double Uinf = cols[Q_H * nxg + nxg - 1];
+            const double c2 = (double)(2 * (nzg - 2)), c1 = (double)(nzg - 2);
+            for (int i = tid; i < nx; i += 256) {
+                const double tw = 0.5 * (0.027 * pow(Uinf * m.x[i] / a.nu, -1.0 / 7.0)) * (Uinf * Uinf);
+                const double um = cols[Q_UW * nxg + i] / c2;
+                double al;
+                if (a.nut_flag == 2) {
+                    const double ur = cols[Q_UR * nxg + i] / c2;
+                    const double fd = a.nu * ur / (tw * m.y[2]);
+                    al = a.nu * um / (fd * tw);
+                } else {
+                    al = a.nu * um / (a.frac_vis * tw);
+                }
+                if (al < 0.0) al = 1e-3;
+                if (fabs(al) < 1e-8) al = 1e-3;
+                a.ax[i] = al;
+                const double mt = cols[Q_TAU * nxg + i] / c1, duv = cols[Q_DUVB * nxg + i] / c1,
+                             du = cols[Q_DUB * nxg + i] / c1;
+                a.cols[Q_H * nxg + i] = a.model == 12 ? (a.nu * du - tw - mt) / (al * duv)
+                                                      : (a.nu * du - tw - mt + um * beta_prev) / duv;
+            }
+            __syncthreads();
+            for (int i = 1 + tid; i <= nxg - 2; i += 256) {
+                double al = 0.5 * (cols[Q_H * nxg + i - 1] + cols[Q_H * nxg + i]);
+                if (a.model == 12 && fabs(al) > 2e-3) al = 2e-3 * al / fabs(al);
+                a.ay[i] = al;
+                a.az[i] = al;
+            }
+            __syncthreads();
+            if (tid == 0) {
+                a.ay[0] = a.ay[1], a.az[0] = a.az[1];
+                a.ay[nxg - 1] = a.ay[nxg - 2], a.az[nxg - 1] = a.az[nxg - 2];
+            }
         } else if (a.model == 11 || a.model == 15) {
             // compute_alpha_u_momentum_wall_model (:1444-1500),
             // compute_exact_Neumann_from_Cf (:2003-2066)
@@ -281,12 +350,22 @@ double *wm_alloc(size_t n) {
 
 bool wall_model_robin(int model) { return wm_robin(model); }
 
-void wall_model_set(Handle &h, int model, int istress, const double am[3], double astd, double freq) {
+void wall_model_set(Handle &h, int model, int istress, const double am[3], double astd, double freq, int nut_flag,
+                    double frac_vis) {
     const Dims &d = h.d;
-    const bool ok = model == 0 || model == 4 || model == 5 || model == 6 || model == 8 || model == 11 || model == 15;
+    const bool ok = model == 0 || model == 4 || model == 5 || model == 6 || model == 8 || model == 9 || model == 11 ||
+                    model == 12 || model == 15;
     if (!ok) {
         last_error() = "bl_set_wall_model: iwall_model " + std::to_string(model) +
-                       " is not built (0, 4, 5, 6, 8, 11 and 15 are)";
+                       " is not built (0, 4, 5, 6, 8, 9, 11, 12 and 15 are)";
+        throw ArgError();
+    }
+    if ((model == 9 || model == 12) && nut_flag != 0 && nut_flag != 2) {
+        last_error() = "bl_set_wall_model: iwall_model_nut must be 0 (fixed viscous fraction) or 2 (dynamic)";
+        throw ArgError();
+    }
+    if ((model == 9 || model == 12) && nut_flag == 0 && frac_vis == 0.0) {
+        last_error() = "bl_set_wall_model: models 9 and 12 with iwall_model_nut 0 need frac_vis != 0";
         throw ArgError();
     }
     if (istress != 0 && istress != 1) {
@@ -325,6 +404,8 @@ void wall_model_set(Handle &h, int model, int istress, const double am[3], doubl
     for (int c = 0; c < 3; ++c) h.wm_am[c] = am[c];
     h.wm_std = astd;
     h.wm_freq = freq;
+    h.wm_nut_flag = nut_flag;
+    h.wm_frac_vis = frac_vis;
     h.bc.iwall = model;
     h.bc.wm_ax = L.ax;
     h.bc.wm_ay = L.ay;
@@ -343,6 +424,8 @@ void launch_wall_model(Handle &h, const Fld3<T> &cur, const Fld3<T> &old, bool b
     a.istress = h.wm_istress;
     a.les = h.les;
     a.nut_dirichlet = h.nut_bc;
+    a.nut_flag = h.wm_nut_flag;
+    a.frac_vis = h.wm_frac_vis;
     a.beta_only = beta_only ? 1 : 0;
     a.nu = h.nu;
     a.dPdx = h.dPdx;

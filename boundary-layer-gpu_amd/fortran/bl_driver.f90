@@ -119,6 +119,10 @@ module bl_state
   ! alpha_mean_x, alpha_mean_y, alpha_mean_z, alpha_std, freq_mult
   integer :: istress_model = 0
   real(real64) :: alphas(5) = 0d0
+  ! models 9, 12: WMnutflag (iwall_model_nut) and WMnut (frac_vis_wall_model,
+  ! params.f90:159-160)
+  integer :: iwall_model_nut = 0
+  real(real64) :: frac_vis = 0d0
   integer :: nstep_init = 0, nstep_init_input = -45, istep = 0
   real(real64) :: nu = 0d0, cfl = -1d-4, dPdx = 0d0, dPdz = 0d0, t = 0d0, dt = 0d0
   real(real64) :: boxsize(4) = 0d0
@@ -199,6 +203,8 @@ contains
     call get_int('WM', iwall_model, f)
     call get_int('TauwModel', istress_model, f)
     call get_dbl_arr('alphas', alphas, f)
+    call get_int('WMnutflag', iwall_model_nut, f)
+    call get_dbl('WMnut', frac_vis, f)
     call get_int('init_step', nstep_init_input, f)
     call get_int('init_rand', random_init, f)
     call get_str('filein', filein, f1)
@@ -214,7 +220,7 @@ contains
     ! this build runs the RK path of the reference with the GPU boundary
     ! conditions (time_integration.f90:145-465)
     if (itime_step /= 2 .and. itime_step /= 3) stop 'Error: only RKscheme = 2 or 3 is built'
-    if (all(iwall_model /= [0, 4, 5, 6, 8, 11, 15])) stop 'Error: WM must be 0, 4, 5, 6, 8, 11 or 15'
+    if (all(iwall_model /= [0, 4, 5, 6, 8, 9, 11, 12, 15])) stop 'Error: WM must be 0, 4, 5, 6, 8, 9, 11, 12 or 15'
     if (les_model < 0 .or. les_model > 3) stop 'Error: LES must be 0, 1, 2 or 3'
     if (iprecision /= 0 .and. iprecision /= 1) stop 'Error: precision must be 0 (FP64) or 1 (mixed)'
     if (inflow_flag < 1 .or. inflow_flag > 5) stop 'Error: inflow_flag must be 1 to 5'
@@ -690,6 +696,8 @@ program bl_driver
      wmp%alpha_mean_z = alphas(3)
      wmp%alpha_std = alphas(4)
      wmp%freq_mult = alphas(5)
+     wmp%iwall_model_nut = iwall_model_nut
+     wmp%frac_vis = frac_vis
      call bl_check(bl_set_wall_model(h, wmp), 'bl_set_wall_model')
   end if
   write(*,*) 'grid ', nx, ny, nz, ' RKscheme ', itime_step, ' LES ', les_model

@@ -42,6 +42,8 @@ module blgpu_api
   type, bind(C) :: bl_wall_params
      integer(c_int) :: iwall_model, istress_model
      real(c_double) :: alpha_mean_x, alpha_mean_y, alpha_mean_z, alpha_std, freq_mult
+     integer(c_int) :: iwall_model_nut
+     real(c_double) :: frac_vis
   end type bl_wall_params
 
   interface

@@ -154,12 +154,20 @@ int bl_set_boundary(bl_handle *h, const bl_boundary *b);
  *   6, 8: log-layer slip length with utau = dPdx^0.5, kappa 0.38, B 5.2,
  *      with the wall value of avg_nu_t (6) or nu_t = kappa utau y (8)
  *      (compute_log_layer_wall_model :987-1026); dPdx > 0;
+ *   9, 12: slip lengths from the Cf = 0.027 Re_x^(-1/7) correlation with
+ *      U_inf = U(2, nyg, 2): alpha_x = nu <U>_wall/(frac tau_w) with the fixed
+ *      viscous fraction frac_vis (iwall_model_nut 0) or the dynamic one
+ *      nu <U>_rows 3:4/(tau_w y(3)) (2); alpha_y = alpha_z from the wall
+ *      stress balance, averaged over the two columns beside each face
+ *      (compute_alpha_v_from_Cf :1532-1697 with the step-start V and the
+ *      last boundary call's beta_y; compute_alpha_u_from_Cf :1263-1424,
+ *      |alpha_y| <= 2e-3);
  *   11: Neumann dU/dy = (utau_model^2 + <tau_uv>_z)/nu, V = W = 0
  *      (compute_alpha_u_momentum_wall_model :1444-1500);
  *   15: Neumann dU/dy = 1/<(nu + nu_t)/tau_w>_z with tau_w from
  *      Cf = 0.027 Re_x^(-1/7), V = W = 0 (compute_exact_Neumann_from_Cf
  *      :2003-2066).
- * With 4, 5, 6, 8 the wall is Robin for U and W (apply_Robin_bc_y_bottom
+ * With 4, 5, 6, 8, 9, 12 the wall is Robin for U and W (apply_Robin_bc_y_bottom
  * :1107-1132) and pseudo-Robin for V from the step-start field with its wall
  * mean removed (apply_pseudo_Robin_bc_y_bottom :1152-1180).  istress_model 1
  * updates utau_model by Newton's method on the law of the wall at yg(3)
@@ -167,11 +175,14 @@ int bl_set_boundary(bl_handle *h, const bl_boundary *b);
  * utau_ref (initialization.f90:745).  avg_nu_t(1, :) of models 4 and 6 is the
  * constant-coefficient Smagorinsky's wall-parallel mean (LES 1,
  * subgrid.f90:648-660) and 0 for the other eddy viscosities, whose routines
- * do not write it.  Models 1, 2, 3, 7, 9, 12, 13, 14 are not built and give
- * BL_EINVAL.  Resets the slip lengths to 0 and utau_model to utau_ref. */
+ * do not write it.  Models 1, 2, 3, 7 (test-filtered slip lengths) and 13, 14
+ * (Cf-fitted wall blowing, whose alpha_z = alpha_x/10 assigns arrays of
+ * different shapes, wallmodel.f90:1933) are not built and give BL_EINVAL.  Resets the slip lengths to 0 and utau_model to utau_ref. */
 typedef struct bl_wall_params {
     int iwall_model, istress_model;
     double alpha_mean_x, alpha_mean_y, alpha_mean_z, alpha_std, freq_mult;
+    int iwall_model_nut;  /* WMnutflag (models 9, 12): 0 fixed, 2 dynamic viscous fraction */
+    double frac_vis;      /* WMnut (models 9, 12): the fixed viscous fraction */
 } bl_wall_params;
 int bl_set_wall_model(bl_handle *h, const bl_wall_params *p);
 /* compute_wall_model (wallmodel.f90:25-144) once on the current fields at

@@ -77,7 +77,8 @@ int orc_set_boundary(orc_state *s, int inflow_flag, int top_flag, double amplitu
  * Vmean_resc_T(ny), Umean_inlet_T(nyg), Vmean_inlet_T(ny). */
 void orc_lund_means(orc_state *s, double *umr, double *vmr, double *umi, double *vmi);
 /* Wall model (wallmodel.f90:25-144) and its bottom-wall conditions
- * (boundary_conditions.f90:73-108): iwall_model 0, 4, 5, 6, 8, 11, 15;
+ * (boundary_conditions.f90:73-108): iwall_model 0, 4, 5, 6, 8, 9, 11, 12, 15
+ * (9, 12: nut_flag = WMnutflag 0 or 2, frac_vis = WMnut);
  * istress_model 1 = law of the wall for utau_model.  alphas(5) of the .turbb
  * as amx, amy, amz, astd, freq_mult.  Resets the slip lengths to 0 and
  * utau_model to utau_ref (initialization.f90:745-746).  Returns 0, or -1
@@ -86,10 +87,12 @@ void orc_lund_means(orc_state *s, double *umr, double *vmr, double *umi, double 
  * iwall_model > 0; orc_wall_model does it once on the current fields at the
  * current t. */
 int orc_set_wall_model(orc_state *s, int iwall, int istress, double amx, double amy, double amz, double astd,
-                       double freq_mult);
+                       double freq_mult, int nut_flag, double frac_vis);
 void orc_wall_model(orc_state *s);
 /* alpha_x(nx), alpha_y(nxg), alpha_z(nxg), utau_model(nx) */
 void orc_get_wall_model(const orc_state *s, double *ax, double *ay, double *az, double *utau);
+/* beta_y of the last pseudo-Robin wall condition (model 9 reads it) */
+double orc_wall_beta(const orc_state *s);
 
 /* compute_rhs_u/v/w (equations.f90:24-415) on the current U,V,W into
  * full-extent arrays ru(nx,nyg,nzg), rv(nxg,ny,nzg), rw(nxg,nyg,nz)

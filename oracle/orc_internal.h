@@ -41,7 +41,8 @@ struct orc_state {
     double *Umean_resc_T, *Vmean_resc_T, *Umean_inlet_T, *Vmean_inlet_T;
     double *Ut_in, *Vt_in, *Wt_in;
     /* wall model (orc_wall.c) */
-    int iwall, istress;
+    int iwall, istress, nut_flag;
+    double frac_vis;
     double alpha_mean_x, alpha_mean_y, alpha_mean_z, alpha_std, freq_mult, beta_y;
     double *alpha_x, *alpha_y, *alpha_z, *utau_model, *utau_ref;
 };

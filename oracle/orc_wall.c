@@ -16,6 +16,9 @@
  *     (compute_constant_alpha :952-961);
  *     6, 8: log-layer slip length with the wall value of avg_nu_t or with
  *     nu_t = kappa utau y (compute_log_layer_wall_model :987-1026);
+ *     9, 12: slip lengths from the Cf correlation and the wall stress
+ *     balance (compute_alpha_v_from_Cf :1532-1697, compute_alpha_u_from_Cf
+ *     :1263-1424);
  *     11: dU/dy at the wall from utau_model and the resolved tau_uv
  *     (compute_alpha_u_momentum_wall_model :1444-1500);
  *     15: dU/dy at the wall from the turbulent Cf correlation
@@ -40,12 +43,17 @@
 #define B_LAW 5.20     /* B_model, :1966 */
 
 static int model_ok(int iwall) {
-    return iwall == 0 || iwall == 4 || iwall == 5 || iwall == 6 || iwall == 8 || iwall == 11 || iwall == 15;
+    return iwall == 0 || iwall == 4 || iwall == 5 || iwall == 6 || iwall == 8 || iwall == 9 || iwall == 11 ||
+           iwall == 12 || iwall == 15;
 }
 
 int orc_set_wall_model(orc_state *s, int iwall, int istress, double amx, double amy, double amz, double astd,
-                       double freq_mult) {
+                       double freq_mult, int nut_flag, double frac_vis) {
     if (!model_ok(iwall)) return -1;
+    if ((iwall == 9 || iwall == 12) && nut_flag != 0 && nut_flag != 2) return -1;
+    s->nut_flag = nut_flag;
+    s->frac_vis = frac_vis;
+    s->beta_y = 0.0;
     s->iwall = iwall;
     s->istress = istress;
     s->alpha_mean_x = amx;
@@ -84,6 +92,8 @@ void orc_get_wall_model(const orc_state *s, double *ax, double *ay, double *az, 
     memcpy(az, s->alpha_z, sizeof(double) * s->nxg);
     memcpy(utau, s->utau_model, sizeof(double) * s->nx);
 }
+
+double orc_wall_beta(const orc_state *s) { return s->beta_y; }
 
 /* f_law_of_wall, df_law_of_wall (Newton_solver.f90:98-118) */
 static double law_f(double u, double ygm, double nu, double Um) {
@@ -212,6 +222,65 @@ static void model_exact_neumann(orc_state *s) {
     memset(s->alpha_z, 0, sizeof(double) * s->nxg);
 }
 
+/* compute_alpha_v_from_Cf (wallmodel.f90:1532-1697, model 9) and
+ * compute_alpha_u_from_Cf (:1263-1424, model 12) */
+static void model_cf_slip(orc_state *s) {
+    const int nx = s->nx, nxg = s->nxg, nzg = s->nzg, m12 = s->iwall == 12;
+    const double *U = s->U, *V = m12 ? s->V : s->Vo;
+    const double Uinf = U[IU(2, s->nyg, 2)];
+    const double c2 = (double)(2 * (nzg - 2)), c1 = (double)(nzg - 2);
+    double *h = orc__dalloc(nx);
+    for (int i = 1; i <= nx; i++) {
+        double Cf = 0.027 * pow(Uinf * X(i) / s->nu, -1.0 / 7.0);
+        double tw = 0.5 * Cf * pow(Uinf, 2.0);
+        /* Part 0: alpha_x from the viscous fraction */
+        double um = 0.0, ur = 0.0;
+        for (int k = 2; k <= nzg - 1; k++) {
+            um += U[IU(i, 1, k)] + U[IU(i, 2, k)];
+            ur += U[IU(i, 3, k)] + U[IU(i, 4, k)];
+        }
+        um = um / c2;
+        ur = ur / c2;
+        double a;
+        if (s->nut_flag == 2) {
+            double fd = s->nu * ur / (tw * Y(3));
+            a = s->nu * um / (fd * tw);
+        } else {
+            a = s->nu * um / (s->frac_vis * tw);
+        }
+        if (a < 0.0) a = 1e-3;
+        if (fabs(a) < 1e-8) a = 1e-3;
+        s->alpha_x[i - 1] = a;
+        /* Parts 1-3: <tau_uv>, <dU/dy dV/dy> (9: <U dVo/dy>), <dU/dy> at the wall */
+        double mt = 0.0, duv = 0.0, du = 0.0;
+        for (int k = 2; k <= nzg - 1; k++) {
+            double g = (U[IU(i, 2, k)] - U[IU(i, 1, k)]) / (YG(2) - YG(1));
+            double sv = m12 ? suv(s, i, 1, k) : 0.5 * ((U[IU(i, 2, k)] - U[IU(i, 1, k)]) / (YG(2) - YG(1)));
+            mt += -2.0 * nut_edge(s, i, k) * sv;
+            double v1 = 0.5 * (V[IV(i, 1, k)] + V[IV(i + 1, 1, k)]);
+            double v2 = 0.5 * (V[IV(i, 2, k)] + V[IV(i + 1, 2, k)]);
+            duv += m12 ? (U[IU(i, 2, k)] - U[IU(i, 1, k)]) / (YG(2) - YG(1)) * (v2 - v1) / (Y(2) - Y(1))
+                       : 0.5 * (U[IU(i, 2, k)] + U[IU(i, 1, k)]) * (v2 - v1) / (Y(2) - Y(1));
+            du += g;
+        }
+        mt = mt / c1;
+        duv = duv / c1;
+        du = du / c1;
+        /* Part 4's per-column quotient */
+        h[i - 1] = m12 ? (s->nu * du - tw - mt) / (a * duv) : (s->nu * du - tw - mt + um * s->beta_y) / duv;
+    }
+    for (int i = 2; i <= nxg - 1; i++) s->alpha_y[i - 1] = 0.5 * (h[i - 2] + h[i - 1]);
+    s->alpha_y[0] = s->alpha_y[1];
+    s->alpha_y[nxg - 1] = s->alpha_y[nxg - 2];
+    if (m12)
+        for (int i = 1; i <= nxg; i++) {
+            double a = s->alpha_y[i - 1];
+            if (fabs(a) > 2e-3) s->alpha_y[i - 1] = 2e-3 * a / fabs(a);
+        }
+    memcpy(s->alpha_z, s->alpha_y, sizeof(double) * nxg);
+    free(h);
+}
+
 /* the factor log(Delta utau/nu e^(B kappa)) of compute_log_layer_wall_model,
  * kappa = 0.38, B = 5.2, Delta = y(2) - y(1), utau = dPdx^0.5 */
 static double log_factor(const orc_state *s, double *utau) {
@@ -239,6 +308,8 @@ void orc_wall_model(orc_state *s) {
         const double L = log_factor(s, &utau);
         const double a = s->iwall == 6 ? (avg_face(s, 1) + s->nu) / (0.38 * utau) * L : (Y(2) - Y(1)) * L;
         fill3(s, a);
+    } else if (s->iwall == 9 || s->iwall == 12) {
+        model_cf_slip(s);
     } else if (s->iwall == 11) {
         model_u_momentum(s);
     } else if (s->iwall == 15) {

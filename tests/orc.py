@@ -41,9 +41,10 @@ def lib():
             "orc_step": (None, [_P]),
             "orc_set_boundary": (_I, [_P, _I, _I, _D, ctypes.c_ulonglong, _D, _D, _D, _D, _I, _D, _D, _D]),
             "orc_lund_means": (None, [_P, _P, _P, _P, _P]),
-            "orc_set_wall_model": (_I, [_P, _I, _I, _D, _D, _D, _D, _D]),
+            "orc_set_wall_model": (_I, [_P, _I, _I, _D, _D, _D, _D, _D, _I, _D]),
             "orc_wall_model": (None, [_P]),
             "orc_get_wall_model": (None, [_P, _P, _P, _P, _P]),
+            "orc_wall_beta": (_D, [_P]),
             "orc_rhs": (None, [_P, _P, _P, _P]),
             "orc_apply_bc": (None, [_P, _I, _D]),
             "orc_set_old": (None, [_P, _P, _P, _P]),
@@ -226,14 +227,17 @@ class Oracle:
         return a, b, c, d
 
     def set_wall_model(self, iwall_model=0, istress_model=0, alpha_mean_x=0.0, alpha_mean_y=0.0, alpha_mean_z=0.0,
-                       alpha_std=0.0, freq_mult=0.0):
+                       alpha_std=0.0, freq_mult=0.0, iwall_model_nut=0, frac_vis=0.0):
         rc = lib().orc_set_wall_model(self.h, iwall_model, istress_model, alpha_mean_x, alpha_mean_y, alpha_mean_z,
-                                      alpha_std, freq_mult)
+                                      alpha_std, freq_mult, iwall_model_nut, frac_vis)
         if rc != 0:
             raise ValueError("orc_set_wall_model: unsupported model")
 
     def wall_model(self):
         lib().orc_wall_model(self.h)
+
+    def wall_beta(self):
+        return lib().orc_wall_beta(self.h)
 
     def get_wall_model(self):
         """alpha_x(nx), alpha_y(nxg), alpha_z(nxg), utau_model(nx)"""

@@ -280,7 +280,7 @@ def test_driver_requires_input_file(tmp_path):
 @needs_driver
 @pytest.mark.parametrize("key,val,msg", [
     ("RKscheme", "1", "only RKscheme = 2 or 3"),
-    ("WM", "1", "WM must be 0, 4, 5, 6, 8, 11 or 15"),
+    ("WM", "1", "WM must be 0, 4, 5, 6, 8, 9, 11, 12 or 15"),
     ("LES", "5", "LES must be 0, 1, 2 or 3"),
     ("inflow_flag", "3", "Lund inflow (3, 5) starts from a flow field file"),
     ("top_flag", "5", "top_flag"),

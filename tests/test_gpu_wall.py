@@ -19,25 +19,30 @@ import sys
 import numpy as np
 import pytest
 
-from . import cases
+from . import cases, orc
 from .blgpu_import import blgpu, solver_for
 
 pytestmark = pytest.mark.gpu
 
 KAPPA, B = 0.38, 5.2
 
-# name: (case keywords, wall-model keywords, law-of-the-wall field)
+# name: (case keywords, wall-model keywords, field: None, "law" (U row 3 on
+# the log law of a chosen utau(x)) or "cf" (a positive wall mean of U))
 MODELS = {
     "robin5": (dict(dPdx=4e-4), dict(iwall_model=5, alpha_mean_x=3e-4, alpha_mean_y=2e-4, alpha_mean_z=4e-4,
-                                     alpha_std=0.3, freq_mult=0.2), False),
-    "momentum4": (dict(dPdx=-0.1, noise=0.05), dict(iwall_model=4), False),
-    "log6": (dict(dPdx=2.5e-3), dict(iwall_model=6), False),
-    "log6_smag": (dict(dPdx=2.5e-3, les=1), dict(iwall_model=6), False),
-    "log8": (dict(dPdx=2.5e-3), dict(iwall_model=8), False),
-    "u_momentum11": (dict(), dict(iwall_model=11, istress_model=1), True),
-    "u_momentum11_dsm": (dict(les=2), dict(iwall_model=11, istress_model=1), True),
-    "neumann15": (dict(), dict(iwall_model=15), False),
-    "neumann15_smag": (dict(dPdx=1e-3, les=1), dict(iwall_model=15), False),
+                                     alpha_std=0.3, freq_mult=0.2), None),
+    "momentum4": (dict(dPdx=-0.1, noise=0.05), dict(iwall_model=4), None),
+    "log6": (dict(dPdx=2.5e-3), dict(iwall_model=6), None),
+    "log6_smag": (dict(dPdx=2.5e-3, les=1), dict(iwall_model=6), None),
+    "log8": (dict(dPdx=2.5e-3), dict(iwall_model=8), None),
+    "cf9": (dict(), dict(iwall_model=9, iwall_model_nut=0, frac_vis=0.05), "cf"),
+    "cf9_dsm_dyn": (dict(les=2), dict(iwall_model=9, iwall_model_nut=2), "cf"),
+    "u_momentum11": (dict(), dict(iwall_model=11, istress_model=1), "law"),
+    "u_momentum11_dsm": (dict(les=2), dict(iwall_model=11, istress_model=1), "law"),
+    "cf12": (dict(), dict(iwall_model=12, iwall_model_nut=0, frac_vis=0.05), "cf"),
+    "cf12_smag_dyn": (dict(dPdx=1e-3, les=1), dict(iwall_model=12, iwall_model_nut=2), "cf"),
+    "neumann15": (dict(), dict(iwall_model=15), None),
+    "neumann15_smag": (dict(dPdx=1e-3, les=1), dict(iwall_model=15), None),
 }
 
 
@@ -51,27 +56,36 @@ def rel(a, b):
     return np.linalg.norm(a - b) / (nb if nb > 0 else 1.0)
 
 
-def _setup(name, precision=None):
+def _oracle_setup(name):
     ckw, wkw, law = MODELS[name]
     args = dict(nx=30, ny=18, nz=10, noise=0.02, seed=8, cfl=-1e-4)
     args.update(ckw)
     c = cases.laminar_small(**args)
     o = c.oracle()
     o.set_nut_bc(1)
-    if law:
+    if law == "law":
         # U row 3 at the log-law velocity of a chosen utau(x)
         U, V, W = o.fields()
-        from . import orc
         _, yg = orc.centers(c.y)
         ut = 0.045 + 0.005 * np.sin(np.linspace(0, 3, c.x.size))
         U[:, 2, :] = (ut / KAPPA * np.log(yg[2] * ut / c.nu) + B * ut)[:, None]
         o.set_fields(U, V, W)
+    elif law == "cf":
+        U, V, W = o.fields()
+        U[:, 1, :] = np.abs(U[:, 1, :]) + 0.002
+        U[:, 0, :] = 0.3 * U[:, 1, :]
+        o.set_fields(U, V, W)
+    o.set_wall_model(**wkw)
+    return c, o
+
+
+def _setup(name, precision=None):
+    c, o = _oracle_setup(name)
     kw = dict(nut_bc=1)
     if precision is not None:
         kw["precision"] = precision
     s, _ = solver_for(c, o, **kw)
-    o.set_wall_model(**wkw)
-    s.set_wall_model(**wkw)
+    s.set_wall_model(**MODELS[name][1])
     return c, o, s
 
 
@@ -79,6 +93,11 @@ def _setup(name, precision=None):
 def test_wall_model_matches_oracle(name):
     c, o, s = _setup(name)
     o.t = s.t = 0.37
+    # the step-start field (model 9 and the pseudo-Robin V read it)
+    U0, V0, W0 = o.fields()
+    Vo = V0 * 0.98
+    o.set_old(U0, Vo, W0)
+    s.set_old(U0, Vo, W0)
     if c.les:
         o.eddy_viscosity()
         s.eddy_viscosity()
@@ -91,10 +110,6 @@ def test_wall_model_matches_oracle(name):
     ax = o.get_wall_model()[0]
     assert np.abs(ax).max() > 0
     # one boundary call on the model's slip lengths
-    U0, V0, W0 = o.fields()
-    Vo = V0 * 0.98
-    o.set_old(U0, Vo, W0)
-    s.set_old(U0, Vo, W0)
     o.apply_bc(1, 1e-4)
     s.apply_bc(1, 1e-4)
     for comp, (a, b) in zip("UVW", zip(s.fields(), o.fields())):
@@ -105,22 +120,56 @@ def test_wall_model_matches_oracle(name):
     o.close()
 
 
+def _oracle_sensitivity(name, nsteps):
+    """Relative L2 change of the oracle's U, V, W after nsteps when its
+    initial U is perturbed by 1e-16 relative: the Cf-fitted models divide by
+    column means of wall gradient products that are near zero in places on
+    this laminar field, so the model itself amplifies round-off (model 12:
+    1e-12 after one step, 5e-11 after two; model 9 more, and its V wall
+    blows the flow up by the third step)."""
+    _, o = _oracle_setup(name)
+    U, V, W = o.fields()
+    rng = np.random.default_rng(3)
+    o.set_fields(U * (1 + 1e-16 * rng.standard_normal(U.shape)), V, W)
+    _, o2 = _oracle_setup(name)
+    for _ in range(nsteps):
+        for oo in (o, o2):
+            oo.compute_dt()
+            oo.step()
+    e = max(rel(a, b) for a, b in zip(o.fields(), o2.fields()))
+    o.close()
+    o2.close()
+    return e
+
+
 @pytest.mark.parametrize("name", list(MODELS))
 def test_steps_with_wall_model(name):
+    """Three RK3 steps (one for model 9) within 1e-12, or for the Cf-fitted
+    models 9 and 12 within 100x the oracle's own response to a 1e-16
+    perturbation of its initial U, whichever is larger: the device rounds
+    differently at every operation of every stage, not only once in the
+    initial field (measured: 0.02-0.5x that response for model 12, 10-20x
+    for model 9's single step)."""
+    nsteps = 1 if name.startswith("cf9") else 3
+    tol = 1e-12
+    if name.startswith("cf"):
+        sens = _oracle_sensitivity(name, nsteps)
+        say("steps", name, "oracle sensitivity", sens)
+        tol = max(tol, 100.0 * sens)
     c, o, s = _setup(name)
-    for _ in range(3):
+    for _ in range(nsteps):
         o.compute_dt()
         o.step()
         s.compute_dt()
         s.step()
     for comp, (a, b) in zip("UVW", zip(s.fields(), o.fields())):
         e = rel(a, b)
-        say("steps", name, comp, e)
-        assert e < 1e-12, comp
+        say("steps", name, comp, e, "bar", tol)
+        assert e < tol, comp
     for what, a, b in zip(("alpha_x", "alpha_y", "alpha_z"), s.get_wall_model()[:3], o.get_wall_model()[:3]):
         e = np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)
         say("steps", name, what, e)
-        assert e < 1e-11, what
+        assert e < max(1e-11, 1e3 * tol), what
     div, odiv = s.max_divergence(), o.max_divergence()
     say("steps", name, "max divergence", div, "oracle", odiv)
     assert div < max(1e-12, 2.0 * odiv)
@@ -164,8 +213,9 @@ def test_mixed_robin_wall():
 def test_rejected_wall_models_keep_the_handle():
     c = cases.laminar_small(nx=30, ny=18, nz=10, noise=0.02, seed=8, cfl=-1e-4)
     s, o = solver_for(c)
-    for bad in (dict(iwall_model=1), dict(iwall_model=9), dict(iwall_model=13), dict(iwall_model=6),
-                dict(iwall_model=5, istress_model=2)):
+    for bad in (dict(iwall_model=1), dict(iwall_model=7), dict(iwall_model=13), dict(iwall_model=6),
+                dict(iwall_model=5, istress_model=2), dict(iwall_model=9, iwall_model_nut=1),
+                dict(iwall_model=12, frac_vis=0.0)):
         with pytest.raises(blgpu.BLError):
             s.set_wall_model(**bad)
     s.compute_dt()

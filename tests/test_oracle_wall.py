@@ -18,6 +18,13 @@ and of the bottom-wall conditions that use them (boundary_conditions.f90:
   nu_t = 0) and the Cf correlation's tau_w/nu (15);
 - model 4 (channel momentum balance): alpha^2 <dU/dy dV/dy> = nu <dU/dy> -
   dPdx Ly/2 over both walls (nu_t = 0);
+- models 9, 12 (Cf-fitted slip): alpha_x is the slip length whose viscous
+  stress nu <U>_wall/alpha_x is frac_vis tau_w of White's correlation with
+  U_inf = U(2, nyg, 2), and with the dynamic fraction (WMnutflag 2) it is
+  y(3) <U>_wall/<U>_rows3:4, in which nu and tau_w cancel; alpha_y is the
+  two-column mean of the slip length that closes the wall stress balance
+  nu <dU/dy> - tau_w - <tau_uv> (+ beta_y <U>_wall for 9) = alpha_x alpha_y
+  <dU/dy dV/dy> (12) or alpha_y <U dVo/dy> (9), 12 clipped to 2e-3;
 - a step with a Robin wall keeps the divergence at round-off.
 """
 import math
@@ -216,10 +223,92 @@ def test_step_with_robin_wall_keeps_divergence():
     o.close()
 
 
+def _cf_field(c, o):
+    """U's first row made positive and its wall ghost row 0.3 of it, so the
+    wall mean of U is positive; U_inf = U(2, nyg, 2) set to 1.02"""
+    U, V, W = o.fields()
+    U[:, 1, :] = np.abs(U[:, 1, :]) + 0.002
+    U[:, 0, :] = 0.3 * U[:, 1, :]
+    U[1, -1, 1] = 1.02
+    o.set_fields(U, V, W)
+    return U, V, W
+
+
+def _white_tau(c, uinf):
+    return 0.5 * 0.027 * (uinf * c.x / c.nu) ** (-1.0 / 7.0) * uinf ** 2
+
+
+@pytest.mark.parametrize("model", [9, 12])
+def test_cf_fitted_slip_models(model):
+    c = _case(noise=0.05)
+    o = c.oracle()
+    U, V, W = _cf_field(c, o)
+    Vo = 0.9 * V + 1e-4
+    o.set_old(U, Vo, W)
+    _, yg = _geo(c)
+    y = c.y
+    nzg = U.shape[2]
+    ks = slice(1, nzg - 1)
+    um = 0.5 * (U[:, 0, ks] + U[:, 1, ks]).mean(axis=1)
+    tw = _white_tau(c, 1.02)
+    frac = 0.05
+    o.set_wall_model(model, iwall_model_nut=0, frac_vis=frac)
+    beta = 0.0
+    if model == 9:
+        # model 9 reads beta_y of the last pseudo-Robin wall condition: one
+        # model + boundary call sets it, then the field is put back
+        o.wall_model()
+        o.apply_bc(1, 1e-4)
+        o.set_fields(U, V, W)
+        beta = o.wall_beta()
+        assert beta != 0.0
+    o.wall_model()
+    ax, ay, az, _ = o.get_wall_model()
+    # viscous stress of the slip velocity = frac tau_w
+    assert np.all(um > 0)
+    assert np.allclose(c.nu * um / ax, frac * tw, rtol=1e-13)
+    # the stress balance per column, averaged over the two columns of a face
+    g = ((U[:, 1, ks] - U[:, 0, ks]) / (yg[1] - yg[0])).mean(axis=1)
+    VV = V if model == 12 else Vo
+    vx = 0.5 * (VV[:-1] + VV[1:])
+    dv = (vx[:, 1, ks] - vx[:, 0, ks]) / (y[1] - y[0])
+    if model == 12:
+        prod = (((U[:, 1, ks] - U[:, 0, ks]) / (yg[1] - yg[0])) * dv).mean(axis=1)
+        h = (c.nu * g - tw) / (ax * prod)
+    else:
+        prod = (0.5 * (U[:, 1, ks] + U[:, 0, ks]) * dv).mean(axis=1)
+        h = (c.nu * g - tw + um * beta) / prod
+    want = np.empty(ay.size)
+    want[1:-1] = 0.5 * (h[:-1] + h[1:])
+    want[0], want[-1] = want[1], want[-2]
+    if model == 12:
+        want = np.where(np.abs(want) > 2e-3, 2e-3 * np.sign(want), want)
+    assert np.allclose(ay, want, rtol=1e-11, atol=1e-18)
+    assert np.array_equal(ay, az)
+    o.close()
+
+
+def test_cf_model12_dynamic_fraction_and_clip():
+    c = _case(noise=0.05)
+    o = c.oracle()
+    U, V, W = _cf_field(c, o)
+    o.set_wall_model(12, iwall_model_nut=2)
+    o.wall_model()
+    ax, ay, _, _ = o.get_wall_model()
+    nzg = U.shape[2]
+    ks = slice(1, nzg - 1)
+    um = 0.5 * (U[:, 0, ks] + U[:, 1, ks]).mean(axis=1)
+    ur = 0.5 * (U[:, 2, ks] + U[:, 3, ks]).mean(axis=1)
+    assert np.allclose(ax, c.y[2] * um / ur, rtol=1e-13)
+    assert np.abs(ay).max() <= 2e-3
+    assert np.any(np.abs(ay) == 2e-3)  # the clip is reached in this case
+    o.close()
+
+
 def test_unsupported_models_rejected():
     c = _case()
     o = c.oracle()
-    for m in (1, 2, 3, 7, 9, 12, 13, 14, 16):
+    for m in (1, 2, 3, 7, 13, 14, 16):
         with pytest.raises(ValueError):
             o.set_wall_model(m)
     o.close()
