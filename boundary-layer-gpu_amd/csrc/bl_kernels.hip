@@ -40,15 +40,22 @@ __device__ __forceinline__ T sq(T a) {
 // instead of two (old and F1): a21 = a31 = 1/4 (initialization.f90:526,528)
 // makes Uo + dt a21 F1 and Uo + dt a31 F1 the same field, and nothing
 // writes G between stages 2 and 3.  A block owns a
-// 64 (x) x 4 RPT (y) column of cells and a chunk of z planes; U, V, W of
-// planes k-1, k, k+1 live in an LDS ring with a one-cell halo in x and y, so
-// every velocity value is read from HBM once per chunk (plus halo) instead of
-// once per neighbour.  Loads are x-fastest and 512-byte aligned rows.
-// Thread (tx, ty) computes cells (i0+tx, j0+RPT ty+r), r < RPT.  FP64: 64 x 8
-// tiles (RPT = 2, 47.5 KB of LDS, 164 VGPRs: 3 blocks per CU); FP32: 64 x 12
-// tiles (RPT = 3, 33 KB, 161 VGPRs: 3 blocks per CU; 64 x 16 tiles held 2
-// blocks per CU and took 3 % longer at 814x257x193).  128-wide FP32 tiles
-// (fewer x-halo lines) measured slower.
+// 64 (x) x 4 RPT (y) column of cells and a chunk of z planes; U and V of
+// planes k, k+1 and W of planes k-1, k, k+1 live in an LDS ring with a
+// one-cell halo in x and y (the stencil reads U and V of plane k-1 only at
+// the cell itself, which each thread carries in registers from the previous
+// plane), so every velocity value is read from HBM once per chunk (plus
+// halo) instead of once per neighbour.  Loads are x-fastest and 512-byte
+// aligned rows.  The x and y metrics of the tile are staged in LDS once:
+// read from global memory inside each equation's branch they cost a load
+// latency per branch and plane (2.69 -> 2.00 ms of RK stages per FP64 step,
+// 1.71 -> 1.47 in FP32, at 814x257x193).  Thread (tx, ty) computes cells
+// (i0+tx, j0+RPT ty+r), r < RPT.  FP64: 64 x 8 tiles (RPT = 2, 39.5 KB of
+// LDS, 140 VGPRs: 3 blocks per CU; RPT = 1 took 16 % longer, and 4 blocks
+// per CU forced by a 128-VGPR bound spilled and took 12 % longer); FP32:
+// 64 x 12 tiles (RPT = 3, 27 KB; 64 x 16 tiles held 2 blocks per CU and
+// took 3 % longer).  128-wide FP32 tiles (fewer x-halo lines) measured
+// slower.
 // ------------------------------------------------------------------------
 template <class T>
 struct MarchTile {
@@ -70,8 +77,15 @@ __global__ __launch_bounds__(256) void k_rk_march(
     T *__restrict__ Fov, T *__restrict__ Fow, T c0, T c1, T dt, T g2) {
     constexpr int RPT = MarchTile<T>::RPT, kTY = MarchTile<T>::TY, kPlane = MarchTile<T>::Plane;
     constexpr int kTX = MarchTile<T>::TX, kHX = MarchTile<T>::HX;
-    __shared__ T sU[3][kPlane], sV[3][kPlane], sW[3][kPlane];
+    // U and V: planes k and k+1 (their plane k-1 value at the thread's own
+    // cells is carried in registers); W: planes k-1, k, k+1
+    __shared__ T sU[2][kPlane], sV[2][kPlane], sW[3][kPlane];
     __shared__ T sN[LES ? 3 : 1][LES ? kPlane : 1];
+    // the tile's x metrics (rdx(i), rdx(i+1), rdxg(i), rdxg(i-1)) and y metrics
+    // (wy0(j), wy1(j), wy0(j-1), wy1(j-1), rdy(j), rdy(j+1), rdyg(j),
+    // rdyg(j-1)): constant along the march, read from LDS with the velocities
+    // instead of from global memory inside each equation's branch
+    __shared__ T sMx[4][kTX], sMy[8][kTY];
     const int tid = threadIdx.x;
     const int tx = tid % kTX, ty = tid / kTX;
     const int nbx = (d.nxg - 1 + kTX - 1) / kTX, nby = (d.nyg - 2 + kTY - 1) / kTY;
@@ -109,24 +123,53 @@ __global__ __launch_bounds__(256) void k_rk_march(
             if constexpr (LES) rn[q] = NT[base + off[q]];
         }
     };
-    auto commit = [&](int slot) {
+    auto commit = [&](int plane) {
+        const int s2 = plane & 1, s3 = plane % 3;
 #pragma unroll
         for (int q = 0; q < kPer; ++q) {
             const int e = tid + 256 * q;
             if (e < kPlane) {
-                sU[slot][e] = ru[q];
-                sV[slot][e] = rv[q];
-                sW[slot][e] = rw[q];
-                if constexpr (LES) sN[slot][e] = rn[q];
+                sU[s2][e] = ru[q];
+                sV[s2][e] = rv[q];
+                sW[s3][e] = rw[q];
+                if constexpr (LES) sN[s3][e] = rn[q];
             }
         }
     };
+    if (tid < kTX) {
+        const int ii = max(1, i0 + tid);  // entries of cells outside the grid are clamped and never used
+        sMx[0][tid] = m.rdx[min(ii, d.nx - 1)];
+        sMx[1][tid] = m.rdx[min(ii + 1, d.nx - 1)];
+        sMx[2][tid] = m.rdxg[min(ii, d.nxg - 1)];
+        sMx[3][tid] = m.rdxg[min(ii - 1, d.nxg - 1)];
+    } else if (tid < kTX + kTY) {
+        const int jt = tid - kTX, jj = j0 + jt;
+        sMy[0][jt] = m.wy0[min(jj, d.ny - 1)];
+        sMy[1][jt] = m.wy1[min(jj, d.ny - 1)];
+        sMy[2][jt] = m.wy0[min(jj - 1, d.ny - 1)];
+        sMy[3][jt] = m.wy1[min(jj - 1, d.ny - 1)];
+        sMy[4][jt] = m.rdy[min(jj, d.ny - 1)];
+        sMy[5][jt] = m.rdy[min(jj + 1, d.ny - 1)];
+        sMy[6][jt] = m.rdyg[min(jj, d.nyg - 1)];
+        sMy[7][jt] = m.rdyg[min(jj - 1, d.nyg - 1)];
+    }
     fetch(kb - 1);
-    commit((kb - 1) % 3);
+    commit(kb - 1);
     fetch(kb);
-    commit(kb % 3);
+    commit(kb);
     fetch(kb + 1);
     const int i = i0 + tx;
+    // U and V of plane kb-1 at the thread's cells, before plane kb+1 takes
+    // their slot
+    __syncthreads();
+    T ubr[RPT], vbr[RPT];
+#pragma unroll
+    for (int r = 0; r < RPT; ++r) {
+        const int e = (RPT * ty + r + 1) * kHX + (tx + 1);
+        ubr[r] = sU[(kb - 1) & 1][e];
+        vbr[r] = sV[(kb - 1) & 1][e];
+    }
+    __syncthreads();  // before plane kb+1 overwrites that slot
     // a first stage starts from the field it differentiates: the old values
     // are then the LDS centre values and are not read again
     const bool old_is_cur = MODE == 0 && Uo == U && Vo == V && Wo == W;
@@ -171,42 +214,51 @@ __global__ __launch_bounds__(256) void k_rk_march(
                 }
             }
         }
-        commit((k + 1) % 3);
+        commit(k + 1);
         __syncthreads();
         if (k + 2 <= ke) fetch(k + 2);
         if (ahead && k + 1 < ke) fetch_old(k + 1);
+        // W (and nu_t): three slots; U, V: two
         const int sm = (k - 1) % 3, sc = k % 3, sp = (k + 1) % 3;
+        const int uc = k & 1, up1 = (k + 1) & 1;
         const T rdzk = m.rdz[k], rdzgk = m.rdzg[k], rdzgm = m.rdzg[k - 1];
 #pragma unroll
         for (int r = 0; r < RPT; ++r) {
             const int jt = RPT * ty + r;     // row in the tile
             const int j = j0 + jt;
-            if (i < 1 || i > d.nxg - 2 || j > d.nyg - 2) continue;
             const int e = (jt + 1) * kHX + (tx + 1);  // LDS index of (i, j)
+            // U, V of plane k-1 at this cell, and plane k's for the next plane
+            const T ubc = ubr[r], vbc = vbr[r];
+            ubr[r] = sU[uc][e];
+            vbr[r] = sV[uc][e];
+            if (i < 1 || i > d.nxg - 2 || j > d.nyg - 2) continue;
+            const T x_rdx = sMx[0][tx], x_rdx1 = sMx[1][tx], x_rdxg = sMx[2][tx], x_rdxgm = sMx[3][tx];
+            const T y_w0 = sMy[0][jt], y_w1 = sMy[1][jt], y_w0m = sMy[2][jt], y_w1m = sMy[3][jt];
+            const T y_rdy = sMy[4][jt], y_rdy1 = sMy[5][jt], y_rdyg = sMy[6][jt], y_rdygm = sMy[7][jt];
             const long long p = fidx(d, i, j, k);
             // ---- u (equations.f90:47-94, 149-154)
             if (i <= d.nx - 2) {
                 T rr = 0.0;
                 if (i < d.nx - 2) {
-                    const T up = sU[sc][e], ue = sU[sc][e + 1], uw = sU[sc][e - 1];
-                    const T un = sU[sc][e + kHX], us = sU[sc][e - kHX];
-                    const T ut = sU[sp][e], ub = sU[sm][e];
-                    rr = -(sq(0.5 * (up + ue)) - sq(0.5 * (uw + up))) * m.rdxg[i];
-                    rr = rr - ((m.wy0[j] * up + m.wy1[j] * un) * 0.5 * (sV[sc][e] + sV[sc][e + 1]) -
-                               (m.wy0[j - 1] * us + m.wy1[j - 1] * up) * 0.5 *
-                                   (sV[sc][e - kHX] + sV[sc][e + 1 - kHX])) *
-                                  m.rdy[j];
+                    const T up = sU[uc][e], ue = sU[uc][e + 1], uw = sU[uc][e - 1];
+                    const T un = sU[uc][e + kHX], us = sU[uc][e - kHX];
+                    const T ut = sU[up1][e], ub = ubc;
+                    rr = -(sq(0.5 * (up + ue)) - sq(0.5 * (uw + up))) * x_rdxg;
+                    rr = rr - ((y_w0 * up + y_w1 * un) * 0.5 * (sV[uc][e] + sV[uc][e + 1]) -
+                               (y_w0m * us + y_w1m * up) * 0.5 *
+                                   (sV[uc][e - kHX] + sV[uc][e + 1 - kHX])) *
+                                  y_rdy;
                     rr = rr - (0.5 * (up + ut) * 0.5 * (sW[sc][e] + sW[sc][e + 1]) -
                                0.5 * (ub + up) * 0.5 * (sW[sm][e] + sW[sm][e + 1])) *
                                   rdzk;
                     if constexpr (!LES) {
                         rr = rr + dPdx +
-                             nu * ((m.rdx[i + 1] * (ue - up) - m.rdx[i] * (up - uw)) * m.rdxg[i] +
-                                   ((un - up) * m.rdyg[j] - (up - us) * m.rdyg[j - 1]) * m.rdy[j] +
+                             nu * ((x_rdx1 * (ue - up) - x_rdx * (up - uw)) * x_rdxg +
+                                   ((un - up) * y_rdyg - (up - us) * y_rdygm) * y_rdy +
                                    (rdzgk * (ut - up) - rdzgm * (up - ub)) * rdzk);
                     } else {
                         const T *N = sN[sc], *Nm = sN[sm], *Np = sN[sp];
-                        const T a0 = m.wy0[j - 1], a1 = m.wy1[j - 1], b0 = m.wy0[j], b1 = m.wy1[j];
+                        const T a0 = y_w0m, a1 = y_w1m, b0 = y_w0, b1 = y_w1;
                         const T nx1 = nu + N[e];
                         const T nx2 = nu + N[e + 1];
                         const T ny1 = nu + 0.5 * (a0 * N[e - kHX] + a1 * N[e] + a0 * N[e + 1 - kHX] + a1 * N[e + 1]);
@@ -214,15 +266,15 @@ __global__ __launch_bounds__(256) void k_rk_march(
                         const T nz1 = nu + 0.25 * (N[e] + Nm[e] + N[e + 1] + Nm[e + 1]);
                         const T nz2 = nu + 0.25 * (N[e] + Np[e] + N[e + 1] + Np[e + 1]);
                         rr = rr + dPdx +
-                             (nx2 * m.rdx[i + 1] * (ue - up) - nx1 * m.rdx[i] * (up - uw)) * m.rdxg[i] +
-                             (ny2 * ((un - up) * m.rdyg[j]) - ny1 * ((up - us) * m.rdyg[j - 1])) * m.rdy[j] +
+                             (nx2 * x_rdx1 * (ue - up) - nx1 * x_rdx * (up - uw)) * x_rdxg +
+                             (ny2 * ((un - up) * y_rdyg) - ny1 * ((up - us) * y_rdygm)) * y_rdy +
                              (nz2 * rdzgk * (ut - up) - nz1 * rdzgm * (up - ub)) * rdzk;
                     }
                 }
                 if (MODE == 2) {
                     Fou[p] = rr;
                 } else {
-                    const T base = old_is_cur ? sU[sc][e] : uo[r][0];
+                    const T base = old_is_cur ? sU[uc][e] : uo[r][0];
                     if (FOUT == 1) Fou[p] = rr;
                     if (FOUT == 2) Fou[p] = base + g2 * rr;
                     Un[p] = (MODE == 0) ? base + c0 * rr
@@ -233,38 +285,38 @@ __global__ __launch_bounds__(256) void k_rk_march(
             if (j <= d.ny - 2) {
                 T rr = 0.0;
                 if (i < d.nxg - 2) {
-                    const T vp = sV[sc][e], ve = sV[sc][e + 1], vw = sV[sc][e - 1];
-                    const T vn = sV[sc][e + kHX], vs = sV[sc][e - kHX];
-                    const T vt = sV[sp][e], vb = sV[sm][e];
-                    rr = -(sq(0.5 * (vp + vn)) - sq(0.5 * (vs + vp))) * m.rdyg[j];
-                    rr = rr - ((m.wy0[j] * sU[sc][e] + m.wy1[j] * sU[sc][e + kHX]) * 0.5 * (vp + ve) -
-                               (m.wy0[j] * sU[sc][e - 1] + m.wy1[j] * sU[sc][e - 1 + kHX]) * 0.5 * (vw + vp)) *
-                                  m.rdx[i];
-                    rr = rr - (0.5 * (vp + vt) * (m.wy0[j] * sW[sc][e] + m.wy1[j] * sW[sc][e + kHX]) -
-                               0.5 * (vb + vp) * (m.wy0[j] * sW[sm][e] + m.wy1[j] * sW[sm][e + kHX])) *
+                    const T vp = sV[uc][e], ve = sV[uc][e + 1], vw = sV[uc][e - 1];
+                    const T vn = sV[uc][e + kHX], vs = sV[uc][e - kHX];
+                    const T vt = sV[up1][e], vb = vbc;
+                    rr = -(sq(0.5 * (vp + vn)) - sq(0.5 * (vs + vp))) * y_rdyg;
+                    rr = rr - ((y_w0 * sU[uc][e] + y_w1 * sU[uc][e + kHX]) * 0.5 * (vp + ve) -
+                               (y_w0 * sU[uc][e - 1] + y_w1 * sU[uc][e - 1 + kHX]) * 0.5 * (vw + vp)) *
+                                  x_rdx;
+                    rr = rr - (0.5 * (vp + vt) * (y_w0 * sW[sc][e] + y_w1 * sW[sc][e + kHX]) -
+                               0.5 * (vb + vp) * (y_w0 * sW[sm][e] + y_w1 * sW[sm][e + kHX])) *
                                   rdzk;
                     if constexpr (!LES) {
-                        rr = rr + nu * ((m.rdxg[i] * (ve - vp) - m.rdxg[i - 1] * (vp - vw)) * m.rdx[i] +
-                                        (m.rdy[j + 1] * (vn - vp) - m.rdy[j] * (vp - vs)) * m.rdyg[j] +
+                        rr = rr + nu * ((x_rdxg * (ve - vp) - x_rdxgm * (vp - vw)) * x_rdx +
+                                        (y_rdy1 * (vn - vp) - y_rdy * (vp - vs)) * y_rdyg +
                                         (rdzgk * (vt - vp) - rdzgm * (vp - vb)) * rdzk);
                     } else {
                         const T *N = sN[sc], *Nm = sN[sm], *Np = sN[sp];
-                        const T b0 = m.wy0[j], b1 = m.wy1[j];
+                        const T b0 = y_w0, b1 = y_w1;
                         const T nx1 = nu + 0.5 * (b0 * N[e - 1] + b1 * N[e - 1 + kHX] + b0 * N[e] + b1 * N[e + kHX]);
                         const T nx2 = nu + 0.5 * (b0 * N[e] + b1 * N[e + kHX] + b0 * N[e + 1] + b1 * N[e + 1 + kHX]);
                         const T ny1 = nu + N[e];
                         const T ny2 = nu + N[e + kHX];
                         const T nz1 = nu + 0.5 * (b0 * Nm[e] + b1 * Nm[e + kHX] + b0 * N[e] + b1 * N[e + kHX]);
                         const T nz2 = nu + 0.5 * (b0 * N[e] + b1 * N[e + kHX] + b0 * Np[e] + b1 * Np[e + kHX]);
-                        rr = rr + m.rdx[i] * (nx2 * m.rdxg[i] * (ve - vp) - nx1 * m.rdxg[i - 1] * (vp - vw)) +
-                             m.rdyg[j] * (ny2 * m.rdy[j + 1] * (vn - vp) - ny1 * m.rdy[j] * (vp - vs)) +
+                        rr = rr + x_rdx * (nx2 * x_rdxg * (ve - vp) - nx1 * x_rdxgm * (vp - vw)) +
+                             y_rdyg * (ny2 * y_rdy1 * (vn - vp) - ny1 * y_rdy * (vp - vs)) +
                              rdzk * (nz2 * rdzgk * (vt - vp) - nz1 * rdzgm * (vp - vb));
                     }
                 }
                 if (MODE == 2) {
                     Fov[p] = rr;
                 } else {
-                    const T base = old_is_cur ? sV[sc][e] : uo[r][1];
+                    const T base = old_is_cur ? sV[uc][e] : uo[r][1];
                     if (FOUT == 1) Fov[p] = rr;
                     if (FOUT == 2) Fov[p] = base + g2 * rr;
                     Vn[p] = (MODE == 0) ? base + c0 * rr
@@ -279,20 +331,20 @@ __global__ __launch_bounds__(256) void k_rk_march(
                     const T wn = sW[sc][e + kHX], ws = sW[sc][e - kHX];
                     const T wt = sW[sp][e], wb = sW[sm][e];
                     rr = -(sq(0.5 * (wp + wt)) - sq(0.5 * (wb + wp))) * rdzgk;
-                    rr = rr - (0.5 * (sU[sc][e] + sU[sp][e]) * 0.5 * (wp + we) -
-                               0.5 * (sU[sc][e - 1] + sU[sp][e - 1]) * 0.5 * (ww + wp)) *
-                                  m.rdx[i];
-                    rr = rr - (0.5 * (sV[sc][e] + sV[sp][e]) * (m.wy0[j] * wp + m.wy1[j] * wn) -
-                               0.5 * (sV[sc][e - kHX] + sV[sp][e - kHX]) * (m.wy0[j - 1] * ws + m.wy1[j - 1] * wp)) *
-                                  m.rdy[j];
+                    rr = rr - (0.5 * (sU[uc][e] + sU[up1][e]) * 0.5 * (wp + we) -
+                               0.5 * (sU[uc][e - 1] + sU[up1][e - 1]) * 0.5 * (ww + wp)) *
+                                  x_rdx;
+                    rr = rr - (0.5 * (sV[uc][e] + sV[up1][e]) * (y_w0 * wp + y_w1 * wn) -
+                               0.5 * (sV[uc][e - kHX] + sV[up1][e - kHX]) * (y_w0m * ws + y_w1m * wp)) *
+                                  y_rdy;
                     if constexpr (!LES) {
                         rr = rr + dPdz +
-                             nu * ((m.rdxg[i] * (we - wp) - m.rdxg[i - 1] * (wp - ww)) * m.rdx[i] +
-                                   ((wn - wp) * m.rdyg[j] - (wp - ws) * m.rdyg[j - 1]) * m.rdy[j] +
+                             nu * ((x_rdxg * (we - wp) - x_rdxgm * (wp - ww)) * x_rdx +
+                                   ((wn - wp) * y_rdyg - (wp - ws) * y_rdygm) * y_rdy +
                                    (m.rdz[k + 1] * (wt - wp) - rdzk * (wp - wb)) * rdzgk);
                     } else {
                         const T *N = sN[sc], *Np = sN[sp];
-                        const T a0 = m.wy0[j - 1], a1 = m.wy1[j - 1], b0 = m.wy0[j], b1 = m.wy1[j];
+                        const T a0 = y_w0m, a1 = y_w1m, b0 = y_w0, b1 = y_w1;
                         const T nx1 = nu + 0.25 * (N[e] + Np[e] + N[e - 1] + Np[e - 1]);
                         const T nx2 = nu + 0.25 * (N[e] + Np[e] + N[e + 1] + Np[e + 1]);
                         const T ny1 = nu + 0.5 * (a0 * N[e - kHX] + a1 * N[e] + a0 * Np[e - kHX] + a1 * Np[e]);
@@ -300,8 +352,8 @@ __global__ __launch_bounds__(256) void k_rk_march(
                         const T nz1 = nu + N[e];
                         const T nz2 = nu + Np[e];
                         rr = rr + dPdz +
-                             (nx2 * m.rdxg[i] * (we - wp) - nx1 * m.rdxg[i - 1] * (wp - ww)) * m.rdx[i] +
-                             (ny2 * ((wn - wp) * m.rdyg[j]) - ny1 * ((wp - ws) * m.rdyg[j - 1])) * m.rdy[j] +
+                             (nx2 * x_rdxg * (we - wp) - nx1 * x_rdxgm * (wp - ww)) * x_rdx +
+                             (ny2 * ((wn - wp) * y_rdyg) - ny1 * ((wp - ws) * y_rdygm)) * y_rdy +
                              (nz2 * m.rdz[k + 1] * (wt - wp) - nz1 * rdzk * (wp - wb)) * rdzgk;
                     }
                 }
