@@ -173,12 +173,14 @@ __global__ __launch_bounds__(256) void k_rk_march(
     // a first stage starts from the field it differentiates: the old values
     // are then the LDS centre values and are not read again
     const bool old_is_cur = MODE == 0 && Uo == U && Vo == V && Wo == W;
-    // MODE 0 with a separate old field (G, or the RK2 second stage):
-    // its three values per cell are loaded one plane ahead of their use
-    // (the registers fit without lowering occupancy; MODE 1's six do not)
-    constexpr bool kAheadOld = MODE == 0 && FOUT == 0;
-    const bool ahead = kAheadOld && !old_is_cur;
-    T nuo[kAheadOld ? RPT : 1][3] = {};
+    // MODE 0 with a separate old field (G, or the RK2 second stage) and
+    // MODE 1: the old values per cell (and MODE 1's F1) are loaded one plane
+    // ahead of their use; with the metrics in LDS MODE 1's six values fit
+    // the registers at 3 blocks per CU too (FP32 RK stages 1.47 -> 1.39 ms
+    // per step at 814x257x193)
+    constexpr bool kAheadOld = (MODE == 0 && FOUT == 0) || MODE == 1;
+    const bool ahead = kAheadOld && (MODE == 1 || !old_is_cur);
+    T nuo[kAheadOld ? RPT : 1][3] = {}, nfo[MODE == 1 ? RPT : 1][3] = {};
     auto fetch_old = [&](int k) {
 #pragma unroll
         for (int r = 0; r < (kAheadOld ? RPT : 1); ++r) {
@@ -186,6 +188,11 @@ __global__ __launch_bounds__(256) void k_rk_march(
             nuo[r][0] = Uo[p];
             nuo[r][1] = Vo[p];
             nuo[r][2] = Wo[p];
+            if constexpr (MODE == 1) {
+                nfo[r][0] = F1u[p];
+                nfo[r][1] = F1v[p];
+                nfo[r][2] = F1w[p];
+            }
         }
     };
     if (ahead) fetch_old(kb);
@@ -197,7 +204,10 @@ __global__ __launch_bounds__(256) void k_rk_march(
 #pragma unroll
             for (int r = 0; r < (kAheadOld ? RPT : 1); ++r)
 #pragma unroll
-                for (int c = 0; c < 3; ++c) uo[r][c] = nuo[r][c];
+                for (int c = 0; c < 3; ++c) {
+                    uo[r][c] = nuo[r][c];
+                    if constexpr (MODE == 1) fo[r][c] = nfo[r][c];
+                }
         } else if (MODE == 1 || (MODE == 0 && !old_is_cur)) {
 #pragma unroll
             for (int r = 0; r < RPT; ++r) {
